@@ -1,0 +1,229 @@
+/*
+ * mpm_gpu.h -- C ABI of the MI355X (gfx950) MPM snow substep.
+ *
+ * The library replaces the CPU substep behind the reference's
+ * MPM_Example / MPM_Driver surface.  Every entry point below names the
+ * reference interface it stands in for (paths are relative to the
+ * reference repository SoldierDown/Non_Ficks_Diffusion).
+ *
+ * Conventions
+ *   - extern "C", plain pointers and sizes; no C++ or torch types.
+ *   - Every function returns an int status: MPM_OK (0) on success, a
+ *     negative MPM_E_* code otherwise; mpm_last_error() gives the text.
+ *     (The reference returns void everywhere and ignores failures, e.g.
+ *     the CG result at MPM_Example.cpp:711.)
+ *   - Host buffers are owned by the caller; device memory is owned by the
+ *     library.  One HIP stream per handle.  A handle is not thread safe:
+ *     one host thread drives one handle, as one thread drives
+ *     MPM_Driver in the reference (mpm_3d/main.cpp:16-24).
+ *   - Particle arrays on the host are particle-major: x[i*dim+a],
+ *     F[(i*dim+a)*dim+b] (row a, column b of particle i's F).
+ *   - Domain is [0,1]^dim with counts[0] cells per axis (cubic cells,
+ *     dx = 1/counts[0]); grid nodes sit at cell centres
+ *     (mpm_3d/Standard_Tests/Standard_Tests.h:44, MPM_Particle.h:81-83).
+ */
+#ifndef MPM_GPU_H
+#define MPM_GPU_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define MPM_OK 0
+#define MPM_E_INVALID (-1)       /* bad argument */
+#define MPM_E_HIP (-2)           /* HIP runtime error */
+#define MPM_E_CAPACITY (-3)      /* particle or block capacity exceeded */
+#define MPM_E_NOT_READY (-4)     /* call order violated (e.g. substep before initialize) */
+#define MPM_E_NONFINITE (-5)     /* NaN/Inf detected in particle state */
+#define MPM_E_BUDGET (-6)        /* serving admission: over the HBM budget */
+
+#define MPM_MAX_PLANES 8
+
+/* Node flag bits (the reference uses Nova's SPGrid flags,
+ * Flag_Setup_Helper.h:32 sets Cell_Type_Interior where m > 0). */
+#define MPM_FLAG_INTERIOR 1u
+
+/* MPM_Plane_Barrier (MPM_Plane_Barrier.h:13-32): half-space collider with
+ * Coulomb friction mu; the node is inside when (x - surface).normal < 0. */
+typedef struct mpm_plane {
+    float normal[3];
+    float surface[3];
+    float mu;
+    int32_t sticky; /* unused, as in the reference (MPM_Plane_Barrier.h:23) */
+} mpm_plane;
+
+/* Snow material (MPM_Constitutive_Model.h:78-93), per scene instead of per
+ * particle.  mu0/lambda0 from Compute_Lame_Parameters
+ * (MPM_Constitutive_Model.cpp:22-27). */
+typedef struct mpm_material {
+    float mu0;
+    float lambda0;
+    float stretching_yield;  /* 1 + theta_s */
+    float compression_yield; /* 1 - theta_c */
+    float hardening_factor;  /* xi */
+    float hardening_max;     /* FLT_MAX in the reference */
+    int32_t plastic;
+} mpm_material;
+
+/* Scene description: what MPM_Example::Parse_Options and the scene hook
+ * Initialize_Particles set up (MPM_Example.cpp:73-119, 925-935;
+ * mpm_3d/Standard_Tests/Standard_Tests.h:39-154), with the reference's
+ * hard-coded constants turned into fields. */
+typedef struct mpm_scene_desc {
+    int32_t dim;                /* 2 or 3 */
+    int32_t counts;             /* cells per axis (-size) */
+    float gravity[3];           /* (0,-2,0) hard-coded at MPM_Example.cpp:81 */
+    float flip;                 /* 0.9 (MPM_Example.cpp:82) */
+    float cfl;                  /* 0.1 (-cfl default, MPM_Example.cpp:862) */
+    float min_dt, max_dt;       /* 1e-6, 5e-3 (MPM_Driver.cpp:58-59) */
+    float frame_rate;           /* frames per second of Time_At_Frame */
+    float cg_tolerance;         /* 1e-7 in the reference (MPM_Example.cpp:78) */
+    int32_t cg_max_iterations;  /* 10000 in the reference (MPM_Example.cpp:79) */
+    int32_t cg_min_iterations;  /* 0 (MPM_Example.cpp:711) */
+    int32_t nplanes;
+    mpm_plane planes[MPM_MAX_PLANES];
+    mpm_material material;
+    /* 1: particle->grid scatters skip nodes whose x index lies outside
+     * [0, counts-1], which is what the reference's x-slab crop does
+     * (Tools/Cropped_Influence_Iterator.h:31-42 with the slab bounds of
+     * MPM_Example.cpp:468-474).  0: every ghost node receives scatters. */
+    int32_t crop_ghost_x;
+    /* CG Project (Implicit_Force_Helper/Project_Helper.h:30-45).  The
+     * reference's predicate compares Nova's Vector with a scalar
+     * (Project_Helper.h:39), whose meaning Nova fixes:
+     * 1 = no-op (the predicate is false for every plane the reference and
+     *     the scene presets define; the presets' default),
+     * 0 = zero the CG vector at every node inside a plane
+     *     ((x_i - s).n < 0). */
+    int32_t project_mode;
+} mpm_scene_desc;
+
+typedef struct mpm_step_stats {
+    int32_t cg_iterations;      /* completed CG updates (I) */
+    int32_t cg_multiplies;      /* operator applications (M = I + 1) */
+    int32_t cg_converged;
+    int32_t active_blocks;      /* 4^3 (3D) / 8^2 (2D) blocks with storage */
+    int64_t live_particles;     /* after removal */
+    double cg_residual;         /* convergence norm at exit (L-inf of |r|) */
+    double max_speed;           /* max |V| over particles after the substep */
+} mpm_step_stats;
+
+typedef struct mpm_context* mpm_handle;
+
+/* ---- library ---------------------------------------------------------- */
+const char* mpm_last_error(void);
+/* Writes "<device name>;<CUs>;<arch>" into buf. */
+int mpm_device_info(char* buf, int32_t len);
+
+/* ---- scene lifecycle (MPM_Example constructor / destructor,
+ *      MPM_Example.cpp:73-135) ------------------------------------------ */
+/* Creates a scene with room for `capacity` particles. */
+int mpm_create(const mpm_scene_desc* desc, int64_t capacity, mpm_handle* out);
+int mpm_destroy(mpm_handle h);
+
+/* Replaces MPM_Example::particles being filled by Initialize_Particles
+ * (MPM_Example.h:306, mpm_3d/Standard_Tests/Standard_Tests.h:47-154).
+ * volume may be NULL: the particles then wait for
+ * Process_Waiting_Particles (MPM_Example.cpp:827-840) in mpm_initialize.
+ * F and Jp may be NULL (identity and 1). */
+int mpm_upload_particles(mpm_handle h, int64_t n, const float* x, const float* v,
+                         const float* F, const float* Jp, const float* mass,
+                         const float* volume);
+
+/* Copies live particles back in upload order (ids ascending); removed
+ * particles are skipped, their ids missing from `ids`.  Any output
+ * pointer may be NULL.  Replaces reading MPM_Example::particles
+ * (MPM_Example.h:271) and Write_Output_Files (MPM_Example.cpp:1049-1063). */
+int mpm_download_particles(mpm_handle h, int64_t cap, float* x, float* v, float* F,
+                           float* Jp, float* mass, float* volume, int32_t* ids,
+                           int64_t* n_out);
+
+/* MPM_Driver::Initialize (MPM_Driver.cpp:18-32): one grid build, P2G and
+ * Process_Waiting_Particles for particles uploaded without volume. */
+int mpm_initialize(mpm_handle h);
+
+/* MPM_Driver::Advance_Step (MPM_Driver.cpp:103-116): one substep with the
+ * given dt.  Asynchronous unless stats != NULL (then it waits and fills
+ * stats). */
+int mpm_substep(mpm_handle h, float dt, mpm_step_stats* stats);
+
+/* Substep whose dt is computed on the device from the current max speed
+ * (MPM_Driver.cpp:55-80): dt = clamp(cfl*dx/max(vmax,1e-2), min_dt, max_dt),
+ * then shortened so the substep ends at or before target_time.  time is
+ * device-resident; *time_inout is read at the first call of a frame and
+ * written back when stats are requested. */
+int mpm_advance_to_time(mpm_handle h, double time, double target_time,
+                        int32_t* substeps_out, double* time_out, mpm_step_stats* stats);
+
+/* MPM_Example::Max_Particle_Velocity (MPM_Example.cpp:375-387). */
+int mpm_max_particle_velocity(mpm_handle h, float* out);
+
+int mpm_synchronize(mpm_handle h);
+
+/* ---- per-stage entry points for parity (MPM_Driver.cpp:103-116) ------ */
+enum mpm_stage {
+    MPM_STAGE_BIN = 0,          /* Initialize_SPGrid + Update_Particle_Weights + Group_Particles
+                                   + Rasterize_Voxels (MPM_Example.cpp:325-331, 418-427, 464-485, 291-311) */
+    MPM_STAGE_P2G = 1,          /* Rasterize incl. flags and v/=m (MPM_Example.cpp:520-547) */
+    MPM_STAGE_WAITING = 2,      /* Process_Waiting_Particles (MPM_Example.cpp:827-840) */
+    MPM_STAGE_CONSTITUTIVE = 3, /* Update_Constitutive_Model_State (MPM_Example.cpp:563-571) */
+    MPM_STAGE_EXPLICIT_FORCE = 4, /* Apply_Explicit_Force (MPM_Example.cpp:757-790) */
+    MPM_STAGE_COLLISION = 5,    /* Grid_Based_Collision (MPM_Example.cpp:802-806) */
+    MPM_STAGE_CG = 6,           /* RHS + Reset_Solver_Channels + CG solve (MPM_Example.cpp:700-711) */
+    MPM_STAGE_G2P = 7           /* G2P, F and X update, removal (MPM_Example.cpp:631-660) */
+};
+int mpm_run_stage(mpm_handle h, int32_t stage, float dt);
+
+/* Grid channels readable after a stage (names follow MPM_Data.h:12-46). */
+enum mpm_channel {
+    MPM_CH_MASS = 0,            /* ch0 */
+    MPM_CH_VELOCITY = 1,        /* ch1..3 (momentum after scatter, velocity after P2G) */
+    MPM_CH_VELOCITY_STAR = 2,   /* ch4..6 (also the CG solution) */
+    MPM_CH_FORCE = 3,           /* ch7..9 */
+    MPM_CH_CG_R = 4,            /* ch19..21 */
+    MPM_CH_CG_S = 5,            /* ch16..18 */
+    MPM_CH_CG_Q = 6             /* ch13..15 */
+};
+/* Dense copy of a grid channel over node indices [-1, counts]^dim, i.e.
+ * (counts+2)^dim nodes, x slowest; vector channels give dim values per
+ * node.  Nodes outside active blocks read 0. */
+int mpm_read_grid(mpm_handle h, int32_t channel, float* out, int64_t out_len);
+/* Dense node flags over the same index range. */
+int mpm_read_flags(mpm_handle h, uint32_t* out, int64_t out_len);
+/* Binning result: active block coordinates (dim ints each, block index =
+ * (node index + B) / B with B = 4 in 3D, 8 in 2D), particles homed in each
+ * block, and for every live particle its home block's ordinal and its id.
+ * Any pointer may be NULL; counts are returned through nblocks/nparticles. */
+int mpm_read_bins(mpm_handle h, int64_t cap_blocks, int32_t* block_coords,
+                  int32_t* block_particle_counts, int64_t cap_particles,
+                  int32_t* particle_block, int32_t* particle_ids, int32_t* nblocks,
+                  int64_t* nparticles);
+/* CG diagnostics of the last solve. */
+int mpm_cg_stats(mpm_handle h, mpm_step_stats* stats);
+
+/* ---- timing (HIP events on the handle's stream) ------------------------ */
+/* Enables per-kernel-class event timing; mpm_kernel_times returns the
+ * accumulated milliseconds and launch counts per class since the last
+ * reset.  Class ids: see mpm_kernel_class_name. */
+int mpm_enable_timing(mpm_handle h, int32_t on);
+int mpm_kernel_times(mpm_handle h, int32_t nclasses, double* ms, int64_t* launches);
+const char* mpm_kernel_class_name(int32_t cls);
+int32_t mpm_kernel_class_count(void);
+
+/* ---- scenes (scene hook Initialize_Particles) ------------------------- */
+/* Fills desc with the preset of a named scene and returns the particle
+ * count it generates.  Known names: "cfg0_disk2d", "cfg1_two_balls",
+ * "cfg2_avalanche", "ref3d_test18", "ref2d_test18", "blob", "slab". */
+int mpm_scene_preset(const char* name, int32_t counts, uint64_t seed, mpm_scene_desc* desc,
+                     int64_t* n_particles);
+/* Generates the particles of a named scene (same arguments as the preset);
+ * arrays must hold n_particles entries. */
+int mpm_scene_generate(const char* name, int32_t counts, uint64_t seed, int64_t cap, float* x,
+                       float* v, float* mass, int64_t* n_out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MPM_GPU_H */

@@ -1,0 +1,300 @@
+"""Python mirror of the libmpm_gpu C ABI (include/mpm_gpu.h).
+
+The MI355X MPM snow substep is C++/HIP behind a C ABI; this module binds
+that ABI with ctypes for tests and benchmarks.  It mirrors the reference's
+MPM_Example / MPM_Driver surface (MPM_Example.h:229-336,
+MPM_Driver.h:13-36): a `Scene` owns the device-resident particles and
+grid of one simulation; `initialize()` is MPM_Driver::Initialize,
+`substep(dt)` is MPM_Driver::Advance_Step, `run_stage` runs one of its
+stages.  There is no CPU fallback: importing works anywhere, but every
+call that reaches the device raises if the HIP library is missing or the
+runtime has no GPU.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Dict, Optional, Tuple
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libmpm_gpu.so")
+
+MPM_MAX_PLANES = 8
+MPM_FLAG_INTERIOR = 1
+
+STAGE_BIN, STAGE_P2G, STAGE_WAITING, STAGE_CONSTITUTIVE = 0, 1, 2, 3
+STAGE_EXPLICIT_FORCE, STAGE_COLLISION, STAGE_CG, STAGE_G2P = 4, 5, 6, 7
+
+CH_MASS, CH_VELOCITY, CH_VELOCITY_STAR, CH_FORCE = 0, 1, 2, 3
+CH_CG_R, CH_CG_S, CH_CG_Q = 4, 5, 6
+
+
+class MPMError(RuntimeError):
+    """A libmpm_gpu call returned a non-zero status."""
+
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"libmpm_gpu error {code}: {msg}")
+        self.code = code
+
+
+class Plane(ctypes.Structure):
+    _fields_ = [("normal", ctypes.c_float * 3), ("surface", ctypes.c_float * 3),
+                ("mu", ctypes.c_float), ("sticky", ctypes.c_int32)]
+
+
+class Material(ctypes.Structure):
+    _fields_ = [("mu0", ctypes.c_float), ("lambda0", ctypes.c_float),
+                ("stretching_yield", ctypes.c_float), ("compression_yield", ctypes.c_float),
+                ("hardening_factor", ctypes.c_float), ("hardening_max", ctypes.c_float),
+                ("plastic", ctypes.c_int32)]
+
+
+class SceneDesc(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("counts", ctypes.c_int32),
+                ("gravity", ctypes.c_float * 3), ("flip", ctypes.c_float), ("cfl", ctypes.c_float),
+                ("min_dt", ctypes.c_float), ("max_dt", ctypes.c_float), ("frame_rate", ctypes.c_float),
+                ("cg_tolerance", ctypes.c_float), ("cg_max_iterations", ctypes.c_int32),
+                ("cg_min_iterations", ctypes.c_int32), ("nplanes", ctypes.c_int32),
+                ("planes", Plane * MPM_MAX_PLANES), ("material", Material),
+                ("crop_ghost_x", ctypes.c_int32), ("project_mode", ctypes.c_int32)]
+
+    def copy(self) -> "SceneDesc":
+        d = SceneDesc()
+        ctypes.memmove(ctypes.byref(d), ctypes.byref(self), ctypes.sizeof(SceneDesc))
+        return d
+
+
+class StepStats(ctypes.Structure):
+    _fields_ = [("cg_iterations", ctypes.c_int32), ("cg_multiplies", ctypes.c_int32),
+                ("cg_converged", ctypes.c_int32), ("active_blocks", ctypes.c_int32),
+                ("live_particles", ctypes.c_int64), ("cg_residual", ctypes.c_double),
+                ("max_speed", ctypes.c_double)]
+
+    def as_dict(self) -> Dict[str, float]:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+_FP = ctypes.POINTER(ctypes.c_float)
+_IP = ctypes.POINTER(ctypes.c_int32)
+_UP = ctypes.POINTER(ctypes.c_uint32)
+_LP = ctypes.POINTER(ctypes.c_int64)
+_H = ctypes.c_void_p
+
+# symbol -> (restype, argtypes); every function declared in include/mpm_gpu.h
+SIGNATURES = {
+    "mpm_last_error": (ctypes.c_char_p, []),
+    "mpm_device_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32]),
+    "mpm_create": (ctypes.c_int, [ctypes.POINTER(SceneDesc), ctypes.c_int64, ctypes.POINTER(_H)]),
+    "mpm_destroy": (ctypes.c_int, [_H]),
+    "mpm_upload_particles": (ctypes.c_int, [_H, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP]),
+    "mpm_download_particles": (ctypes.c_int, [_H, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _IP, _LP]),
+    "mpm_initialize": (ctypes.c_int, [_H]),
+    "mpm_substep": (ctypes.c_int, [_H, ctypes.c_float, ctypes.POINTER(StepStats)]),
+    "mpm_advance_to_time": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_double, _IP,
+                                           ctypes.POINTER(ctypes.c_double), ctypes.POINTER(StepStats)]),
+    "mpm_max_particle_velocity": (ctypes.c_int, [_H, _FP]),
+    "mpm_synchronize": (ctypes.c_int, [_H]),
+    "mpm_run_stage": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_float]),
+    "mpm_read_grid": (ctypes.c_int, [_H, ctypes.c_int32, _FP, ctypes.c_int64]),
+    "mpm_read_flags": (ctypes.c_int, [_H, _UP, ctypes.c_int64]),
+    "mpm_read_bins": (ctypes.c_int, [_H, ctypes.c_int64, _IP, _IP, ctypes.c_int64, _IP, _IP, _IP, _LP]),
+    "mpm_cg_stats": (ctypes.c_int, [_H, ctypes.POINTER(StepStats)]),
+    "mpm_enable_timing": (ctypes.c_int, [_H, ctypes.c_int32]),
+    "mpm_kernel_times": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), _LP]),
+    "mpm_kernel_class_name": (ctypes.c_char_p, [ctypes.c_int32]),
+    "mpm_kernel_class_count": (ctypes.c_int32, []),
+    "mpm_scene_preset": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint64,
+                                        ctypes.POINTER(SceneDesc), _LP]),
+    "mpm_scene_generate": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64,
+                                          _FP, _FP, _FP, _LP]),
+}
+
+_lib: Optional[ctypes.CDLL] = None
+
+
+def lib() -> ctypes.CDLL:
+    """Loads libmpm_gpu.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc: int) -> None:
+    if rc != 0:
+        raise MPMError(rc, lib().mpm_last_error().decode())
+
+
+def _fp(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_FP)
+
+
+def _f32(a) -> np.ndarray:
+    return np.ascontiguousarray(a, dtype=np.float32)
+
+
+def device_info() -> str:
+    buf = ctypes.create_string_buffer(256)
+    _check(lib().mpm_device_info(buf, 256))
+    return buf.value.decode()
+
+
+def scene_preset(name: str, counts: int = 0, seed: int = 0) -> Tuple[SceneDesc, int]:
+    d = SceneDesc()
+    n = ctypes.c_int64()
+    _check(lib().mpm_scene_preset(name.encode(), counts, seed, ctypes.byref(d), ctypes.byref(n)))
+    return d, n.value
+
+
+def scene_generate(name: str, counts: int = 0, seed: int = 0):
+    """Returns (desc, x[n,dim], v[n,dim], mass[n]) of a named scene (scene hook
+    Initialize_Particles, mpm_3d/Standard_Tests/Standard_Tests.h:47-154)."""
+    d, n = scene_preset(name, counts, seed)
+    x = np.zeros((n, d.dim), np.float32)
+    v = np.zeros((n, d.dim), np.float32)
+    m = np.zeros(n, np.float32)
+    out = ctypes.c_int64()
+    _check(lib().mpm_scene_generate(name.encode(), counts, seed, n, _fp(x), _fp(v), _fp(m), ctypes.byref(out)))
+    return d, x, v, m
+
+
+def kernel_class_names():
+    L = lib()
+    return [L.mpm_kernel_class_name(k).decode() for k in range(L.mpm_kernel_class_count())]
+
+
+class Scene:
+    """One device-resident simulation (MPM_Example + MPM_Driver on the GPU)."""
+
+    def __init__(self, desc: SceneDesc, capacity: int):
+        self.desc = desc.copy()
+        self.dim = desc.dim
+        self.counts = desc.counts
+        self.capacity = int(capacity)
+        h = _H()
+        _check(lib().mpm_create(ctypes.byref(self.desc), int(capacity), ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().mpm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # -- particles ---------------------------------------------------------
+    def upload(self, x, v, mass, F=None, Jp=None, volume=None) -> None:
+        x, v, mass = _f32(x), _f32(v), _f32(mass)
+        n = mass.shape[0]
+        F = None if F is None else _f32(F)
+        Jp = None if Jp is None else _f32(Jp)
+        volume = None if volume is None else _f32(volume)
+        _check(lib().mpm_upload_particles(self._h, n, _fp(x), _fp(v), _fp(F), _fp(Jp), _fp(mass), _fp(volume)))
+
+    def download(self, cap: Optional[int] = None) -> Dict[str, np.ndarray]:
+        d = self.dim
+        if cap is None:
+            cap = self.capacity
+        out = {"x": np.zeros((cap, d), np.float32), "v": np.zeros((cap, d), np.float32),
+               "F": np.zeros((cap, d, d), np.float32), "Jp": np.zeros(cap, np.float32),
+               "mass": np.zeros(cap, np.float32), "volume": np.zeros(cap, np.float32),
+               "ids": np.zeros(cap, np.int32)}
+        n = ctypes.c_int64()
+        _check(lib().mpm_download_particles(self._h, cap, _fp(out["x"]), _fp(out["v"]), _fp(out["F"]),
+                                            _fp(out["Jp"]), _fp(out["mass"]), _fp(out["volume"]),
+                                            out["ids"].ctypes.data_as(_IP), ctypes.byref(n)))
+        return {k: a[:n.value] for k, a in out.items()}
+
+    # -- driver ------------------------------------------------------------
+    def initialize(self) -> None:
+        _check(lib().mpm_initialize(self._h))
+
+    def substep(self, dt: float, stats: bool = False):
+        if stats:
+            s = StepStats()
+            _check(lib().mpm_substep(self._h, float(dt), ctypes.byref(s)))
+            return s.as_dict()
+        _check(lib().mpm_substep(self._h, float(dt), None))
+        return None
+
+    def advance_to_time(self, time: float, target: float):
+        n = ctypes.c_int32()
+        t = ctypes.c_double()
+        s = StepStats()
+        _check(lib().mpm_advance_to_time(self._h, time, target, ctypes.byref(n), ctypes.byref(t), ctypes.byref(s)))
+        return t.value, n.value, s.as_dict()
+
+    def max_particle_velocity(self) -> float:
+        v = ctypes.c_float()
+        _check(lib().mpm_max_particle_velocity(self._h, ctypes.byref(v)))
+        return v.value
+
+    def synchronize(self) -> None:
+        _check(lib().mpm_synchronize(self._h))
+
+    def run_stage(self, stage: int, dt: float = 0.0) -> None:
+        _check(lib().mpm_run_stage(self._h, int(stage), float(dt)))
+
+    def stats(self) -> Dict[str, float]:
+        s = StepStats()
+        _check(lib().mpm_cg_stats(self._h, ctypes.byref(s)))
+        return s.as_dict()
+
+    # -- inspection --------------------------------------------------------
+    def _dense_nodes(self) -> int:
+        return (self.counts + 2) ** self.dim
+
+    def read_grid(self, channel: int) -> np.ndarray:
+        w = 1 if channel == CH_MASS else self.dim
+        out = np.zeros(self._dense_nodes() * w, np.float32)
+        _check(lib().mpm_read_grid(self._h, channel, _fp(out), out.size))
+        shape = (self.counts + 2,) * self.dim + ((w,) if w > 1 else ())
+        return out.reshape(shape)
+
+    def read_flags(self) -> np.ndarray:
+        out = np.zeros(self._dense_nodes(), np.uint32)
+        _check(lib().mpm_read_flags(self._h, out.ctypes.data_as(_UP), out.size))
+        return out.reshape((self.counts + 2,) * self.dim)
+
+    def read_bins(self):
+        nb = ctypes.c_int32()
+        npart = ctypes.c_int64()
+        _check(lib().mpm_read_bins(self._h, 0, None, None, 0, None, None, ctypes.byref(nb), ctypes.byref(npart)))
+        coords = np.zeros((max(nb.value, 1), self.dim), np.int32)
+        counts = np.zeros(max(nb.value, 1), np.int32)
+        pblock = np.zeros(max(npart.value, 1), np.int32)
+        pids = np.zeros(max(npart.value, 1), np.int32)
+        _check(lib().mpm_read_bins(self._h, coords.shape[0], coords.ctypes.data_as(_IP), counts.ctypes.data_as(_IP),
+                                   pids.shape[0], pblock.ctypes.data_as(_IP), pids.ctypes.data_as(_IP),
+                                   ctypes.byref(nb), ctypes.byref(npart)))
+        return coords[:nb.value], counts[:nb.value], pblock[:npart.value], pids[:npart.value]
+
+    # -- timing ------------------------------------------------------------
+    def enable_timing(self, on: bool = True) -> None:
+        _check(lib().mpm_enable_timing(self._h, 1 if on else 0))
+
+    def kernel_times(self) -> Dict[str, Tuple[float, int]]:
+        names = kernel_class_names()
+        ms = (ctypes.c_double * len(names))()
+        n = (ctypes.c_int64 * len(names))()
+        _check(lib().mpm_kernel_times(self._h, len(names), ms, n))
+        return {names[k]: (ms[k], n[k]) for k in range(len(names))}

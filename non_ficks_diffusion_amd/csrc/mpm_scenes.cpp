@@ -1,0 +1,308 @@
+// Scene hook of the GPU path: the seeded particle set-ups that the
+// reference builds in Standard_Tests::Initialize_Particles
+// (mpm_3d/Standard_Tests/Standard_Tests.h:47-154,
+// mpm_2d/Standard_Tests/Standard_Tests.h:47-323), plus the snow scenes the
+// benchmark configurations name.
+//
+// Sampling: stratified jitter.  Every cell of the shape's bounding box is
+// split into ppc sub-cells (2 per axis), each sub-cell gets one uniform
+// sample from a counter-based generator keyed by (seed, cell, sub-cell),
+// and samples inside the shape are kept.  Particle mass is
+// density * dx^dim / ppc.  The reference's own test 18 scenes draw a fixed
+// number of uniform samples inside a sphere (Random_Numbers::
+// Get_Vector_In_Sphere); those are reproduced by rejection sampling from
+// the same counter-based generator (Nova's generator is not available, so
+// the positions are new, the statistics are the same).
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+#include <algorithm>
+#include <cfloat>
+
+#include "../../include/mpm_gpu.h"
+#include "mpm_internal.h"
+
+namespace {
+
+inline uint64_t mix64(uint64_t z) {
+    // SplitMix64 finaliser (Steele, Lea, Flood 2014).
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline uint64_t key3(uint64_t seed, uint64_t a, uint64_t b) {
+    return mix64(mix64(mix64(seed) ^ a) ^ b);
+}
+// 24 random bits -> float in [0,1)
+inline float unit_float(uint64_t h) { return (float)(h >> 40) * (1.0f / 16777216.0f); }
+
+enum ShapeKind { SPHERE = 0, OBOX = 1 };
+struct Shape {
+    ShapeKind kind;
+    double c[3];
+    double r;            // sphere radius
+    double axis[3][3];   // box axes (unit)
+    double half[3];      // box half extents
+    float v[3];          // initial velocity
+    bool inside(const double* p, int dim) const {
+        if (kind == SPHERE) {
+            double s = 0;
+            for (int a = 0; a < dim; ++a) s += (p[a] - c[a]) * (p[a] - c[a]);
+            return s <= r * r;
+        }
+        for (int k = 0; k < dim; ++k) {
+            double s = 0;
+            for (int a = 0; a < dim; ++a) s += (p[a] - c[a]) * axis[k][a];
+            if (std::fabs(s) > half[k]) return false;
+        }
+        return true;
+    }
+    void bounds(int dim, double* lo, double* hi) const {
+        for (int a = 0; a < dim; ++a) {
+            double e;
+            if (kind == SPHERE) e = r;
+            else {
+                e = 0;
+                for (int k = 0; k < dim; ++k) e += std::fabs(axis[k][a]) * half[k];
+            }
+            lo[a] = c[a] - e;
+            hi[a] = c[a] + e;
+        }
+    }
+};
+
+struct Spec {
+    mpm_scene_desc desc;
+    std::vector<Shape> shapes;
+    double density = 400.0;
+    int fixed_count = 0;     // >0: uniform samples in each sphere (reference test 18 style)
+    double fixed_mass_density = 2.0;
+};
+
+void snow_material(mpm_material& m, double E, double nu, double theta_c, double theta_s, double xi) {
+    m.mu0 = (float)(E / (2.0 * (1.0 + nu)));
+    m.lambda0 = (float)(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+    m.compression_yield = (float)(1.0 - theta_c);
+    m.stretching_yield = (float)(1.0 + theta_s);
+    m.hardening_factor = (float)xi;
+    m.hardening_max = FLT_MAX;
+    m.plastic = 1;
+}
+
+void base_desc(mpm_scene_desc& d, int dim, int counts) {
+    std::memset(&d, 0, sizeof(d));
+    d.dim = dim;
+    d.counts = counts;
+    d.gravity[1] = -9.8f;
+    d.flip = 0.9f;
+    d.cfl = 0.1f;
+    d.min_dt = 1e-6f;
+    d.max_dt = 5e-3f;
+    d.frame_rate = 24.0f;
+    d.cg_tolerance = 1e-6f;
+    d.cg_max_iterations = 50;
+    d.cg_min_iterations = 0;
+    d.crop_ghost_x = 1;
+    d.project_mode = 1;
+    // rho=400, E=1.4e5, nu=0.2, theta_c=2.5e-2, theta_s=7.5e-3, xi=10 (Stomakhin et al. 2013)
+    snow_material(d.material, 1.4e5, 0.2, 2.5e-2, 7.5e-3, 10.0);
+}
+
+void add_plane(mpm_scene_desc& d, double nx, double ny, double nz, double sx, double sy, double sz) {
+    mpm_plane& p = d.planes[d.nplanes++];
+    double l = std::sqrt(nx * nx + ny * ny + nz * nz);
+    p.normal[0] = (float)(nx / l);
+    p.normal[1] = (float)(ny / l);
+    p.normal[2] = (float)(nz / l);
+    p.surface[0] = (float)sx;
+    p.surface[1] = (float)sy;
+    p.surface[2] = (float)sz;
+    p.mu = 0.0f;
+    p.sticky = 0;
+}
+
+Shape sphere(double x, double y, double z, double r, float vx, float vy, float vz) {
+    Shape s{};
+    s.kind = SPHERE;
+    s.c[0] = x; s.c[1] = y; s.c[2] = z;
+    s.r = r;
+    s.v[0] = vx; s.v[1] = vy; s.v[2] = vz;
+    return s;
+}
+
+bool build_spec(const std::string& name, int counts, uint64_t seed, Spec& sp) {
+    mpm_scene_desc& d = sp.desc;
+    if (name == "cfg0_disk2d") {
+        // 2D snow disk r=0.1 at (0.5,0.7), V=(0,-2), ppc 4, ground y=0.05.
+        base_desc(d, 2, counts > 0 ? counts : 64);
+        add_plane(d, 0, 1, 0, 0, 0.05, 0);
+        sp.shapes.push_back(sphere(0.5, 0.7, 0, 0.1, 0.f, -2.f, 0.f));
+    } else if (name == "cfg1_two_balls") {
+        // Two snowballs r=0.12 at x=0.3 and x=0.7 moving head-on at 3 m/s each, ground y=0.05.
+        base_desc(d, 3, counts > 0 ? counts : 128);
+        add_plane(d, 0, 1, 0, 0, 0.05, 0);
+        sp.shapes.push_back(sphere(0.3, 0.5, 0.5, 0.12, 3.f, 0.f, 0.f));
+        sp.shapes.push_back(sphere(0.7, 0.5, 0.5, 0.12, -3.f, 0.f, 0.f));
+    } else if (name == "cfg2_avalanche") {
+        // Snow slab 0.6 x 0.1 x 0.125 resting on a 30-degree incline; ground y=0.05 and
+        // a wall at x=0.05 catch the slide.
+        base_desc(d, 3, counts > 0 ? counts : 256);
+        const double s30 = 0.5, c30 = std::sqrt(3.0) / 2.0;
+        const double p0[3] = {0.1, 0.1, 0.5};
+        add_plane(d, -s30, c30, 0, p0[0], p0[1], p0[2]);
+        add_plane(d, 0, 1, 0, 0, 0.05, 0);
+        add_plane(d, 1, 0, 0, 0.05, 0, 0);
+        Shape s{};
+        s.kind = OBOX;
+        const double t[3] = {c30, s30, 0}, n[3] = {-s30, c30, 0}, z[3] = {0, 0, 1};
+        for (int a = 0; a < 3; ++a) {
+            s.c[a] = p0[a] + 0.6 * t[a] + 0.05 * n[a];
+            s.axis[0][a] = t[a];
+            s.axis[1][a] = n[a];
+            s.axis[2][a] = z[a];
+        }
+        s.half[0] = 0.3; s.half[1] = 0.05; s.half[2] = 0.0625;
+        sp.shapes.push_back(s);
+    } else if (name == "ref3d_test18" || name == "ref2d_test18") {
+        // Reference test 18: two balls r=0.1 at x=0.8 / 0.2 with V=(-2,0.1) / (2,0.1),
+        // E=30, nu=0.4, yields 1.01 / 0.95, xi=10, mass density 2, ground y=0.1,
+        // g=(0,-2,0), CG tolerance 1e-7 (mpm_3d/Standard_Tests/Standard_Tests.h:51-112,
+        // mpm_2d/Standard_Tests/Standard_Tests.h:190-246, MPM_Example.cpp:77-82).
+        const int dim = name == "ref3d_test18" ? 3 : 2;
+        base_desc(d, dim, counts > 0 ? counts : (dim == 3 ? 32 : 64));
+        d.gravity[1] = -2.0f;
+        d.cg_tolerance = 1e-7f;
+        d.cg_max_iterations = 10000;
+        d.material.mu0 = (float)(30.0 / (2.0 * 1.4));
+        d.material.lambda0 = (float)(30.0 * 0.4 / (1.4 * 0.2));
+        d.material.stretching_yield = 1.01f;
+        d.material.compression_yield = 0.95f;
+        d.material.hardening_factor = 10.0f;
+        add_plane(d, 0, 1, 0, 0, 0.1, 0);
+        sp.shapes.push_back(sphere(0.8, 0.5, 0.5, 0.1, -2.f, 0.1f, 0.f));
+        sp.shapes.push_back(sphere(0.2, 0.5, 0.5, 0.1, 2.f, 0.1f, 0.f));
+        sp.fixed_count = dim == 3 ? 20000 : 2000;
+        sp.fixed_mass_density = 2.0;
+    } else if (name == "blob" || name == "slab") {
+        // Seeded random snow blob (sphere) or slab (box) for batched and served scenes.
+        base_desc(d, 3, counts > 0 ? counts : 64);
+        add_plane(d, 0, 1, 0, 0, 0.05, 0);
+        const double u0 = unit_float(key3(seed, 11, 0)), u1 = unit_float(key3(seed, 11, 1));
+        const double u2 = unit_float(key3(seed, 11, 2)), u3 = unit_float(key3(seed, 11, 3));
+        if (name == "blob") {
+            const double r = 0.08 + 0.10 * u0;
+            sp.shapes.push_back(sphere(0.3 + 0.4 * u1, 0.35 + 0.3 * u2, 0.3 + 0.4 * u3, r,
+                                       (float)(2.0 * (u1 - 0.5)), (float)(-1.0 - u2), 0.f));
+        } else {
+            Shape s{};
+            s.kind = OBOX;
+            for (int a = 0; a < 3; ++a)
+                for (int k = 0; k < 3; ++k) s.axis[k][a] = (a == k) ? 1.0 : 0.0;
+            s.half[0] = 0.12 + 0.15 * u0;
+            s.half[1] = 0.03 + 0.04 * u1;
+            s.half[2] = 0.08 + 0.1 * u2;
+            s.c[0] = 0.5; s.c[1] = 0.06 + s.half[1] + 0.2 * u3; s.c[2] = 0.5;
+            s.v[0] = 0.f; s.v[1] = -0.5f; s.v[2] = 0.f;
+            sp.shapes.push_back(s);
+        }
+    } else {
+        return false;
+    }
+    return true;
+}
+
+// Generates particles; if x == nullptr only counts.
+int64_t generate(const Spec& sp, uint64_t seed, int64_t cap, float* x, float* v, float* mass) {
+    const mpm_scene_desc& d = sp.desc;
+    const int dim = d.dim;
+    const int n = d.counts;
+    const double dx = 1.0 / n;
+    int64_t count = 0;
+    auto emit = [&](const double* p, const float* vel, float m) {
+        if (x && count < cap) {
+            for (int a = 0; a < dim; ++a) {
+                x[count * dim + a] = (float)p[a];
+                v[count * dim + a] = vel[a];
+            }
+            mass[count] = m;
+        }
+        ++count;
+    };
+    if (sp.fixed_count > 0) {
+        for (size_t si = 0; si < sp.shapes.size(); ++si) {
+            const Shape& s = sp.shapes[si];
+            const double vol = dim == 3 ? 4.0 / 3.0 * M_PI * s.r * s.r * s.r : M_PI * s.r * s.r;
+            const float m = (float)(sp.fixed_mass_density * vol / sp.fixed_count);
+            uint64_t attempt = 0;
+            for (int i = 0; i < sp.fixed_count;) {
+                double p[3] = {0, 0, 0};
+                for (int a = 0; a < dim; ++a)
+                    p[a] = s.c[a] + s.r * (2.0 * unit_float(key3(seed ^ (si * 0x1000193ull), attempt, a)) - 1.0);
+                ++attempt;
+                if (!s.inside(p, dim)) continue;
+                emit(p, s.v, m);
+                ++i;
+            }
+        }
+        return count;
+    }
+    const int sub = 2;  // 2 sub-cells per axis: ppc 4 (2D) / 8 (3D)
+    const int ppc = dim == 3 ? 8 : 4;
+    const float m = (float)(sp.density * std::pow(dx, dim) / ppc);
+    for (size_t si = 0; si < sp.shapes.size(); ++si) {
+        const Shape& s = sp.shapes[si];
+        double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+        s.bounds(dim, lo, hi);
+        int clo[3] = {0, 0, 0}, chi[3] = {0, 0, 0};
+        for (int a = 0; a < dim; ++a) {
+            clo[a] = std::max(1, (int)std::floor(lo[a] / dx) - 1);
+            chi[a] = std::min(n - 2, (int)std::floor(hi[a] / dx) + 1);
+        }
+        const int zlo = dim == 3 ? clo[2] : 0, zhi = dim == 3 ? chi[2] : 0;
+        for (int i = clo[0]; i <= chi[0]; ++i)
+            for (int j = clo[1]; j <= chi[1]; ++j)
+                for (int k = zlo; k <= zhi; ++k) {
+                    const uint64_t cell = ((uint64_t)i * n + j) * n + k;
+                    const int nsub = dim == 3 ? 8 : 4;
+                    for (int q = 0; q < nsub; ++q) {
+                        const int qi[3] = {q & 1, (q >> 1) & 1, (q >> 2) & 1};
+                        const int ci[3] = {i, j, k};
+                        double p[3] = {0, 0, 0};
+                        for (int a = 0; a < dim; ++a) {
+                            const double u = unit_float(key3(seed + 0x51ull * si, cell, (uint64_t)(q * 4 + a)));
+                            p[a] = (ci[a] + (qi[a] + u) / sub) * dx;
+                        }
+                        if (!s.inside(p, dim)) continue;
+                        emit(p, s.v, m);
+                    }
+                }
+    }
+    return count;
+}
+
+}  // namespace
+
+extern "C" int mpm_scene_preset(const char* name, int32_t counts, uint64_t seed,
+                                mpm_scene_desc* desc, int64_t* n_particles) {
+    if (!name || !desc) return mpm_set_error(MPM_E_INVALID, "mpm_scene_preset: null argument");
+    Spec sp;
+    if (!build_spec(name, counts, seed, sp))
+        return mpm_set_error(MPM_E_INVALID, std::string("unknown scene ") + name);
+    *desc = sp.desc;
+    if (n_particles) *n_particles = generate(sp, seed, 0, nullptr, nullptr, nullptr);
+    return MPM_OK;
+}
+
+extern "C" int mpm_scene_generate(const char* name, int32_t counts, uint64_t seed, int64_t cap,
+                                  float* x, float* v, float* mass, int64_t* n_out) {
+    if (!name || !x || !v || !mass) return mpm_set_error(MPM_E_INVALID, "mpm_scene_generate: null argument");
+    Spec sp;
+    if (!build_spec(name, counts, seed, sp))
+        return mpm_set_error(MPM_E_INVALID, std::string("unknown scene ") + name);
+    const int64_t n = generate(sp, seed, cap, x, v, mass);
+    if (n_out) *n_out = n;
+    if (n > cap) return mpm_set_error(MPM_E_CAPACITY, "mpm_scene_generate: capacity too small");
+    return MPM_OK;
+}

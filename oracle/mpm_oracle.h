@@ -1,0 +1,96 @@
+/*
+ * mpm_oracle.h -- CPU oracle for the MPM snow substep.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a C++/OpenMP restatement of the
+ * reference's algorithm (SoldierDown/Non_Ficks_Diffusion, which cannot be
+ * built here: its Nova dependency is not vendored).  It is the parity
+ * checker for the GPU path and the CPU baseline that bench.py times; the
+ * product never links, loads or calls it.
+ *
+ * Every stage follows the reference stage for stage: AoS particles,
+ * x-slab binning with `threads` slabs, a dense grid over the particle
+ * bounding box rebuilt every substep, slab-owned cropped scatters, and
+ * Nova's unpreconditioned CG with the MPM_CG_System callbacks.  Pieces
+ * that live in Nova are restated from mathematics (see mpm_oracle.cpp).
+ *
+ * Particle arrays use the same layout as include/mpm_gpu.h (particle
+ * major; F row-major per particle).
+ */
+#ifndef MPM_ORACLE_H
+#define MPM_ORACLE_H
+#include <stdint.h>
+#include "../include/mpm_gpu.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* option bits */
+#define ORACLE_DOT_FP64 1    /* CG inner products and norm accumulate in double */
+#define ORACLE_SVD_FP64 2    /* 3x3 / 2x2 SVD computed in double */
+#define ORACLE_JP_SCALAR 4   /* carry det(Fp) as a scalar instead of the Fp matrix */
+
+typedef struct oracle_ctx* oracle_handle;
+
+/* precision: 4 = float state (the reference build), 8 = double (FP64 oracle) */
+oracle_handle oracle_create(const mpm_scene_desc* desc, int32_t precision, int32_t threads,
+                            int32_t options);
+void oracle_destroy(oracle_handle h);
+/* volume may be NULL (particles then wait for Process_Waiting_Particles);
+ * F, Jp may be NULL (identity, 1). */
+void oracle_set_particles(oracle_handle h, int64_t n, const float* x, const float* v,
+                          const float* F, const float* Jp, const float* mass, const float* volume);
+int64_t oracle_num_particles(oracle_handle h);
+int64_t oracle_num_simulated(oracle_handle h);
+/* All particles in creation order; valid[i] = 0 for removed ones. */
+void oracle_get_particles(oracle_handle h, double* x, double* v, double* F, double* Jp,
+                          double* mass, double* volume, uint8_t* valid);
+
+/* MPM_Driver::Initialize (MPM_Driver.cpp:18-32) */
+void oracle_initialize(oracle_handle h);
+/* MPM_Driver::Advance_Step (MPM_Driver.cpp:103-116) */
+void oracle_advance_step(oracle_handle h, double dt);
+/* One stage of Advance_Step; stage ids are enum mpm_stage of mpm_gpu.h. */
+void oracle_run_stage(oracle_handle h, int32_t stage, double dt);
+/* Populate_Simulated_Particles (MPM_Example.cpp:347-355) */
+void oracle_populate_simulated(oracle_handle h);
+/* MPM_Example::Max_Particle_Velocity (MPM_Example.cpp:375-387) */
+double oracle_max_particle_velocity(oracle_handle h);
+/* MPM_Driver::Advance_To_Target_Time (MPM_Driver.cpp:55-80); returns the new time. */
+double oracle_advance_to_target_time(oracle_handle h, double time, double target_time,
+                                     int32_t* substeps);
+
+/* Dense grid channel over node indices [-1, counts]^dim (x slowest);
+ * vector channels give dim values per node (enum mpm_channel). */
+void oracle_get_grid(oracle_handle h, int32_t channel, double* out);
+void oracle_get_flags(oracle_handle h, uint32_t* out);
+/* Applies the CG operator (MPM_CG_System::Multiply followed by Project)
+ * to a dense grid vector x (same layout as oracle_get_grid of a vector
+ * channel) using the current grid and particle state; writes q. */
+void oracle_apply_operator(oracle_handle h, double dt, const double* x, double* q);
+/* Interior-masked M inner product of two dense vectors (Inner_Product_Helper). */
+double oracle_inner_product(oracle_handle h, const double* a, const double* b);
+void oracle_cg_stats(oracle_handle h, int32_t* iterations, int32_t* multiplies, int32_t* converged,
+                     double* residual);
+/* Bounding-box cell range of the last grid build (Rasterize_Voxels). */
+void oracle_bbox_cells(oracle_handle h, int32_t* lo, int32_t* hi);
+
+/* ---- per-particle maths, for closed-form and invariant tests --------- */
+/* Quadratic B-spline weights of a particle at X (MPM_Particle.h:79-100):
+ * closest cell, w[3*dim] (row k = node c-1+k), dw[3*dim]. */
+void oracle_weights(int32_t dim, int32_t counts, const double* X, int32_t* cell, double* w,
+                    double* dw);
+/* 3x3 (dim 3) or 2x2 SVD, row-major matrices: F = U diag(S) V^T. */
+void oracle_svd(int32_t dim, const double* F, double* U, double* S, double* V);
+/* MPM_Constitutive_Model::Precompute + P + Psi for one particle in double:
+ * F, Fp-determinant in; F, Jp out, R, P, psi out. */
+void oracle_constitutive(int32_t dim, const mpm_material* mat, const double* F_in, double Jp_in,
+                         double* F_out, double* Jp_out, double* R, double* P, double* psi);
+/* MPM_Constitutive_Model::Times_dP_dF at state (F, R, lambda, mu). */
+void oracle_dP(int32_t dim, const double* F, const double* R, double lambda, double mu,
+               const double* dF, double* dP);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
