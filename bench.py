@@ -1,0 +1,168 @@
+#!/usr/bin/env python3
+"""Throughput benchmark of the MI355X MPM snow substep.
+
+Workload: the 3D sparse avalanche -- a 0.6 x 0.1 x 0.125 snow slab (8
+particles per cell, about 1.0M particles) on a 30-degree incline in a 256^3
+grid, snow material (rho 400, E 1.4e5, nu 0.2, theta_c 2.5e-2,
+theta_s 7.5e-3, xi 10), CG tolerance 1e-6 / at most 50 iterations.  A step
+is one substep (MPM_Driver::Advance_Step) in fixed-dt mode: the dt of the
+first CFL substep, kept for every timed substep.  Particle state, grid and
+CG vectors stay resident in HBM; the timed region contains only kernel work.
+
+Prints one JSON line: particle-substeps/s, the roofline of the dominant
+kernel (CG operator) measured with HIP events on the scene's stream, and the
+CPU baseline (the oracle, the C++/OpenMP restatement of the reference CPU
+path, timed on the host cores on a bounded sample of the same workload).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+WORKLOAD = "cfg2_avalanche"
+COUNTS = 256
+
+
+def canonical_bytes(N, Nn, I, M):
+    """Algorithmic HBM bytes (FP32 = 4 B; see DESIGN.md, 'Measurement')."""
+    return {
+        "substep": N * (320 + 60 * M) + Nn * (200 + 64 * M + 164 * I),
+        "cg_force": N * 60 + Nn * 24,      # per operator application: particles + gather s + scatter q
+        "p2g": N * 32 + Nn * 16,
+        "g2p": N * 124 + Nn * 24,
+        "constitutive_force": N * (80 + 64) + Nn * 12,
+    }
+
+
+def cfl_dt(desc, vmax):
+    dx = 1.0 / desc.counts
+    return float(max(desc.min_dt, min(desc.max_dt, desc.cfl * dx / max(vmax, 1e-2))))
+
+
+def cpu_baseline(desc, x, v, m, dt, substeps=2):
+    """Times the oracle (reference CPU path restated) on the host cores."""
+    import oracle as O
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    o = O.Oracle(desc, 4, cores, O.SVD_FP64)  # float state, float CG dot products (as the reference)
+    o.set_particles(x, v, m)
+    o.initialize()
+    o.advance_step(dt)  # warm-up substep
+    t0 = time.perf_counter()
+    for _ in range(substeps):
+        o.advance_step(dt)
+    el = time.perf_counter() - t0
+    live = o.particles()["x"].shape[0]
+    return {"value": live * substeps / el, "unit": "particle-substeps/s", "cores": cores, "kind": "port",
+            "sample": f"{WORKLOAD} {COUNTS}^3, {len(m)} particles, 1 warm-up + {substeps} fixed-dt substeps "
+                      f"(dt {dt:.2e}, CG cap {desc.cg_max_iterations}), {el:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=30)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--counts", type=int, default=COUNTS)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-substeps", type=int, default=2)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs each scene on a single MI355X "
+              f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
+        sys.exit(2)
+
+    import non_ficks_diffusion_amd as M
+
+    desc, x, v, m = M.scene_generate(WORKLOAD, args.counts, 0)
+    n0 = len(m)
+    s = M.Scene(desc, n0)
+    s.upload(x, v, m)
+    s.initialize()
+    dt = cfl_dt(desc, s.max_particle_velocity())
+
+    for _ in range(args.warmup):
+        s.substep(dt)
+    s.synchronize()
+    live0 = s.stats()["live_particles"]
+    # timed region: K substeps, synchronised on both sides
+    t0 = time.perf_counter()
+    live_sum = 0
+    for _ in range(args.steps):
+        s.substep(dt)
+    s.synchronize()
+    el = time.perf_counter() - t0
+    st = s.stats()
+    # live particles before the timed region plus after; removal is rare and
+    # monotone, so the average bounds the per-substep count
+    live_sum = args.steps * 0.5 * (live0 + st["live_particles"])
+    value = live_sum / el
+    ms_per_step = el * 1e3 / args.steps
+
+    # per-kernel HIP-event timing pass (outside the timed region)
+    s.enable_timing(True)
+    tsteps = 3
+    stats = []
+    for _ in range(tsteps):
+        stats.append(s.substep(dt, stats=True))
+    kt = s.kernel_times()
+    s.enable_timing(False)
+    N = stats[-1]["live_particles"]
+    Nn = stats[-1]["active_blocks"] * 64
+    I = stats[-1]["cg_iterations"]
+    Mul = stats[-1]["cg_multiplies"]
+    cb = canonical_bytes(N, Nn, I, Mul)
+    ms_cg, n_cg = kt["cg_force"]
+    avg_cg_ms = ms_cg / max(n_cg, 1)
+    achieved = cb["cg_force"] / (avg_cg_ms * 1e-3) / 1e9
+    breakdown = {k: round(msv / tsteps, 4) for k, (msv, n) in kt.items() if n}
+    sub_ms = sum(msv for msv, _ in kt.values()) / tsteps
+    per_kernel_gbs = {}
+    for k in ("p2g", "g2p", "constitutive_force"):
+        msv, n = kt[k]
+        if n:
+            per_kernel_gbs[k] = round(cb[k] / (msv / n * 1e-3) / 1e9, 1)
+    per_kernel_gbs["cg_force"] = round(achieved, 1)
+
+    out = {
+        "metric": "particle-substeps/s, 3D 1M snow",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{WORKLOAD} (3D sparse avalanche, 30-degree incline)", "grid": f"{args.counts}^3",
+                   "particles": int(N), "active_blocks": int(stats[-1]["active_blocks"]),
+                   "active_block_fraction": round(stats[-1]["active_blocks"] / ((args.counts // 4) ** 3), 4),
+                   "dt": dt, "cg_iterations": int(I), "cg_multiplies": int(Mul),
+                   "cg_tolerance": float(desc.cg_tolerance), "cg_max_iterations": int(desc.cg_max_iterations),
+                   "precision": "FP32 state, FP64 CG reductions and SVD", "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "kernel": "cg_force", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "avg_launch_ms": round(avg_cg_ms, 5), "algorithmic_bytes_per_launch": int(cb["cg_force"])},
+        "substep_roofline": {"algorithmic_bytes": int(cb["substep"]), "kernel_ms": round(sub_ms, 4),
+                             "achieved_GBs": round(cb["substep"] / (sub_ms * 1e-3) / 1e9, 1)},
+        "kernel_GBs": per_kernel_gbs,
+        "kernel_ms_per_substep": breakdown,
+    }
+    s.close()
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(desc, x, v, m, dt, args.cpu_substeps)
+        out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

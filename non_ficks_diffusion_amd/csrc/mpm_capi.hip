@@ -57,12 +57,13 @@ struct mpm_context {
     float* rcache = nullptr;
     int *slot_count = nullptr, *slot_touched = nullptr, *slot_active = nullptr;
     int *block_slot = nullptr, *block_pstart = nullptr, *block_nbr = nullptr, *scan_part = nullptr;
+    int *cell_count = nullptr, *cell_start = nullptr;
     float *g_m = nullptr, *g_v = nullptr, *g_vs = nullptr, *g_q = nullptr, *g_s = nullptr, *g_r = nullptr;
     unsigned* g_flags = nullptr;
     mpm::Counters* ctr = nullptr;
     mpm::CGState* cg = nullptr;
     double* red = nullptr;
-    unsigned* sync = nullptr;
+    float* tiles = nullptr;
     int* h_done = nullptr;  // pinned
     hipEvent_t done_ev[2] = {nullptr, nullptr};
     int grid_particle = 0, grid_block = 0, grid_node = 0;
@@ -140,6 +141,8 @@ mpm::KP make_kp(const mpm_context* c, float dt) {
     p.block_slot = c->block_slot;
     p.block_pstart = c->block_pstart;
     p.block_nbr = c->block_nbr;
+    p.cell_count = c->cell_count;
+    p.cell_start = c->cell_start;
     p.bcap = c->bcap;
     p.scan_part = c->scan_part;
     p.nparts = c->nparts;
@@ -154,7 +157,9 @@ mpm::KP make_kp(const mpm_context* c, float dt) {
     p.ctr = c->ctr;
     p.cg = c->cg;
     p.red = c->red;
-    p.sync = c->sync;
+    p.red_stride = c->grid_node;
+    p.red_count = c->grid_node;
+    p.tiles = c->tiles;
     p.dt = dt;
     return p;
 }
@@ -221,42 +226,46 @@ void stage_waiting(mpm_context* c) {
     c->waiting = false;
 }
 
+// Apply_Force's implicit solve (MPM_Example.cpp:700-711): CG set-up, restart
+// (r = b - A x, Project) and up to cg_max_iterations iterations.
 template <int D>
-int stage_cg(mpm_context* c, float dt) {
+int stage_cg(mpm_context* c, float dt, bool do_init) {
     mpm::KP p = make_kp(c, dt);
-    timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, false, false, true); });
+    if (do_init)
+        timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, false, false, true, false); });
     timed(c, KC_CG_FORCE, [&] { mpm::launch_cg_force<D>(p, c->stream, c->grid_block, true); });
     timed(c, KC_CG_NODE, [&] { mpm::launch_cg_restart_node<D>(p, c->stream, c->grid_node); });
     const int max_it = c->desc.cg_max_iterations;
-    auto iteration = [&] {
-        timed(c, KC_CG_NODE, [&] { mpm::launch_cg_update_s<D>(p, c->stream, c->grid_node); });
+    auto iteration = [&](int k) {
+        timed(c, KC_CG_NODE, [&] { mpm::launch_cg_update_s<D>(p, c->stream, c->grid_node, k); });
         timed(c, KC_CG_FORCE, [&] { mpm::launch_cg_force<D>(p, c->stream, c->grid_block, false); });
         timed(c, KC_CG_NODE, [&] { mpm::launch_cg_node1<D>(p, c->stream, c->grid_node); });
-        timed(c, KC_CG_NODE, [&] { mpm::launch_cg_node2<D>(p, c->stream, c->grid_node); });
+        timed(c, KC_CG_NODE, [&] { mpm::launch_cg_node2<D>(p, c->stream, c->grid_node, k); });
     };
     if (max_it <= 64) {
         // every iteration is launched; kernels after convergence return at once
-        for (int k = 0; k < max_it; ++k) iteration();
-        return MPM_OK;
-    }
-    // long solves: launch in chunks and stop once a chunk two back reported convergence
-    const int chunk = 8;
-    int slot = 0;
-    bool pending[2] = {false, false};
-    for (int k = 0; k < max_it;) {
-        const int n = std::min(chunk, max_it - k);
-        for (int j = 0; j < n; ++j) iteration();
-        k += n;
-        HIPCHECK(hipMemcpyAsync(&c->h_done[slot], &c->cg->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-        HIPCHECK(hipEventRecord(c->done_ev[slot], c->stream));
-        pending[slot] = true;
-        slot ^= 1;
-        if (pending[slot]) {
-            HIPCHECK(hipEventSynchronize(c->done_ev[slot]));
-            pending[slot] = false;
-            if (c->h_done[slot]) break;
+        for (int k = 0; k < max_it; ++k) iteration(k);
+    } else {
+        // long solves: launch in chunks and stop once a chunk two back reported convergence
+        const int chunk = 8;
+        int slot = 0;
+        bool pending[2] = {false, false};
+        for (int k = 0; k < max_it;) {
+            const int n = std::min(chunk, max_it - k);
+            for (int j = 0; j < n; ++j) iteration(k + j);
+            k += n;
+            HIPCHECK(hipMemcpyAsync(&c->h_done[slot], &c->cg->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+            HIPCHECK(hipEventRecord(c->done_ev[slot], c->stream));
+            pending[slot] = true;
+            slot ^= 1;
+            if (pending[slot]) {
+                HIPCHECK(hipEventSynchronize(c->done_ev[slot]));
+                pending[slot] = false;
+                if (c->h_done[slot]) break;
+            }
         }
     }
+    timed(c, KC_CG_NODE, [&] { mpm::launch_cg_finalize(p, c->stream, max_it); });
     return MPM_OK;
 }
 
@@ -279,12 +288,12 @@ int run_stage(mpm_context* c, int stage, float dt) {
             break;
         case MPM_STAGE_EXPLICIT_FORCE:
             timed(c, KC_PFORCE, [&] { mpm::launch_particle_force<D>(p, c->stream, c->grid_block, false, true); });
-            timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, true, false, false); });
+            timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, true, false, false, true); });
             break;
         case MPM_STAGE_COLLISION:
-            timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, false, true, false); });
+            timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, false, true, false, false); });
             break;
-        case MPM_STAGE_CG: return stage_cg<D>(c, dt);
+        case MPM_STAGE_CG: return stage_cg<D>(c, dt, true);
         case MPM_STAGE_G2P: stage_g2p<D>(c, dt); break;
         default: return mpm_set_error(MPM_E_INVALID, "unknown stage");
     }
@@ -299,8 +308,8 @@ int substep(mpm_context* c, float dt) {
     stage_p2g<D>(c);
     mpm::KP p = make_kp(c, dt);
     timed(c, KC_PFORCE, [&] { mpm::launch_particle_force<D>(p, c->stream, c->grid_block, true, true); });
-    timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, true, true, false); });
-    int rc = stage_cg<D>(c, dt);
+    timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, true, true, true, false); });
+    int rc = stage_cg<D>(c, dt, false);
     if (rc != MPM_OK) return rc;
     stage_g2p<D>(c, dt);
     return MPM_OK;
@@ -379,7 +388,10 @@ int mpm_create(const mpm_scene_desc* desc, int64_t capacity, mpm_handle* out) {
         return mpm_set_error(MPM_E_INVALID, "mpm_create: grid too large");
     }
     c->pcap = capacity;
-    c->bcap = (int)std::min<int64_t>(c->nslots, (int64_t)(c->D == 3 ? 8 : 4) * capacity + 1);
+    // Block capacity: every slot of small grids, else capacity/8 blocks -- about
+    // 16x the active blocks (home blocks plus halo) of a scene packed at 8
+    // particles per cell.  Overflow makes the substep a no-op and is reported.
+    c->bcap = (int)std::min<int64_t>(c->nslots, std::max<int64_t>(4096, capacity / 8));
     c->gcap = (int64_t)c->bcap * 64;
     c->nparts = (c->nslots + mpm::SCAN_TILE - 1) / mpm::SCAN_TILE;
     const int D = c->D;
@@ -399,6 +411,9 @@ int mpm_create(const mpm_scene_desc* desc, int64_t capacity, mpm_handle* out) {
     A(dalloc(c, &c->block_slot, (size_t)c->bcap));
     A(dalloc(c, &c->block_pstart, (size_t)c->bcap + 1));
     A(dalloc(c, &c->block_nbr, (size_t)c->bcap * (D == 3 ? 27 : 9)));
+    A(dalloc(c, &c->cell_count, (size_t)c->nslots * 64));
+    A(dalloc(c, &c->cell_start, (size_t)c->bcap * 65 + 1));
+    A(dalloc(c, &c->tiles, (size_t)c->bcap * (D + 1) * (D == 3 ? 216 : 100)));
     A(dalloc(c, &c->scan_part, (size_t)2 * c->nparts));
     A(dalloc(c, &c->g_m, (size_t)c->gcap));
     A(dalloc(c, &c->g_v, (size_t)D * c->gcap));
@@ -410,10 +425,9 @@ int mpm_create(const mpm_scene_desc* desc, int64_t capacity, mpm_handle* out) {
     A(dalloc(c, &c->ctr, 1));
     A(dalloc(c, &c->cg, 1));
     c->grid_particle = (int)std::min<int64_t>(std::max<int64_t>((c->pcap + mpm::WG - 1) / mpm::WG, 8), 4096);
-    c->grid_block = (int)std::min<int64_t>(((int64_t)c->bcap + 7) / 8 * 8, 2048);
-    c->grid_node = (int)std::min<int64_t>(std::max<int64_t>((c->gcap + mpm::WG - 1) / mpm::WG, 1), 1024);
-    A(dalloc(c, &c->red, (size_t)2 * c->grid_node));
-    A(dalloc(c, &c->sync, 1));
+    c->grid_block = (int)std::min<int64_t>(((int64_t)c->bcap + 7) / 8 * 8, 8192);
+    c->grid_node = (int)std::min<int64_t>(std::max<int64_t>((c->gcap + mpm::WG - 1) / mpm::WG, 1), 256);
+    A(dalloc(c, &c->red, (size_t)3 * c->grid_node));
     if (rc == MPM_OK) {
         void* hp = nullptr;
         if (hipHostMalloc(&hp, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");

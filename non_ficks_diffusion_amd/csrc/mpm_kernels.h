@@ -24,8 +24,10 @@ struct Counters {
     int pad;
 };
 
+// CG scalars.  rho of iteration k lives in rho_hist[k & 1] so a kernel never
+// reads a slot that another work-group of the same kernel writes.
 struct CGState {
-    double rho, rho_old, alpha, beta, conv, rr, sq;
+    double rho_hist[2], alpha, beta, conv, rr, sq;
     int iter, done, converged, multiplies;
 };
 
@@ -58,6 +60,8 @@ struct KP {
     int* block_slot;
     int* block_pstart;
     int* block_nbr;
+    int* cell_count;   // per slot, 64 cell counters (zero between substeps)
+    int* cell_start;   // per active block, 65 particle offsets (cell ranges)
     int bcap;
     int* scan_part;  // 2 * nparts
     int nparts;
@@ -70,11 +74,13 @@ struct KP {
     float* g_r;
     unsigned* g_flags;
     int64_t gcap;
+    float* tiles;     // per active block, (D+1) * (B+2)^D floats
     // control
     Counters* ctr;
     CGState* cg;
-    double* red;      // 2 * max node work-groups partials
-    unsigned* sync;   // last-block counter
+    double* red;      // 3 * red_stride partials: s.q | r.r | max |r|^2
+    int red_stride;   // node-kernel work-groups
+    int red_count;
     float dt;
 };
 
@@ -87,12 +93,13 @@ template <int D> void launch_p2g(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_normalize(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_init_volume(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_particle_force(const KP& p, hipStream_t s, int grid, bool constitutive, bool force);
-template <int D> void launch_grid_update(const KP& p, hipStream_t s, int grid, bool force, bool collision, bool cg_init);
+template <int D> void launch_grid_update(const KP& p, hipStream_t s, int grid, bool force, bool collision, bool cg_init, bool write_f);
 template <int D> void launch_cg_force(const KP& p, hipStream_t s, int grid, bool restart);
 template <int D> void launch_cg_restart_node(const KP& p, hipStream_t s, int grid);
-template <int D> void launch_cg_update_s(const KP& p, hipStream_t s, int grid);
+template <int D> void launch_cg_update_s(const KP& p, hipStream_t s, int grid, int k);
 template <int D> void launch_cg_node1(const KP& p, hipStream_t s, int grid);
-template <int D> void launch_cg_node2(const KP& p, hipStream_t s, int grid);
+template <int D> void launch_cg_node2(const KP& p, hipStream_t s, int grid, int k);
+void launch_cg_finalize(const KP& p, hipStream_t s, int k_end);
 template <int D> void launch_g2p(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_max_speed(const KP& p, hipStream_t s, int grid);
 

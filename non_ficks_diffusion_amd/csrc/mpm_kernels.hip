@@ -9,11 +9,18 @@
 //              8^2 (2D) node blocks; a dense slot table maps block
 //              coordinates to a compacted ordinal; every channel is an
 //              array of ordinal*64 + local-node entries.
-// Block kernels (P2G, force, CG force, G2P) run one work-group per active
-// block: the block's particles touch only its nodes and a one-node halo,
-// a (B+2)^D tile that lives in LDS.  Scatters accumulate with LDS float
-// atomics and flush the tile once with global float atomics; gathers load
-// the tile once from the 3^D neighbour blocks.
+//   tiles      per active block, the (B+2)^D node tile (block plus a
+//              one-node halo) of each scattered channel.
+// Block kernels (P2G, force, CG operator, G2P) run one work-group per
+// active block.  A block's particles touch only its nodes and the one-node
+// halo, so scatters accumulate into an LDS tile and the work-group writes
+// the whole tile to the tile buffer with plain stores; the next node kernel
+// sums, for every node, the (at most 2^D) tiles that cover it in a fixed
+// order.  There are no global atomics on the scatter path, and the
+// cross-block sum is deterministic.  Gathers load the tile once from the
+// 3^D neighbour blocks.  Reductions (CG dot products and norms) write one
+// FP64 partial per work-group; the next kernel reduces the partials in a
+// fixed order in every work-group, so no fences or atomics are needed.
 #include "mpm_kernels.h"
 
 namespace mpm {
@@ -41,6 +48,12 @@ __device__ inline void slot_coords(int slot, int nb, int* b) {
         b[1] = slot % nb;
         b[0] = slot / nb;
     }
+}
+
+template <int D>
+__device__ inline void local_coords(int l, int* lc) {
+    if constexpr (D == 3) { lc[0] = l >> 4; lc[1] = (l >> 2) & 3; lc[2] = l & 3; }
+    else { lc[0] = l >> 3; lc[1] = l & 7; }
 }
 
 // tile index -> (neighbour ordinal in 0..3^D-1, local node in 0..63, tile coords)
@@ -126,16 +139,69 @@ __device__ inline bool particle_weights(const KP& p, int i, const int* bb, float
     return ok;
 }
 
-// x crop of the reference's slab decomposition: scatters never reach nodes
-// whose x index is outside [0, counts-1].
-template <int D>
-__device__ inline bool flush_allowed(const KP& p, const int* bb, const int* tc) {
-    if (!p.crop_ghost_x) return true;
-    constexpr int B = Dims<D>::B;
-    const int ix = B * bb[0] - 1 + tc[0] - B;
-    return ix >= 0 && ix <= p.counts - 1;
+// Writes the C-channel LDS tile of block b to its tile slot (plain stores).
+template <int D, int C>
+__device__ inline void store_tile(const KP& p, int b, const float* lds) {
+    constexpr int TN = Dims<D>::TN;
+    float* dst = p.tiles + (int64_t)b * C * TN;
+    for (int t = threadIdx.x; t < C * TN; t += (int)blockDim.x) dst[t] = lds[t];
 }
 
+// Sum, for node n, of channel c over every tile covering it, in a fixed
+// order.  With crop_ghost_x, nodes whose x index is outside [0, counts-1]
+// receive nothing (the reference's slab crop,
+// Tools/Cropped_Influence_Iterator.h:31-42).
+template <int D, int C>
+__device__ inline void gather_node_tiles(const KP& p, int n, float* out) {
+    using Dm = Dims<D>;
+    constexpr int B = Dm::B, TE = Dm::TE, TN = Dm::TN, NBR = Dm::NBR;
+    const int b = n >> 6;
+    int lc[D];
+    local_coords<D>(n & 63, lc);
+#pragma unroll
+    for (int c = 0; c < C; ++c) out[c] = 0.0f;
+    if (p.crop_ghost_x) {
+        int bb[D];
+        slot_coords<D>(p.block_slot[b], p.nb, bb);
+        const int ix = B * bb[0] + lc[0] - B;
+        if (ix < 0 || ix > p.counts - 1) return;
+    }
+    int od[D][2], td[D][2], nopt[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        od[a][0] = 1;
+        td[a][0] = lc[a] + 1;
+        nopt[a] = 1;
+        if (lc[a] == 0) { od[a][1] = 0; td[a][1] = TE - 1; nopt[a] = 2; }
+        else if (lc[a] == B - 1) { od[a][1] = 2; td[a][1] = 0; nopt[a] = 2; }
+        else { od[a][1] = 1; td[a][1] = lc[a] + 1; }
+    }
+    const int* nbr = p.block_nbr + (int64_t)b * NBR;
+    if constexpr (D == 3) {
+        for (int i = 0; i < nopt[0]; ++i)
+            for (int j = 0; j < nopt[1]; ++j)
+                for (int k = 0; k < nopt[2]; ++k) {
+                    const int blk = nbr[(od[0][i] * 3 + od[1][j]) * 3 + od[2][k]];
+                    if (blk < 0) continue;
+                    const int t = (td[0][i] * TE + td[1][j]) * TE + td[2][k];
+                    const float* src = p.tiles + (int64_t)blk * C * TN + t;
+#pragma unroll
+                    for (int c = 0; c < C; ++c) out[c] += src[c * TN];
+                }
+    } else {
+        for (int i = 0; i < nopt[0]; ++i)
+            for (int j = 0; j < nopt[1]; ++j) {
+                const int blk = nbr[od[0][i] * 3 + od[1][j]];
+                if (blk < 0) continue;
+                const int t = td[0][i] * TE + td[1][j];
+                const float* src = p.tiles + (int64_t)blk * C * TN + t;
+#pragma unroll
+                for (int c = 0; c < C; ++c) out[c] += src[c * TN];
+            }
+    }
+}
+
+// sum and max over the work-group; result valid in thread 0
 __device__ inline void wg_reduce2(double& sum, double& mx) {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -156,32 +222,20 @@ __device__ inline void wg_reduce2(double& sum, double& mx) {
     }
 }
 
-// Publishes the work-group's partials and returns true in the last
-// work-group to arrive; that group then sees every partial.
-__device__ inline bool publish_and_check_last(const KP& p, double sum, double mx) {
-    __shared__ int is_last;
-    if (threadIdx.x == 0) {
-        p.red[blockIdx.x] = sum;
-        p.red[gridDim.x + blockIdx.x] = mx;
-        __threadfence();
-        const unsigned prev = __hip_atomic_fetch_add(p.sync, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (prev == gridDim.x - 1) ? 1 : 0;
-    }
-    __syncthreads();
-    if (is_last) __threadfence();
-    return is_last != 0;
-}
-
-// Sums the partials in a fixed order (deterministic) inside the last group.
-__device__ inline void gather_partials(const KP& p, double& sum, double& mx) {
+// Reduces `count` partials (sum array, max array) in a fixed order; the
+// result is broadcast to every thread of the work-group.
+__device__ inline void reduce_partials(const double* sums, const double* maxs, int count, double& sum, double& mx) {
     double s = 0.0, m = 0.0;
-    for (int k = threadIdx.x; k < (int)gridDim.x; k += WG) {
-        s += __hip_atomic_load(&p.red[k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        m = fmax(m, __hip_atomic_load(&p.red[gridDim.x + k], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+    for (int k = threadIdx.x; k < count; k += WG) {
+        s += sums[k];
+        if (maxs) m = fmax(m, maxs[k]);
     }
     wg_reduce2(s, m);
-    sum = s;
-    mx = m;
+    __shared__ double bs, bm;
+    if (threadIdx.x == 0) { bs = s; bm = m; }
+    __syncthreads();
+    sum = bs;
+    mx = bm;
 }
 
 __device__ inline bool inside_plane(const KP& p, int id, const float* x, int D) {
@@ -193,12 +247,9 @@ __device__ inline bool inside_plane(const KP& p, int id, const float* x, int D) 
 template <int D>
 __device__ inline void node_center(const KP& p, int n, float* x) {
     constexpr int B = Dims<D>::B;
-    const int b = n >> 6, l = n & 63;
-    int bb[D];
-    slot_coords<D>(p.block_slot[b], p.nb, bb);
-    int lc[D];
-    if constexpr (D == 3) { lc[0] = l >> 4; lc[1] = (l >> 2) & 3; lc[2] = l & 3; }
-    else { lc[0] = l >> 3; lc[1] = l & 7; }
+    int bb[D], lc[D];
+    slot_coords<D>(p.block_slot[n >> 6], p.nb, bb);
+    local_coords<D>(n & 63, lc);
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         const int i = B * bb[a] + lc[a] - B;
@@ -213,37 +264,73 @@ template <int D>
 __global__ __launch_bounds__(WG) void k_bin(KP p) {
     constexpr int B = Dims<D>::B;
     const int n = p.ctr->n_live;
-    for (int i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) {
-        if (p.keycur[i] < 0) continue;
+    const int lane = threadIdx.x & 63;
+    // whole waves iterate together so the ballots below are convergent
+    for (int base = blockIdx.x * WG + (threadIdx.x & ~63); base < n; base += gridDim.x * WG) {
+        const int i = base + lane;
+        bool valid = i < n && p.keycur[i] >= 0;
         int c[D];
-        bool ok = true, finite = true;
+        bool finite = true;
+        if (valid) {
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            const float X = p.pcur[(PF<D>::X + a) * p.pcap + i];
-            finite = finite && isfinite(X);
-            c[a] = finite ? cell_of(X, p.inv_dx) : -1;
-            ok = ok && finite && c[a] >= 0 && c[a] <= p.counts - 1;
+            for (int a = 0; a < D; ++a) {
+                const float X = p.pcur[(PF<D>::X + a) * p.pcap + i];
+                finite = finite && isfinite(X);
+                c[a] = finite ? cell_of(X, p.inv_dx) : -1;
+                valid = valid && finite && c[a] >= 0 && c[a] <= p.counts - 1;
+            }
+            if (!valid) {
+                p.keycur[i] = -1;
+                if (!finite) atomicOr(&p.ctr->nonfinite, 1);
+            }
         }
-        if (!ok) {
-            p.keycur[i] = -1;
-            if (!finite) atomicOr(&p.ctr->nonfinite, 1);
-            continue;
+        int slot = -1, key = -1;
+        if (valid) {
+            slot = 0;
+            int cell = 0;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                slot = slot * p.nb + (c[a] + B) / B;
+                cell = cell * B + (c[a] + B) % B;
+            }
+            key = slot * 64 + cell;
         }
-        int slot = 0;
+        // wave-aggregated atomics: one per distinct (block, cell) for the rank in
+        // the cell, one per distinct block for the block total
+        int rank = 0;
+        unsigned long long todo = __ballot(valid);
+        while (todo) {
+            const int leader = __ffsll((long long)todo) - 1;
+            const int lkey = __shfl(key, leader);
+            const unsigned long long grp = __ballot(valid && key == lkey);
+            int r0 = 0;
+            if (lane == leader) r0 = atomicAdd(&p.cell_count[lkey], __popcll(grp));
+            r0 = __shfl(r0, leader);
+            if (valid && key == lkey) rank = r0 + __popcll(grp & ((1ull << lane) - 1ull));
+            todo &= ~grp;
+        }
+        todo = __ballot(valid);
+        while (todo) {
+            const int leader = __ffsll((long long)todo) - 1;
+            const int lslot = __shfl(slot, leader);
+            const unsigned long long grp = __ballot(valid && slot == lslot);
+            if (lane == leader) atomicAdd(&p.slot_count[lslot], __popcll(grp));
+            todo &= ~grp;
+        }
+        if (valid) {
+            p.prank[i] = rank;
+            p.keycur[i] = key;
+            int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
 #pragma unroll
-        for (int a = 0; a < D; ++a) slot = slot * p.nb + (c[a] + B) / B;
-        p.prank[i] = atomicAdd(&p.slot_count[slot], 1);
-        p.keycur[i] = slot;
-        int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
-#pragma unroll
-        for (int a = 0; a < D; ++a) { lo[a] = (c[a] + B - 1) / B; hi[a] = (c[a] + B + 1) / B; }
-        if constexpr (D == 3) {
-            for (int x = lo[0]; x <= hi[0]; ++x)
-                for (int y = lo[1]; y <= hi[1]; ++y)
-                    for (int z = lo[2]; z <= hi[2]; ++z) p.slot_touched[(x * p.nb + y) * p.nb + z] = 1;
-        } else {
-            for (int x = lo[0]; x <= hi[0]; ++x)
-                for (int y = lo[1]; y <= hi[1]; ++y) p.slot_touched[x * p.nb + y] = 1;
+            for (int a = 0; a < D; ++a) { lo[a] = (c[a] + B - 1) / B; hi[a] = (c[a] + B + 1) / B; }
+            if constexpr (D == 3) {
+                for (int x = lo[0]; x <= hi[0]; ++x)
+                    for (int y = lo[1]; y <= hi[1]; ++y)
+                        for (int z = lo[2]; z <= hi[2]; ++z) p.slot_touched[(x * p.nb + y) * p.nb + z] = 1;
+            } else {
+                for (int x = lo[0]; x <= hi[0]; ++x)
+                    for (int y = lo[1]; y <= hi[1]; ++y) p.slot_touched[x * p.nb + y] = 1;
+            }
         }
     }
 }
@@ -316,12 +403,17 @@ __global__ __launch_bounds__(WG) void k_scan_top(KP p) {
     }
     if (threadIdx.x == 0) {
         p.ctr->n_prev = p.ctr->n_live;
-        p.ctr->n_live = tc;
-        p.ctr->nblocks = ta;
         p.ctr->max_v2 = 0u;
         p.ctr->n_removed = 0;
-        if (ta > p.bcap) p.ctr->overflow = 1;
-        p.block_pstart[ta < p.bcap ? ta : p.bcap] = tc;
+        if (ta > p.bcap) {
+            // the substep becomes a no-op: particles keep their order and state
+            p.ctr->overflow = 1;
+            p.ctr->nblocks = 0;
+        } else {
+            p.ctr->n_live = tc;
+            p.ctr->nblocks = ta;
+            p.block_pstart[ta] = tc;
+        }
     }
 }
 
@@ -360,17 +452,19 @@ __global__ __launch_bounds__(WG) void k_scan_apply(KP p) {
     }
 }
 
-// neighbour table + clearing of the scatter channels of every active block
+// Per active block (one wave, lane = cell): neighbour table, and the cell
+// ranges -- an exclusive scan of the block's 64 cell counts -- after which
+// the cell counters are cleared for the next binning.
 template <int D>
 __global__ __launch_bounds__(WG) void k_block_setup(KP p) {
     using Dm = Dims<D>;
-    if (p.ctr->overflow) return;
     const int nblocks = p.ctr->nblocks;
     const int group = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int b = blockIdx.x * (WG / 64) + group; b < nblocks; b += gridDim.x * (WG / 64)) {
+        const int slot = p.block_slot[b];
         if (lane < Dm::NBR) {
             int bb[D];
-            slot_coords<D>(p.block_slot[b], p.nb, bb);
+            slot_coords<D>(slot, p.nb, bb);
             int o[3];
             if constexpr (D == 3) { o[0] = lane / 9; o[1] = (lane / 3) % 3; o[2] = lane % 3; }
             else { o[0] = lane / 3; o[1] = lane % 3; }
@@ -384,100 +478,128 @@ __global__ __launch_bounds__(WG) void k_block_setup(KP p) {
             }
             p.block_nbr[b * Dm::NBR + lane] = in ? p.slot_active[s] : -1;
         }
-        const int64_t n = (int64_t)b * 64 + lane;
-        p.g_m[n] = 0.0f;
-        p.g_flags[n] = 0u;
+        const int cnt = p.cell_count[slot * 64 + lane];
+        int x = cnt;
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            p.g_v[a * p.gcap + n] = 0.0f;
-            p.g_q[a * p.gcap + n] = 0.0f;
+        for (int off = 1; off < 64; off <<= 1) {
+            const int y = __shfl_up(x, off);
+            if (lane >= off) x += y;
         }
+        const int start = p.block_pstart[b] + x - cnt;
+        p.cell_start[b * 65 + lane] = start;
+        if (lane == 63) p.cell_start[b * 65 + 64] = start + cnt;
+        p.cell_count[slot * 64 + lane] = 0;
     }
 }
 
+// scatter into (block, cell) order
 template <int D>
 __global__ __launch_bounds__(WG) void k_reorder(KP p) {
-    if (p.ctr->overflow) return;
     const int n = p.ctr->n_prev;
+    const bool overflow = p.ctr->overflow != 0;
     for (int i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) {
         const int key = p.keycur[i];
-        if (key < 0) continue;
-        const int dest = p.block_pstart[p.slot_active[key]] + p.prank[i];
+        int dest;
+        if (overflow) dest = i;  // identity copy: the buffer swap leaves the state unchanged
+        else if (key < 0) continue;
+        else dest = p.cell_start[p.slot_active[key >> 6] * 65 + (key & 63)] + p.prank[i];
 #pragma unroll
         for (int f = 0; f < PF<D>::N; ++f) p.palt[f * p.pcap + dest] = p.pcur[f * p.pcap + i];
         p.idalt[dest] = p.idcur[i];
-        p.keyalt[dest] = 0;
+        p.keyalt[dest] = overflow ? key : 0;
     }
+}
+
+// Block kernels run one wave per active block with lane c owning cell c
+// (particles are in (block, cell) order, cell ranges in cell_start).  All
+// particles of a lane share one 3^D stencil, so the lane accumulates its
+// cell's scatter in registers and adds it to the LDS tile once: for a fixed
+// stencil offset the 64 lanes hit 64 distinct tile nodes (no conflicts).
+constexpr int BWG = 64;
+
+// Weights of particle i; false if it is not in cell l of block bb (only after
+// particles moved without re-binning: the particle is skipped).
+template <int D>
+__device__ inline bool cell_particle(const KP& p, int i, const int* bb, const int* l, float (*w)[3], float (*dw)[3],
+                                     float* X) {
+    int lp[D];
+    if (!particle_weights<D>(p, i, bb, w, dw, lp, X)) return false;
+    bool same = true;
+#pragma unroll
+    for (int a = 0; a < D; ++a) same = same && lp[a] == l[a];
+    return same;
 }
 
 // ============================================================================
 // P2G (Rasterize, MPM_Example.cpp:520-537): m_i += w m_p, (mv)_i += w m_p V_p
 // ============================================================================
 template <int D>
-__global__ __launch_bounds__(WG) void k_p2g(KP p) {
+__global__ __launch_bounds__(BWG) void k_p2g(KP p) {
     using Dm = Dims<D>;
-    constexpr int TN = Dm::TN, NBR = Dm::NBR;
-    __shared__ float tile[D + 1][TN];
-    __shared__ int nbr[NBR];
-    if (p.ctr->overflow) return;
+    constexpr int TN = Dm::TN, ST = Dm::ST;
+    __shared__ float tile[(D + 1) * TN];
     const int nblocks = p.ctr->nblocks;
+    const int c = threadIdx.x;
+    int l[D];
+    local_coords<D>(c, l);
     for (int b = xcd_wg(); b < nblocks; b += gridDim.x) {
-        for (int t = threadIdx.x; t < (D + 1) * TN; t += WG) (&tile[0][0])[t] = 0.0f;
-        if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
+        for (int t = c; t < (D + 1) * TN; t += BWG) tile[t] = 0.0f;
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
-        __syncthreads();
-        const int i1 = p.block_pstart[b + 1];
-        for (int i = p.block_pstart[b] + threadIdx.x; i < i1; i += WG) {
+        float acc[ST][D + 1];
+#pragma unroll
+        for (int s = 0; s < ST; ++s)
+#pragma unroll
+            for (int k = 0; k <= D; ++k) acc[s][k] = 0.0f;
+        const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
+        for (int i = i0; i < i1; ++i) {
             float w[D][3], dw[D][3], X[D];
-            int l[D];
-            if (!particle_weights<D>(p, i, bb, w, dw, l, X)) continue;
+            if (!cell_particle<D>(p, i, bb, l, w, dw, X)) continue;
             const float m = p.pcur[PF<D>::M * p.pcap + i];
             float mv[D];
 #pragma unroll
             for (int a = 0; a < D; ++a) mv[a] = m * p.pcur[(PF<D>::V + a) * p.pcap + i];
 #pragma unroll
-            for (int s = 0; s < Dm::ST; ++s) {
+            for (int s = 0; s < ST; ++s) {
                 int off[D];
-                const int t = tile_index<D>(l, s, off);
+                tile_index<D>(l, s, off);
                 const float wt = stencil_weight<D>(w, off);
-                atomicAdd(&tile[0][t], wt * m);
+                acc[s][0] += wt * m;
 #pragma unroll
-                for (int a = 0; a < D; ++a) atomicAdd(&tile[1 + a][t], wt * mv[a]);
+                for (int a = 0; a < D; ++a) acc[s][1 + a] += wt * mv[a];
             }
         }
         __syncthreads();
-        for (int t = threadIdx.x; t < TN; t += WG) {
-            const float mm = tile[0][t];
-            if (mm == 0.0f) continue;
-            int nb, local, tc[D];
-            tile_map<D>(t, nb, local, tc);
-            const int blk = nbr[nb];
-            if (blk < 0 || !flush_allowed<D>(p, bb, tc)) continue;
-            const int64_t node = (int64_t)blk * 64 + local;
-            unsafeAtomicAdd(&p.g_m[node], mm);
+        if (i1 > i0) {
 #pragma unroll
-            for (int a = 0; a < D; ++a) unsafeAtomicAdd(&p.g_v[a * p.gcap + node], tile[1 + a][t]);
+            for (int s = 0; s < ST; ++s) {
+                int off[D];
+                const int t = tile_index<D>(l, s, off);
+#pragma unroll
+                for (int k = 0; k <= D; ++k) atomicAdd(&tile[k * TN + t], acc[s][k]);
+            }
         }
+        __syncthreads();
+        store_tile<D, D + 1>(p, b, tile);
         __syncthreads();
     }
 }
 
-// Flag_Setup_Helper (Flag_Setup_Helper.h:25-35) + Velocity_Normalization_Helper
-// (Velocity_Normalization_Helper.h:39-52)
+// Tile sum of P2G + Flag_Setup_Helper (Flag_Setup_Helper.h:25-35) +
+// Velocity_Normalization_Helper (Velocity_Normalization_Helper.h:39-52)
 template <int D>
 __global__ __launch_bounds__(WG) void k_normalize(KP p) {
-    if (p.ctr->overflow) return;
     const int nn = p.ctr->nblocks * 64;
     for (int n = blockIdx.x * WG + threadIdx.x; n < nn; n += gridDim.x * WG) {
-        const float m = p.g_m[n];
+        float v[D + 1];
+        gather_node_tiles<D, D + 1>(p, n, v);
+        const float m = v[0];
         const bool interior = m > 0.0f;
+        p.g_m[n] = m;
         p.g_flags[n] = interior ? MPM_FLAG_INTERIOR_DEV : 0u;
-        if (interior) {
-            const float inv = 1.0f / m;
+        const float inv = interior ? 1.0f / m : 0.0f;
 #pragma unroll
-            for (int a = 0; a < D; ++a) p.g_v[a * p.gcap + n] *= inv;
-        }
+        for (int a = 0; a < D; ++a) p.g_v[a * p.gcap + n] = interior ? v[1 + a] * inv : v[1 + a];
     }
 }
 
@@ -485,7 +607,7 @@ __global__ __launch_bounds__(WG) void k_normalize(KP p) {
 template <int D, int NCH>
 __device__ inline void gather_tile(const KP& p, const int* nbr, float (*tile)[Dims<D>::TN], const float* const* src) {
     constexpr int TN = Dims<D>::TN;
-    for (int t = threadIdx.x; t < TN; t += WG) {
+    for (int t = threadIdx.x; t < TN; t += (int)blockDim.x) {
         int nb, local, tc[D];
         tile_map<D>(t, nb, local, tc);
         const int blk = nbr[nb];
@@ -496,29 +618,30 @@ __device__ inline void gather_tile(const KP& p, const int* nbr, float (*tile)[Di
 
 // Process_Waiting_Particles (MPM_Example.cpp:827-840): V0 = m / (sum w m_i / dx^d)
 template <int D>
-__global__ __launch_bounds__(WG) void k_init_volume(KP p) {
+__global__ __launch_bounds__(BWG) void k_init_volume(KP p) {
     using Dm = Dims<D>;
     constexpr int TN = Dm::TN, NBR = Dm::NBR;
     __shared__ float tile[1][TN];
     __shared__ int nbr[NBR];
-    if (p.ctr->overflow) return;
     const int nblocks = p.ctr->nblocks;
+    const int c = threadIdx.x;
+    int l[D];
+    local_coords<D>(c, l);
     float vol = 1.0f;
     for (int a = 0; a < D; ++a) vol *= p.dx;
     const float inv_vol = 1.0f / vol;
     for (int b = xcd_wg(); b < nblocks; b += gridDim.x) {
-        if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
+        if (c < NBR) nbr[c] = p.block_nbr[b * NBR + c];
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
         __syncthreads();
         const float* src[1] = {p.g_m};
         gather_tile<D, 1>(p, nbr, tile, src);
         __syncthreads();
-        const int i1 = p.block_pstart[b + 1];
-        for (int i = p.block_pstart[b] + threadIdx.x; i < i1; i += WG) {
+        const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
+        for (int i = i0; i < i1; ++i) {
             float w[D][3], dw[D][3], X[D];
-            int l[D];
-            if (!particle_weights<D>(p, i, bb, w, dw, l, X)) continue;
+            if (!cell_particle<D>(p, i, bb, l, w, dw, X)) continue;
             float rho = 0.0f;
 #pragma unroll
             for (int s = 0; s < Dm::ST; ++s) {
@@ -538,23 +661,26 @@ __global__ __launch_bounds__(WG) void k_init_volume(KP p) {
 // (MPM_Example.cpp:757-784): f_i += -V0 P F^T grad w + g m w
 // ============================================================================
 template <int D, bool CONST, bool FORCE>
-__global__ __launch_bounds__(WG) void k_particle_force(KP p) {
+__global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
     using Dm = Dims<D>;
-    constexpr int TN = Dm::TN, NBR = Dm::NBR, DD = D * D;
-    __shared__ float tile[D][TN];
-    __shared__ int nbr[NBR];
-    if (p.ctr->overflow) return;
+    constexpr int TN = Dm::TN, DD = D * D, ST = Dm::ST;
+    __shared__ float tile[D * TN];
     const int nblocks = p.ctr->nblocks;
+    const int c = threadIdx.x;
+    int l[D];
+    local_coords<D>(c, l);
     for (int b = xcd_wg(); b < nblocks; b += gridDim.x) {
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
-        if (FORCE) {
-            for (int t = threadIdx.x; t < D * TN; t += WG) (&tile[0][0])[t] = 0.0f;
-            if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
-            __syncthreads();
-        }
-        const int i1 = p.block_pstart[b + 1];
-        for (int i = p.block_pstart[b] + threadIdx.x; i < i1; i += WG) {
+        if (FORCE)
+            for (int t = c; t < D * TN; t += BWG) tile[t] = 0.0f;
+        float acc[ST][D];
+#pragma unroll
+        for (int s = 0; s < ST; ++s)
+#pragma unroll
+            for (int k = 0; k < D; ++k) acc[s][k] = 0.0f;
+        const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
+        for (int i = i0; i < i1; ++i) {
             float F[DD], R[DD];
 #pragma unroll
             for (int k = 0; k < DD; ++k) F[k] = p.pcur[(PF<D>::F + k) * p.pcap + i];
@@ -572,6 +698,8 @@ __global__ __launch_bounds__(WG) void k_particle_force(KP p) {
                 for (int k = 0; k < DD; ++k) R[k] = p.rcache[k * p.pcap + i];
             }
             if (FORCE) {
+                float w[D][3], dw[D][3], X[D];
+                if (!cell_particle<D>(p, i, bb, l, w, dw, X)) continue;
                 StressState<D> st;
                 stress_state<D>(F, R, Jp, p.mat, st);
                 float P[DD];
@@ -582,22 +710,19 @@ __global__ __launch_bounds__(WG) void k_particle_force(KP p) {
 #pragma unroll
                 for (int r = 0; r < D; ++r)
 #pragma unroll
-                    for (int c = 0; c < D; ++c) {
+                    for (int cc = 0; cc < D; ++cc) {
                         float s = 0.0f;
 #pragma unroll
-                        for (int k = 0; k < D; ++k) s += P[r * D + k] * F[c * D + k];
-                        A[r * D + c] = s * V0;
+                        for (int k = 0; k < D; ++k) s += P[r * D + k] * F[cc * D + k];
+                        A[r * D + cc] = s * V0;
                     }
-                float w[D][3], dw[D][3], X[D];
-                int l[D];
-                if (!particle_weights<D>(p, i, bb, w, dw, l, X)) continue;
                 float gm[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) gm[a] = p.gravity[a] * m;
 #pragma unroll
-                for (int s = 0; s < Dm::ST; ++s) {
+                for (int s = 0; s < ST; ++s) {
                     int off[D];
-                    const int t = tile_index<D>(l, s, off);
+                    tile_index<D>(l, s, off);
                     float g[D];
                     stencil_grad<D>(w, dw, off, g);
                     const float wt = stencil_weight<D>(w, off);
@@ -606,44 +731,44 @@ __global__ __launch_bounds__(WG) void k_particle_force(KP p) {
                         float inner = 0.0f;
 #pragma unroll
                         for (int k = 0; k < D; ++k) inner += A[a * D + k] * g[k];
-                        atomicAdd(&tile[a][t], gm[a] * wt - inner);
+                        acc[s][a] += gm[a] * wt - inner;
                     }
                 }
             }
         }
         if (FORCE) {
             __syncthreads();
-            for (int t = threadIdx.x; t < TN; t += WG) {
-                int nb, local, tc[D];
-                tile_map<D>(t, nb, local, tc);
-                const int blk = nbr[nb];
-                if (blk < 0 || !flush_allowed<D>(p, bb, tc)) continue;
-                const int64_t node = (int64_t)blk * 64 + local;
+            if (i1 > i0) {
 #pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    const float v = tile[a][t];
-                    if (v != 0.0f) unsafeAtomicAdd(&p.g_q[a * p.gcap + node], v);
+                for (int s = 0; s < ST; ++s) {
+                    int off[D];
+                    const int t = tile_index<D>(l, s, off);
+#pragma unroll
+                    for (int k = 0; k < D; ++k) atomicAdd(&tile[k * TN + t], acc[s][k]);
                 }
             }
+            __syncthreads();
+            store_tile<D, D>(p, b, tile);
             __syncthreads();
         }
     }
 }
 
 // ============================================================================
-// Grid update: Explicit_Force_Helper (v* = v + dt f / m), plane collision
-// (Grid_Based_Collision_Helper.h:57-85) and the CG set-up (MPM_RHS_Helper,
-// Reset_Solver_Channels: r = b = v*, s = q = 0).
+// Grid update: tile sum of the force, Explicit_Force_Helper (v* = v + dt f/m),
+// plane collision (Grid_Based_Collision_Helper.h:57-85) and the CG set-up
+// (MPM_RHS_Helper, Reset_Solver_Channels: r = b = v*, s = 0).
+// WRITE_F keeps the force in g_q for inspection.
 // ============================================================================
-template <int D, bool FORCE, bool COLL, bool CGINIT>
+template <int D, bool FORCE, bool COLL, bool CGINIT, bool WRITE_F>
 __global__ __launch_bounds__(WG) void k_grid_update(KP p) {
-    if (p.ctr->overflow) return;
     if (CGINIT && blockIdx.x == 0 && threadIdx.x == 0) {
         p.cg->iter = 0;
         p.cg->done = 0;
         p.cg->converged = 0;
         p.cg->multiplies = 0;
-        p.cg->rho = p.cg->rho_old = p.cg->alpha = p.cg->beta = p.cg->conv = p.cg->rr = p.cg->sq = 0.0;
+        p.cg->rho_hist[0] = p.cg->rho_hist[1] = 0.0;
+        p.cg->alpha = p.cg->beta = p.cg->conv = p.cg->rr = p.cg->sq = 0.0;
     }
     const int nn = p.ctr->nblocks * 64;
     const float dt = p.dt;
@@ -651,10 +776,16 @@ __global__ __launch_bounds__(WG) void k_grid_update(KP p) {
         const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
         float vs[D];
         if (FORCE) {
+            float f[D];
+            gather_node_tiles<D, D>(p, n, f);
+            if (WRITE_F) {
+#pragma unroll
+                for (int a = 0; a < D; ++a) p.g_q[a * p.gcap + n] = f[a];
+            }
             if (interior) {
                 const float dtm = dt / p.g_m[n];
 #pragma unroll
-                for (int a = 0; a < D; ++a) vs[a] = p.g_v[a * p.gcap + n] + dtm * p.g_q[a * p.gcap + n];
+                for (int a = 0; a < D; ++a) vs[a] = p.g_v[a * p.gcap + n] + dtm * f[a];
             } else {
 #pragma unroll
                 for (int a = 0; a < D; ++a) vs[a] = 0.0f;
@@ -698,7 +829,6 @@ __global__ __launch_bounds__(WG) void k_grid_update(KP p) {
             for (int a = 0; a < D; ++a) {
                 p.g_r[a * p.gcap + n] = interior ? vs[a] : 0.0f;
                 p.g_s[a * p.gcap + n] = 0.0f;
-                p.g_q[a * p.gcap + n] = 0.0f;
             }
         }
     }
@@ -709,36 +839,43 @@ __global__ __launch_bounds__(WG) void k_grid_update(KP p) {
 //   T = sum_i grad w_ip (x) u_i;  dF = T^T F;  A = V0 F dP(dF)^T;  q_i += A^T grad w_ip
 // ============================================================================
 template <int D, bool RESTART>
-__global__ __launch_bounds__(WG) void k_cg_force(KP p) {
+__global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
     using Dm = Dims<D>;
-    constexpr int TN = Dm::TN, NBR = Dm::NBR, DD = D * D;
+    constexpr int TN = Dm::TN, NBR = Dm::NBR, DD = D * D, ST = Dm::ST;
     __shared__ float ut[D][TN];
-    __shared__ float qt[D][TN];
+    __shared__ float qt[D * TN];
     __shared__ int nbr[NBR];
-    if (p.ctr->overflow || p.cg->done) return;
+    if (p.cg->done) return;
     const int nblocks = p.ctr->nblocks;
     const float* u = RESTART ? p.g_vs : p.g_s;
+    const int c = threadIdx.x;
+    int l[D];
+    local_coords<D>(c, l);
     for (int b = xcd_wg(); b < nblocks; b += gridDim.x) {
-        if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
+        if (c < NBR) nbr[c] = p.block_nbr[b * NBR + c];
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
-        for (int t = threadIdx.x; t < D * TN; t += WG) (&qt[0][0])[t] = 0.0f;
+        for (int t = c; t < D * TN; t += BWG) qt[t] = 0.0f;
         __syncthreads();
         const float* src[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) src[a] = u + a * p.gcap;
         gather_tile<D, D>(p, nbr, ut, src);
         __syncthreads();
-        const int i1 = p.block_pstart[b + 1];
-        for (int i = p.block_pstart[b] + threadIdx.x; i < i1; i += WG) {
+        float acc[ST][D];
+#pragma unroll
+        for (int s = 0; s < ST; ++s)
+#pragma unroll
+            for (int k = 0; k < D; ++k) acc[s][k] = 0.0f;
+        const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
+        for (int i = i0; i < i1; ++i) {
             float w[D][3], dw[D][3], X[D];
-            int l[D];
-            if (!particle_weights<D>(p, i, bb, w, dw, l, X)) continue;
+            if (!cell_particle<D>(p, i, bb, l, w, dw, X)) continue;
             float T[DD];
 #pragma unroll
             for (int k = 0; k < DD; ++k) T[k] = 0.0f;
 #pragma unroll
-            for (int s = 0; s < Dm::ST; ++s) {
+            for (int s = 0; s < ST; ++s) {
                 int off[D];
                 const int t = tile_index<D>(l, s, off);
                 float g[D];
@@ -746,7 +883,7 @@ __global__ __launch_bounds__(WG) void k_cg_force(KP p) {
 #pragma unroll
                 for (int a = 0; a < D; ++a)
 #pragma unroll
-                    for (int c = 0; c < D; ++c) T[a * D + c] += g[a] * ut[c][t];
+                    for (int cc = 0; cc < D; ++cc) T[a * D + cc] += g[a] * ut[cc][t];
             }
             float F[DD], R[DD];
 #pragma unroll
@@ -760,11 +897,11 @@ __global__ __launch_bounds__(WG) void k_cg_force(KP p) {
 #pragma unroll
             for (int r = 0; r < D; ++r)
 #pragma unroll
-                for (int c = 0; c < D; ++c) {
+                for (int cc = 0; cc < D; ++cc) {
                     float s = 0.0f;
 #pragma unroll
-                    for (int k = 0; k < D; ++k) s += T[k * D + r] * F[k * D + c];
-                    dF[r * D + c] = s;
+                    for (int k = 0; k < D; ++k) s += T[k * D + r] * F[k * D + cc];
+                    dF[r * D + cc] = s;
                 }
             StressState<D> st;
             stress_state<D>(F, R, Jp, p.mat, st);
@@ -774,40 +911,38 @@ __global__ __launch_bounds__(WG) void k_cg_force(KP p) {
 #pragma unroll
             for (int r = 0; r < D; ++r)
 #pragma unroll
-                for (int c = 0; c < D; ++c) {
+                for (int cc = 0; cc < D; ++cc) {
                     float s = 0.0f;
 #pragma unroll
-                    for (int k = 0; k < D; ++k) s += F[r * D + k] * dP[c * D + k];
-                    A[r * D + c] = s * V0;
+                    for (int k = 0; k < D; ++k) s += F[r * D + k] * dP[cc * D + k];
+                    A[r * D + cc] = s * V0;
                 }
 #pragma unroll
-            for (int s = 0; s < Dm::ST; ++s) {
+            for (int s = 0; s < ST; ++s) {
                 int off[D];
-                const int t = tile_index<D>(l, s, off);
+                tile_index<D>(l, s, off);
                 float g[D];
                 stencil_grad<D>(w, dw, off, g);
 #pragma unroll
-                for (int c = 0; c < D; ++c) {
+                for (int cc = 0; cc < D; ++cc) {
                     float v = 0.0f;
 #pragma unroll
-                    for (int r = 0; r < D; ++r) v += A[r * D + c] * g[r];
-                    atomicAdd(&qt[c][t], v);
+                    for (int r = 0; r < D; ++r) v += A[r * D + cc] * g[r];
+                    acc[s][cc] += v;
                 }
             }
         }
-        __syncthreads();
-        for (int t = threadIdx.x; t < TN; t += WG) {
-            int nb, local, tc[D];
-            tile_map<D>(t, nb, local, tc);
-            const int blk = nbr[nb];
-            if (blk < 0 || !flush_allowed<D>(p, bb, tc)) continue;
-            const int64_t node = (int64_t)blk * 64 + local;
+        if (i1 > i0) {
 #pragma unroll
-            for (int a = 0; a < D; ++a) {
-                const float v = qt[a][t];
-                if (v != 0.0f) unsafeAtomicAdd(&p.g_q[a * p.gcap + node], v);
+            for (int s = 0; s < ST; ++s) {
+                int off[D];
+                const int t = tile_index<D>(l, s, off);
+#pragma unroll
+                for (int k = 0; k < D; ++k) atomicAdd(&qt[k * TN + t], acc[s][k]);
             }
         }
+        __syncthreads();
+        store_tile<D, D>(p, b, qt);
         __syncthreads();
     }
 }
@@ -824,24 +959,25 @@ __device__ inline bool project_zero(const KP& p, int n) {
 }
 
 // CG restart: q = dt^2/m q + x (Multiply_Helper), r = b - q, Project(r);
-// then the convergence norm and r.r.
+// partials of r.r (M-weighted) and max |r|^2.
 template <int D>
 __global__ __launch_bounds__(WG) void k_cg_restart_node(KP p) {
-    if (p.ctr->overflow) return;
     const int nn = p.ctr->nblocks * 64;
     const float sds = p.dt * p.dt;
+    if (blockIdx.x == 0 && threadIdx.x == 0) p.cg->multiplies = 1;
     double sum = 0.0, mx = 0.0;
     for (int n = blockIdx.x * WG + threadIdx.x; n < nn; n += gridDim.x * WG) {
         const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
-        float r[D];
+        float q[D], r[D];
+        gather_node_tiles<D, D>(p, n, q);
 #pragma unroll
         for (int a = 0; a < D; ++a) r[a] = p.g_r[a * p.gcap + n];
         if (interior) {
             const float c = sds / p.g_m[n];
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                const float q = c * p.g_q[a * p.gcap + n] + p.g_vs[a * p.gcap + n];
-                r[a] = r[a] - q;
+                q[a] = c * q[a] + p.g_vs[a * p.gcap + n];
+                r[a] = r[a] - q[a];
             }
         }
         if (project_zero<D>(p, n)) {
@@ -851,7 +987,7 @@ __global__ __launch_bounds__(WG) void k_cg_restart_node(KP p) {
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             p.g_r[a * p.gcap + n] = r[a];
-            p.g_q[a * p.gcap + n] = 0.0f;
+            p.g_q[a * p.gcap + n] = q[a];
         }
         if (interior) {
             const double m = (double)p.g_m[n];
@@ -866,56 +1002,64 @@ __global__ __launch_bounds__(WG) void k_cg_restart_node(KP p) {
         }
     }
     wg_reduce2(sum, mx);
-    if (publish_and_check_last(p, sum, mx)) {
-        gather_partials(p, sum, mx);
-        if (threadIdx.x == 0) {
-            CGState* cg = p.cg;
-            cg->multiplies = 1;
-            cg->iter = 0;
-            cg->conv = sqrt(mx);
-            cg->rr = sum;
-            if (cg->conv <= p.cg_tol && (0 >= p.cg_min || cg->conv < (double)FLT_EPSILON)) {
-                cg->done = 1;
-                cg->converged = 1;
-            } else if (p.cg_max == 0) {
-                cg->done = 1;
-            } else {
-                cg->rho = sum;
-            }
-            *p.sync = 0u;
-        }
+    if (threadIdx.x == 0) {
+        p.red[p.red_stride + blockIdx.x] = sum;
+        p.red[2 * p.red_stride + blockIdx.x] = mx;
     }
 }
 
-// s = r (first iteration) or s = (rho/rho_old) s + r
+// Convergence test of iteration k and s = r (k = 0) or s = (rho/rho_old) s + r.
+// Every work-group reduces the r.r / max|r|^2 partials of the previous
+// kernel itself (same order, same result); work-group 0 records the state.
 template <int D>
-__global__ __launch_bounds__(WG) void k_cg_update_s(KP p) {
-    if (p.ctr->overflow || p.cg->done) return;
+__global__ __launch_bounds__(WG) void k_cg_update_s(KP p, int k) {
+    CGState* cg = p.cg;
+    if (cg->done) return;
+    double rr, mx;
+    reduce_partials(p.red + p.red_stride, p.red + 2 * p.red_stride, p.red_count, rr, mx);
+    const double conv = sqrt(mx);
+    bool done = false, converged = false;
+    if (conv <= p.cg_tol && (k >= p.cg_min || conv < (double)FLT_EPSILON)) {
+        done = converged = true;
+    }
+    const double rho_old = cg->rho_hist[(k + 1) & 1];
+    const double beta = k == 0 ? 0.0 : rr / rho_old;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cg->iter = k;
+        cg->conv = conv;
+        cg->rr = rr;
+        if (done) {
+            cg->done = 1;
+            cg->converged = converged ? 1 : 0;
+        } else {
+            cg->rho_hist[k & 1] = rr;
+            cg->beta = beta;
+        }
+    }
+    if (done) return;
     const int nn = p.ctr->nblocks * 64;
-    const bool first = p.cg->iter == 0;
-    const float beta = (float)p.cg->beta;
+    const float fb = (float)beta;
     for (int n = blockIdx.x * WG + threadIdx.x; n < nn; n += gridDim.x * WG) {
         if (!(p.g_flags[n] & MPM_FLAG_INTERIOR_DEV)) continue;
 #pragma unroll
         for (int a = 0; a < D; ++a) {
             const float r = p.g_r[a * p.gcap + n];
-            p.g_s[a * p.gcap + n] = first ? r : beta * p.g_s[a * p.gcap + n] + r;
+            p.g_s[a * p.gcap + n] = k == 0 ? r : fb * p.g_s[a * p.gcap + n] + r;
         }
     }
 }
 
-// q = dt^2/m q + s, Project(q), s.q
+// tile sum of the operator, q = dt^2/m q + s, Project(q); partials of s.q
 template <int D>
 __global__ __launch_bounds__(WG) void k_cg_node1(KP p) {
-    if (p.ctr->overflow || p.cg->done) return;
+    if (p.cg->done) return;
     const int nn = p.ctr->nblocks * 64;
     const float sds = p.dt * p.dt;
     double sum = 0.0, mx = 0.0;
     for (int n = blockIdx.x * WG + threadIdx.x; n < nn; n += gridDim.x * WG) {
         const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
         float q[D];
-#pragma unroll
-        for (int a = 0; a < D; ++a) q[a] = p.g_q[a * p.gcap + n];
+        gather_node_tiles<D, D>(p, n, q);
         if (interior) {
             const float c = sds / p.g_m[n];
 #pragma unroll
@@ -936,65 +1080,63 @@ __global__ __launch_bounds__(WG) void k_cg_node1(KP p) {
         }
     }
     wg_reduce2(sum, mx);
-    if (publish_and_check_last(p, sum, mx)) {
-        gather_partials(p, sum, mx);
-        if (threadIdx.x == 0) {
-            CGState* cg = p.cg;
-            cg->multiplies += 1;
-            cg->sq = sum;
-            cg->alpha = sum != 0.0 ? cg->rho / sum : (double)FLT_MAX;
-            *p.sync = 0u;
-        }
-    }
+    if (threadIdx.x == 0) p.red[blockIdx.x] = sum;
 }
 
-// x += alpha s, r -= alpha q, then the convergence test of the next iteration
+// alpha = rho / s.q;  x += alpha s, r -= alpha q; partials of r.r and max|r|^2
 template <int D>
-__global__ __launch_bounds__(WG) void k_cg_node2(KP p) {
-    if (p.ctr->overflow || p.cg->done) return;
+__global__ __launch_bounds__(WG) void k_cg_node2(KP p, int k) {
+    CGState* cg = p.cg;
+    if (cg->done) return;
+    double sq, unused;
+    reduce_partials(p.red, nullptr, p.red_count, sq, unused);
+    const double alpha_d = sq != 0.0 ? cg->rho_hist[k & 1] / sq : (double)FLT_MAX;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        cg->sq = sq;
+        cg->alpha = alpha_d;
+        cg->multiplies += 1;
+    }
+    const float alpha = (float)alpha_d;
     const int nn = p.ctr->nblocks * 64;
-    const float alpha = (float)p.cg->alpha;
     double sum = 0.0, mx = 0.0;
     for (int n = blockIdx.x * WG + threadIdx.x; n < nn; n += gridDim.x * WG) {
         const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
-        if (interior) {
-            const double m = (double)p.g_m[n];
-            double rr = 0.0, r2 = 0.0;
+        if (!interior) continue;
+        const double m = (double)p.g_m[n];
+        double rr = 0.0, r2 = 0.0;
 #pragma unroll
-            for (int a = 0; a < D; ++a) {
-                const float s = p.g_s[a * p.gcap + n], q = p.g_q[a * p.gcap + n];
-                p.g_vs[a * p.gcap + n] = alpha * s + p.g_vs[a * p.gcap + n];
-                const float r = -alpha * q + p.g_r[a * p.gcap + n];
-                p.g_r[a * p.gcap + n] = r;
-                rr += (double)r * ((double)r * m);
-                r2 += (double)r * (double)r;
-            }
-            sum += rr;
-            mx = fmax(mx, r2);
+        for (int a = 0; a < D; ++a) {
+            const float s = p.g_s[a * p.gcap + n], q = p.g_q[a * p.gcap + n];
+            p.g_vs[a * p.gcap + n] = alpha * s + p.g_vs[a * p.gcap + n];
+            const float r = -alpha * q + p.g_r[a * p.gcap + n];
+            p.g_r[a * p.gcap + n] = r;
+            rr += (double)r * ((double)r * m);
+            r2 += (double)r * (double)r;
         }
-#pragma unroll
-        for (int a = 0; a < D; ++a) p.g_q[a * p.gcap + n] = 0.0f;
+        sum += rr;
+        mx = fmax(mx, r2);
     }
     wg_reduce2(sum, mx);
-    if (publish_and_check_last(p, sum, mx)) {
-        gather_partials(p, sum, mx);
-        if (threadIdx.x == 0) {
-            CGState* cg = p.cg;
-            cg->iter += 1;
-            cg->conv = sqrt(mx);
-            cg->rr = sum;
-            if (cg->conv <= p.cg_tol && (cg->iter >= p.cg_min || cg->conv < (double)FLT_EPSILON)) {
-                cg->done = 1;
-                cg->converged = 1;
-            } else if (cg->iter == p.cg_max) {
-                cg->done = 1;
-            } else {
-                cg->rho_old = cg->rho;
-                cg->rho = sum;
-                cg->beta = cg->rho / cg->rho_old;
-            }
-            *p.sync = 0u;
-        }
+    if (threadIdx.x == 0) {
+        p.red[p.red_stride + blockIdx.x] = sum;
+        p.red[2 * p.red_stride + blockIdx.x] = mx;
+    }
+}
+
+// After the last launched iteration: convergence test of iteration k_end
+// (one work-group).
+__global__ __launch_bounds__(WG) void k_cg_finalize(KP p, int k_end) {
+    CGState* cg = p.cg;
+    if (cg->done) return;
+    double rr, mx;
+    reduce_partials(p.red + p.red_stride, p.red + 2 * p.red_stride, p.red_count, rr, mx);
+    if (threadIdx.x == 0) {
+        const double conv = sqrt(mx);
+        cg->iter = k_end;
+        cg->conv = conv;
+        cg->rr = rr;
+        cg->converged = (conv <= p.cg_tol && (k_end >= p.cg_min || conv < (double)FLT_EPSILON)) ? 1 : 0;
+        cg->done = 1;
     }
 }
 
@@ -1002,17 +1144,21 @@ __global__ __launch_bounds__(WG) void k_cg_node2(KP p) {
 // G2P (MPM_Example.cpp:631-660): PIC/FLIP velocity, F and X update, removal
 // ============================================================================
 template <int D>
-__global__ __launch_bounds__(WG) void k_g2p(KP p) {
+__global__ __launch_bounds__(BWG) void k_g2p(KP p) {
     using Dm = Dims<D>;
     constexpr int TN = Dm::TN, NBR = Dm::NBR, DD = D * D;
     __shared__ float vt[2 * D][TN];
     __shared__ int nbr[NBR];
-    if (p.ctr->overflow) return;
     const int nblocks = p.ctr->nblocks;
     const float dt = p.dt, flip = p.flip;
     const float lo = 0.0f + 0.5f * p.dx, hi = 1.0f - 0.5f * p.dx;
+    unsigned best = 0u;
+    int removed = 0;
+    const int c = threadIdx.x;
+    int l[D];
+    local_coords<D>(c, l);
     for (int b = xcd_wg(); b < nblocks; b += gridDim.x) {
-        if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
+        if (c < NBR) nbr[c] = p.block_nbr[b * NBR + c];
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
         __syncthreads();
@@ -1024,11 +1170,10 @@ __global__ __launch_bounds__(WG) void k_g2p(KP p) {
         }
         gather_tile<D, 2 * D>(p, nbr, vt, src);
         __syncthreads();
-        const int i1 = p.block_pstart[b + 1];
-        for (int i = p.block_pstart[b] + threadIdx.x; i < i1; i += WG) {
+        const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
+        for (int i = i0; i < i1; ++i) {
             float w[D][3], dw[D][3], X[D];
-            int l[D];
-            if (!particle_weights<D>(p, i, bb, w, dw, l, X)) continue;
+            if (!cell_particle<D>(p, i, bb, l, w, dw, X)) continue;
             float vpic[D], vflip[D], G[DD];
 #pragma unroll
             for (int a = 0; a < D; ++a) {
@@ -1051,7 +1196,7 @@ __global__ __launch_bounds__(WG) void k_g2p(KP p) {
                         vpic[a] += wt * vs;
                         vflip[a] += wt * (vs - v);
 #pragma unroll
-                        for (int c = 0; c < D; ++c) G[a * D + c] += vs * g[c];
+                        for (int cc = 0; cc < D; ++cc) G[a * D + cc] += vs * g[cc];
                     }
                 }
             }
@@ -1061,11 +1206,11 @@ __global__ __launch_bounds__(WG) void k_g2p(KP p) {
 #pragma unroll
             for (int r = 0; r < D; ++r)
 #pragma unroll
-                for (int c = 0; c < D; ++c) {
+                for (int cc = 0; cc < D; ++cc) {
                     float s = 0.0f;
 #pragma unroll
-                    for (int k = 0; k < D; ++k) s += (dt * G[r * D + k]) * F[k * D + c];
-                    p.pcur[(PF<D>::F + r * D + c) * p.pcap + i] = F[r * D + c] + s;
+                    for (int k = 0; k < D; ++k) s += (dt * G[r * D + k]) * F[k * D + cc];
+                    p.pcur[(PF<D>::F + r * D + cc) * p.pcap + i] = F[r * D + cc] + s;
                 }
             bool inside = true;
             float v2 = 0.0f;
@@ -1080,10 +1225,22 @@ __global__ __launch_bounds__(WG) void k_g2p(KP p) {
             }
             if (!inside) {
                 p.keycur[i] = -1;
-                atomicAdd(&p.ctr->n_removed, 1);
-            } else atomicMax(&p.ctr->max_v2, __float_as_uint(v2));
+                ++removed;
+            } else {
+                best = max(best, __float_as_uint(v2));
+            }
         }
         __syncthreads();
+    }
+    // one atomic per wave for the max speed and the removal count
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        best = max(best, (unsigned)__shfl_xor((int)best, off));
+        removed += __shfl_xor(removed, off);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (best) atomicMax(&p.ctr->max_v2, best);
+        if (removed) atomicAdd(&p.ctr->n_removed, removed);
     }
 }
 
@@ -1117,30 +1274,31 @@ template <int D> void launch_scan(const KP& p, hipStream_t s) {
 }
 template <int D> void launch_block_setup(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_block_setup<D>, dim3(grid), dim3(WG), 0, s, p); }
 template <int D> void launch_reorder(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_reorder<D>, dim3(grid), dim3(WG), 0, s, p); }
-template <int D> void launch_p2g(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_p2g<D>, dim3(grid), dim3(WG), 0, s, p); }
+template <int D> void launch_p2g(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_p2g<D>, dim3(grid), dim3(BWG), 0, s, p); }
 template <int D> void launch_normalize(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_normalize<D>, dim3(grid), dim3(WG), 0, s, p); }
-template <int D> void launch_init_volume(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_init_volume<D>, dim3(grid), dim3(WG), 0, s, p); }
+template <int D> void launch_init_volume(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_init_volume<D>, dim3(grid), dim3(BWG), 0, s, p); }
 template <int D> void launch_particle_force(const KP& p, hipStream_t s, int grid, bool constitutive, bool force) {
-    if (constitutive && force) hipLaunchKernelGGL((k_particle_force<D, true, true>), dim3(grid), dim3(WG), 0, s, p);
-    else if (constitutive) hipLaunchKernelGGL((k_particle_force<D, true, false>), dim3(grid), dim3(WG), 0, s, p);
-    else if (force) hipLaunchKernelGGL((k_particle_force<D, false, true>), dim3(grid), dim3(WG), 0, s, p);
+    if (constitutive && force) hipLaunchKernelGGL((k_particle_force<D, true, true>), dim3(grid), dim3(BWG), 0, s, p);
+    else if (constitutive) hipLaunchKernelGGL((k_particle_force<D, true, false>), dim3(grid), dim3(BWG), 0, s, p);
+    else if (force) hipLaunchKernelGGL((k_particle_force<D, false, true>), dim3(grid), dim3(BWG), 0, s, p);
 }
-template <int D> void launch_grid_update(const KP& p, hipStream_t s, int grid, bool force, bool collision, bool cg_init) {
-    if (force && collision && cg_init) hipLaunchKernelGGL((k_grid_update<D, true, true, true>), dim3(grid), dim3(WG), 0, s, p);
-    else if (force && !collision && !cg_init) hipLaunchKernelGGL((k_grid_update<D, true, false, false>), dim3(grid), dim3(WG), 0, s, p);
-    else if (!force && collision && !cg_init) hipLaunchKernelGGL((k_grid_update<D, false, true, false>), dim3(grid), dim3(WG), 0, s, p);
-    else if (!force && !collision && cg_init) hipLaunchKernelGGL((k_grid_update<D, false, false, true>), dim3(grid), dim3(WG), 0, s, p);
-    else if (force && collision && !cg_init) hipLaunchKernelGGL((k_grid_update<D, true, true, false>), dim3(grid), dim3(WG), 0, s, p);
+template <int D> void launch_grid_update(const KP& p, hipStream_t s, int grid, bool force, bool collision, bool cg_init, bool write_f) {
+    if (force && collision && !cg_init && !write_f) hipLaunchKernelGGL((k_grid_update<D, true, true, false, false>), dim3(grid), dim3(WG), 0, s, p);
+    else if (force && !collision && !cg_init && write_f) hipLaunchKernelGGL((k_grid_update<D, true, false, false, true>), dim3(grid), dim3(WG), 0, s, p);
+    else if (!force && collision && !cg_init) hipLaunchKernelGGL((k_grid_update<D, false, true, false, false>), dim3(grid), dim3(WG), 0, s, p);
+    else if (!force && !collision && cg_init) hipLaunchKernelGGL((k_grid_update<D, false, false, true, false>), dim3(grid), dim3(WG), 0, s, p);
+    else hipLaunchKernelGGL((k_grid_update<D, true, true, true, false>), dim3(grid), dim3(WG), 0, s, p);
 }
 template <int D> void launch_cg_force(const KP& p, hipStream_t s, int grid, bool restart) {
-    if (restart) hipLaunchKernelGGL((k_cg_force<D, true>), dim3(grid), dim3(WG), 0, s, p);
-    else hipLaunchKernelGGL((k_cg_force<D, false>), dim3(grid), dim3(WG), 0, s, p);
+    if (restart) hipLaunchKernelGGL((k_cg_force<D, true>), dim3(grid), dim3(BWG), 0, s, p);
+    else hipLaunchKernelGGL((k_cg_force<D, false>), dim3(grid), dim3(BWG), 0, s, p);
 }
 template <int D> void launch_cg_restart_node(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_cg_restart_node<D>, dim3(grid), dim3(WG), 0, s, p); }
-template <int D> void launch_cg_update_s(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_cg_update_s<D>, dim3(grid), dim3(WG), 0, s, p); }
+template <int D> void launch_cg_update_s(const KP& p, hipStream_t s, int grid, int k) { hipLaunchKernelGGL(k_cg_update_s<D>, dim3(grid), dim3(WG), 0, s, p, k); }
 template <int D> void launch_cg_node1(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_cg_node1<D>, dim3(grid), dim3(WG), 0, s, p); }
-template <int D> void launch_cg_node2(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_cg_node2<D>, dim3(grid), dim3(WG), 0, s, p); }
-template <int D> void launch_g2p(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_g2p<D>, dim3(grid), dim3(WG), 0, s, p); }
+template <int D> void launch_cg_node2(const KP& p, hipStream_t s, int grid, int k) { hipLaunchKernelGGL(k_cg_node2<D>, dim3(grid), dim3(WG), 0, s, p, k); }
+void launch_cg_finalize(const KP& p, hipStream_t s, int k_end) { hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(WG), 0, s, p, k_end); }
+template <int D> void launch_g2p(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_g2p<D>, dim3(grid), dim3(BWG), 0, s, p); }
 template <int D> void launch_max_speed(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_max_speed<D>, dim3(grid), dim3(WG), 0, s, p); }
 
 #define MPM_INSTANTIATE(D)                                                                   \
@@ -1152,12 +1310,12 @@ template <int D> void launch_max_speed(const KP& p, hipStream_t s, int grid) { h
     template void launch_normalize<D>(const KP&, hipStream_t, int);                          \
     template void launch_init_volume<D>(const KP&, hipStream_t, int);                        \
     template void launch_particle_force<D>(const KP&, hipStream_t, int, bool, bool);         \
-    template void launch_grid_update<D>(const KP&, hipStream_t, int, bool, bool, bool);      \
+    template void launch_grid_update<D>(const KP&, hipStream_t, int, bool, bool, bool, bool); \
     template void launch_cg_force<D>(const KP&, hipStream_t, int, bool);                     \
     template void launch_cg_restart_node<D>(const KP&, hipStream_t, int);                    \
-    template void launch_cg_update_s<D>(const KP&, hipStream_t, int);                        \
+    template void launch_cg_update_s<D>(const KP&, hipStream_t, int, int);                   \
     template void launch_cg_node1<D>(const KP&, hipStream_t, int);                           \
-    template void launch_cg_node2<D>(const KP&, hipStream_t, int);                           \
+    template void launch_cg_node2<D>(const KP&, hipStream_t, int, int);                           \
     template void launch_g2p<D>(const KP&, hipStream_t, int);                                \
     template void launch_max_speed<D>(const KP&, hipStream_t, int);
 MPM_INSTANTIATE(2)
