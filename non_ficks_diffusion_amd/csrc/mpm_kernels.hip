@@ -510,12 +510,36 @@ __global__ __launch_bounds__(WG) void k_reorder(KP p) {
     }
 }
 
-// Block kernels run one wave per active block with lane c owning cell c
-// (particles are in (block, cell) order, cell ranges in cell_start).  All
-// particles of a lane share one 3^D stencil, so the lane accumulates its
-// cell's scatter in registers and adds it to the LDS tile once: for a fixed
-// stencil offset the 64 lanes hit 64 distinct tile nodes (no conflicts).
-constexpr int BWG = 64;
+// Block kernels run NW waves per active block.  In scatter kernels lane c of
+// every wave owns cell c of the block (particles are in (block, cell) order,
+// cell ranges in cell_start) and wave w takes the cell's particles w, w+NW,
+// ...  All particles of a lane share one 3^D stencil and, for a fixed stencil
+// offset, the 64 lanes of a wave address 64 distinct tile nodes.  Each wave
+// therefore scatters into its own LDS tile with plain read-add-write steps
+// (one stencil node per step; LDS operations of a wave execute in order), and
+// the NW wave tiles are summed when the block's tile is stored.  No LDS or
+// global atomics are used.
+constexpr int NW = 4;
+constexpr int BWG = 64 * NW;
+
+// Compiler-only memory barrier between two read-add-write steps of a wave
+// scatter: a lane's step s+1 may read the node another lane wrote in step s,
+// so no LDS access may move across it (a wave's LDS operations execute in
+// issue order).
+#define MPM_STEP_FENCE() __asm__ __volatile__("" ::: "memory")
+
+// Sum of the NW wave tiles (C channels each) of block b into its tile slot.
+template <int D, int C>
+__device__ inline void store_wave_tiles(const KP& p, int b, const float* lds) {
+    constexpr int TN = Dims<D>::TN;
+    float* dst = p.tiles + (int64_t)b * C * TN;
+    for (int t = threadIdx.x; t < C * TN; t += BWG) {
+        float v = 0.0f;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) v += lds[w * C * TN + t];
+        dst[t] = v;
+    }
+}
 
 // Weights of particle i; false if it is not in cell l of block bb (only after
 // particles moved without re-binning: the particle is skipped).
@@ -536,23 +560,20 @@ __device__ inline bool cell_particle(const KP& p, int i, const int* bb, const in
 template <int D>
 __global__ __launch_bounds__(BWG) void k_p2g(KP p) {
     using Dm = Dims<D>;
-    constexpr int TN = Dm::TN, ST = Dm::ST;
-    __shared__ float tile[(D + 1) * TN];
+    constexpr int TN = Dm::TN, ST = Dm::ST, C = D + 1;
+    __shared__ float tile[NW * C * TN];
     const int nblocks = p.ctr->nblocks;
-    const int c = threadIdx.x;
+    const int c = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float* mine = tile + wv * C * TN;
     int l[D];
     local_coords<D>(c, l);
     for (int b = xcd_wg(); b < nblocks; b += gridDim.x) {
-        for (int t = c; t < (D + 1) * TN; t += BWG) tile[t] = 0.0f;
+        for (int t = threadIdx.x; t < NW * C * TN; t += BWG) tile[t] = 0.0f;
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
-        float acc[ST][D + 1];
-#pragma unroll
-        for (int s = 0; s < ST; ++s)
-#pragma unroll
-            for (int k = 0; k <= D; ++k) acc[s][k] = 0.0f;
+        __syncthreads();
         const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
-        for (int i = i0; i < i1; ++i) {
+        for (int i = i0 + wv; i < i1; i += NW) {
             float w[D][3], dw[D][3], X[D];
             if (!cell_particle<D>(p, i, bb, l, w, dw, X)) continue;
             const float m = p.pcur[PF<D>::M * p.pcap + i];
@@ -562,25 +583,16 @@ __global__ __launch_bounds__(BWG) void k_p2g(KP p) {
 #pragma unroll
             for (int s = 0; s < ST; ++s) {
                 int off[D];
-                tile_index<D>(l, s, off);
-                const float wt = stencil_weight<D>(w, off);
-                acc[s][0] += wt * m;
-#pragma unroll
-                for (int a = 0; a < D; ++a) acc[s][1 + a] += wt * mv[a];
-            }
-        }
-        __syncthreads();
-        if (i1 > i0) {
-#pragma unroll
-            for (int s = 0; s < ST; ++s) {
-                int off[D];
                 const int t = tile_index<D>(l, s, off);
+                const float wt = stencil_weight<D>(w, off);
+                mine[t] += wt * m;
 #pragma unroll
-                for (int k = 0; k <= D; ++k) atomicAdd(&tile[k * TN + t], acc[s][k]);
+                for (int a = 0; a < D; ++a) mine[(1 + a) * TN + t] += wt * mv[a];
+                MPM_STEP_FENCE();
             }
         }
         __syncthreads();
-        store_tile<D, D + 1>(p, b, tile);
+        store_wave_tiles<D, C>(p, b, tile);
         __syncthreads();
     }
 }
@@ -624,24 +636,22 @@ __global__ __launch_bounds__(BWG) void k_init_volume(KP p) {
     __shared__ float tile[1][TN];
     __shared__ int nbr[NBR];
     const int nblocks = p.ctr->nblocks;
-    const int c = threadIdx.x;
-    int l[D];
-    local_coords<D>(c, l);
     float vol = 1.0f;
     for (int a = 0; a < D; ++a) vol *= p.dx;
     const float inv_vol = 1.0f / vol;
     for (int b = xcd_wg(); b < nblocks; b += gridDim.x) {
-        if (c < NBR) nbr[c] = p.block_nbr[b * NBR + c];
+        if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
         __syncthreads();
         const float* src[1] = {p.g_m};
         gather_tile<D, 1>(p, nbr, tile, src);
         __syncthreads();
-        const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
-        for (int i = i0; i < i1; ++i) {
+        const int i1 = p.block_pstart[b + 1];
+        for (int i = p.block_pstart[b] + (int)threadIdx.x; i < i1; i += BWG) {
             float w[D][3], dw[D][3], X[D];
-            if (!cell_particle<D>(p, i, bb, l, w, dw, X)) continue;
+            int l[D];
+            if (!particle_weights<D>(p, i, bb, w, dw, l, X)) continue;
             float rho = 0.0f;
 #pragma unroll
             for (int s = 0; s < Dm::ST; ++s) {
@@ -664,23 +674,21 @@ template <int D, bool CONST, bool FORCE>
 __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
     using Dm = Dims<D>;
     constexpr int TN = Dm::TN, DD = D * D, ST = Dm::ST;
-    __shared__ float tile[D * TN];
+    __shared__ float tile[NW * D * TN];
     const int nblocks = p.ctr->nblocks;
-    const int c = threadIdx.x;
+    const int c = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float* mine = tile + wv * D * TN;
     int l[D];
     local_coords<D>(c, l);
     for (int b = xcd_wg(); b < nblocks; b += gridDim.x) {
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
-        if (FORCE)
-            for (int t = c; t < D * TN; t += BWG) tile[t] = 0.0f;
-        float acc[ST][D];
-#pragma unroll
-        for (int s = 0; s < ST; ++s)
-#pragma unroll
-            for (int k = 0; k < D; ++k) acc[s][k] = 0.0f;
+        if (FORCE) {
+            for (int t = threadIdx.x; t < NW * D * TN; t += BWG) tile[t] = 0.0f;
+            __syncthreads();
+        }
         const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
-        for (int i = i0; i < i1; ++i) {
+        for (int i = i0 + wv; i < i1; i += NW) {
             float F[DD], R[DD];
 #pragma unroll
             for (int k = 0; k < DD; ++k) F[k] = p.pcur[(PF<D>::F + k) * p.pcap + i];
@@ -722,7 +730,7 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
 #pragma unroll
                 for (int s = 0; s < ST; ++s) {
                     int off[D];
-                    tile_index<D>(l, s, off);
+                    const int t = tile_index<D>(l, s, off);
                     float g[D];
                     stencil_grad<D>(w, dw, off, g);
                     const float wt = stencil_weight<D>(w, off);
@@ -731,24 +739,15 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
                         float inner = 0.0f;
 #pragma unroll
                         for (int k = 0; k < D; ++k) inner += A[a * D + k] * g[k];
-                        acc[s][a] += gm[a] * wt - inner;
+                        mine[a * TN + t] += gm[a] * wt - inner;
                     }
+                    MPM_STEP_FENCE();
                 }
             }
         }
         if (FORCE) {
             __syncthreads();
-            if (i1 > i0) {
-#pragma unroll
-                for (int s = 0; s < ST; ++s) {
-                    int off[D];
-                    const int t = tile_index<D>(l, s, off);
-#pragma unroll
-                    for (int k = 0; k < D; ++k) atomicAdd(&tile[k * TN + t], acc[s][k]);
-                }
-            }
-            __syncthreads();
-            store_tile<D, D>(p, b, tile);
+            store_wave_tiles<D, D>(p, b, tile);
             __syncthreads();
         }
     }
@@ -843,32 +842,28 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
     using Dm = Dims<D>;
     constexpr int TN = Dm::TN, NBR = Dm::NBR, DD = D * D, ST = Dm::ST;
     __shared__ float ut[D][TN];
-    __shared__ float qt[D * TN];
+    __shared__ float qt[NW * D * TN];
     __shared__ int nbr[NBR];
     if (p.cg->done) return;
     const int nblocks = p.ctr->nblocks;
     const float* u = RESTART ? p.g_vs : p.g_s;
-    const int c = threadIdx.x;
+    const int c = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float* mine = qt + wv * D * TN;
     int l[D];
     local_coords<D>(c, l);
     for (int b = xcd_wg(); b < nblocks; b += gridDim.x) {
-        if (c < NBR) nbr[c] = p.block_nbr[b * NBR + c];
+        if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
-        for (int t = c; t < D * TN; t += BWG) qt[t] = 0.0f;
+        for (int t = threadIdx.x; t < NW * D * TN; t += BWG) qt[t] = 0.0f;
         __syncthreads();
         const float* src[D];
 #pragma unroll
         for (int a = 0; a < D; ++a) src[a] = u + a * p.gcap;
         gather_tile<D, D>(p, nbr, ut, src);
         __syncthreads();
-        float acc[ST][D];
-#pragma unroll
-        for (int s = 0; s < ST; ++s)
-#pragma unroll
-            for (int k = 0; k < D; ++k) acc[s][k] = 0.0f;
         const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
-        for (int i = i0; i < i1; ++i) {
+        for (int i = i0 + wv; i < i1; i += NW) {
             float w[D][3], dw[D][3], X[D];
             if (!cell_particle<D>(p, i, bb, l, w, dw, X)) continue;
             float T[DD];
@@ -920,7 +915,7 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
 #pragma unroll
             for (int s = 0; s < ST; ++s) {
                 int off[D];
-                tile_index<D>(l, s, off);
+                const int t = tile_index<D>(l, s, off);
                 float g[D];
                 stencil_grad<D>(w, dw, off, g);
 #pragma unroll
@@ -928,21 +923,13 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
                     float v = 0.0f;
 #pragma unroll
                     for (int r = 0; r < D; ++r) v += A[r * D + cc] * g[r];
-                    acc[s][cc] += v;
+                    mine[cc * TN + t] += v;
                 }
-            }
-        }
-        if (i1 > i0) {
-#pragma unroll
-            for (int s = 0; s < ST; ++s) {
-                int off[D];
-                const int t = tile_index<D>(l, s, off);
-#pragma unroll
-                for (int k = 0; k < D; ++k) atomicAdd(&qt[k * TN + t], acc[s][k]);
+                MPM_STEP_FENCE();
             }
         }
         __syncthreads();
-        store_tile<D, D>(p, b, qt);
+        store_wave_tiles<D, D>(p, b, qt);
         __syncthreads();
     }
 }
@@ -1154,11 +1141,8 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
     const float lo = 0.0f + 0.5f * p.dx, hi = 1.0f - 0.5f * p.dx;
     unsigned best = 0u;
     int removed = 0;
-    const int c = threadIdx.x;
-    int l[D];
-    local_coords<D>(c, l);
     for (int b = xcd_wg(); b < nblocks; b += gridDim.x) {
-        if (c < NBR) nbr[c] = p.block_nbr[b * NBR + c];
+        if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
         __syncthreads();
@@ -1170,10 +1154,11 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
         }
         gather_tile<D, 2 * D>(p, nbr, vt, src);
         __syncthreads();
-        const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
-        for (int i = i0; i < i1; ++i) {
+        const int i1 = p.block_pstart[b + 1];
+        for (int i = p.block_pstart[b] + (int)threadIdx.x; i < i1; i += BWG) {
             float w[D][3], dw[D][3], X[D];
-            if (!cell_particle<D>(p, i, bb, l, w, dw, X)) continue;
+            int l[D];
+            if (!particle_weights<D>(p, i, bb, w, dw, l, X)) continue;
             float vpic[D], vflip[D], G[DD];
 #pragma unroll
             for (int a = 0; a < D; ++a) {
