@@ -40,6 +40,7 @@ _SIG = {
     "oracle_get_flags": (None, [_H, ctypes.POINTER(ctypes.c_uint32)]),
     "oracle_apply_operator": (None, [_H, ctypes.c_double, _DP, _DP]),
     "oracle_inner_product": (ctypes.c_double, [_H, _DP, _DP]),
+    "oracle_set_grid": (None, [_H, ctypes.c_int32, _DP]),
     "oracle_cg_stats": (None, [_H, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32),
                                ctypes.POINTER(ctypes.c_int32), _DP]),
     "oracle_bbox_cells": (None, [_H, ctypes.POINTER(ctypes.c_int32), ctypes.POINTER(ctypes.c_int32)]),
@@ -146,6 +147,10 @@ class Oracle:
         out = np.zeros((self.counts + 2) ** self.dim, np.uint32)
         lib().oracle_get_flags(self._h, out.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)))
         return out.reshape((self.counts + 2,) * self.dim)
+
+    def set_grid(self, channel: int, values: np.ndarray) -> None:
+        v = np.ascontiguousarray(values, dtype=np.float64)
+        lib().oracle_set_grid(self._h, channel, _dp(v))
 
     def apply_operator(self, dt: float, x: np.ndarray) -> np.ndarray:
         x = np.ascontiguousarray(x, dtype=np.float64)

@@ -511,6 +511,7 @@ struct OracleBase {
     virtual void get_grid(int channel, double* out) const = 0;
     virtual void get_flags(uint32_t* out) const = 0;
     virtual void apply_operator(double dt, const double* x, double* q) = 0;
+    virtual void set_grid(int channel, const double* in) = 0;
     virtual double inner_product(const double* a, const double* b) const = 0;
     virtual void cg_stats(int* it, int* mult, int* conv, double* res) const = 0;
     virtual void bbox_cells(int* lo, int* hi) const = 0;
@@ -1226,6 +1227,19 @@ struct Oracle : OracleBase {
             for (int a = 0; a < d; ++a) grid[lin].ch[ch + a] = ok ? (T)src[dense_index(c) * d + a] : T(0);
         }
     }
+    void set_grid(int channel, const double* in) override {
+        int width;
+        const int base = channel_base(channel, &width);
+        for (size_t lin = 0; lin < grid.size(); ++lin) {
+            int c[3] = {0, 0, 0};
+            node_coord((int64_t)lin, c);
+            bool ok = true;
+            for (int a = 0; a < d; ++a) ok = ok && c[a] >= -1 && c[a] <= counts;
+            if (!ok) continue;
+            const int64_t o = dense_index(c);
+            for (int w = 0; w < width; ++w) grid[lin].ch[base + w] = (T)in[o * width + w];
+        }
+    }
     void apply_operator(double dt, const double* x, double* q) override {
         load_dense(CH_Z, x);
         Multiply(CH_Z, CH_Q, (T)dt);
@@ -1345,6 +1359,7 @@ void oracle_get_grid(oracle_handle h, int32_t channel, double* out) { h->impl->g
 void oracle_get_flags(oracle_handle h, uint32_t* out) { h->impl->get_flags(out); }
 void oracle_apply_operator(oracle_handle h, double dt, const double* x, double* q) { h->impl->apply_operator(dt, x, q); }
 double oracle_inner_product(oracle_handle h, const double* a, const double* b) { return h->impl->inner_product(a, b); }
+void oracle_set_grid(oracle_handle h, int32_t channel, const double* in) { h->impl->set_grid(channel, in); }
 void oracle_cg_stats(oracle_handle h, int32_t* iterations, int32_t* multiplies, int32_t* converged, double* residual) {
     int it = 0, m = 0, c = 0;
     h->impl->cg_stats(&it, &m, &c, residual);

@@ -68,6 +68,9 @@ void oracle_get_flags(oracle_handle h, uint32_t* out);
  * to a dense grid vector x (same layout as oracle_get_grid of a vector
  * channel) using the current grid and particle state; writes q. */
 void oracle_apply_operator(oracle_handle h, double dt, const double* x, double* q);
+/* Overwrites a grid channel (same dense layout as oracle_get_grid) on the
+ * nodes of the current grid; for tests that prescribe a grid field. */
+void oracle_set_grid(oracle_handle h, int32_t channel, const double* in);
 /* Interior-masked M inner product of two dense vectors (Inner_Product_Helper). */
 double oracle_inner_product(oracle_handle h, const double* a, const double* b);
 void oracle_cg_stats(oracle_handle h, int32_t* iterations, int32_t* multiplies, int32_t* converged,

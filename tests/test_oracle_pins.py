@@ -1,0 +1,369 @@
+"""Pins the CPU oracle (CPU only).
+
+The reference cannot be built here (its Nova dependency is absent) and holds
+no golden vectors, so every function of the oracle is pinned by what the
+mathematics fixes: closed forms written independently of the oracle's own
+expressions, library routines (numpy's SVD and linear solver), invariants and
+brute force on tiny inputs.  Each check is chosen so that a plausible slip
+(a dropped term, a wrong sign or index, a transposed operand) fails it.
+"""
+import numpy as np
+import pytest
+
+import non_ficks_diffusion_amd as M
+import oracle as O
+
+RNG = np.random.default_rng(1234)
+
+
+def rotation(dim):
+    """Uniform random rotation (det +1)."""
+    Q, _ = np.linalg.qr(RNG.standard_normal((dim, dim)))
+    if np.linalg.det(Q) < 0:
+        Q[:, 0] = -Q[:, 0]
+    return Q
+
+
+def bspline(r):
+    """Textbook quadratic B-spline N(r) (Steffen et al. 2008), by cases."""
+    r = abs(r)
+    if r < 0.5:
+        return 0.75 - r * r
+    if r < 1.5:
+        return 0.5 * (1.5 - r) ** 2
+    return 0.0
+
+
+def bspline_d(r):
+    s = np.sign(r)
+    r = abs(r)
+    if r < 0.5:
+        return -2.0 * r * s
+    if r < 1.5:
+        return -(1.5 - r) * s
+    return 0.0
+
+
+# --------------------------------------------------------------------------- weights
+@pytest.mark.parametrize("dim", [2, 3])
+def test_weights_closed_form_and_moments(dim):
+    n = 64
+    dx = 1.0 / n
+    for _ in range(200):
+        X = RNG.uniform(2 * dx, 1 - 2 * dx, dim)
+        cell, w, dw = O.weights(dim, n, X)
+        assert np.array_equal(cell, np.floor(X * n).astype(int))
+        for a in range(dim):
+            for k in range(3):
+                node = (cell[a] - 1 + k + 0.5) * dx
+                r = (X[a] - node) / dx
+                assert abs(w[k, a] - bspline(r)) < 1e-12
+                assert abs(dw[k, a] - bspline_d(r) / dx) < 1e-9
+        # tensor-product moments over the 3^d stencil
+        idx = np.array(np.meshgrid(*[range(3)] * dim, indexing="ij")).reshape(dim, -1).T
+        W = np.array([np.prod([w[i[a], a] for a in range(dim)]) for i in idx])
+        G = np.array([[np.prod([(dw if a == b else w)[i[a], a] for a in range(dim)]) for b in range(dim)] for i in idx])
+        xi = np.array([[(cell[a] - 1 + i[a] + 0.5) * dx for a in range(dim)] for i in idx])
+        assert abs(W.sum() - 1) < 1e-12                                   # partition of unity
+        assert np.allclose((W[:, None] * xi).sum(0), X, atol=1e-14)      # linear reproduction
+        assert np.allclose(G.sum(0), 0, atol=1e-9)                        # sum grad w = 0
+        assert np.allclose(np.einsum("ka,kb->ab", xi, G), np.eye(dim), atol=1e-9)  # sum x (x) grad w = I
+
+
+# --------------------------------------------------------------------------- SVD
+def check_svd(F):
+    U, S, V = O.svd(F)
+    d = F.shape[0]
+    assert np.allclose(U @ np.diag(S) @ V.T, F, atol=1e-12)
+    assert np.allclose(U.T @ U, np.eye(d), atol=1e-12)
+    assert np.allclose(V.T @ V, np.eye(d), atol=1e-12)
+    assert abs(np.linalg.det(U) - 1) < 1e-12 and abs(np.linalg.det(V) - 1) < 1e-12
+    assert all(S[i] >= S[i + 1] - 1e-12 for i in range(d - 2))
+    assert S[d - 2] >= abs(S[d - 1]) - 1e-12
+    # the sign of det F is carried by the smallest value; magnitudes match LAPACK
+    assert np.sign(np.prod(S)) == np.sign(np.linalg.det(F)) or abs(np.linalg.det(F)) < 1e-14
+    assert np.allclose(np.sort(np.abs(S)), np.sort(np.linalg.svd(F, compute_uv=False)), atol=1e-12)
+    return U, S, V
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_svd_random_and_special(dim):
+    for _ in range(300):
+        check_svd(np.eye(dim) + 0.3 * RNG.standard_normal((dim, dim)))
+    check_svd(np.diag(np.arange(1, dim + 1, dtype=float)[::-1]))       # diagonal
+    Q = rotation(dim)
+    U, S, V = check_svd(Q)                                              # pure rotation
+    assert np.allclose(S, 1, atol=1e-12)
+    inv = np.eye(dim)
+    inv[0, 0] = -1
+    _, S, _ = check_svd(inv @ (np.eye(dim) + 0.1 * RNG.standard_normal((dim, dim))))  # inverted
+    assert S[-1] < 0
+    check_svd(np.eye(dim) * 0.97)                                        # repeated
+    check_svd(np.outer(RNG.standard_normal(dim), RNG.standard_normal(dim)))  # rank one
+
+
+# --------------------------------------------------------------------------- constitutive
+def snow(plastic=1):
+    d, _ = M.scene_preset("cfg2_avalanche", 32, 0)
+    m = d.material
+    m.plastic = plastic
+    return m
+
+
+def test_plasticity_diagonal_closed_form():
+    mat = snow()
+    for F in (np.diag([1.02, 1.0, 0.96]), np.diag([0.9, 1.004, 1.1])):
+        Q, P2 = rotation(3), rotation(3)
+        Fr = Q @ F @ P2.T
+        Fo, Jp, R, P, psi = O.constitutive(mat, Fr, 1.0)
+        s = np.clip(np.diag(F), mat.compression_yield, mat.stretching_yield)
+        assert np.allclose(Fo, Q @ np.diag(s) @ P2.T, atol=1e-12)
+        assert abs(Jp - np.prod(np.diag(F)) / np.prod(s)) < 1e-12
+        assert np.allclose(R, Q @ P2.T, atol=1e-12)
+        # hardening: lambda = lambda0 exp(xi (1 - Jp)); P = 2mu(F-R) + lambda(J-1)cof(F)
+        e = np.exp(mat.hardening_factor * (1 - Jp))
+        J = np.prod(s)
+        cof = J * np.linalg.inv(Fo).T
+        assert np.allclose(P, 2 * mat.mu0 * e * (Fo - R) + mat.lambda0 * e * (J - 1) * cof, rtol=1e-10, atol=1e-6)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_stress_is_energy_gradient(dim):
+    """P equals dPsi/dF (central differences) and vanishes on rotations."""
+    mat = snow(plastic=0)
+    for _ in range(20):
+        F = np.eye(dim) + 0.05 * RNG.standard_normal((dim, dim))
+        _, _, _, P, _ = O.constitutive(mat, F, 1.0)
+        h = 1e-6
+        G = np.zeros_like(F)
+        for i in range(dim):
+            for j in range(dim):
+                E = np.zeros_like(F)
+                E[i, j] = h
+                G[i, j] = (O.constitutive(mat, F + E, 1.0)[4] - O.constitutive(mat, F - E, 1.0)[4]) / (2 * h)
+        assert np.allclose(P, G, rtol=1e-6, atol=1e-3)
+    assert np.allclose(O.constitutive(mat, rotation(dim), 1.0)[3], 0, atol=1e-8)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_dP_is_directional_derivative_and_symmetric(dim):
+    mat = snow(plastic=0)
+    lam, mu = mat.lambda0, mat.mu0
+    for _ in range(20):
+        F = np.eye(dim) + 0.05 * RNG.standard_normal((dim, dim))
+        _, _, R, _, _ = O.constitutive(mat, F, 1.0)
+        A = RNG.standard_normal((dim, dim))
+        B = RNG.standard_normal((dim, dim))
+        h = 1e-6
+        fd = (O.constitutive(mat, F + h * A, 1.0)[3] - O.constitutive(mat, F - h * A, 1.0)[3]) / (2 * h)
+        dPA = O.dP(F, R, lam, mu, A)
+        assert np.allclose(dPA, fd, rtol=1e-5, atol=1e-2 * np.abs(fd).max() * 1e-3 + 1e-4)
+        dPB = O.dP(F, R, lam, mu, B)
+        assert abs(np.sum(A * dPB) - np.sum(B * dPA)) < 1e-8 * (np.abs(dPA).sum() * np.abs(B).sum())
+
+
+# --------------------------------------------------------------------------- scenes used below
+def tiny_scene(dim=3, n=6, counts=16, planes=0, seed=0, v_scale=0.0, F_scale=0.0, E_scale=1.0):
+    d, _ = M.scene_preset("cfg2_avalanche" if dim == 3 else "cfg0_disk2d", counts, 0)
+    d.nplanes = planes
+    d.material.mu0 *= E_scale
+    d.material.lambda0 *= E_scale
+    rng = np.random.default_rng(seed)
+    x = (0.5 + 0.05 * rng.standard_normal((n, dim))).astype(np.float32)
+    v = (v_scale * rng.standard_normal((n, dim))).astype(np.float32)
+    m = np.full(n, 1e-4, np.float32)
+    F = (np.tile(np.eye(dim), (n, 1, 1)) + F_scale * rng.standard_normal((n, dim, dim))).astype(np.float32)
+    return d, x, v, m, F
+
+
+def test_cg_operator_symmetric_and_matches_dense_solve():
+    """A = I + dt^2 M^-1 K is M-symmetric, and CG run to tight tolerance on a
+    tiny scene reproduces the dense solve of A x = b (numpy)."""
+    d, x, v, m, F = tiny_scene(n=12, v_scale=0.3)
+    d.cg_tolerance = 1e-13
+    d.cg_max_iterations = 2000
+    o = O.Oracle(d, 8, 1, O.DOT_FP64 | O.SVD_FP64)
+    o.set_particles(x, v, m, F=F)
+    o.initialize()
+    dt = 2e-3
+    for st in (M.STAGE_BIN, M.STAGE_P2G, M.STAGE_CONSTITUTIVE, M.STAGE_EXPLICIT_FORCE, M.STAGE_COLLISION):
+        o.run_stage(st, dt)
+    mass = o.grid(M.CH_MASS)
+    interior = mass > 0
+    idx = np.argwhere(interior)
+    shape = mass.shape + (3,)
+    N = len(idx) * 3
+    A = np.zeros((N, N))
+    for k in range(N):
+        e = np.zeros(shape)
+        e[tuple(idx[k // 3]) + (k % 3,)] = 1.0
+        q = o.apply_operator(dt, e.ravel()).reshape(shape)
+        A[:, k] = q[interior].ravel()
+    Mv = np.repeat(mass[interior], 3)
+    MA = Mv[:, None] * A
+    assert np.abs(MA - MA.T).max() < 1e-12 * np.abs(MA).max()
+    b = o.grid(M.CH_VELOCITY_STAR)[interior].ravel()
+    xs = np.linalg.solve(A, b)
+    o.run_stage(M.STAGE_CG, dt)
+    st = o.cg_stats()
+    assert st["cg_converged"] == 1
+    assert st["cg_multiplies"] == st["cg_iterations"] + 1
+    xc = o.grid(M.CH_VELOCITY_STAR)[interior].ravel()
+    assert np.allclose(xc, xs, atol=1e-10)
+    # operator is identity for rigid translation (K v = 0 for uniform v)
+    u = np.zeros(shape)
+    u[interior] = [0.3, -0.2, 0.1]
+    q = o.apply_operator(dt, u.ravel()).reshape(shape)
+    assert np.allclose(q[interior], u[interior], atol=1e-9)
+
+
+def test_collision_closed_form():
+    d, x, v, m, F = tiny_scene(n=20, v_scale=1.0)
+    d.nplanes = 2
+    for i, (nrm, srf, mu) in enumerate((((0.0, 1.0, 0.0), (0, 0.5, 0), 0.0), ((-0.6, 0.8, 0.0), (0.5, 0.5, 0.5), 0.3))):
+        d.planes[i].normal[:] = nrm
+        d.planes[i].surface[:] = srf
+        d.planes[i].mu = mu
+    o = O.Oracle(d, 8, 1, O.DOT_FP64 | O.SVD_FP64)
+    o.set_particles(x, v, m, F=F)
+    o.initialize()
+    for st in (M.STAGE_BIN, M.STAGE_P2G, M.STAGE_CONSTITUTIVE, M.STAGE_EXPLICIT_FORCE):
+        o.run_stage(st, 1e-3)
+    before = o.grid(M.CH_VELOCITY_STAR)
+    flags = o.flags() & 1
+    o.run_stage(M.STAGE_COLLISION, 1e-3)
+    after = o.grid(M.CH_VELOCITY_STAR)
+    n = d.counts
+    checked = 0
+    for ijk in np.argwhere(flags):
+        xc = (ijk - 1 + 0.5) / n
+        vel = before[tuple(ijk)].copy()
+        for p in range(d.nplanes):
+            nrm = np.array(d.planes[p].normal[:3])
+            srf = np.array(d.planes[p].surface[:3])
+            if np.dot(xc - srf, nrm) < 0:
+                vn = np.dot(vel, nrm)
+                if vn < 0:
+                    vt = vel - vn * nrm
+                    t = np.linalg.norm(vt)
+                    mu = d.planes[p].mu
+                    vel = vt + vt / t * vn * mu if -vn * mu < t else np.zeros(3)
+                    checked += 1
+        assert np.allclose(after[tuple(ijk)], vel, atol=1e-12)
+    assert checked > 0
+
+
+def test_g2p_reproduces_linear_field():
+    """With grid v = v* = a + B x_i, G2P gives V_pic = a + B x_p, grad v = B:
+    Fe_new = Fe + dt B Fe, X_new = X + dt V_pic, V_new = V (when V = V_pic)."""
+    d, x, v, m, F = tiny_scene(n=15)
+    a = np.array([0.3, -0.1, 0.2])
+    B = 0.5 * RNG.standard_normal((3, 3))
+    v = (a + x.astype(np.float64) @ B.T).astype(np.float32)
+    o = O.Oracle(d, 8, 1, O.DOT_FP64 | O.SVD_FP64)
+    o.set_particles(x, v, m, F=F, volume=np.full(len(m), 1e-6, np.float32))
+    o.run_stage(M.STAGE_BIN)
+    o.run_stage(M.STAGE_P2G)
+    n = d.counts
+    g = np.indices((n + 2,) * 3).reshape(3, -1).T
+    xc = (g - 1 + 0.5) / n
+    field = (a + xc @ B.T).reshape((n + 2,) * 3 + (3,))
+    o.set_grid(M.CH_VELOCITY, field)
+    o.set_grid(M.CH_VELOCITY_STAR, field)
+    dt = 1e-3
+    p0 = o.particles()
+    o.run_stage(M.STAGE_G2P, dt)
+    p1 = o.particles()
+    vpic = a + p0["x"] @ B.T
+    assert np.allclose(p1["v"], 0.9 * p0["v"] + 0.1 * vpic, atol=1e-12)
+    assert np.allclose(p1["x"], p0["x"] + dt * vpic, atol=1e-12)
+    assert np.allclose(p1["F"], p0["F"] + dt * np.einsum("ij,pjk->pik", B, p0["F"]), atol=1e-12)
+
+
+# --------------------------------------------------------------------------- whole substep invariants
+def test_free_fall_without_elasticity():
+    """E = 0: every particle gains exactly g dt (no planes), x moves by the PIC velocity."""
+    d, x, v, m, F = tiny_scene(n=40, v_scale=0.0, E_scale=0.0)
+    o = O.Oracle(d, 8, 1, O.DOT_FP64 | O.SVD_FP64)
+    o.set_particles(x, v, m)
+    o.initialize()
+    dt = 1e-3
+    o.advance_step(dt)
+    p = o.particles()
+    g = np.array(d.gravity[:3])
+    assert np.allclose(p["v"], g * dt, atol=1e-12)
+    assert np.allclose(p["x"], x + g * dt * dt, atol=1e-12)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_momentum_and_mass_conservation(dim):
+    """No planes, no gravity: P2G conserves mass and momentum exactly (in
+    exact arithmetic); a full substep conserves total particle momentum."""
+    d, x, v, m, F = tiny_scene(dim=dim, n=60, v_scale=0.5, F_scale=0.01)
+    d.gravity[:] = (0, 0, 0)
+    d.cg_tolerance = 1e-12
+    d.cg_max_iterations = 500
+    o = O.Oracle(d, 8, 1, O.DOT_FP64 | O.SVD_FP64)
+    o.set_particles(x, v, m, F=F)
+    o.initialize()
+    o.run_stage(M.STAGE_BIN)
+    o.run_stage(M.STAGE_P2G)
+    gm = o.grid(M.CH_MASS)
+    gv = o.grid(M.CH_VELOCITY)
+    assert abs(gm.sum() - m.astype(np.float64).sum()) < 1e-13
+    p_mom = (m[:, None] * v.astype(np.float64)).sum(0)
+    assert np.allclose((gm[..., None] * gv).reshape(-1, dim).sum(0), p_mom, atol=1e-14)
+    o2 = O.Oracle(d, 8, 1, O.DOT_FP64 | O.SVD_FP64)
+    o2.set_particles(x, v, m, F=F)
+    o2.initialize()
+    o2.advance_step(1e-3)
+    p = o2.particles()
+    assert np.allclose((p["mass"][:, None] * p["v"]).sum(0), p_mom, atol=1e-10)
+
+
+def test_float_oracle_tracks_double_oracle():
+    """The FP32 build (the reference's precision) stays within FP32 rounding
+    of the FP64 build over 10 substeps of a real scene."""
+    d, x, v, m = M.scene_generate("cfg0_disk2d", 64, 0)
+    res = []
+    for prec in (4, 8):
+        o = O.Oracle(d, prec, 2, O.DOT_FP64 | O.SVD_FP64)
+        o.set_particles(x, v, m)
+        o.initialize()
+        for _ in range(10):
+            o.advance_step(5e-4)
+        res.append(o.particles())
+    for k in ("x", "v", "F"):
+        err = np.linalg.norm(res[0][k] - res[1][k]) / np.linalg.norm(res[1][k])
+        assert err < 1e-5, (k, err)
+
+
+def test_slab_binning_covers_every_stencil():
+    """Group_Particles: for any thread count, the cropped stencils of a
+    particle over the slabs it is binned into tile its stencil exactly, so P2G
+    does not depend on the slab count beyond rounding."""
+    d, x, v, m = M.scene_generate("cfg1_two_balls", 32, 0)
+    grids = []
+    for threads in (1, 3, 8):
+        o = O.Oracle(d, 8, threads, O.DOT_FP64 | O.SVD_FP64)
+        o.set_particles(x, v, m)
+        o.initialize()
+        grids.append(o.grid(M.CH_MASS))
+    assert np.allclose(grids[0], grids[1], rtol=1e-13, atol=1e-20)
+    assert np.allclose(grids[0], grids[2], rtol=1e-13, atol=1e-20)
+
+
+def test_driver_dt_sequence():
+    """Advance_To_Target_Time (MPM_Driver.cpp:55-80): substeps of
+    clamp(cfl dx / max(vmax, 1e-2), min_dt, max_dt) that end exactly at the
+    frame time, the last two shortened by the halving rule."""
+    d, x, v, m = M.scene_generate("cfg0_disk2d", 64, 0)
+    o = O.Oracle(d, 4, 1, O.DOT_FP64 | O.SVD_FP64)
+    o.set_particles(x, v, m)
+    o.initialize()
+    vmax = o.max_particle_velocity()
+    assert abs(vmax - np.linalg.norm(v, axis=1).max()) < 1e-6
+    t, n = o.advance_to_target_time(0.0, 1.0 / 24)
+    assert abs(t - 1.0 / 24) < 1e-6
+    dt0 = max(d.min_dt, min(d.max_dt, d.cfl / d.counts / max(vmax, 1e-2)))
+    assert n >= int(np.floor((1.0 / 24) / dt0 * 0.999)) - 1
