@@ -26,12 +26,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-WORKLOAD = "cfg2_avalanche"
+WORKLOAD = "avalanche"
 COUNTS = 256
 
 
 def canonical_bytes(N, Nn, I, M):
-    """Algorithmic HBM bytes (FP32 = 4 B; see DESIGN.md, 'Measurement')."""
+    """Algorithmic HBM bytes per substep: every particle and grid field read or written once (FP32 = 4 B)."""
     return {
         "substep": N * (320 + 60 * M) + Nn * (200 + 64 * M + 164 * I),
         "cg_force": N * 60 + Nn * 24,      # per operator application: particles + gather s + scatter q

@@ -149,11 +149,12 @@ int mpm_initialize(mpm_handle h);
  * stats). */
 int mpm_substep(mpm_handle h, float dt, mpm_step_stats* stats);
 
-/* Substep whose dt is computed on the device from the current max speed
- * (MPM_Driver.cpp:55-80): dt = clamp(cfl*dx/max(vmax,1e-2), min_dt, max_dt),
- * then shortened so the substep ends at or before target_time.  time is
- * device-resident; *time_inout is read at the first call of a frame and
- * written back when stats are requested. */
+/* MPM_Driver::Advance_To_Target_Time (MPM_Driver.cpp:55-80): substeps from
+ * `time` until `target_time`, each with dt = clamp(cfl*dx/max(vmax,1e-2),
+ * min_dt, max_dt) from the device-reduced max speed, shortened or halved
+ * near the target exactly as the reference does (float arithmetic).
+ * Returns the substep count and the final time; fills stats of the last
+ * substep when stats != NULL. */
 int mpm_advance_to_time(mpm_handle h, double time, double target_time,
                         int32_t* substeps_out, double* time_out, mpm_step_stats* stats);
 
@@ -214,8 +215,8 @@ int32_t mpm_kernel_class_count(void);
 
 /* ---- scenes (scene hook Initialize_Particles) ------------------------- */
 /* Fills desc with the preset of a named scene and returns the particle
- * count it generates.  Known names: "cfg0_disk2d", "cfg1_two_balls",
- * "cfg2_avalanche", "ref3d_test18", "ref2d_test18", "blob", "slab". */
+ * count it generates.  Known names: "disk2d", "two_balls",
+ * "avalanche", "ref3d_test18", "ref2d_test18", "blob", "slab". */
 int mpm_scene_preset(const char* name, int32_t counts, uint64_t seed, mpm_scene_desc* desc,
                      int64_t* n_particles);
 /* Generates the particles of a named scene (same arguments as the preset);

@@ -145,18 +145,18 @@ Shape sphere(double x, double y, double z, double r, float vx, float vy, float v
 
 bool build_spec(const std::string& name, int counts, uint64_t seed, Spec& sp) {
     mpm_scene_desc& d = sp.desc;
-    if (name == "cfg0_disk2d") {
+    if (name == "disk2d") {
         // 2D snow disk r=0.1 at (0.5,0.7), V=(0,-2), ppc 4, ground y=0.05.
         base_desc(d, 2, counts > 0 ? counts : 64);
         add_plane(d, 0, 1, 0, 0, 0.05, 0);
         sp.shapes.push_back(sphere(0.5, 0.7, 0, 0.1, 0.f, -2.f, 0.f));
-    } else if (name == "cfg1_two_balls") {
+    } else if (name == "two_balls") {
         // Two snowballs r=0.12 at x=0.3 and x=0.7 moving head-on at 3 m/s each, ground y=0.05.
         base_desc(d, 3, counts > 0 ? counts : 128);
         add_plane(d, 0, 1, 0, 0, 0.05, 0);
         sp.shapes.push_back(sphere(0.3, 0.5, 0.5, 0.12, 3.f, 0.f, 0.f));
         sp.shapes.push_back(sphere(0.7, 0.5, 0.5, 0.12, -3.f, 0.f, 0.f));
-    } else if (name == "cfg2_avalanche") {
+    } else if (name == "avalanche") {
         // Snow slab 0.6 x 0.1 x 0.125 resting on a 30-degree incline; ground y=0.05 and
         // a wall at x=0.05 catch the slide.
         base_desc(d, 3, counts > 0 ? counts : 256);

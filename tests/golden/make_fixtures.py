@@ -19,9 +19,9 @@ import oracle as O  # noqa: E402
 
 CASES = [
     # name, scene, counts, dt, substeps
-    ("cfg0_disk2d_64_5steps", "cfg0_disk2d", 64, 5e-4, 5),
-    ("cfg1_two_balls_16_3steps", "cfg1_two_balls", 16, 1e-3, 3),
-    ("cfg2_avalanche_32_3steps", "cfg2_avalanche", 32, 1e-3, 3),
+    ("disk2d_64_5steps", "disk2d", 64, 5e-4, 5),
+    ("two_balls_16_3steps", "two_balls", 16, 1e-3, 3),
+    ("avalanche_32_3steps", "avalanche", 32, 1e-3, 3),
 ]
 
 

@@ -51,8 +51,8 @@ def test_struct_layouts_match_header(tmp_path):
 
 
 def test_scene_generation_deterministic_and_sized():
-    d, x, v, m = M.scene_generate("cfg2_avalanche", 64, 0)
-    d2, x2, v2, m2 = M.scene_generate("cfg2_avalanche", 64, 0)
+    d, x, v, m = M.scene_generate("avalanche", 64, 0)
+    d2, x2, v2, m2 = M.scene_generate("avalanche", 64, 0)
     assert np.array_equal(x, x2) and np.array_equal(m, m2)
     assert d.counts == 64 and d.dim == 3
     # particles inside [dx/2, 1-dx/2] and ppc 8 mass
@@ -61,9 +61,9 @@ def test_scene_generation_deterministic_and_sized():
     assert np.allclose(m, 400 * dx ** 3 / 8)
     # slab volume 0.6 x 0.1 x 0.125 at 8 per cell
     assert abs(len(m) - 0.6 * 0.1 * 0.125 * 64 ** 3 * 8) / len(m) < 0.02
-    _, n_full = M.scene_preset("cfg2_avalanche", 256, 0)
+    _, n_full = M.scene_preset("avalanche", 256, 0)
     assert 0.99e6 < n_full < 1.02e6
-    _, n0 = M.scene_preset("cfg0_disk2d", 64, 0)
+    _, n0 = M.scene_preset("disk2d", 64, 0)
     assert 480 < n0 < 550
     _, xa, _, _ = M.scene_generate("blob", 64, 1)
     _, xb, _, _ = M.scene_generate("blob", 64, 2)
@@ -71,7 +71,7 @@ def test_scene_generation_deterministic_and_sized():
 
 
 def test_presets_bound_dt_by_elastic_wave_speed():
-    d, _ = M.scene_preset("cfg2_avalanche", 256, 0)
+    d, _ = M.scene_preset("avalanche", 256, 0)
     cp = np.sqrt((d.material.lambda0 + 2 * d.material.mu0) / 400.0)
     assert abs(d.max_dt - (1.0 / 256) / cp) / d.max_dt < 1e-5
     d, _ = M.scene_preset("ref3d_test18", 32, 0)
@@ -80,7 +80,7 @@ def test_presets_bound_dt_by_elastic_wave_speed():
 
 def test_errors_are_status_codes():
     # invalid descriptors are rejected with a status and a message, no crash
-    d, _ = M.scene_preset("cfg0_disk2d", 64, 0)
+    d, _ = M.scene_preset("disk2d", 64, 0)
     d.dim = 5
     h = ctypes.c_void_p()
     rc = M.lib().mpm_create(ctypes.byref(d), 10, ctypes.byref(h))

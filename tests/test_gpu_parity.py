@@ -14,8 +14,8 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-SCENES_3D = [("cfg1_two_balls", 32), ("cfg2_avalanche", 64), ("ref3d_test18", 32)]
-SCENES_2D = [("cfg0_disk2d", 64), ("ref2d_test18", 64)]
+SCENES_3D = [("two_balls", 32), ("avalanche", 64), ("ref3d_test18", 32)]
+SCENES_2D = [("disk2d", 64), ("ref2d_test18", 64)]
 ALL = SCENES_3D + SCENES_2D
 MATCHED = O.DOT_FP64 | O.SVD_FP64
 
@@ -150,7 +150,7 @@ def run_both_stages(s, o, dt):
     return errs
 
 
-@pytest.mark.parametrize("name,counts", [("cfg1_two_balls", 32), ("cfg0_disk2d", 64), ("cfg2_avalanche", 64)])
+@pytest.mark.parametrize("name,counts", [("two_balls", 32), ("disk2d", 64), ("avalanche", 64)])
 def test_stage_by_stage(name, counts):
     d, x, v, m, s, o = make(name, counts, cg_max=20)
     errs = run_both_stages(s, o, 1e-3)
@@ -159,8 +159,8 @@ def test_stage_by_stage(name, counts):
         assert e < 1e-5, (k, e)
 
 
-@pytest.mark.parametrize("name,counts,dt", [("cfg1_two_balls", 32, 1e-3), ("cfg0_disk2d", 64, 1e-3),
-                                            ("cfg2_avalanche", 64, 1e-3), ("ref3d_test18", 32, 2e-3)])
+@pytest.mark.parametrize("name,counts,dt", [("two_balls", 32, 1e-3), ("disk2d", 64, 1e-3),
+                                            ("avalanche", 64, 1e-3), ("ref3d_test18", 32, 2e-3)])
 def test_one_substep(name, counts, dt):
     d, x, v, m, s, o = make(name, counts)
     s.initialize()
@@ -177,7 +177,7 @@ def test_one_substep(name, counts, dt):
         assert e < 1e-5, (k, e)
 
 
-@pytest.mark.parametrize("name,counts,dt,steps", [("cfg0_disk2d", 64, 2e-3, 100), ("cfg1_two_balls", 32, 1e-3, 100)])
+@pytest.mark.parametrize("name,counts,dt,steps", [("disk2d", 64, 2e-3, 100), ("two_balls", 32, 1e-3, 100)])
 def test_hundred_substeps(name, counts, dt, steps):
     d, x, v, m, s, o = make(name, counts)
     s.initialize()

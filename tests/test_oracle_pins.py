@@ -104,7 +104,7 @@ def test_svd_random_and_special(dim):
 
 # --------------------------------------------------------------------------- constitutive
 def snow(plastic=1):
-    d, _ = M.scene_preset("cfg2_avalanche", 32, 0)
+    d, _ = M.scene_preset("avalanche", 32, 0)
     m = d.material
     m.plastic = plastic
     return m
@@ -164,7 +164,7 @@ def test_dP_is_directional_derivative_and_symmetric(dim):
 
 # --------------------------------------------------------------------------- scenes used below
 def tiny_scene(dim=3, n=6, counts=16, planes=0, seed=0, v_scale=0.0, F_scale=0.0, E_scale=1.0):
-    d, _ = M.scene_preset("cfg2_avalanche" if dim == 3 else "cfg0_disk2d", counts, 0)
+    d, _ = M.scene_preset("avalanche" if dim == 3 else "disk2d", counts, 0)
     d.nplanes = planes
     d.material.mu0 *= E_scale
     d.material.lambda0 *= E_scale
@@ -324,7 +324,7 @@ def test_momentum_and_mass_conservation(dim):
 def test_float_oracle_tracks_double_oracle():
     """The FP32 build (the reference's precision) stays within FP32 rounding
     of the FP64 build over 10 substeps of a real scene."""
-    d, x, v, m = M.scene_generate("cfg0_disk2d", 64, 0)
+    d, x, v, m = M.scene_generate("disk2d", 64, 0)
     res = []
     for prec in (4, 8):
         o = O.Oracle(d, prec, 2, O.DOT_FP64 | O.SVD_FP64)
@@ -342,7 +342,7 @@ def test_slab_binning_covers_every_stencil():
     """Group_Particles: for any thread count, the cropped stencils of a
     particle over the slabs it is binned into tile its stencil exactly, so P2G
     does not depend on the slab count beyond rounding."""
-    d, x, v, m = M.scene_generate("cfg1_two_balls", 32, 0)
+    d, x, v, m = M.scene_generate("two_balls", 32, 0)
     grids = []
     for threads in (1, 3, 8):
         o = O.Oracle(d, 8, threads, O.DOT_FP64 | O.SVD_FP64)
@@ -357,7 +357,7 @@ def test_driver_dt_sequence():
     """Advance_To_Target_Time (MPM_Driver.cpp:55-80): substeps of
     clamp(cfl dx / max(vmax, 1e-2), min_dt, max_dt) that end exactly at the
     frame time, the last two shortened by the halving rule."""
-    d, x, v, m = M.scene_generate("cfg0_disk2d", 64, 0)
+    d, x, v, m = M.scene_generate("disk2d", 64, 0)
     o = O.Oracle(d, 4, 1, O.DOT_FP64 | O.SVD_FP64)
     o.set_particles(x, v, m)
     o.initialize()
