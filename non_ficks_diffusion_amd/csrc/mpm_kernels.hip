@@ -30,12 +30,18 @@ struct PF {
     static constexpr int X = 0, V = D, F = 2 * D, J = 2 * D + D * D, M = J + 1, VOL = J + 2, N = J + 3;
 };
 
-// Work-groups are dispatched round-robin over the 8 XCDs; map them so
-// each XCD walks a contiguous range of blocks (neighbouring blocks share
-// halo nodes in that XCD's L2).  Requires gridDim.x % 8 == 0.
-__device__ inline int xcd_wg() {
-    const int G = (int)gridDim.x, w = (int)blockIdx.x;
-    return (w & 7) * (G >> 3) + (w >> 3);
+// Work-groups are dispatched round-robin over the 8 XCDs (work-group w runs
+// on XCD w % 8).  Block loops give each XCD one contiguous eighth of the
+// active blocks, so every XCD has work whatever the block count, and
+// neighbouring blocks (which share halo nodes) meet in the same L2.
+// Requires gridDim.x % 8 == 0.
+struct XcdRange {
+    int first, end, step;
+};
+__device__ inline XcdRange xcd_blocks(int n) {
+    const int x = (int)(blockIdx.x & 7), k = (int)(blockIdx.x >> 3);
+    const int lo = (int)(((int64_t)n * x) >> 3), hi = (int)(((int64_t)n * (x + 1)) >> 3);
+    return {lo + k, hi, (int)(gridDim.x >> 3)};
 }
 
 template <int D>
@@ -567,7 +573,8 @@ __global__ __launch_bounds__(BWG) void k_p2g(KP p) {
     float* mine = tile + wv * C * TN;
     int l[D];
     local_coords<D>(c, l);
-    for (int b = xcd_wg(); b < nblocks; b += gridDim.x) {
+    const XcdRange xr = xcd_blocks(nblocks);
+    for (int b = xr.first; b < xr.end; b += xr.step) {
         for (int t = threadIdx.x; t < NW * C * TN; t += BWG) tile[t] = 0.0f;
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
@@ -639,7 +646,8 @@ __global__ __launch_bounds__(BWG) void k_init_volume(KP p) {
     float vol = 1.0f;
     for (int a = 0; a < D; ++a) vol *= p.dx;
     const float inv_vol = 1.0f / vol;
-    for (int b = xcd_wg(); b < nblocks; b += gridDim.x) {
+    const XcdRange xr = xcd_blocks(nblocks);
+    for (int b = xr.first; b < xr.end; b += xr.step) {
         if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
@@ -680,7 +688,8 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
     float* mine = tile + wv * D * TN;
     int l[D];
     local_coords<D>(c, l);
-    for (int b = xcd_wg(); b < nblocks; b += gridDim.x) {
+    const XcdRange xr = xcd_blocks(nblocks);
+    for (int b = xr.first; b < xr.end; b += xr.step) {
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
         if (FORCE) {
@@ -851,7 +860,8 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
     float* mine = qt + wv * D * TN;
     int l[D];
     local_coords<D>(c, l);
-    for (int b = xcd_wg(); b < nblocks; b += gridDim.x) {
+    const XcdRange xr = xcd_blocks(nblocks);
+    for (int b = xr.first; b < xr.end; b += xr.step) {
         if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
@@ -1141,7 +1151,8 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
     const float lo = 0.0f + 0.5f * p.dx, hi = 1.0f - 0.5f * p.dx;
     unsigned best = 0u;
     int removed = 0;
-    for (int b = xcd_wg(); b < nblocks; b += gridDim.x) {
+    const XcdRange xr = xcd_blocks(nblocks);
+    for (int b = xr.first; b < xr.end; b += xr.step) {
         if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
