@@ -55,6 +55,7 @@ struct mpm_context {
     int cur = 0;
     int* prank = nullptr;
     float* rcache = nullptr;
+    float* pop = nullptr;
     int *slot_count = nullptr, *slot_touched = nullptr, *slot_active = nullptr;
     int *block_slot = nullptr, *block_pstart = nullptr, *block_nbr = nullptr, *scan_part = nullptr;
     int *cell_count = nullptr, *cell_start = nullptr;
@@ -63,6 +64,8 @@ struct mpm_context {
     mpm::Counters* ctr = nullptr;
     mpm::CGState* cg = nullptr;
     double* red = nullptr;
+    unsigned* wg_part = nullptr;
+    int cg_predict = 0;  // iterations launched before the done flag was seen, last solve
     float* tiles = nullptr;
     int* h_done = nullptr;  // pinned
     hipEvent_t done_ev[2] = {nullptr, nullptr};
@@ -134,6 +137,7 @@ mpm::KP make_kp(const mpm_context* c, float dt) {
     p.keyalt = c->keybuf[c->cur ^ 1];
     p.prank = c->prank;
     p.rcache = c->rcache;
+    p.pop = c->pop;
     p.pcap = c->pcap;
     p.slot_count = c->slot_count;
     p.slot_touched = c->slot_touched;
@@ -159,6 +163,8 @@ mpm::KP make_kp(const mpm_context* c, float dt) {
     p.red = c->red;
     p.red_stride = c->grid_node;
     p.red_count = c->grid_node;
+    p.wg_part = c->wg_part;
+    p.wg_part_count = c->grid_block;
     p.tiles = c->tiles;
     p.dt = dt;
     return p;
@@ -242,29 +248,35 @@ int stage_cg(mpm_context* c, float dt, bool do_init) {
         timed(c, KC_CG_NODE, [&] { mpm::launch_cg_node1<D>(p, c->stream, c->grid_node); });
         timed(c, KC_CG_NODE, [&] { mpm::launch_cg_node2<D>(p, c->stream, c->grid_node, k); });
     };
-    if (max_it <= 64) {
-        // every iteration is launched; kernels after convergence return at once
-        for (int k = 0; k < max_it; ++k) iteration(k);
-    } else {
-        // long solves: launch in chunks and stop once a chunk two back reported convergence
-        const int chunk = 8;
-        int slot = 0;
-        bool pending[2] = {false, false};
-        for (int k = 0; k < max_it;) {
-            const int n = std::min(chunk, max_it - k);
-            for (int j = 0; j < n; ++j) iteration(k + j);
-            k += n;
-            HIPCHECK(hipMemcpyAsync(&c->h_done[slot], &c->cg->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
-            HIPCHECK(hipEventRecord(c->done_ev[slot], c->stream));
-            pending[slot] = true;
-            slot ^= 1;
-            if (pending[slot]) {
-                HIPCHECK(hipEventSynchronize(c->done_ev[slot]));
-                pending[slot] = false;
-                if (c->h_done[slot]) break;
+    // Iterations after convergence return at once, so launching a few too
+    // many is harmless.  The first batch is the previous solve's iteration
+    // count; after it the host polls the done flag one chunk behind the GPU
+    // (double-buffered events), so the queue never runs dry while it waits.
+    const int chunk = 4;
+    int first = std::min(max_it, std::max(c->cg_predict, chunk));
+    int slot = 0, launched = 0;
+    int boundary[2] = {0, 0};
+    bool pending[2] = {false, false};
+    while (launched < max_it) {
+        const int n = launched == 0 ? first : std::min(chunk, max_it - launched);
+        for (int j = 0; j < n; ++j) iteration(launched + j);
+        launched += n;
+        HIPCHECK(hipMemcpyAsync(&c->h_done[slot], &c->cg->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipEventRecord(c->done_ev[slot], c->stream));
+        pending[slot] = true;
+        boundary[slot] = launched;
+        slot ^= 1;
+        if (pending[slot]) {
+            HIPCHECK(hipEventSynchronize(c->done_ev[slot]));
+            pending[slot] = false;
+            if (c->h_done[slot]) {
+                // done by the first boundary: try one fewer next time
+                c->cg_predict = boundary[slot] == first ? first - 1 : boundary[slot];
+                break;
             }
         }
     }
+    if (launched >= max_it) c->cg_predict = max_it;
     timed(c, KC_CG_NODE, [&] { mpm::launch_cg_finalize(p, c->stream, max_it); });
     return MPM_OK;
 }
@@ -405,6 +417,7 @@ int mpm_create(const mpm_scene_desc* desc, int64_t capacity, mpm_handle* out) {
     }
     A(dalloc(c, &c->prank, (size_t)c->pcap));
     A(dalloc(c, &c->rcache, (size_t)D * D * c->pcap));
+    A(dalloc(c, &c->pop, (size_t)mpm::op_record_floats(D) * c->pcap));
     A(dalloc(c, &c->slot_count, (size_t)c->nslots));
     A(dalloc(c, &c->slot_touched, (size_t)c->nslots));
     A(dalloc(c, &c->slot_active, (size_t)c->nslots));
@@ -426,8 +439,10 @@ int mpm_create(const mpm_scene_desc* desc, int64_t capacity, mpm_handle* out) {
     A(dalloc(c, &c->cg, 1));
     c->grid_particle = (int)std::min<int64_t>(std::max<int64_t>((c->pcap + mpm::WG - 1) / mpm::WG, 8), 4096);
     c->grid_block = (int)std::min<int64_t>(((int64_t)c->bcap + 7) / 8 * 8, 8192);
-    c->grid_node = (int)std::min<int64_t>(std::max<int64_t>((c->gcap + mpm::WG - 1) / mpm::WG, 1), 256);
+    // node kernels: one thread per node up to 1024 work-groups (a multiple of 8 for the XCD ranges)
+    c->grid_node = (int)std::min<int64_t>(((c->gcap + mpm::WG - 1) / mpm::WG + 7) / 8 * 8, 1024);
     A(dalloc(c, &c->red, (size_t)3 * c->grid_node));
+    A(dalloc(c, &c->wg_part, (size_t)2 * c->grid_block));
     if (rc == MPM_OK) {
         void* hp = nullptr;
         if (hipHostMalloc(&hp, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");

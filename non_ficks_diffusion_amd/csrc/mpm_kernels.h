@@ -7,6 +7,8 @@
 namespace mpm {
 
 constexpr int WG = 256;          // work-group size of every kernel
+// floats per particle of the CG force operator record (see OpRec)
+constexpr int op_record_floats(int D) { return D == 3 ? 48 : 12; }
 constexpr int SCAN_PER_THREAD = 16;
 constexpr int SCAN_TILE = WG * SCAN_PER_THREAD;
 constexpr int MAX_PLANES = 8;
@@ -52,6 +54,8 @@ struct KP {
     int* keyalt;
     int* prank;
     float* rcache;   // rotation R per particle (sorted order), field-major
+    float* pop;      // per particle (sorted order) OPW<D> floats: the symmetric
+                     // D^2 x D^2 operator of the CG force (upper triangle) + X
     int64_t pcap;
     // slots and blocks
     int* slot_count;
@@ -81,6 +85,8 @@ struct KP {
     double* red;      // 3 * red_stride partials: s.q | r.r | max |r|^2
     int red_stride;   // node-kernel work-groups
     int red_count;
+    unsigned* wg_part;  // 2 * wg_part_count: per-work-group max |v|^2 | removals of G2P
+    int wg_part_count;
     float dt;
 };
 

@@ -43,6 +43,13 @@ __device__ inline XcdRange xcd_blocks(int n) {
     const int lo = (int)(((int64_t)n * x) >> 3), hi = (int)(((int64_t)n * (x + 1)) >> 3);
     return {lo + k, hi, (int)(gridDim.x >> 3)};
 }
+// Node loops (one thread per node): the nodes of the same per-XCD block
+// range, so a node kernel reads the tiles its XCD's block kernel wrote.
+__device__ inline XcdRange xcd_nodes(int nblocks) {
+    const XcdRange r = xcd_blocks(nblocks);
+    const int lo = r.first - (int)(blockIdx.x >> 3);
+    return {lo * 64 + (int)(blockIdx.x >> 3) * WG + (int)threadIdx.x, r.end * 64, r.step * WG};
+}
 
 template <int D>
 __device__ inline void slot_coords(int slot, int nb, int* b) {
@@ -561,6 +568,84 @@ __device__ inline bool cell_particle(const KP& p, int i, const int* bb, const in
 }
 
 // ============================================================================
+// Linearised force operator per particle.  MPM_CG_System::Force
+// (MPM_CG_System.h:150-215) maps the node field u to forces through
+//   T = sum_i grad w_ip (x) u_i,  dF = T^T F,  A = V0 F dP(dF)^T,
+//   q_i += A^T grad w_ip.
+// T -> A is linear and fixed for the whole solve, so it is evaluated once
+// per substep as a D^2 x D^2 matrix (columns = A of the unit T's), which is
+// symmetric (dP is the Hessian of the energy: V0 dP(dF):dF' = V0 dP(dF'):dF)
+// and is stored after the particle position as its upper triangle.
+// One CG application then costs D^4 multiply-adds per particle instead of
+// re-deriving the stress state, and reads one contiguous record.
+// ============================================================================
+template <int D> struct OpRec;
+template <> struct OpRec<3> { static constexpr int L = 45, W = 48; };  // X[3], L[45]: 12 x float4
+template <> struct OpRec<2> { static constexpr int L = 10, W = 12; };  // X[2], L[10]: 3 x float4
+static_assert(OpRec<3>::W == op_record_floats(3) && OpRec<2>::W == op_record_floats(2), "operator record size");
+
+template <int D>
+__device__ constexpr int op_index(int r, int c) {
+    // upper triangle, row-major, of a D^2 x D^2 matrix (r <= c)
+    return r * D * D - (r * (r - 1)) / 2 + (c - r);
+}
+
+template <int D>
+__device__ inline void store_operator(const KP& p, int i, const StressState<D>& st, float V0, const float* X) {
+    constexpr int DD = D * D;
+    float L[DD][DD];
+#pragma unroll
+    for (int j = 0; j < DD; ++j) {
+        // T = e_j: dF = T^T F has row (j % D) equal to row (j / D) of F
+        float dF[DD];
+#pragma unroll
+        for (int k = 0; k < DD; ++k) dF[k] = 0.0f;
+#pragma unroll
+        for (int cc = 0; cc < D; ++cc) dF[(j % D) * D + cc] = st.F[(j / D) * D + cc];
+        float dP[DD];
+        dP_of<D>(st, dF, dP);
+#pragma unroll
+        for (int r = 0; r < D; ++r)
+#pragma unroll
+            for (int cc = 0; cc < D; ++cc) {
+                float s = 0.0f;
+#pragma unroll
+                for (int k = 0; k < D; ++k) s += st.F[r * D + k] * dP[cc * D + k];
+                L[r * D + cc][j] = s * V0;
+            }
+    }
+    float rec[OpRec<D>::W];
+#pragma unroll
+    for (int a = 0; a < D; ++a) rec[a] = X[a];
+#pragma unroll
+    for (int r = 0; r < DD; ++r)
+#pragma unroll
+        for (int c = r; c < DD; ++c) rec[D + op_index<D>(r, c)] = 0.5f * (L[r][c] + L[c][r]);
+#pragma unroll
+    for (int k = D + OpRec<D>::L; k < OpRec<D>::W; ++k) rec[k] = 0.0f;
+    float4* dst = reinterpret_cast<float4*>(p.pop + (int64_t)i * OpRec<D>::W);
+#pragma unroll
+    for (int k = 0; k < OpRec<D>::W / 4; ++k) dst[k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
+}
+
+// A = L T for the stored operator; `tri` is the upper triangle of the record,
+// walked entry by entry so only T, A and the entry in flight are live.
+template <int D>
+__device__ inline void apply_operator(const float* tri, const float* T, float* A) {
+    constexpr int DD = D * D;
+#pragma unroll
+    for (int r = 0; r < DD; ++r) A[r] = 0.0f;
+#pragma unroll
+    for (int r = 0; r < DD; ++r)
+#pragma unroll
+        for (int c = r; c < DD; ++c) {
+            const float v = tri[op_index<D>(r, c)];
+            A[r] += v * T[c];
+            if (c != r) A[c] += v * T[r];
+        }
+}
+
+// ============================================================================
 // P2G (Rasterize, MPM_Example.cpp:520-537): m_i += w m_p, (mv)_i += w m_p V_p
 // ============================================================================
 template <int D>
@@ -609,7 +694,8 @@ __global__ __launch_bounds__(BWG) void k_p2g(KP p) {
 template <int D>
 __global__ __launch_bounds__(WG) void k_normalize(KP p) {
     const int nn = p.ctr->nblocks * 64;
-    for (int n = blockIdx.x * WG + threadIdx.x; n < nn; n += gridDim.x * WG) {
+    const XcdRange nr = xcd_nodes(nn >> 6);
+    for (int n = nr.first; n < nr.end; n += nr.step) {
         float v[D + 1];
         gather_node_tiles<D, D + 1>(p, n, v);
         const float m = v[0];
@@ -710,6 +796,12 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
                     p.rcache[k * p.pcap + i] = R[k];
                 }
                 p.pcur[PF<D>::J * p.pcap + i] = Jp;
+                StressState<D> so;
+                stress_state<D>(F, R, Jp, p.mat, so);
+                float Xo[D];
+#pragma unroll
+                for (int a = 0; a < D; ++a) Xo[a] = p.pcur[(PF<D>::X + a) * p.pcap + i];
+                store_operator<D>(p, i, so, p.pcur[PF<D>::VOL * p.pcap + i], Xo);
             } else {
 #pragma unroll
                 for (int k = 0; k < DD; ++k) R[k] = p.rcache[k * p.pcap + i];
@@ -780,7 +872,8 @@ __global__ __launch_bounds__(WG) void k_grid_update(KP p) {
     }
     const int nn = p.ctr->nblocks * 64;
     const float dt = p.dt;
-    for (int n = blockIdx.x * WG + threadIdx.x; n < nn; n += gridDim.x * WG) {
+    const XcdRange nr = xcd_nodes(nn >> 6);
+    for (int n = nr.first; n < nr.end; n += nr.step) {
         const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
         float vs[D];
         if (FORCE) {
@@ -874,8 +967,20 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
         __syncthreads();
         const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
         for (int i = i0 + wv; i < i1; i += NW) {
-            float w[D][3], dw[D][3], X[D];
-            if (!cell_particle<D>(p, i, bb, l, w, dw, X)) continue;
+            const float* rec = p.pop + (int64_t)i * OpRec<D>::W;
+            float w[D][3], dw[D][3];
+            bool same = true;
+            {
+                const float4 x4 = *reinterpret_cast<const float4*>(rec);
+                const float X[3] = {x4.x, x4.y, x4.z};
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    int cl;
+                    bspline_weights(X[a], p.dx, p.inv_dx, cl, w[a], dw[a]);
+                    same = same && cl + Dm::B - Dm::B * bb[a] == l[a];
+                }
+            }
+            if (!same) continue;
             float T[DD];
 #pragma unroll
             for (int k = 0; k < DD; ++k) T[k] = 0.0f;
@@ -890,38 +995,8 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
 #pragma unroll
                     for (int cc = 0; cc < D; ++cc) T[a * D + cc] += g[a] * ut[cc][t];
             }
-            float F[DD], R[DD];
-#pragma unroll
-            for (int k = 0; k < DD; ++k) {
-                F[k] = p.pcur[(PF<D>::F + k) * p.pcap + i];
-                R[k] = p.rcache[k * p.pcap + i];
-            }
-            const float Jp = p.pcur[PF<D>::J * p.pcap + i];
-            const float V0 = p.pcur[PF<D>::VOL * p.pcap + i];
-            float dF[DD];
-#pragma unroll
-            for (int r = 0; r < D; ++r)
-#pragma unroll
-                for (int cc = 0; cc < D; ++cc) {
-                    float s = 0.0f;
-#pragma unroll
-                    for (int k = 0; k < D; ++k) s += T[k * D + r] * F[k * D + cc];
-                    dF[r * D + cc] = s;
-                }
-            StressState<D> st;
-            stress_state<D>(F, R, Jp, p.mat, st);
-            float dP[DD];
-            dP_of<D>(st, dF, dP);
-            float A[DD];  // V0 F dP^T
-#pragma unroll
-            for (int r = 0; r < D; ++r)
-#pragma unroll
-                for (int cc = 0; cc < D; ++cc) {
-                    float s = 0.0f;
-#pragma unroll
-                    for (int k = 0; k < D; ++k) s += F[r * D + k] * dP[cc * D + k];
-                    A[r * D + cc] = s * V0;
-                }
+            float A[DD];  // V0 F dP(T^T F)^T
+            apply_operator<D>(rec + D, T, A);
 #pragma unroll
             for (int s = 0; s < ST; ++s) {
                 int off[D];
@@ -963,7 +1038,8 @@ __global__ __launch_bounds__(WG) void k_cg_restart_node(KP p) {
     const float sds = p.dt * p.dt;
     if (blockIdx.x == 0 && threadIdx.x == 0) p.cg->multiplies = 1;
     double sum = 0.0, mx = 0.0;
-    for (int n = blockIdx.x * WG + threadIdx.x; n < nn; n += gridDim.x * WG) {
+    const XcdRange nr = xcd_nodes(nn >> 6);
+    for (int n = nr.first; n < nr.end; n += nr.step) {
         const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
         float q[D], r[D];
         gather_node_tiles<D, D>(p, n, q);
@@ -1036,7 +1112,8 @@ __global__ __launch_bounds__(WG) void k_cg_update_s(KP p, int k) {
     if (done) return;
     const int nn = p.ctr->nblocks * 64;
     const float fb = (float)beta;
-    for (int n = blockIdx.x * WG + threadIdx.x; n < nn; n += gridDim.x * WG) {
+    const XcdRange nr = xcd_nodes(nn >> 6);
+    for (int n = nr.first; n < nr.end; n += nr.step) {
         if (!(p.g_flags[n] & MPM_FLAG_INTERIOR_DEV)) continue;
 #pragma unroll
         for (int a = 0; a < D; ++a) {
@@ -1053,7 +1130,8 @@ __global__ __launch_bounds__(WG) void k_cg_node1(KP p) {
     const int nn = p.ctr->nblocks * 64;
     const float sds = p.dt * p.dt;
     double sum = 0.0, mx = 0.0;
-    for (int n = blockIdx.x * WG + threadIdx.x; n < nn; n += gridDim.x * WG) {
+    const XcdRange nr = xcd_nodes(nn >> 6);
+    for (int n = nr.first; n < nr.end; n += nr.step) {
         const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
         float q[D];
         gather_node_tiles<D, D>(p, n, q);
@@ -1096,7 +1174,8 @@ __global__ __launch_bounds__(WG) void k_cg_node2(KP p, int k) {
     const float alpha = (float)alpha_d;
     const int nn = p.ctr->nblocks * 64;
     double sum = 0.0, mx = 0.0;
-    for (int n = blockIdx.x * WG + threadIdx.x; n < nn; n += gridDim.x * WG) {
+    const XcdRange nr = xcd_nodes(nn >> 6);
+    for (int n = nr.first; n < nr.end; n += nr.step) {
         const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
         if (!interior) continue;
         const double m = (double)p.g_m[n];
@@ -1228,15 +1307,59 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
         }
         __syncthreads();
     }
-    // one atomic per wave for the max speed and the removal count
+    // per-work-group partials (plain stores, no same-address atomics);
+    // k_g2p_finish folds them into the counters
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         best = max(best, (unsigned)__shfl_xor((int)best, off));
         removed += __shfl_xor(removed, off);
     }
+    __shared__ unsigned wbest[NW];
+    __shared__ int wrem[NW];
     if ((threadIdx.x & 63) == 0) {
-        if (best) atomicMax(&p.ctr->max_v2, best);
-        if (removed) atomicAdd(&p.ctr->n_removed, removed);
+        wbest[threadIdx.x >> 6] = best;
+        wrem[threadIdx.x >> 6] = removed;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned bb = 0u;
+        int rr = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) {
+            bb = max(bb, wbest[w]);
+            rr += wrem[w];
+        }
+        p.wg_part[blockIdx.x] = bb;
+        p.wg_part[p.wg_part_count + blockIdx.x] = (unsigned)rr;
+    }
+}
+
+// One work-group: max speed and removal count of the last G2P, in a fixed
+// order, into the counters read by the host and the next binning.
+__global__ __launch_bounds__(WG) void k_g2p_finish(KP p, int nwg) {
+    unsigned best = 0u, removed = 0u;
+    for (int i = threadIdx.x; i < nwg; i += WG) {
+        best = max(best, p.wg_part[i]);
+        removed += p.wg_part[p.wg_part_count + i];
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        best = max(best, (unsigned)__shfl_xor((int)best, off));
+        removed += (unsigned)__shfl_xor((int)removed, off);
+    }
+    __shared__ unsigned sb[WG / 64], sr[WG / 64];
+    if ((threadIdx.x & 63) == 0) {
+        sb[threadIdx.x >> 6] = best;
+        sr[threadIdx.x >> 6] = removed;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int w = 1; w < WG / 64; ++w) {
+            best = max(best, sb[w]);
+            removed += sr[w];
+        }
+        p.ctr->max_v2 = best;
+        p.ctr->n_removed = (int)removed;
     }
 }
 
@@ -1294,7 +1417,10 @@ template <int D> void launch_cg_update_s(const KP& p, hipStream_t s, int grid, i
 template <int D> void launch_cg_node1(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_cg_node1<D>, dim3(grid), dim3(WG), 0, s, p); }
 template <int D> void launch_cg_node2(const KP& p, hipStream_t s, int grid, int k) { hipLaunchKernelGGL(k_cg_node2<D>, dim3(grid), dim3(WG), 0, s, p, k); }
 void launch_cg_finalize(const KP& p, hipStream_t s, int k_end) { hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(WG), 0, s, p, k_end); }
-template <int D> void launch_g2p(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_g2p<D>, dim3(grid), dim3(BWG), 0, s, p); }
+template <int D> void launch_g2p(const KP& p, hipStream_t s, int grid) {
+    hipLaunchKernelGGL(k_g2p<D>, dim3(grid), dim3(BWG), 0, s, p);
+    hipLaunchKernelGGL(k_g2p_finish, dim3(1), dim3(WG), 0, s, p, grid);
+}
 template <int D> void launch_max_speed(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_max_speed<D>, dim3(grid), dim3(WG), 0, s, p); }
 
 #define MPM_INSTANTIATE(D)                                                                   \
