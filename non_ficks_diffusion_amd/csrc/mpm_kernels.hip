@@ -645,6 +645,115 @@ __device__ inline void apply_operator(const float* tri, const float* T, float* A
         }
 }
 
+// Separable stencil contractions for the CG operator.  With
+// grad w = (dwx wy wz, wx dwy wz, wx wy dwz) the 3^D-node sums factor axis
+// by axis: 270 multiply-adds per particle in 3D instead of ~400 for the
+// node-by-node form.  `base` is the tile index of the particle's stencil
+// corner (its cell - 1 in every axis).
+template <int D>
+__device__ inline void grad_gather(const float (*w)[3], const float (*dw)[3], int base,
+                                   const float (*ut)[Dims<D>::TN], float* T) {
+    constexpr int TE = Dims<D>::TE;
+    if constexpr (D == 3) {
+        float y00[3][3], y10[3][3], y01[3][3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) y00[i][cc] = y10[i][cc] = y01[i][cc] = 0.0f;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int t0 = base + (i * TE + j) * TE;
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    const float u0 = ut[cc][t0], u1 = ut[cc][t0 + 1], u2 = ut[cc][t0 + 2];
+                    const float z0 = w[2][0] * u0 + w[2][1] * u1 + w[2][2] * u2;
+                    const float z1 = dw[2][0] * u0 + dw[2][1] * u1 + dw[2][2] * u2;
+                    y00[i][cc] += w[1][j] * z0;
+                    y10[i][cc] += dw[1][j] * z0;
+                    y01[i][cc] += w[1][j] * z1;
+                }
+            }
+        }
+#pragma unroll
+        for (int cc = 0; cc < 3; ++cc) {
+            T[0 * 3 + cc] = dw[0][0] * y00[0][cc] + dw[0][1] * y00[1][cc] + dw[0][2] * y00[2][cc];
+            T[1 * 3 + cc] = w[0][0] * y10[0][cc] + w[0][1] * y10[1][cc] + w[0][2] * y10[2][cc];
+            T[2 * 3 + cc] = w[0][0] * y01[0][cc] + w[0][1] * y01[1][cc] + w[0][2] * y01[2][cc];
+        }
+    } else {
+        float z0[3][2], z1[3][2];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int t0 = base + i * TE;
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                const float u0 = ut[cc][t0], u1 = ut[cc][t0 + 1], u2 = ut[cc][t0 + 2];
+                z0[i][cc] = w[1][0] * u0 + w[1][1] * u1 + w[1][2] * u2;
+                z1[i][cc] = dw[1][0] * u0 + dw[1][1] * u1 + dw[1][2] * u2;
+            }
+        }
+#pragma unroll
+        for (int cc = 0; cc < 2; ++cc) {
+            T[0 * 2 + cc] = dw[0][0] * z0[0][cc] + dw[0][1] * z0[1][cc] + dw[0][2] * z0[2][cc];
+            T[1 * 2 + cc] = w[0][0] * z1[0][cc] + w[0][1] * z1[1][cc] + w[0][2] * z1[2][cc];
+        }
+    }
+}
+
+// q_node[cc] += sum_r A[r][cc] grad_r w_node over the stencil, into a
+// per-wave LDS tile (channel-major, TN floats per channel).  Lanes of a wave
+// hold distinct cells, so one stencil node is read-modified-written by at
+// most one lane per step; the compiler fence keeps the steps in order.
+template <int D>
+__device__ inline void grad_scatter(const float (*w)[3], const float (*dw)[3], int base, const float* A, float* q) {
+    constexpr int TE = Dims<D>::TE, TN = Dims<D>::TN;
+    if constexpr (D == 3) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            float x0[3], x1[3], x2[3];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                x0[cc] = A[0 * 3 + cc] * dw[0][i];
+                x1[cc] = A[1 * 3 + cc] * w[0][i];
+                x2[cc] = A[2 * 3 + cc] * w[0][i];
+            }
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                float ya[3], yb[3];
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    ya[cc] = w[1][j] * x0[cc] + dw[1][j] * x1[cc];
+                    yb[cc] = w[1][j] * x2[cc];
+                }
+                const int t0 = base + (i * TE + j) * TE;
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) q[cc * TN + t0 + k] += w[2][k] * ya[cc] + dw[2][k] * yb[cc];
+                    MPM_STEP_FENCE();
+                }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            float x0[2], x1[2];
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                x0[cc] = A[0 * 2 + cc] * dw[0][i];
+                x1[cc] = A[1 * 2 + cc] * w[0][i];
+            }
+            const int t0 = base + i * TE;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc) q[cc * TN + t0 + j] += w[1][j] * x0[cc] + dw[1][j] * x1[cc];
+                MPM_STEP_FENCE();
+            }
+        }
+    }
+}
+
 // ============================================================================
 // P2G (Rasterize, MPM_Example.cpp:520-537): m_i += w m_p, (mv)_i += w m_p V_p
 // ============================================================================
@@ -942,7 +1051,7 @@ __global__ __launch_bounds__(WG) void k_grid_update(KP p) {
 template <int D, bool RESTART>
 __global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
     using Dm = Dims<D>;
-    constexpr int TN = Dm::TN, NBR = Dm::NBR, DD = D * D, ST = Dm::ST;
+    constexpr int TN = Dm::TN, NBR = Dm::NBR, DD = D * D;
     __shared__ float ut[D][TN];
     __shared__ float qt[NW * D * TN];
     __shared__ int nbr[NBR];
@@ -953,6 +1062,11 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
     float* mine = qt + wv * D * TN;
     int l[D];
     local_coords<D>(c, l);
+    int base;  // tile index of this cell's stencil corner
+    {
+        int off0[D];
+        base = tile_index<D>(l, 0, off0);
+    }
     const XcdRange xr = xcd_blocks(nblocks);
     for (int b = xr.first; b < xr.end; b += xr.step) {
         if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
@@ -982,36 +1096,10 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
             }
             if (!same) continue;
             float T[DD];
-#pragma unroll
-            for (int k = 0; k < DD; ++k) T[k] = 0.0f;
-#pragma unroll
-            for (int s = 0; s < ST; ++s) {
-                int off[D];
-                const int t = tile_index<D>(l, s, off);
-                float g[D];
-                stencil_grad<D>(w, dw, off, g);
-#pragma unroll
-                for (int a = 0; a < D; ++a)
-#pragma unroll
-                    for (int cc = 0; cc < D; ++cc) T[a * D + cc] += g[a] * ut[cc][t];
-            }
+            grad_gather<D>(w, dw, base, ut, T);
             float A[DD];  // V0 F dP(T^T F)^T
             apply_operator<D>(rec + D, T, A);
-#pragma unroll
-            for (int s = 0; s < ST; ++s) {
-                int off[D];
-                const int t = tile_index<D>(l, s, off);
-                float g[D];
-                stencil_grad<D>(w, dw, off, g);
-#pragma unroll
-                for (int cc = 0; cc < D; ++cc) {
-                    float v = 0.0f;
-#pragma unroll
-                    for (int r = 0; r < D; ++r) v += A[r * D + cc] * g[r];
-                    mine[cc * TN + t] += v;
-                }
-                MPM_STEP_FENCE();
-            }
+            grad_scatter<D>(w, dw, base, A, mine);
         }
         __syncthreads();
         store_wave_tiles<D, D>(p, b, qt);
