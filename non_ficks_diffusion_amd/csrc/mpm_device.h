@@ -225,7 +225,7 @@ __device__ inline void svd2_fp64(const double* F, double* U, double* sig, double
 // Fe = U Sigma V^T, and return R = U V^T.
 // ---------------------------------------------------------------------------
 template <int D>
-__device__ inline void plasticity(float* Fe, float& Jp, float* R, const Material& mat) {
+__device__ inline void plasticity(float* Fe, float& Jp, float* R, const Material& mat, float* Uout = nullptr) {
     constexpr int DD = D * D;
     double F[DD], U[DD], V[DD], s[D];
 #pragma unroll
@@ -261,6 +261,10 @@ __device__ inline void plasticity(float* Fe, float& Jp, float* R, const Material
             for (int k = 0; k < D; ++k) a += U[D * i + k] * V[D * j + k];
             R[D * i + j] = (float)a;
         }
+    if (Uout) {
+#pragma unroll
+        for (int i = 0; i < DD; ++i) Uout[i] = (float)U[i];
+    }
 }
 
 // Lame parameters after hardening (MPM_Constitutive_Model.cpp:42-47).

@@ -8,7 +8,7 @@ namespace mpm {
 
 constexpr int WG = 256;          // work-group size of every kernel
 // floats per particle of the CG force operator record (see OpRec)
-constexpr int op_record_floats(int D) { return D == 3 ? 48 : 12; }
+constexpr int op_record_floats(int D) { return D == 3 ? 24 : 12; }
 constexpr int SCAN_PER_THREAD = 16;
 constexpr int SCAN_TILE = WG * SCAN_PER_THREAD;
 constexpr int MAX_PLANES = 8;
@@ -54,8 +54,8 @@ struct KP {
     int* keyalt;
     int* prank;
     float* rcache;   // rotation R per particle (sorted order), field-major
-    float* pop;      // per particle (sorted order) OPW<D> floats: the symmetric
-                     // D^2 x D^2 operator of the CG force (upper triangle) + X
+    float* pop;      // per particle (sorted order) op_record_floats(D) floats:
+                     // X and the linearised CG force operator
     int64_t pcap;
     // slots and blocks
     int* slot_count;

@@ -580,8 +580,11 @@ __device__ inline bool cell_particle(const KP& p, int i, const int* bb, const in
 // re-deriving the stress state, and reads one contiguous record.
 // ============================================================================
 template <int D> struct OpRec;
-template <> struct OpRec<3> { static constexpr int L = 45, W = 48; };  // X[3], L[45]: 12 x float4
-template <> struct OpRec<2> { static constexpr int L = 10, W = 12; };  // X[2], L[10]: 3 x float4
+// 3D: X[3], q[4] (unit quaternion of U), the 3x3 block on the diagonal
+// entries (upper triangle, 6) and the three 2x2 pair blocks (3 each), pad 2.
+template <> struct OpRec<3> { static constexpr int W = 24; };  // 6 x float4
+// 2D: X[2], the full symmetric 4x4 operator (upper triangle, 10).
+template <> struct OpRec<2> { static constexpr int L = 10, W = 12; };  // 3 x float4
 static_assert(OpRec<3>::W == op_record_floats(3) && OpRec<2>::W == op_record_floats(2), "operator record size");
 
 template <int D>
@@ -590,59 +593,192 @@ __device__ constexpr int op_index(int r, int c) {
     return r * D * D - (r * (r - 1)) / 2 + (c - r);
 }
 
+// A = V0 F dP(T^T F)^T, the reference's per-particle force map
 template <int D>
-__device__ inline void store_operator(const KP& p, int i, const StressState<D>& st, float V0, const float* X) {
+__device__ inline void force_map(const StressState<D>& st, float V0, const float* T, float* A) {
     constexpr int DD = D * D;
-    float L[DD][DD];
+    float dF[DD];
 #pragma unroll
-    for (int j = 0; j < DD; ++j) {
-        // T = e_j: dF = T^T F has row (j % D) equal to row (j / D) of F
-        float dF[DD];
+    for (int r = 0; r < D; ++r)
 #pragma unroll
-        for (int k = 0; k < DD; ++k) dF[k] = 0.0f;
+        for (int cc = 0; cc < D; ++cc) {
+            float s = 0.0f;
 #pragma unroll
-        for (int cc = 0; cc < D; ++cc) dF[(j % D) * D + cc] = st.F[(j / D) * D + cc];
-        float dP[DD];
-        dP_of<D>(st, dF, dP);
+            for (int k = 0; k < D; ++k) s += T[k * D + r] * st.F[k * D + cc];
+            dF[r * D + cc] = s;
+        }
+    float dP[DD];
+    dP_of<D>(st, dF, dP);
 #pragma unroll
-        for (int r = 0; r < D; ++r)
+    for (int r = 0; r < D; ++r)
 #pragma unroll
-            for (int cc = 0; cc < D; ++cc) {
-                float s = 0.0f;
+        for (int cc = 0; cc < D; ++cc) {
+            float s = 0.0f;
 #pragma unroll
-                for (int k = 0; k < D; ++k) s += st.F[r * D + k] * dP[cc * D + k];
-                L[r * D + cc][j] = s * V0;
-            }
+            for (int k = 0; k < D; ++k) s += st.F[r * D + k] * dP[cc * D + k];
+            A[r * D + cc] = s * V0;
+        }
+}
+
+// rotation matrix (row-major) <-> unit quaternion (w, x, y, z)
+__device__ inline void quat_of(const float* U, float* q) {
+    const float tr = U[0] + U[4] + U[8];
+    float w, x, y, z;
+    if (tr > 0.0f) {
+        const float s = 2.0f * sqrtf(tr + 1.0f);
+        w = 0.25f * s; x = (U[7] - U[5]) / s; y = (U[2] - U[6]) / s; z = (U[3] - U[1]) / s;
+    } else if (U[0] > U[4] && U[0] > U[8]) {
+        const float s = 2.0f * sqrtf(1.0f + U[0] - U[4] - U[8]);
+        w = (U[7] - U[5]) / s; x = 0.25f * s; y = (U[1] + U[3]) / s; z = (U[2] + U[6]) / s;
+    } else if (U[4] > U[8]) {
+        const float s = 2.0f * sqrtf(1.0f + U[4] - U[0] - U[8]);
+        w = (U[2] - U[6]) / s; x = (U[1] + U[3]) / s; y = 0.25f * s; z = (U[5] + U[7]) / s;
+    } else {
+        const float s = 2.0f * sqrtf(1.0f + U[8] - U[0] - U[4]);
+        w = (U[3] - U[1]) / s; x = (U[2] + U[6]) / s; y = (U[5] + U[7]) / s; z = 0.25f * s;
     }
+    const float n = 1.0f / sqrtf(w * w + x * x + y * y + z * z);
+    q[0] = w * n; q[1] = x * n; q[2] = y * n; q[3] = z * n;
+}
+__device__ inline void rot_of(const float* q, float* U) {
+    const float w = q[0], x = q[1], y = q[2], z = q[3];
+    U[0] = 1.0f - 2.0f * (y * y + z * z); U[1] = 2.0f * (x * y - w * z);        U[2] = 2.0f * (x * z + w * y);
+    U[3] = 2.0f * (x * y + w * z);        U[4] = 1.0f - 2.0f * (x * x + z * z); U[5] = 2.0f * (y * z - w * x);
+    U[6] = 2.0f * (x * z - w * y);        U[7] = 2.0f * (y * z + w * x);        U[8] = 1.0f - 2.0f * (x * x + y * y);
+}
+
+// Ahat = U^T A U (3x3)
+__device__ inline void rotate_in(const float* U, const float* A, float* out) {
+    float t[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) t[i * 3 + j] = A[i * 3 + 0] * U[0 * 3 + j] + A[i * 3 + 1] * U[1 * 3 + j] + A[i * 3 + 2] * U[2 * 3 + j];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) out[i * 3 + j] = U[0 * 3 + i] * t[0 * 3 + j] + U[1 * 3 + i] * t[1 * 3 + j] + U[2 * 3 + i] * t[2 * 3 + j];
+}
+// A = U Ahat U^T (3x3)
+__device__ inline void rotate_out(const float* U, const float* Ah, float* out) {
+    float t[9];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) t[i * 3 + j] = U[i * 3 + 0] * Ah[0 * 3 + j] + U[i * 3 + 1] * Ah[1 * 3 + j] + U[i * 3 + 2] * Ah[2 * 3 + j];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) out[i * 3 + j] = t[i * 3 + 0] * U[j * 3 + 0] + t[i * 3 + 1] * U[j * 3 + 1] + t[i * 3 + 2] * U[j * 3 + 2];
+}
+
+// The record of particle i.  3D: in the frame of U (F = U S V^T) the map
+// T^ = U^T T U -> A^ = U^T A U of an isotropic energy decouples into a
+// 3x3 block on the diagonal entries and one 2x2 block per off-diagonal
+// pair (a,b),(b,a); the blocks are read off the reference's force map
+// applied to the rotated unit inputs u_a u_b^T and symmetrised.
+template <int D>
+__device__ inline void store_operator(const KP& p, int i, const StressState<D>& st, float V0, const float* X,
+                                      const float* U) {
+    constexpr int DD = D * D;
     float rec[OpRec<D>::W];
 #pragma unroll
+    for (int k = 0; k < OpRec<D>::W; ++k) rec[k] = 0.0f;
+#pragma unroll
     for (int a = 0; a < D; ++a) rec[a] = X[a];
+    if constexpr (D == 3) {
+        quat_of(U, rec + 3);
+        float Ur[9];
+        rot_of(rec + 3, Ur);  // the rotation the kernels will rebuild
+        float M[3][3], Pr[3][2][2];
 #pragma unroll
-    for (int r = 0; r < DD; ++r)
+        for (int a = 0; a < 3; ++a)
 #pragma unroll
-        for (int c = r; c < DD; ++c) rec[D + op_index<D>(r, c)] = 0.5f * (L[r][c] + L[c][r]);
+            for (int b = 0; b < 3; ++b) {
+                float T[9], A[9], Ah[9];
 #pragma unroll
-    for (int k = D + OpRec<D>::L; k < OpRec<D>::W; ++k) rec[k] = 0.0f;
+                for (int r = 0; r < 3; ++r)
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) T[r * 3 + cc] = Ur[r * 3 + a] * Ur[cc * 3 + b];
+                force_map<3>(st, V0, T, A);
+                rotate_in(Ur, A, Ah);
+                if (a == b) {
+#pragma unroll
+                    for (int c = 0; c < 3; ++c) M[c][a] = Ah[c * 3 + c];
+                } else {
+                    const int pr = (a + b) - 1;         // (0,1)->0, (0,2)->1, (1,2)->2
+                    const int lo = a < b ? a : b, hi = a < b ? b : a;
+                    const int col = a < b ? 0 : 1;      // input (T^_lo,hi, T^_hi,lo)
+                    Pr[pr][0][col] = Ah[lo * 3 + hi];
+                    Pr[pr][1][col] = Ah[hi * 3 + lo];
+                }
+            }
+        rec[7] = M[0][0]; rec[8] = 0.5f * (M[0][1] + M[1][0]); rec[9] = 0.5f * (M[0][2] + M[2][0]);
+        rec[10] = M[1][1]; rec[11] = 0.5f * (M[1][2] + M[2][1]); rec[12] = M[2][2];
+#pragma unroll
+        for (int pr = 0; pr < 3; ++pr) {
+            rec[13 + 3 * pr] = Pr[pr][0][0];
+            rec[14 + 3 * pr] = 0.5f * (Pr[pr][0][1] + Pr[pr][1][0]);
+            rec[15 + 3 * pr] = Pr[pr][1][1];
+        }
+    } else {
+        float L[DD][DD];
+#pragma unroll
+        for (int j = 0; j < DD; ++j) {
+            float T[DD], A[DD];
+#pragma unroll
+            for (int k = 0; k < DD; ++k) T[k] = k == j ? 1.0f : 0.0f;
+            force_map<D>(st, V0, T, A);
+#pragma unroll
+            for (int r = 0; r < DD; ++r) L[r][j] = A[r];
+        }
+#pragma unroll
+        for (int r = 0; r < DD; ++r)
+#pragma unroll
+            for (int c = r; c < DD; ++c) rec[D + op_index<D>(r, c)] = 0.5f * (L[r][c] + L[c][r]);
+    }
     float4* dst = reinterpret_cast<float4*>(p.pop + (int64_t)i * OpRec<D>::W);
 #pragma unroll
     for (int k = 0; k < OpRec<D>::W / 4; ++k) dst[k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
 }
 
-// A = L T for the stored operator; `tri` is the upper triangle of the record,
-// walked entry by entry so only T, A and the entry in flight are live.
+// A = L T from a particle's operator record
 template <int D>
-__device__ inline void apply_operator(const float* tri, const float* T, float* A) {
+__device__ inline void apply_operator(const float* rec, const float* T, float* A) {
     constexpr int DD = D * D;
+    if constexpr (D == 3) {
+        const float4 r1 = *reinterpret_cast<const float4*>(rec + 4);   // q1 q2 q3 m00
+        const float4 r0 = *reinterpret_cast<const float4*>(rec);       // X0 X1 X2 q0
+        const float4 r2 = *reinterpret_cast<const float4*>(rec + 8);   // m01 m02 m11 m12
+        const float4 r3 = *reinterpret_cast<const float4*>(rec + 12);  // m22 p0 p0 p0
+        const float4 r4 = *reinterpret_cast<const float4*>(rec + 16);  // p1 p1 p1 p2
+        const float4 r5 = *reinterpret_cast<const float4*>(rec + 20);  // p2 p2 - -
+        const float q[4] = {r0.w, r1.x, r1.y, r1.z};
+        float U[9], Th[9], Ah[9];
+        rot_of(q, U);
+        rotate_in(U, T, Th);
+        const float m00 = r1.w, m01 = r2.x, m02 = r2.y, m11 = r2.z, m12 = r2.w, m22 = r3.x;
+        Ah[0] = m00 * Th[0] + m01 * Th[4] + m02 * Th[8];
+        Ah[4] = m01 * Th[0] + m11 * Th[4] + m12 * Th[8];
+        Ah[8] = m02 * Th[0] + m12 * Th[4] + m22 * Th[8];
+        // pairs (0,1), (0,2), (1,2): (Ah_lo,hi, Ah_hi,lo) = [[a b][b c]] (Th_lo,hi, Th_hi,lo)
+        Ah[1] = r3.y * Th[1] + r3.z * Th[3];  Ah[3] = r3.z * Th[1] + r3.w * Th[3];
+        Ah[2] = r4.x * Th[2] + r4.y * Th[6];  Ah[6] = r4.y * Th[2] + r4.z * Th[6];
+        Ah[5] = r4.w * Th[5] + r5.x * Th[7];  Ah[7] = r5.x * Th[5] + r5.y * Th[7];
+        rotate_out(U, Ah, A);
+    } else {
+        const float* tri = rec + D;
 #pragma unroll
-    for (int r = 0; r < DD; ++r) A[r] = 0.0f;
+        for (int r = 0; r < DD; ++r) A[r] = 0.0f;
 #pragma unroll
-    for (int r = 0; r < DD; ++r)
+        for (int r = 0; r < DD; ++r)
 #pragma unroll
-        for (int c = r; c < DD; ++c) {
-            const float v = tri[op_index<D>(r, c)];
-            A[r] += v * T[c];
-            if (c != r) A[c] += v * T[r];
-        }
+            for (int c = r; c < DD; ++c) {
+                const float v = tri[op_index<D>(r, c)];
+                A[r] += v * T[c];
+                if (c != r) A[c] += v * T[r];
+            }
+    }
 }
 
 // Separable stencil contractions for the CG operator.  With
@@ -898,7 +1034,8 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
             for (int k = 0; k < DD; ++k) F[k] = p.pcur[(PF<D>::F + k) * p.pcap + i];
             float Jp = p.pcur[PF<D>::J * p.pcap + i];
             if (CONST) {
-                plasticity<D>(F, Jp, R, p.mat);
+                float U[DD];
+                plasticity<D>(F, Jp, R, p.mat, U);
 #pragma unroll
                 for (int k = 0; k < DD; ++k) {
                     p.pcur[(PF<D>::F + k) * p.pcap + i] = F[k];
@@ -910,7 +1047,7 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
                 float Xo[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) Xo[a] = p.pcur[(PF<D>::X + a) * p.pcap + i];
-                store_operator<D>(p, i, so, p.pcur[PF<D>::VOL * p.pcap + i], Xo);
+                store_operator<D>(p, i, so, p.pcur[PF<D>::VOL * p.pcap + i], Xo, U);
             } else {
 #pragma unroll
                 for (int k = 0; k < DD; ++k) R[k] = p.rcache[k * p.pcap + i];
@@ -1098,7 +1235,7 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
             float T[DD];
             grad_gather<D>(w, dw, base, ut, T);
             float A[DD];  // V0 F dP(T^T F)^T
-            apply_operator<D>(rec + D, T, A);
+            apply_operator<D>(rec, T, A);
             grad_scatter<D>(w, dw, base, A, mine);
         }
         __syncthreads();

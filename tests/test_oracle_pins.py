@@ -162,6 +162,61 @@ def test_dP_is_directional_derivative_and_symmetric(dim):
         assert abs(np.sum(A * dPB) - np.sum(B * dPA)) < 1e-8 * (np.abs(dPA).sum() * np.abs(B).sum())
 
 
+def test_force_map_blocks_in_svd_frame():
+    """The GPU stores each particle's linearised force map T -> A = V0 F dP(T^T F)^T
+    (T = sum_i grad w_i (x) u_i) in the frame of U (F = U S V^T) as a 3x3 block on
+    the diagonal entries plus one symmetric 2x2 block per off-diagonal pair.  That
+    layout is exact only if the map is symmetric and has exactly that block
+    pattern in the U frame; check both on the oracle's dP, with and without
+    inverted (negative-determinant) and plastically clamped states."""
+    mat = snow(plastic=1)
+    lam, mu = mat.lambda0, mat.mu0
+    diag = [0, 4, 8]
+    mask = np.zeros((9, 9), bool)
+    for a in diag:
+        for b in diag:
+            mask[a, b] = True
+    for pair in [(1, 3), (2, 6), (5, 7)]:
+        for x in pair:
+            for y in pair:
+                mask[x, y] = True
+    for k in range(20):
+        F = np.eye(3) + 0.3 * RNG.standard_normal((3, 3))
+        if k % 5 == 0:
+            F[:, 0] *= -1.0  # inverted element
+        Fo, _, R, _, _ = O.constitutive(mat, F, 1.0)
+        U, _, _ = O.svd(Fo)
+        assert np.linalg.det(U) > 0.0
+
+        def fmap(T):
+            return 0.7 * Fo @ O.dP(Fo, R, lam, mu, T.T @ Fo).T
+
+        L = np.zeros((9, 9))
+        Lh = np.zeros((9, 9))
+        for j in range(9):
+            E = np.zeros(9)
+            E[j] = 1.0
+            L[:, j] = fmap(E.reshape(3, 3)).ravel()
+            Lh[:, j] = (U.T @ fmap(U @ E.reshape(3, 3) @ U.T) @ U).ravel()
+        scale = np.abs(L).max()
+        assert np.abs(L - L.T).max() < 1e-12 * scale
+        assert np.abs(Lh[~mask]).max() < 1e-12 * scale
+        assert np.abs(Lh - Lh.T).max() < 1e-12 * scale
+        # a planted mistake -- dropping the rotation term dR -- breaks the
+        # symmetry the layout relies on
+        def fmap_no_dR(T):
+            dF = T.T @ Fo
+            J = np.linalg.det(Fo)
+            H = J * np.linalg.inv(Fo).T
+            return 0.7 * Fo @ (2 * mu * dF + lam * np.sum(H * dF) * H).T
+        Lbad = np.zeros((9, 9))
+        for j in range(9):
+            E = np.zeros(9)
+            E[j] = 1.0
+            Lbad[:, j] = fmap_no_dR(E.reshape(3, 3)).ravel()
+        assert np.abs(Lbad - L).max() > 1e-6 * scale
+
+
 # --------------------------------------------------------------------------- scenes used below
 def tiny_scene(dim=3, n=6, counts=16, planes=0, seed=0, v_scale=0.0, F_scale=0.0, E_scale=1.0):
     d, _ = M.scene_preset("avalanche" if dim == 3 else "disk2d", counts, 0)
