@@ -64,6 +64,19 @@ def cpu_baseline(desc, x, v, m, dt, substeps=2):
                       f"(dt {dt:.2e}, CG cap {desc.cg_max_iterations}), {el:.1f} s"}
 
 
+def copy_calibration():
+    """A copy kernel of known size: 1 GiB read + 1 GiB written per launch."""
+    import torch
+    n = (1 << 30) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read_per_launch": n * 4, "copy_bytes_written_per_launch": n * 4, "launches": 3}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,11 +85,18 @@ def main():
     ap.add_argument("--counts", type=int, default=COUNTS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-substeps", type=int, default=2)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 1 GiB device-to-device copy (3 times), the reference kernel "
+                         "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs each scene on a single MI355X "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
+
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     import non_ficks_diffusion_amd as M
 

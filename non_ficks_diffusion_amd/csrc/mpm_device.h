@@ -16,6 +16,10 @@ struct Dims<3> {
     static constexpr int NB = 64;        // nodes per block
     static constexpr int TE = 6;         // tile edge: block plus one halo node per side
     static constexpr int TN = 216;       // tile nodes
+    // LDS copy of a tile: node (x,y,z) at x*LX + y*LY + z.  With these
+    // strides the 64 cells of a block (lanes of a wave) fall on distinct
+    // banks of the 32-bank LDS for every stencil offset.
+    static constexpr int LX = 52, LY = 8, LTN = 6 * 52;
     static constexpr int ST = 27;        // stencil nodes
     static constexpr int NBR = 27;       // neighbour blocks incl. self
     static constexpr int DD = 9;
@@ -26,6 +30,7 @@ struct Dims<2> {
     static constexpr int NB = 64;
     static constexpr int TE = 10;
     static constexpr int TN = 100;
+    static constexpr int LX = 40, LY = 1, LTN = 10 * 40;  // node (x,y) at x*LX + y
     static constexpr int ST = 9;
     static constexpr int NBR = 9;
     static constexpr int DD = 4;

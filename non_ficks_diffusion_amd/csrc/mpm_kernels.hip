@@ -93,21 +93,28 @@ __device__ inline void tile_map(int t, int& nbr, int& local, int* tc) {
     }
 }
 
-// stencil offsets of node s (last axis fastest) -> tile index relative to the local cell
+// stencil offsets of node s (last axis fastest) -> LDS tile index of that node for local cell l
 template <int D>
 __device__ inline int tile_index(const int* l, int s, int* off) {
     using Dm = Dims<D>;
-    constexpr int TE = Dm::TE;
     if constexpr (D == 3) {
         off[0] = s / 9;
         off[1] = (s / 3) % 3;
         off[2] = s % 3;
-        return ((l[0] + off[0]) * TE + (l[1] + off[1])) * TE + (l[2] + off[2]);
+        return (l[0] + off[0]) * Dm::LX + (l[1] + off[1]) * Dm::LY + (l[2] + off[2]);
     } else {
         off[0] = s / 3;
         off[1] = s % 3;
-        return (l[0] + off[0]) * TE + (l[1] + off[1]);
+        return (l[0] + off[0]) * Dm::LX + (l[1] + off[1]);
     }
+}
+
+// LDS index of tile node tc (tile coordinates 0..TE-1)
+template <int D>
+__device__ inline int lds_index(const int* tc) {
+    using Dm = Dims<D>;
+    if constexpr (D == 3) return tc[0] * Dm::LX + tc[1] * Dm::LY + tc[2];
+    else return tc[0] * Dm::LX + tc[1];
 }
 
 // Weight (Tools/Influence_Iterator.h:85-90): left-to-right product over axes.
@@ -150,14 +157,6 @@ __device__ inline bool particle_weights(const KP& p, int i, const int* bb, float
         ok = ok && l[a] >= 0 && l[a] < B;
     }
     return ok;
-}
-
-// Writes the C-channel LDS tile of block b to its tile slot (plain stores).
-template <int D, int C>
-__device__ inline void store_tile(const KP& p, int b, const float* lds) {
-    constexpr int TN = Dims<D>::TN;
-    float* dst = p.tiles + (int64_t)b * C * TN;
-    for (int t = threadIdx.x; t < C * TN; t += (int)blockDim.x) dst[t] = lds[t];
 }
 
 // Sum, for node n, of channel c over every tile covering it, in a fixed
@@ -541,15 +540,20 @@ constexpr int BWG = 64 * NW;
 // issue order).
 #define MPM_STEP_FENCE() __asm__ __volatile__("" ::: "memory")
 
-// Sum of the NW wave tiles (C channels each) of block b into its tile slot.
+// Sum of the NW wave tiles (C channels each, LDS layout) of block b into
+// its tile slot (compact layout, node (x,y,z) at (x*TE + y)*TE + z).
 template <int D, int C>
 __device__ inline void store_wave_tiles(const KP& p, int b, const float* lds) {
-    constexpr int TN = Dims<D>::TN;
+    constexpr int TN = Dims<D>::TN, LTN = Dims<D>::LTN;
     float* dst = p.tiles + (int64_t)b * C * TN;
     for (int t = threadIdx.x; t < C * TN; t += BWG) {
+        const int c = t / TN;
+        int nb, local, tc[D];
+        tile_map<D>(t - c * TN, nb, local, tc);
+        const int li = c * LTN + lds_index<D>(tc);
         float v = 0.0f;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) v += lds[w * C * TN + t];
+        for (int w = 0; w < NW; ++w) v += lds[w * C * LTN + li];
         dst[t] = v;
     }
 }
@@ -788,8 +792,8 @@ __device__ inline void apply_operator(const float* rec, const float* T, float* A
 // corner (its cell - 1 in every axis).
 template <int D>
 __device__ inline void grad_gather(const float (*w)[3], const float (*dw)[3], int base,
-                                   const float (*ut)[Dims<D>::TN], float* T) {
-    constexpr int TE = Dims<D>::TE;
+                                   const float (*ut)[Dims<D>::LTN], float* T) {
+    constexpr int LX = Dims<D>::LX, LY = Dims<D>::LY;
     if constexpr (D == 3) {
         float y00[3][3], y10[3][3], y01[3][3];
 #pragma unroll
@@ -798,7 +802,7 @@ __device__ inline void grad_gather(const float (*w)[3], const float (*dw)[3], in
             for (int cc = 0; cc < 3; ++cc) y00[i][cc] = y10[i][cc] = y01[i][cc] = 0.0f;
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
-                const int t0 = base + (i * TE + j) * TE;
+                const int t0 = base + i * LX + j * LY;
 #pragma unroll
                 for (int cc = 0; cc < 3; ++cc) {
                     const float u0 = ut[cc][t0], u1 = ut[cc][t0 + 1], u2 = ut[cc][t0 + 2];
@@ -820,7 +824,7 @@ __device__ inline void grad_gather(const float (*w)[3], const float (*dw)[3], in
         float z0[3][2], z1[3][2];
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            const int t0 = base + i * TE;
+            const int t0 = base + i * LX;
 #pragma unroll
             for (int cc = 0; cc < 2; ++cc) {
                 const float u0 = ut[cc][t0], u1 = ut[cc][t0 + 1], u2 = ut[cc][t0 + 2];
@@ -842,7 +846,7 @@ __device__ inline void grad_gather(const float (*w)[3], const float (*dw)[3], in
 // most one lane per step; the compiler fence keeps the steps in order.
 template <int D>
 __device__ inline void grad_scatter(const float (*w)[3], const float (*dw)[3], int base, const float* A, float* q) {
-    constexpr int TE = Dims<D>::TE, TN = Dims<D>::TN;
+    constexpr int LX = Dims<D>::LX, LY = Dims<D>::LY, TN = Dims<D>::LTN;
     if constexpr (D == 3) {
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
@@ -858,14 +862,17 @@ __device__ inline void grad_scatter(const float (*w)[3], const float (*dw)[3], i
                 float ya[3], yb[3];
 #pragma unroll
                 for (int cc = 0; cc < 3; ++cc) {
-                    ya[cc] = w[1][j] * x0[cc] + dw[1][j] * x1[cc];
+                    ya[cc] = fmaf(w[1][j], x0[cc], dw[1][j] * x1[cc]);
                     yb[cc] = w[1][j] * x2[cc];
                 }
-                const int t0 = base + (i * TE + j) * TE;
+                const int t0 = base + i * LX + j * LY;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
 #pragma unroll
-                    for (int cc = 0; cc < 3; ++cc) q[cc * TN + t0 + k] += w[2][k] * ya[cc] + dw[2][k] * yb[cc];
+                    for (int cc = 0; cc < 3; ++cc) {
+                        float* dst = q + cc * TN + t0 + k;
+                        *dst = fmaf(w[2][k], ya[cc], fmaf(dw[2][k], yb[cc], *dst));
+                    }
                     MPM_STEP_FENCE();
                 }
             }
@@ -879,11 +886,14 @@ __device__ inline void grad_scatter(const float (*w)[3], const float (*dw)[3], i
                 x0[cc] = A[0 * 2 + cc] * dw[0][i];
                 x1[cc] = A[1 * 2 + cc] * w[0][i];
             }
-            const int t0 = base + i * TE;
+            const int t0 = base + i * LX;
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
 #pragma unroll
-                for (int cc = 0; cc < 2; ++cc) q[cc * TN + t0 + j] += w[1][j] * x0[cc] + dw[1][j] * x1[cc];
+                for (int cc = 0; cc < 2; ++cc) {
+                    float* dst = q + cc * TN + t0 + j;
+                    *dst = fmaf(w[1][j], x0[cc], fmaf(dw[1][j], x1[cc], *dst));
+                }
                 MPM_STEP_FENCE();
             }
         }
@@ -896,16 +906,16 @@ __device__ inline void grad_scatter(const float (*w)[3], const float (*dw)[3], i
 template <int D>
 __global__ __launch_bounds__(BWG) void k_p2g(KP p) {
     using Dm = Dims<D>;
-    constexpr int TN = Dm::TN, ST = Dm::ST, C = D + 1;
-    __shared__ float tile[NW * C * TN];
+    constexpr int LTN = Dm::LTN, ST = Dm::ST, C = D + 1;
+    __shared__ float tile[NW * C * LTN];
     const int nblocks = p.ctr->nblocks;
     const int c = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    float* mine = tile + wv * C * TN;
+    float* mine = tile + wv * C * LTN;
     int l[D];
     local_coords<D>(c, l);
     const XcdRange xr = xcd_blocks(nblocks);
     for (int b = xr.first; b < xr.end; b += xr.step) {
-        for (int t = threadIdx.x; t < NW * C * TN; t += BWG) tile[t] = 0.0f;
+        for (int t = threadIdx.x; t < NW * C * LTN; t += BWG) tile[t] = 0.0f;
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
         __syncthreads();
@@ -924,7 +934,7 @@ __global__ __launch_bounds__(BWG) void k_p2g(KP p) {
                 const float wt = stencil_weight<D>(w, off);
                 mine[t] += wt * m;
 #pragma unroll
-                for (int a = 0; a < D; ++a) mine[(1 + a) * TN + t] += wt * mv[a];
+                for (int a = 0; a < D; ++a) mine[(1 + a) * LTN + t] += wt * mv[a];
                 MPM_STEP_FENCE();
             }
         }
@@ -955,14 +965,15 @@ __global__ __launch_bounds__(WG) void k_normalize(KP p) {
 
 // gather of `nch` grid channel arrays into an LDS tile
 template <int D, int NCH>
-__device__ inline void gather_tile(const KP& p, const int* nbr, float (*tile)[Dims<D>::TN], const float* const* src) {
+__device__ inline void gather_tile(const KP& p, const int* nbr, float (*tile)[Dims<D>::LTN], const float* const* src) {
     constexpr int TN = Dims<D>::TN;
     for (int t = threadIdx.x; t < TN; t += (int)blockDim.x) {
         int nb, local, tc[D];
         tile_map<D>(t, nb, local, tc);
         const int blk = nbr[nb];
+        const int li = lds_index<D>(tc);
 #pragma unroll
-        for (int c = 0; c < NCH; ++c) tile[c][t] = blk >= 0 ? src[c][(int64_t)blk * 64 + local] : 0.0f;
+        for (int c = 0; c < NCH; ++c) tile[c][li] = blk >= 0 ? src[c][(int64_t)blk * 64 + local] : 0.0f;
     }
 }
 
@@ -970,8 +981,8 @@ __device__ inline void gather_tile(const KP& p, const int* nbr, float (*tile)[Di
 template <int D>
 __global__ __launch_bounds__(BWG) void k_init_volume(KP p) {
     using Dm = Dims<D>;
-    constexpr int TN = Dm::TN, NBR = Dm::NBR;
-    __shared__ float tile[1][TN];
+    constexpr int LTN = Dm::LTN, NBR = Dm::NBR;
+    __shared__ float tile[1][LTN];
     __shared__ int nbr[NBR];
     const int nblocks = p.ctr->nblocks;
     float vol = 1.0f;
@@ -1012,11 +1023,11 @@ __global__ __launch_bounds__(BWG) void k_init_volume(KP p) {
 template <int D, bool CONST, bool FORCE>
 __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
     using Dm = Dims<D>;
-    constexpr int TN = Dm::TN, DD = D * D, ST = Dm::ST;
-    __shared__ float tile[NW * D * TN];
+    constexpr int LTN = Dm::LTN, DD = D * D, ST = Dm::ST;
+    __shared__ float tile[NW * D * LTN];
     const int nblocks = p.ctr->nblocks;
     const int c = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    float* mine = tile + wv * D * TN;
+    float* mine = tile + wv * D * LTN;
     int l[D];
     local_coords<D>(c, l);
     const XcdRange xr = xcd_blocks(nblocks);
@@ -1024,7 +1035,7 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
         if (FORCE) {
-            for (int t = threadIdx.x; t < NW * D * TN; t += BWG) tile[t] = 0.0f;
+            for (int t = threadIdx.x; t < NW * D * LTN; t += BWG) tile[t] = 0.0f;
             __syncthreads();
         }
         const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
@@ -1086,7 +1097,7 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
                         float inner = 0.0f;
 #pragma unroll
                         for (int k = 0; k < D; ++k) inner += A[a * D + k] * g[k];
-                        mine[a * TN + t] += gm[a] * wt - inner;
+                        mine[a * LTN + t] += gm[a] * wt - inner;
                     }
                     MPM_STEP_FENCE();
                 }
@@ -1188,15 +1199,15 @@ __global__ __launch_bounds__(WG) void k_grid_update(KP p) {
 template <int D, bool RESTART>
 __global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
     using Dm = Dims<D>;
-    constexpr int TN = Dm::TN, NBR = Dm::NBR, DD = D * D;
-    __shared__ float ut[D][TN];
-    __shared__ float qt[NW * D * TN];
+    constexpr int LTN = Dm::LTN, NBR = Dm::NBR, DD = D * D;
+    __shared__ float ut[D][LTN];
+    __shared__ float qt[NW * D * LTN];
     __shared__ int nbr[NBR];
     if (p.cg->done) return;
     const int nblocks = p.ctr->nblocks;
     const float* u = RESTART ? p.g_vs : p.g_s;
     const int c = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    float* mine = qt + wv * D * TN;
+    float* mine = qt + wv * D * LTN;
     int l[D];
     local_coords<D>(c, l);
     int base;  // tile index of this cell's stencil corner
@@ -1209,7 +1220,7 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
         if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
         slot_coords<D>(p.block_slot[b], p.nb, bb);
-        for (int t = threadIdx.x; t < NW * D * TN; t += BWG) qt[t] = 0.0f;
+        for (int t = threadIdx.x; t < NW * D * LTN; t += BWG) qt[t] = 0.0f;
         __syncthreads();
         const float* src[D];
 #pragma unroll
@@ -1447,8 +1458,8 @@ __global__ __launch_bounds__(WG) void k_cg_finalize(KP p, int k_end) {
 template <int D>
 __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
     using Dm = Dims<D>;
-    constexpr int TN = Dm::TN, NBR = Dm::NBR, DD = D * D;
-    __shared__ float vt[2 * D][TN];
+    constexpr int LTN = Dm::LTN, NBR = Dm::NBR, DD = D * D;
+    __shared__ float vt[2 * D][LTN];
     __shared__ int nbr[NBR];
     const int nblocks = p.ctr->nblocks;
     const float dt = p.dt, flip = p.flip;
