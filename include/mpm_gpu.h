@@ -212,6 +212,13 @@ int mpm_enable_timing(mpm_handle h, int32_t on);
 int mpm_kernel_times(mpm_handle h, int32_t nclasses, double* ms, int64_t* launches);
 const char* mpm_kernel_class_name(int32_t cls);
 int32_t mpm_kernel_class_count(void);
+/* Benchmark support (no counterpart in the reference): launches the CG
+ * operator application (MPM_CG_System::Force, MPM_CG_System.h:150-215) `reps`
+ * times back to back on the current particle state and search direction,
+ * bracketed by two HIP events on the handle's stream, and returns the
+ * average milliseconds per launch.  Needs a completed substep; leaves the
+ * state of that substep unchanged except the operator's tile buffer. */
+int mpm_time_cg_operator(mpm_handle h, int32_t reps, float* avg_ms);
 
 /* ---- scenes (scene hook Initialize_Particles) ------------------------- */
 /* Fills desc with the preset of a named scene and returns the particle

@@ -105,6 +105,7 @@ SIGNATURES = {
     "mpm_kernel_times": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(ctypes.c_double), _LP]),
     "mpm_kernel_class_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "mpm_kernel_class_count": (ctypes.c_int32, []),
+    "mpm_time_cg_operator": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
     "mpm_scene_preset": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint64,
                                         ctypes.POINTER(SceneDesc), _LP]),
     "mpm_scene_generate": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64,
@@ -291,6 +292,12 @@ class Scene:
     # -- timing ------------------------------------------------------------
     def enable_timing(self, on: bool = True) -> None:
         _check(lib().mpm_enable_timing(self._h, 1 if on else 0))
+
+    def time_cg_operator(self, reps: int = 20) -> float:
+        """Average ms of one CG operator launch, timed back to back with HIP events."""
+        ms = ctypes.c_float()
+        _check(lib().mpm_time_cg_operator(self._h, int(reps), ctypes.byref(ms)))
+        return ms.value
 
     def kernel_times(self) -> Dict[str, Tuple[float, int]]:
         names = kernel_class_names()

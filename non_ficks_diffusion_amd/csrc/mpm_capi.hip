@@ -65,7 +65,8 @@ struct mpm_context {
     mpm::CGState* cg = nullptr;
     double* red = nullptr;
     unsigned* wg_part = nullptr;
-    int cg_predict = 0;  // iterations launched before the done flag was seen, last solve
+    int cg_predict = 0;
+    float last_dt = 0.0f;  // dt of the last substep (operator timing)  // iterations launched before the done flag was seen, last solve
     float* tiles = nullptr;
     int* h_done = nullptr;  // pinned
     hipEvent_t done_ev[2] = {nullptr, nullptr};
@@ -566,6 +567,7 @@ int mpm_substep(mpm_handle c, float dt, mpm_step_stats* stats) {
     if (!(dt > 0.0f) || !std::isfinite(dt)) return mpm_set_error(MPM_E_INVALID, "mpm_substep: dt must be positive");
     int rc = c->D == 3 ? substep<3>(c, dt) : substep<2>(c, dt);
     if (rc) return rc;
+    c->last_dt = dt;
     rc = check_launch("mpm_substep");
     if (rc) return rc;
     rc = collect_timing(c);
@@ -788,6 +790,38 @@ int mpm_kernel_times(mpm_handle c, int32_t nclasses, double* ms, int64_t* launch
         if (ms) ms[k] = c->ms[k];
         if (launches) launches[k] = c->launches[k];
     }
+    return MPM_OK;
+}
+
+int mpm_time_cg_operator(mpm_handle c, int32_t reps, float* avg_ms) {
+    if (!c || !avg_ms || reps <= 0) return mpm_set_error(MPM_E_INVALID, "mpm_time_cg_operator: bad argument");
+    if (c->last_dt <= 0.0f) return mpm_set_error(MPM_E_NOT_READY, "mpm_time_cg_operator: run a substep first");
+    // the operator kernel returns at once after a finished solve: clear the
+    // done flag for the timed launches and restore it afterwards
+    int done = 0;
+    HIPCHECK(hipMemcpyAsync(&done, &c->cg->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipMemsetAsync(&c->cg->done, 0, sizeof(int), c->stream));
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    mpm::KP p = make_kp(c, c->last_dt);
+    HIPCHECK(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) {
+        if (c->D == 3) mpm::launch_cg_force<3>(p, c->stream, c->grid_block, false);
+        else mpm::launch_cg_force<2>(p, c->stream, c->grid_block, false);
+    }
+    HIPCHECK(hipEventRecord(e1, c->stream));
+    HIPCHECK(hipEventSynchronize(e1));
+    float ms = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    HIPCHECK(hipEventDestroy(e0));
+    HIPCHECK(hipEventDestroy(e1));
+    HIPCHECK(hipMemcpyAsync(&c->cg->done, &done, sizeof(int), hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    int rc = check_launch("mpm_time_cg_operator");
+    if (rc) return rc;
+    *avg_ms = ms / (float)reps;
     return MPM_OK;
 }
 

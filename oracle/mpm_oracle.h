@@ -15,6 +15,34 @@
  *
  * Particle arrays use the same layout as include/mpm_gpu.h (particle
  * major; F row-major per particle).
+ *
+ * How each part is pinned (the reference ships no golden vectors, and it
+ * cannot be built without Nova; tests/test_oracle_pins.py holds the checks):
+ *   weights          closed forms of MPM_Particle.h:90-100, partition of
+ *                    unity, linear reproduction, sum of gradients = 0
+ *   SVD              F = U S V^T, orthogonality, det U = det V = +1,
+ *                    ordering, sign on the smallest value, special cases,
+ *                    against numpy's SVD
+ *   plasticity       diagonal-F closed form, Jp update identity
+ *   P, dP            P = dPsi/dF and dP = directional derivative of P by
+ *                    finite differences; dP symmetric
+ *   force map        symmetric, block structure in the SVD frame
+ *   CG operator      M-symmetric; solve matches a dense numpy solve; rigid
+ *                    translation maps to itself
+ *   collision        closed form of Grid_Based_Collision_Helper.h:72-80
+ *   P2G / G2P        mass and momentum conservation; G2P reproduces a linear
+ *                    velocity field; free fall without elasticity
+ *   binning          every stencil node has exactly one slab owner for any
+ *                    thread count
+ *   driver dt        sequence of MPM_Driver.cpp:55-80 by hand
+ * Parity unpinned (no reference behaviour to check against; the readings
+ * taken are the scene options of mpm_gpu.h):
+ *   - Project_Helper's predicate (Project_Helper.h:39, Nova operator<):
+ *     project_mode 1 (no-op) by default, 0 = zero inside planes;
+ *   - Nova CG internals beyond the solution: the stop-test order, restart
+ *     at iteration 0, s.q <= 0 handling and the Multiply count;
+ *   - SPGrid handling of stencil writes at ghost index -1 (crop_ghost_x);
+ *   - Time_At_Frame's frame rate (24 fps taken).
  */
 #ifndef MPM_ORACLE_H
 #define MPM_ORACLE_H
