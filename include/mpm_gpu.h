@@ -220,6 +220,36 @@ int32_t mpm_kernel_class_count(void);
  * state of that substep unchanged except the operator's tile buffer. */
 int mpm_time_cg_operator(mpm_handle h, int32_t reps, float* avg_ms);
 
+/* ---- batches of scenes ---------------------------------------------------
+ * No counterpart in the reference, which runs one MPM_Example per process.
+ * A batch is many independent scenes of one dimension that advance together:
+ * every kernel of a substep covers all of them (their block slots and
+ * particles are concatenated with per-scene offsets, and each block reads
+ * its scene's parameters from a scene table), and every scene runs its own
+ * CG solve with its own dot products.  A handle from mpm_create is a batch
+ * of one; every single-scene call above acts on scene 0. */
+int mpm_batch_create(const mpm_scene_desc* descs, const int64_t* capacities, int32_t nscenes,
+                     mpm_handle* out);
+int32_t mpm_batch_size(mpm_handle h);
+/* Particles of one scene (as mpm_upload_particles); every scene is uploaded
+ * before mpm_initialize, which initialises the whole batch. */
+int mpm_batch_upload(mpm_handle h, int32_t scene, int64_t n, const float* x, const float* v,
+                     const float* F, const float* Jp, const float* mass, const float* volume);
+/* One substep of every scene s with dt[s]; dt[s] <= 0 leaves scene s
+ * unchanged (MPM_Driver::Advance_Step per scene). */
+int mpm_batch_substep(mpm_handle h, const float* dt);
+/* Max |V| per scene (nscenes values) after the last substep. */
+int mpm_batch_max_velocity(mpm_handle h, float* out);
+int mpm_batch_stats(mpm_handle h, int32_t scene, mpm_step_stats* stats);
+int mpm_batch_download(mpm_handle h, int32_t scene, int64_t cap, float* x, float* v, float* F,
+                       float* Jp, float* mass, float* volume, int32_t* ids, int64_t* n_out);
+/* MPM_Driver::Advance_To_Target_Time (MPM_Driver.cpp:55-80) for every scene:
+ * each scene takes the reference's dt rule from its own max speed, and
+ * scenes that reach their target idle while the others finish.  time_out
+ * and substeps_out (nscenes entries each) may be NULL. */
+int mpm_batch_advance_to_time(mpm_handle h, const double* time, const double* target,
+                              double* time_out, int32_t* substeps_out);
+
 /* ---- scenes (scene hook Initialize_Particles) ------------------------- */
 /* Fills desc with the preset of a named scene and returns the particle
  * count it generates.  Known names: "disk2d", "two_balls",

@@ -106,6 +106,15 @@ SIGNATURES = {
     "mpm_kernel_class_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "mpm_kernel_class_count": (ctypes.c_int32, []),
     "mpm_time_cg_operator": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
+    "mpm_batch_create": (ctypes.c_int, [ctypes.POINTER(SceneDesc), _LP, ctypes.c_int32, ctypes.POINTER(_H)]),
+    "mpm_batch_size": (ctypes.c_int32, [_H]),
+    "mpm_batch_upload": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP]),
+    "mpm_batch_substep": (ctypes.c_int, [_H, _FP]),
+    "mpm_batch_max_velocity": (ctypes.c_int, [_H, _FP]),
+    "mpm_batch_stats": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(StepStats)]),
+    "mpm_batch_download": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _IP, _LP]),
+    "mpm_batch_advance_to_time": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                                                 ctypes.POINTER(ctypes.c_double), _IP]),
     "mpm_scene_preset": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint64,
                                         ctypes.POINTER(SceneDesc), _LP]),
     "mpm_scene_generate": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64,
@@ -298,6 +307,95 @@ class Scene:
         ms = ctypes.c_float()
         _check(lib().mpm_time_cg_operator(self._h, int(reps), ctypes.byref(ms)))
         return ms.value
+
+    def kernel_times(self) -> Dict[str, Tuple[float, int]]:
+        names = kernel_class_names()
+        ms = (ctypes.c_double * len(names))()
+        n = (ctypes.c_int64 * len(names))()
+        _check(lib().mpm_kernel_times(self._h, len(names), ms, n))
+        return {names[k]: (ms[k], n[k]) for k in range(len(names))}
+
+
+class Batch:
+    """Many independent scenes of one dimension advanced together: every
+    kernel of a substep covers all of them, and each runs its own CG."""
+
+    def __init__(self, descs, capacities):
+        self.descs = [d.copy() for d in descs]
+        self.n = len(self.descs)
+        self.dim = self.descs[0].dim
+        self.capacities = [int(c) for c in capacities]
+        arr = (SceneDesc * self.n)(*self.descs)
+        caps = (ctypes.c_int64 * self.n)(*self.capacities)
+        h = _H()
+        _check(lib().mpm_batch_create(arr, caps, self.n, ctypes.byref(h)))
+        self._h = h
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().mpm_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def upload(self, scene: int, x, v, mass, F=None, Jp=None, volume=None) -> None:
+        x, v, mass = _f32(x), _f32(v), _f32(mass)
+        F = None if F is None else _f32(F)
+        Jp = None if Jp is None else _f32(Jp)
+        volume = None if volume is None else _f32(volume)
+        _check(lib().mpm_batch_upload(self._h, int(scene), mass.shape[0], _fp(x), _fp(v), _fp(F), _fp(Jp), _fp(mass),
+                                      _fp(volume)))
+
+    def initialize(self) -> None:
+        _check(lib().mpm_initialize(self._h))
+
+    def substep(self, dt) -> None:
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float32), (self.n,)), dtype=np.float32)
+        _check(lib().mpm_batch_substep(self._h, _fp(d)))
+
+    def max_velocity(self) -> np.ndarray:
+        out = np.zeros(self.n, np.float32)
+        _check(lib().mpm_batch_max_velocity(self._h, _fp(out)))
+        return out
+
+    def stats(self, scene: int) -> Dict[str, float]:
+        s = StepStats()
+        _check(lib().mpm_batch_stats(self._h, int(scene), ctypes.byref(s)))
+        return s.as_dict()
+
+    def download(self, scene: int, cap: Optional[int] = None) -> Dict[str, np.ndarray]:
+        d = self.dim
+        if cap is None:
+            cap = self.capacities[scene]
+        out = {"x": np.zeros((cap, d), np.float32), "v": np.zeros((cap, d), np.float32),
+               "F": np.zeros((cap, d, d), np.float32), "Jp": np.zeros(cap, np.float32),
+               "mass": np.zeros(cap, np.float32), "volume": np.zeros(cap, np.float32),
+               "ids": np.zeros(cap, np.int32)}
+        n = ctypes.c_int64()
+        _check(lib().mpm_batch_download(self._h, int(scene), cap, _fp(out["x"]), _fp(out["v"]), _fp(out["F"]),
+                                        _fp(out["Jp"]), _fp(out["mass"]), _fp(out["volume"]),
+                                        out["ids"].ctypes.data_as(_IP), ctypes.byref(n)))
+        return {k: a[:n.value] for k, a in out.items()}
+
+    def advance_to_time(self, time, target):
+        t = np.ascontiguousarray(np.broadcast_to(np.asarray(time, np.float64), (self.n,)), dtype=np.float64)
+        g = np.ascontiguousarray(np.broadcast_to(np.asarray(target, np.float64), (self.n,)), dtype=np.float64)
+        t_out = np.zeros(self.n, np.float64)
+        steps = np.zeros(self.n, np.int32)
+        dp = ctypes.POINTER(ctypes.c_double)
+        _check(lib().mpm_batch_advance_to_time(self._h, t.ctypes.data_as(dp), g.ctypes.data_as(dp),
+                                               t_out.ctypes.data_as(dp), steps.ctypes.data_as(_IP)))
+        return t_out, steps
+
+    def synchronize(self) -> None:
+        _check(lib().mpm_synchronize(self._h))
+
+    def enable_timing(self, on: bool = True) -> None:
+        _check(lib().mpm_enable_timing(self._h, 1 if on else 0))
 
     def kernel_times(self) -> Dict[str, Tuple[float, int]]:
         names = kernel_class_names()

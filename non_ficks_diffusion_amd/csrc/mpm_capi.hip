@@ -1,5 +1,6 @@
-// Host side of libmpm_gpu: scene context, device memory, launch sequence
-// of the substep and the C ABI of include/mpm_gpu.h.
+// Host side of libmpm_gpu: batch context (a single scene is a batch of one),
+// device memory, launch sequence of the substep and the C ABI of
+// include/mpm_gpu.h.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -40,39 +41,56 @@ const char* kClassNames[KC_COUNT] = {"bin",          "scan",         "block_setu
                                      "normalize",    "init_volume",  "constitutive_force", "grid_update",
                                      "cg_force",     "cg_node",      "g2p",         "max_speed"};
 
+// Host view of one scene of a batch.
+struct SceneHost {
+    mpm_scene_desc desc{};
+    int64_t capacity = 0;  // particle entries reserved for the scene
+    int64_t pbase = 0;     // its range in the particle buffer before the first binning
+    int64_t n_uploaded = 0;
+    bool uploaded = false;
+    float last_dt = 0.0f;
+};
+
 }  // namespace
 
 struct mpm_context {
-    mpm_scene_desc desc{};
     int D = 3;
+    int nscenes = 0;
+    std::vector<SceneHost> sc;
+    std::vector<mpm::SceneParams> table;  // host copy of the scene table
     int64_t pcap = 0;
-    int bcap = 0, nb = 0, nslots = 0, nparts = 0;
+    int bcap = 0, nslots = 0, nparts = 0;
     int64_t gcap = 0;
     hipStream_t stream = nullptr;
     float* pbuf[2] = {nullptr, nullptr};
     int* idbuf[2] = {nullptr, nullptr};
     int* keybuf[2] = {nullptr, nullptr};
+    int* scbuf[2] = {nullptr, nullptr};
     int cur = 0;
     int* prank = nullptr;
     float* rcache = nullptr;
     float* pop = nullptr;
     int *slot_count = nullptr, *slot_touched = nullptr, *slot_active = nullptr;
-    int *block_slot = nullptr, *block_pstart = nullptr, *block_nbr = nullptr, *scan_part = nullptr;
-    int *cell_count = nullptr, *cell_start = nullptr;
+    int *block_slot = nullptr, *block_scene = nullptr, *block_pstart = nullptr, *block_nbr = nullptr;
+    int *scan_part = nullptr, *cell_count = nullptr, *cell_start = nullptr;
     float *g_m = nullptr, *g_v = nullptr, *g_vs = nullptr, *g_q = nullptr, *g_s = nullptr, *g_r = nullptr;
     unsigned* g_flags = nullptr;
-    mpm::Counters* ctr = nullptr;
-    mpm::CGState* cg = nullptr;
-    double* red = nullptr;
-    unsigned* wg_part = nullptr;
-    int cg_predict = 0;
-    float last_dt = 0.0f;  // dt of the last substep (operator timing)  // iterations launched before the done flag was seen, last solve
     float* tiles = nullptr;
+    mpm::Counters* ctr = nullptr;
+    mpm::SceneParams* scn = nullptr;
+    mpm::SceneState* sst = nullptr;
+    mpm::CGState* cg = nullptr;
+    int* slot_base = nullptr;
+    double *bsum = nullptr, *bmax = nullptr;
+    unsigned* bu = nullptr;
+    int cg_predict = 0;    // iterations launched before the done count was seen, last solve
     int* h_done = nullptr;  // pinned
     hipEvent_t done_ev[2] = {nullptr, nullptr};
     int grid_particle = 0, grid_block = 0, grid_node = 0;
-    bool have_particles = false, waiting = false, initialized = false;
-    bool binned = false;  // particles sit in the block ranges of the last binning
+    bool initialized = false;
+    bool binned = false;       // particles sit in the block ranges of the last binning
+    bool ever_binned = false;  // particle ranges are the binned ones (not the upload ranges)
+    bool stepped = false;      // a substep has run (operator timing needs its state)
     std::vector<void*> allocations;
     // timing
     bool timing = false;
@@ -99,13 +117,15 @@ int dalloc(mpm_context* c, T** p, size_t count) {
 }
 
 int nfields(int D) { return 2 * D + D * D + 3; }
+int block_edge(int D) { return D == 3 ? 4 : 8; }
+int slots_per_axis(int D, int counts) { const int B = block_edge(D); return (counts + 2 * B) / B + 1; }
 
-mpm::KP make_kp(const mpm_context* c, float dt) {
-    mpm::KP p{};
-    const mpm_scene_desc& d = c->desc;
+mpm::SceneParams scene_params(const mpm_scene_desc& d, int slot_base, int nslots) {
+    mpm::SceneParams p{};
     p.counts = d.counts;
-    p.nb = c->nb;
-    p.nslots = c->nslots;
+    p.nb = slots_per_axis(d.dim, d.counts);
+    p.slot_base = slot_base;
+    p.nslots = nslots;
     p.dx = 1.0f / (float)d.counts;
     p.inv_dx = (float)d.counts;
     p.mat.mu0 = d.material.mu0;
@@ -130,20 +150,37 @@ mpm::KP make_kp(const mpm_context* c, float dt) {
     p.cg_tol = (double)d.cg_tolerance;
     p.cg_max = d.cg_max_iterations;
     p.cg_min = d.cg_min_iterations;
+    p.dt = 0.0f;
+    p.active = 0;
+    p.waiting = 0;
+    return p;
+}
+
+mpm::KP make_kp(const mpm_context* c) {
+    mpm::KP p{};
+    p.scn = c->scn;
+    p.sst = c->sst;
+    p.cg = c->cg;
+    p.slot_base = c->slot_base;
+    p.nscenes = c->nscenes;
     p.pcur = c->pbuf[c->cur];
     p.palt = c->pbuf[c->cur ^ 1];
     p.idcur = c->idbuf[c->cur];
     p.idalt = c->idbuf[c->cur ^ 1];
     p.keycur = c->keybuf[c->cur];
     p.keyalt = c->keybuf[c->cur ^ 1];
+    p.scur = c->scbuf[c->cur];
+    p.salt = c->scbuf[c->cur ^ 1];
     p.prank = c->prank;
     p.rcache = c->rcache;
     p.pop = c->pop;
     p.pcap = c->pcap;
+    p.nslots = c->nslots;
     p.slot_count = c->slot_count;
     p.slot_touched = c->slot_touched;
     p.slot_active = c->slot_active;
     p.block_slot = c->block_slot;
+    p.block_scene = c->block_scene;
     p.block_pstart = c->block_pstart;
     p.block_nbr = c->block_nbr;
     p.cell_count = c->cell_count;
@@ -159,15 +196,11 @@ mpm::KP make_kp(const mpm_context* c, float dt) {
     p.g_r = c->g_r;
     p.g_flags = c->g_flags;
     p.gcap = c->gcap;
-    p.ctr = c->ctr;
-    p.cg = c->cg;
-    p.red = c->red;
-    p.red_stride = c->grid_node;
-    p.red_count = c->grid_node;
-    p.wg_part = c->wg_part;
-    p.wg_part_count = c->grid_block;
     p.tiles = c->tiles;
-    p.dt = dt;
+    p.ctr = c->ctr;
+    p.bsum = c->bsum;
+    p.bmax = c->bmax;
+    p.bu = c->bu;
     return p;
 }
 
@@ -206,55 +239,86 @@ int collect_timing(mpm_context* c) {
     return MPM_OK;
 }
 
+// dt and activity of every scene for the next kernels.  dts == nullptr: every
+// scene gets dt_all; force_active makes every scene active whatever its dt
+// (grid build, stage-by-stage calls).
+void set_step(mpm_context* c, const float* dts, float dt_all, bool force_active, bool clear_waiting) {
+    const mpm::KP p = make_kp(c);
+    for (int first = 0; first < c->nscenes; first += mpm::STEP_ARGS_MAX) {
+        mpm::StepArgs a{};
+        a.first = first;
+        a.n = std::min(mpm::STEP_ARGS_MAX, c->nscenes - first);
+        a.force_active = force_active ? 1 : 0;
+        a.clear_waiting = clear_waiting ? 1 : 0;
+        for (int i = 0; i < a.n; ++i) a.dt[i] = dts ? dts[first + i] : dt_all;
+        mpm::launch_set_step(p, c->stream, a);
+    }
+}
+
 // ---- stage sequences --------------------------------------------------------
 template <int D>
 void stage_bin(mpm_context* c) {
-    mpm::KP p = make_kp(c, 0.0f);
+    mpm::KP p = make_kp(c);
     timed(c, KC_BIN, [&] { mpm::launch_bin<D>(p, c->stream, c->grid_particle); });
-    timed(c, KC_SCAN, [&] { mpm::launch_scan<D>(p, c->stream); });
-    timed(c, KC_SETUP, [&] { mpm::launch_block_setup<D>(p, c->stream, c->grid_block); });
+    timed(c, KC_SCAN, [&] { mpm::launch_scan(p, c->stream); });
+    timed(c, KC_SETUP, [&] {
+        mpm::launch_block_setup<D>(p, c->stream, c->grid_block);
+        mpm::launch_scene_ranges(p, c->stream);
+    });
     timed(c, KC_REORDER, [&] { mpm::launch_reorder<D>(p, c->stream, c->grid_particle); });
     c->cur ^= 1;
     c->binned = true;
+    c->ever_binned = true;
 }
 
 template <int D>
 void stage_p2g(mpm_context* c) {
-    mpm::KP p = make_kp(c, 0.0f);
+    mpm::KP p = make_kp(c);
     timed(c, KC_P2G, [&] { mpm::launch_p2g<D>(p, c->stream, c->grid_block); });
     timed(c, KC_NORMALIZE, [&] { mpm::launch_normalize<D>(p, c->stream, c->grid_node); });
 }
 
 template <int D>
 void stage_waiting(mpm_context* c) {
-    if (!c->waiting) return;
-    mpm::KP p = make_kp(c, 0.0f);
+    mpm::KP p = make_kp(c);
     timed(c, KC_INIT_VOLUME, [&] { mpm::launch_init_volume<D>(p, c->stream, c->grid_block); });
-    c->waiting = false;
 }
 
-// Apply_Force's implicit solve (MPM_Example.cpp:700-711): CG set-up, restart
-// (r = b - A x, Project) and up to cg_max_iterations iterations.
+int max_cg_iterations(const mpm_context* c) {
+    int m = 0;
+    for (const SceneHost& s : c->sc) m = std::max(m, s.desc.cg_max_iterations);
+    return m;
+}
+
+// Apply_Force's implicit solve (MPM_Example.cpp:700-711) for every active
+// scene: CG set-up, restart (r = b - A x, Project) and iterations until every
+// scene's solve is done (converged or at its cap).
 template <int D>
-int stage_cg(mpm_context* c, float dt, bool do_init) {
-    mpm::KP p = make_kp(c, dt);
+int stage_cg(mpm_context* c, bool do_init) {
+    mpm::KP p = make_kp(c);
     if (do_init)
         timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, false, false, true, false); });
-    timed(c, KC_CG_FORCE, [&] { mpm::launch_cg_force<D>(p, c->stream, c->grid_block, true); });
-    timed(c, KC_CG_NODE, [&] { mpm::launch_cg_restart_node<D>(p, c->stream, c->grid_node); });
-    const int max_it = c->desc.cg_max_iterations;
+    timed(c, KC_CG_FORCE, [&] { mpm::launch_cg_force<D>(p, c->stream, c->grid_block, true, 0); });
+    timed(c, KC_CG_NODE, [&] {
+        mpm::launch_cg_restart_node<D>(p, c->stream, c->grid_node);
+        mpm::launch_cg_reduce_rr(p, c->stream, 0);
+    });
+    const int max_it = max_cg_iterations(c);
     auto iteration = [&](int k) {
-        timed(c, KC_CG_NODE, [&] { mpm::launch_cg_update_s<D>(p, c->stream, c->grid_node, k); });
-        timed(c, KC_CG_FORCE, [&] { mpm::launch_cg_force<D>(p, c->stream, c->grid_block, false); });
-        timed(c, KC_CG_NODE, [&] { mpm::launch_cg_node1<D>(p, c->stream, c->grid_node); });
-        timed(c, KC_CG_NODE, [&] { mpm::launch_cg_node2<D>(p, c->stream, c->grid_node, k); });
+        timed(c, KC_CG_FORCE, [&] { mpm::launch_cg_force<D>(p, c->stream, c->grid_block, false, k); });
+        timed(c, KC_CG_NODE, [&] {
+            mpm::launch_cg_node1<D>(p, c->stream, c->grid_node, k);
+            mpm::launch_cg_reduce_sq(p, c->stream, k);
+            mpm::launch_cg_node2<D>(p, c->stream, c->grid_node);
+            mpm::launch_cg_reduce_rr(p, c->stream, k + 1);
+        });
     };
-    // Iterations after convergence return at once, so launching a few too
-    // many is harmless.  The first batch is the previous solve's iteration
-    // count; after it the host polls the done flag one chunk behind the GPU
+    // Iterations after every solve is done return at once, so launching a few
+    // too many is harmless.  The first batch is the previous solve's iteration
+    // count; after it the host polls the done count one chunk behind the GPU
     // (double-buffered events), so the queue never runs dry while it waits.
     const int chunk = 4;
-    int first = std::min(max_it, std::max(c->cg_predict, chunk));
+    const int first = std::min(max_it, std::max(c->cg_predict, chunk));
     int slot = 0, launched = 0;
     int boundary[2] = {0, 0};
     bool pending[2] = {false, false};
@@ -262,7 +326,7 @@ int stage_cg(mpm_context* c, float dt, bool do_init) {
         const int n = launched == 0 ? first : std::min(chunk, max_it - launched);
         for (int j = 0; j < n; ++j) iteration(launched + j);
         launched += n;
-        HIPCHECK(hipMemcpyAsync(&c->h_done[slot], &c->cg->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipMemcpyAsync(&c->h_done[slot], &c->ctr->cg_ndone, sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipEventRecord(c->done_ev[slot], c->stream));
         pending[slot] = true;
         boundary[slot] = launched;
@@ -270,7 +334,7 @@ int stage_cg(mpm_context* c, float dt, bool do_init) {
         if (pending[slot]) {
             HIPCHECK(hipEventSynchronize(c->done_ev[slot]));
             pending[slot] = false;
-            if (c->h_done[slot]) {
+            if (c->h_done[slot] >= c->nscenes) {
                 // done by the first boundary: try one fewer next time
                 c->cg_predict = boundary[slot] == first ? first - 1 : boundary[slot];
                 break;
@@ -278,24 +342,27 @@ int stage_cg(mpm_context* c, float dt, bool do_init) {
         }
     }
     if (launched >= max_it) c->cg_predict = max_it;
-    timed(c, KC_CG_NODE, [&] { mpm::launch_cg_finalize(p, c->stream, max_it); });
     return MPM_OK;
 }
 
 template <int D>
-void stage_g2p(mpm_context* c, float dt) {
-    mpm::KP p = make_kp(c, dt);
+void stage_g2p(mpm_context* c) {
+    mpm::KP p = make_kp(c);
     timed(c, KC_G2P, [&] { mpm::launch_g2p<D>(p, c->stream, c->grid_block); });
     c->binned = false;
 }
 
 template <int D>
 int run_stage(mpm_context* c, int stage, float dt) {
-    mpm::KP p = make_kp(c, dt);
+    set_step(c, nullptr, dt, true, false);
+    mpm::KP p = make_kp(c);
     switch (stage) {
         case MPM_STAGE_BIN: stage_bin<D>(c); break;
         case MPM_STAGE_P2G: stage_p2g<D>(c); break;
-        case MPM_STAGE_WAITING: stage_waiting<D>(c); break;
+        case MPM_STAGE_WAITING:
+            stage_waiting<D>(c);
+            set_step(c, nullptr, dt, true, true);
+            break;
         case MPM_STAGE_CONSTITUTIVE:
             timed(c, KC_PFORCE, [&] { mpm::launch_particle_force<D>(p, c->stream, c->grid_block, true, false); });
             break;
@@ -306,33 +373,37 @@ int run_stage(mpm_context* c, int stage, float dt) {
         case MPM_STAGE_COLLISION:
             timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, false, true, false, false); });
             break;
-        case MPM_STAGE_CG: return stage_cg<D>(c, dt, true);
-        case MPM_STAGE_G2P: stage_g2p<D>(c, dt); break;
+        case MPM_STAGE_CG: return stage_cg<D>(c, true);
+        case MPM_STAGE_G2P: stage_g2p<D>(c); break;
         default: return mpm_set_error(MPM_E_INVALID, "unknown stage");
     }
     return MPM_OK;
 }
 
-// MPM_Driver::Advance_Step (MPM_Driver.cpp:103-116) with the constitutive update,
-// explicit force, collision and CG set-up fused.
+// MPM_Driver::Advance_Step (MPM_Driver.cpp:103-116) for every scene with
+// dt > 0, with the constitutive update, explicit force, collision and CG
+// set-up fused.
 template <int D>
-int substep(mpm_context* c, float dt) {
+int substep(mpm_context* c, const float* dts) {
+    set_step(c, dts, 0.0f, false, false);
     stage_bin<D>(c);
     stage_p2g<D>(c);
-    mpm::KP p = make_kp(c, dt);
+    mpm::KP p = make_kp(c);
     timed(c, KC_PFORCE, [&] { mpm::launch_particle_force<D>(p, c->stream, c->grid_block, true, true); });
     timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, true, true, true, false); });
-    int rc = stage_cg<D>(c, dt, false);
+    int rc = stage_cg<D>(c, false);
     if (rc != MPM_OK) return rc;
-    stage_g2p<D>(c, dt);
+    stage_g2p<D>(c);
     return MPM_OK;
 }
 
 template <int D>
 int initialize(mpm_context* c) {
+    set_step(c, nullptr, 0.0f, true, false);
     stage_bin<D>(c);
     stage_p2g<D>(c);
     stage_waiting<D>(c);
+    set_step(c, nullptr, 0.0f, true, true);
     return MPM_OK;
 }
 
@@ -342,30 +413,319 @@ int check_launch(const char* what) {
     return MPM_OK;
 }
 
-int read_counters(mpm_context* c, mpm::Counters& h, mpm::CGState* cg = nullptr) {
+int read_counters(mpm_context* c, mpm::Counters& h) {
     HIPCHECK(hipStreamSynchronize(c->stream));
     HIPCHECK(hipMemcpy(&h, c->ctr, sizeof(h), hipMemcpyDeviceToHost));
-    if (cg) HIPCHECK(hipMemcpy(cg, c->cg, sizeof(*cg), hipMemcpyDeviceToHost));
     return MPM_OK;
 }
 
-int fill_stats(mpm_context* c, mpm_step_stats* s) {
-    mpm::Counters h{};
-    mpm::CGState g{};
-    int rc = read_counters(c, h, &g);
-    if (rc) return rc;
-    s->cg_iterations = g.iter;
-    s->cg_multiplies = g.multiplies;
-    s->cg_converged = g.converged;
-    s->cg_residual = g.conv;
-    s->active_blocks = h.nblocks;
-    s->live_particles = (int64_t)h.n_live - h.n_removed;
-    float v2;
-    std::memcpy(&v2, &h.max_v2, sizeof(v2));
-    s->max_speed = std::sqrt((double)v2);
-    if (h.overflow) return mpm_set_error(MPM_E_CAPACITY, "active blocks exceed the block capacity");
-    if (h.nonfinite) return mpm_set_error(MPM_E_NONFINITE, "non-finite particle position");
+int read_scene(mpm_context* c, int s, mpm::SceneState& st, mpm::CGState* cg) {
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipMemcpy(&st, c->sst + s, sizeof(st), hipMemcpyDeviceToHost));
+    if (cg) HIPCHECK(hipMemcpy(cg, c->cg + s, sizeof(*cg), hipMemcpyDeviceToHost));
     return MPM_OK;
+}
+
+int fill_stats(mpm_context* c, int s, mpm_step_stats* out) {
+    mpm::Counters h{};
+    mpm::SceneState st{};
+    mpm::CGState g{};
+    int rc = read_counters(c, h);
+    if (rc) return rc;
+    rc = read_scene(c, s, st, &g);
+    if (rc) return rc;
+    out->cg_iterations = g.iter;
+    out->cg_multiplies = g.multiplies;
+    out->cg_converged = g.converged;
+    out->cg_residual = g.conv;
+    out->active_blocks = st.block_end - st.block_begin;
+    out->live_particles = c->ever_binned ? (int64_t)(st.pend - st.pbegin) - st.n_removed : c->sc[(size_t)s].n_uploaded;
+    float v2;
+    std::memcpy(&v2, &st.max_v2, sizeof(v2));
+    out->max_speed = std::sqrt((double)v2);
+    if (h.overflow) return mpm_set_error(MPM_E_CAPACITY, "active blocks exceed the block capacity");
+    if (st.nonfinite) return mpm_set_error(MPM_E_NONFINITE, "non-finite particle position");
+    return MPM_OK;
+}
+
+int create_batch(const mpm_scene_desc* descs, const int64_t* capacities, int nscenes, mpm_handle* out) {
+    if (!descs || !capacities || !out || nscenes < 1) return mpm_set_error(MPM_E_INVALID, "mpm_batch_create: bad argument");
+    const int D = descs[0].dim;
+    int64_t pcap = 0, slots = 0;
+    for (int s = 0; s < nscenes; ++s) {
+        const mpm_scene_desc& d = descs[s];
+        if (d.dim != 2 && d.dim != 3) return mpm_set_error(MPM_E_INVALID, "mpm_create: dim must be 2 or 3");
+        if (d.dim != D) return mpm_set_error(MPM_E_INVALID, "mpm_batch_create: scenes of one batch share dim");
+        if (d.counts < 4 || d.counts > 4096) return mpm_set_error(MPM_E_INVALID, "mpm_create: counts out of range");
+        if (d.nplanes < 0 || d.nplanes > MPM_MAX_PLANES) return mpm_set_error(MPM_E_INVALID, "mpm_create: too many planes");
+        if (capacities[s] < 1 || capacities[s] > (int64_t)1 << 30) return mpm_set_error(MPM_E_INVALID, "mpm_create: bad capacity");
+        const int64_t nb = slots_per_axis(D, d.counts);
+        slots += D == 3 ? nb * nb * nb : nb * nb;
+        pcap += capacities[s];
+    }
+    // keys are slot*64 + cell in 32 bits
+    if (slots * 64 > ((int64_t)1 << 31) - 1) return mpm_set_error(MPM_E_INVALID, "mpm_create: grid too large");
+    if (pcap > ((int64_t)1 << 31) - 1) return mpm_set_error(MPM_E_INVALID, "mpm_create: too many particles");
+    mpm_context* c = new mpm_context;
+    c->D = D;
+    c->nscenes = nscenes;
+    c->sc.resize((size_t)nscenes);
+    c->table.resize((size_t)nscenes);
+    std::vector<int> slot_base((size_t)nscenes + 1);
+    int64_t sb = 0, pb = 0;
+    for (int s = 0; s < nscenes; ++s) {
+        const mpm_scene_desc& d = descs[s];
+        const int nb = slots_per_axis(D, d.counts);
+        const int ns = D == 3 ? nb * nb * nb : nb * nb;
+        c->sc[(size_t)s].desc = d;
+        c->sc[(size_t)s].capacity = capacities[s];
+        c->sc[(size_t)s].pbase = pb;
+        c->table[(size_t)s] = scene_params(d, (int)sb, ns);
+        slot_base[(size_t)s] = (int)sb;
+        sb += ns;
+        pb += capacities[s];
+    }
+    slot_base[(size_t)nscenes] = (int)sb;
+    c->nslots = (int)sb;
+    c->pcap = pcap;
+    // Block capacity: every slot of small grids, else pcap/16 blocks -- about
+    // 8x the active blocks (home blocks plus halo) of scenes packed at 8
+    // particles per cell.  Overflow makes the substep a no-op and is reported.
+    c->bcap = (int)std::min<int64_t>(c->nslots, std::max<int64_t>(4096, pcap / 16));
+    c->gcap = (int64_t)c->bcap * 64;
+    c->nparts = (c->nslots + mpm::SCAN_TILE - 1) / mpm::SCAN_TILE;
+    int rc = MPM_OK;
+    auto A = [&](int r) { if (rc == MPM_OK) rc = r; };
+    if (hipStreamCreate(&c->stream) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipStreamCreate failed");
+    for (int k = 0; k < 2; ++k) {
+        A(dalloc(c, &c->pbuf[k], (size_t)nfields(D) * c->pcap));
+        A(dalloc(c, &c->idbuf[k], (size_t)c->pcap));
+        A(dalloc(c, &c->keybuf[k], (size_t)c->pcap));
+        A(dalloc(c, &c->scbuf[k], (size_t)c->pcap));
+    }
+    A(dalloc(c, &c->prank, (size_t)c->pcap));
+    A(dalloc(c, &c->rcache, (size_t)D * D * c->pcap));
+    A(dalloc(c, &c->pop, (size_t)mpm::op_record_floats(D) * c->pcap));
+    A(dalloc(c, &c->slot_count, (size_t)c->nslots));
+    A(dalloc(c, &c->slot_touched, (size_t)c->nslots));
+    A(dalloc(c, &c->slot_active, (size_t)c->nslots));
+    A(dalloc(c, &c->block_slot, (size_t)c->bcap));
+    A(dalloc(c, &c->block_scene, (size_t)c->bcap));
+    A(dalloc(c, &c->block_pstart, (size_t)c->bcap + 1));
+    A(dalloc(c, &c->block_nbr, (size_t)c->bcap * (D == 3 ? 27 : 9)));
+    A(dalloc(c, &c->cell_count, (size_t)c->nslots * 64));
+    A(dalloc(c, &c->cell_start, (size_t)c->bcap * 65 + 1));
+    A(dalloc(c, &c->tiles, (size_t)c->bcap * (D + 1) * (D == 3 ? 216 : 100)));
+    A(dalloc(c, &c->scan_part, (size_t)2 * c->nparts));
+    A(dalloc(c, &c->g_m, (size_t)c->gcap));
+    A(dalloc(c, &c->g_v, (size_t)D * c->gcap));
+    A(dalloc(c, &c->g_vs, (size_t)D * c->gcap));
+    A(dalloc(c, &c->g_q, (size_t)D * c->gcap));
+    A(dalloc(c, &c->g_s, (size_t)D * c->gcap));
+    A(dalloc(c, &c->g_r, (size_t)D * c->gcap));
+    A(dalloc(c, &c->g_flags, (size_t)c->gcap));
+    A(dalloc(c, &c->ctr, 1));
+    A(dalloc(c, &c->scn, (size_t)nscenes));
+    A(dalloc(c, &c->sst, (size_t)nscenes));
+    A(dalloc(c, &c->cg, (size_t)nscenes));
+    A(dalloc(c, &c->slot_base, (size_t)nscenes + 1));
+    A(dalloc(c, &c->bsum, (size_t)c->bcap));
+    A(dalloc(c, &c->bmax, (size_t)c->bcap));
+    A(dalloc(c, &c->bu, (size_t)2 * c->bcap));
+    c->grid_particle = (int)std::min<int64_t>(std::max<int64_t>((c->pcap + mpm::WG - 1) / mpm::WG, 8), 4096);
+    c->grid_block = (int)std::min<int64_t>(((int64_t)c->bcap + 7) / 8 * 8, 8192);
+    // node kernels: one wave per block, up to 1024 work-groups (a multiple of 8 for the XCD ranges)
+    c->grid_node = (int)std::min<int64_t>(((int64_t)c->bcap / (mpm::WG / 64) + 7) / 8 * 8 + 8, 1024);
+    if (rc == MPM_OK) {
+        if (hipMemcpy(c->scn, c->table.data(), sizeof(mpm::SceneParams) * (size_t)nscenes, hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemcpy(c->slot_base, slot_base.data(), sizeof(int) * slot_base.size(), hipMemcpyHostToDevice) != hipSuccess ||
+            hipMemset(c->keybuf[0], 0xff, (size_t)c->pcap * sizeof(int)) != hipSuccess)
+            rc = mpm_set_error(MPM_E_HIP, "scene table upload failed");
+    }
+    if (rc == MPM_OK) {
+        // the particle buffer spans every scene's range; empty entries carry key -1
+        mpm::Counters hc{};
+        hc.n_live = (int)c->pcap;
+        hc.n_prev = (int)c->pcap;
+        if (hipMemcpy(c->ctr, &hc, sizeof(hc), hipMemcpyHostToDevice) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "counter upload failed");
+    }
+    if (rc == MPM_OK) {
+        void* hp = nullptr;
+        if (hipHostMalloc(&hp, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");
+        c->h_done = (int*)hp;
+        for (int k = 0; k < 2 && rc == MPM_OK; ++k)
+            if (hipEventCreateWithFlags(&c->done_ev[k], hipEventDisableTiming) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipEventCreate failed");
+    }
+    if (rc != MPM_OK) {
+        std::string msg = g_err;
+        mpm_destroy(c);
+        return mpm_set_error(rc, msg);
+    }
+    *out = c;
+    return MPM_OK;
+}
+
+// Writes scene s's particles into its range of the (unbinned) buffer.
+int upload_scene(mpm_context* c, int s, int64_t n, const float* x, const float* v, const float* F, const float* Jp,
+                 const float* mass, const float* volume) {
+    SceneHost& sh = c->sc[(size_t)s];
+    if (n < 0 || n > sh.capacity) return mpm_set_error(MPM_E_CAPACITY, "mpm_upload_particles: n exceeds capacity");
+    const int D = c->D, NF = nfields(D);
+    const int64_t cap = sh.capacity;
+    std::vector<float> h((size_t)NF * (size_t)cap, 0.0f);
+    auto f = [&](int field, int64_t i) -> float& { return h[(size_t)field * (size_t)cap + (size_t)i]; };
+    for (int64_t i = 0; i < n; ++i) {
+        for (int a = 0; a < D; ++a) {
+            f(a, i) = x[i * D + a];
+            f(D + a, i) = v[i * D + a];
+        }
+        for (int a = 0; a < D; ++a)
+            for (int b = 0; b < D; ++b) f(2 * D + a * D + b, i) = F ? F[(i * D + a) * D + b] : (a == b ? 1.0f : 0.0f);
+        f(2 * D + D * D, i) = Jp ? Jp[i] : 1.0f;
+        f(2 * D + D * D + 1, i) = mass[i];
+        f(2 * D + D * D + 2, i) = volume ? volume[i] : 0.0f;
+    }
+    std::vector<int> ids((size_t)cap), keys((size_t)cap), scene((size_t)cap, s);
+    for (int64_t i = 0; i < cap; ++i) {
+        ids[(size_t)i] = (int)i;
+        keys[(size_t)i] = i < n ? 0 : -1;
+    }
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    const int64_t b = sh.pbase;
+    for (int fld = 0; fld < NF; ++fld)
+        HIPCHECK(hipMemcpy(c->pbuf[0] + (size_t)fld * c->pcap + b, &h[(size_t)fld * cap], (size_t)cap * sizeof(float),
+                           hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->idbuf[0] + b, ids.data(), (size_t)cap * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->keybuf[0] + b, keys.data(), (size_t)cap * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->scbuf[0] + b, scene.data(), (size_t)cap * sizeof(int), hipMemcpyHostToDevice));
+    const int waiting = volume == nullptr ? 1 : 0;
+    HIPCHECK(hipMemcpy(&c->scn[s].waiting, &waiting, sizeof(int), hipMemcpyHostToDevice));
+    c->table[(size_t)s].waiting = waiting;
+    mpm::SceneState st{};
+    HIPCHECK(hipMemcpy(c->sst + s, &st, sizeof(st), hipMemcpyHostToDevice));
+    sh.uploaded = true;
+    sh.n_uploaded = n;
+    return MPM_OK;
+}
+
+// Per-scene max |V|^2 of the freshly uploaded buffer.
+int upload_max_speed(mpm_context* c) {
+    mpm::KP p = make_kp(c);
+    if (c->D == 3) mpm::launch_max_speed<3>(p, c->stream, c->grid_particle);
+    else mpm::launch_max_speed<2>(p, c->stream, c->grid_particle);
+    return check_launch("max speed");
+}
+
+int download_scene(mpm_context* c, int s, int64_t cap, float* x, float* v, float* F, float* Jp, float* mass,
+                   float* volume, int32_t* ids, int64_t* n_out) {
+    mpm::Counters hc{};
+    int rc = read_counters(c, hc);
+    if (rc) return rc;
+    int64_t b0 = 0, b1 = 0;
+    if (c->ever_binned && !hc.overflow) {
+        mpm::SceneState st{};
+        rc = read_scene(c, s, st, nullptr);
+        if (rc) return rc;
+        b0 = st.pbegin;
+        b1 = st.pend;
+    } else if (c->ever_binned) {  // an overflowing binning kept the buffer as it was
+        b0 = 0;
+        b1 = hc.n_live;
+    } else {
+        b0 = c->sc[(size_t)s].pbase;
+        b1 = b0 + c->sc[(size_t)s].capacity;
+    }
+    const int D = c->D, NF = nfields(D);
+    const int64_t n = std::max<int64_t>(b1 - b0, 0);
+    std::vector<float> h((size_t)NF * (size_t)std::max<int64_t>(n, 1));
+    std::vector<int> id((size_t)std::max<int64_t>(n, 1)), key((size_t)std::max<int64_t>(n, 1)),
+        scn((size_t)std::max<int64_t>(n, 1));
+    if (n > 0) {
+        for (int fld = 0; fld < NF; ++fld)
+            HIPCHECK(hipMemcpy(&h[(size_t)fld * n], c->pbuf[c->cur] + (size_t)fld * c->pcap + b0, (size_t)n * sizeof(float),
+                               hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(id.data(), c->idbuf[c->cur] + b0, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(key.data(), c->keybuf[c->cur] + b0, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(scn.data(), c->scbuf[c->cur] + b0, (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
+    }
+    std::vector<int64_t> order;
+    order.reserve((size_t)n);
+    for (int64_t i = 0; i < n; ++i)
+        if (key[(size_t)i] >= 0 && scn[(size_t)i] == s) order.push_back(i);
+    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return id[(size_t)a] < id[(size_t)b]; });
+    const int64_t live = (int64_t)order.size();
+    if (n_out) *n_out = live;
+    if (live > cap) return mpm_set_error(MPM_E_CAPACITY, "mpm_download_particles: cap too small");
+    auto g = [&](int fld, int64_t i) { return h[(size_t)fld * n + (size_t)i]; };
+    for (int64_t k = 0; k < live; ++k) {
+        const int64_t i = order[(size_t)k];
+        for (int a = 0; a < D; ++a) {
+            if (x) x[k * D + a] = g(a, i);
+            if (v) v[k * D + a] = g(D + a, i);
+        }
+        if (F) for (int a = 0; a < D * D; ++a) F[k * D * D + a] = g(2 * D + a, i);
+        if (Jp) Jp[k] = g(2 * D + D * D, i);
+        if (mass) mass[k] = g(2 * D + D * D + 1, i);
+        if (volume) volume[k] = g(2 * D + D * D + 2, i);
+        if (ids) ids[k] = id[(size_t)i];
+    }
+    return MPM_OK;
+}
+
+int do_substep(mpm_context* c, const float* dts) {
+    int rc = c->D == 3 ? substep<3>(c, dts) : substep<2>(c, dts);
+    if (rc) return rc;
+    for (int s = 0; s < c->nscenes; ++s)
+        if (dts[s] > 0.0f) c->sc[(size_t)s].last_dt = dts[s];
+    c->stepped = true;
+    rc = check_launch("mpm_substep");
+    if (rc) return rc;
+    return collect_timing(c);
+}
+
+int max_velocities(mpm_context* c, float* out) {
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    std::vector<mpm::SceneState> st((size_t)c->nscenes);
+    HIPCHECK(hipMemcpy(st.data(), c->sst, sizeof(mpm::SceneState) * st.size(), hipMemcpyDeviceToHost));
+    for (int s = 0; s < c->nscenes; ++s) {
+        float v2;
+        std::memcpy(&v2, &st[(size_t)s].max_v2, sizeof(v2));
+        out[s] = std::sqrt(v2);
+    }
+    return MPM_OK;
+}
+
+// The reference's dt rule (MPM_Driver.cpp:55-80) for one scene, in float.
+float next_dt(const mpm_scene_desc& d, float max_v, float time, float target, bool& last) {
+    const float dx_min = 1.0f / (float)d.counts;
+    float dt = std::max(d.min_dt, std::min(d.max_dt, d.cfl * dx_min / std::max(max_v, 1e-2f)));
+    last = false;
+    if (target - time < dt * 1.001f) {
+        dt = target - time;
+        last = true;
+    } else if (target - time < 2.0f * dt) {
+        dt = (target - time) * 0.5f;
+    }
+    return dt;
+}
+
+// Dense views of scene 0 over node indices [-1, counts]^dim.
+int dense_index(const mpm_context* c, int slot, int l, int64_t& o) {
+    const int D = c->D, B = block_edge(D), n = c->sc[0].desc.counts;
+    const int nb = c->table[0].nb;
+    int bb[3];
+    int s = slot - c->table[0].slot_base;
+    for (int a = D - 1; a >= 0; --a) { bb[a] = s % nb; s /= nb; }
+    int lc[3];
+    if (D == 3) { lc[0] = l >> 4; lc[1] = (l >> 2) & 3; lc[2] = l & 3; }
+    else { lc[0] = l >> 3; lc[1] = l & 7; }
+    o = 0;
+    for (int a = 0; a < D; ++a) {
+        const int i = B * bb[a] + lc[a] - B;
+        if (i < -1 || i > n) return 0;
+        o = o * (n + 2) + (i + 1);
+    }
+    return 1;
 }
 
 }  // namespace
@@ -386,78 +746,7 @@ int mpm_device_info(char* buf, int32_t len) {
 
 int mpm_create(const mpm_scene_desc* desc, int64_t capacity, mpm_handle* out) {
     if (!desc || !out) return mpm_set_error(MPM_E_INVALID, "mpm_create: null argument");
-    if (desc->dim != 2 && desc->dim != 3) return mpm_set_error(MPM_E_INVALID, "mpm_create: dim must be 2 or 3");
-    if (desc->counts < 4 || desc->counts > 4096) return mpm_set_error(MPM_E_INVALID, "mpm_create: counts out of range");
-    if (desc->nplanes < 0 || desc->nplanes > MPM_MAX_PLANES) return mpm_set_error(MPM_E_INVALID, "mpm_create: too many planes");
-    if (capacity < 1 || capacity > (int64_t)1 << 30) return mpm_set_error(MPM_E_INVALID, "mpm_create: bad capacity");
-    mpm_context* c = new mpm_context;
-    c->desc = *desc;
-    c->D = desc->dim;
-    const int B = c->D == 3 ? 4 : 8;
-    c->nb = (desc->counts + 2 * B) / B + 1;
-    c->nslots = c->D == 3 ? c->nb * c->nb * c->nb : c->nb * c->nb;
-    if ((int64_t)c->nslots * 64 > ((int64_t)1 << 31) - 1) {
-        delete c;
-        return mpm_set_error(MPM_E_INVALID, "mpm_create: grid too large");
-    }
-    c->pcap = capacity;
-    // Block capacity: every slot of small grids, else capacity/8 blocks -- about
-    // 16x the active blocks (home blocks plus halo) of a scene packed at 8
-    // particles per cell.  Overflow makes the substep a no-op and is reported.
-    c->bcap = (int)std::min<int64_t>(c->nslots, std::max<int64_t>(4096, capacity / 8));
-    c->gcap = (int64_t)c->bcap * 64;
-    c->nparts = (c->nslots + mpm::SCAN_TILE - 1) / mpm::SCAN_TILE;
-    const int D = c->D;
-    int rc = MPM_OK;
-    auto A = [&](int r) { if (rc == MPM_OK) rc = r; };
-    if (hipStreamCreate(&c->stream) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipStreamCreate failed");
-    for (int k = 0; k < 2; ++k) {
-        A(dalloc(c, &c->pbuf[k], (size_t)nfields(D) * c->pcap));
-        A(dalloc(c, &c->idbuf[k], (size_t)c->pcap));
-        A(dalloc(c, &c->keybuf[k], (size_t)c->pcap));
-    }
-    A(dalloc(c, &c->prank, (size_t)c->pcap));
-    A(dalloc(c, &c->rcache, (size_t)D * D * c->pcap));
-    A(dalloc(c, &c->pop, (size_t)mpm::op_record_floats(D) * c->pcap));
-    A(dalloc(c, &c->slot_count, (size_t)c->nslots));
-    A(dalloc(c, &c->slot_touched, (size_t)c->nslots));
-    A(dalloc(c, &c->slot_active, (size_t)c->nslots));
-    A(dalloc(c, &c->block_slot, (size_t)c->bcap));
-    A(dalloc(c, &c->block_pstart, (size_t)c->bcap + 1));
-    A(dalloc(c, &c->block_nbr, (size_t)c->bcap * (D == 3 ? 27 : 9)));
-    A(dalloc(c, &c->cell_count, (size_t)c->nslots * 64));
-    A(dalloc(c, &c->cell_start, (size_t)c->bcap * 65 + 1));
-    A(dalloc(c, &c->tiles, (size_t)c->bcap * (D + 1) * (D == 3 ? 216 : 100)));
-    A(dalloc(c, &c->scan_part, (size_t)2 * c->nparts));
-    A(dalloc(c, &c->g_m, (size_t)c->gcap));
-    A(dalloc(c, &c->g_v, (size_t)D * c->gcap));
-    A(dalloc(c, &c->g_vs, (size_t)D * c->gcap));
-    A(dalloc(c, &c->g_q, (size_t)D * c->gcap));
-    A(dalloc(c, &c->g_s, (size_t)D * c->gcap));
-    A(dalloc(c, &c->g_r, (size_t)D * c->gcap));
-    A(dalloc(c, &c->g_flags, (size_t)c->gcap));
-    A(dalloc(c, &c->ctr, 1));
-    A(dalloc(c, &c->cg, 1));
-    c->grid_particle = (int)std::min<int64_t>(std::max<int64_t>((c->pcap + mpm::WG - 1) / mpm::WG, 8), 4096);
-    c->grid_block = (int)std::min<int64_t>(((int64_t)c->bcap + 7) / 8 * 8, 8192);
-    // node kernels: one thread per node up to 1024 work-groups (a multiple of 8 for the XCD ranges)
-    c->grid_node = (int)std::min<int64_t>(((c->gcap + mpm::WG - 1) / mpm::WG + 7) / 8 * 8, 1024);
-    A(dalloc(c, &c->red, (size_t)3 * c->grid_node));
-    A(dalloc(c, &c->wg_part, (size_t)2 * c->grid_block));
-    if (rc == MPM_OK) {
-        void* hp = nullptr;
-        if (hipHostMalloc(&hp, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");
-        c->h_done = (int*)hp;
-        for (int k = 0; k < 2 && rc == MPM_OK; ++k)
-            if (hipEventCreateWithFlags(&c->done_ev[k], hipEventDisableTiming) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipEventCreate failed");
-    }
-    if (rc != MPM_OK) {
-        std::string msg = g_err;
-        mpm_destroy(c);
-        return mpm_set_error(rc, msg);
-    }
-    *out = c;
-    return MPM_OK;
+    return create_batch(desc, &capacity, 1, out);
 }
 
 int mpm_destroy(mpm_handle c) {
@@ -475,84 +764,31 @@ int mpm_destroy(mpm_handle c) {
 int mpm_upload_particles(mpm_handle c, int64_t n, const float* x, const float* v, const float* F, const float* Jp,
                          const float* mass, const float* volume) {
     if (!c || !x || !v || !mass) return mpm_set_error(MPM_E_INVALID, "mpm_upload_particles: null argument");
-    if (n < 0 || n > c->pcap) return mpm_set_error(MPM_E_CAPACITY, "mpm_upload_particles: n exceeds capacity");
-    const int D = c->D, NF = nfields(D);
-    std::vector<float> h((size_t)NF * c->pcap, 0.0f);
-    auto f = [&](int field, int64_t i) -> float& { return h[(size_t)field * c->pcap + i]; };
-    for (int64_t i = 0; i < n; ++i) {
-        for (int a = 0; a < D; ++a) {
-            f(a, i) = x[i * D + a];
-            f(D + a, i) = v[i * D + a];
-        }
-        for (int a = 0; a < D; ++a)
-            for (int b = 0; b < D; ++b) f(2 * D + a * D + b, i) = F ? F[(i * D + a) * D + b] : (a == b ? 1.0f : 0.0f);
-        f(2 * D + D * D, i) = Jp ? Jp[i] : 1.0f;
-        f(2 * D + D * D + 1, i) = mass[i];
-        f(2 * D + D * D + 2, i) = volume ? volume[i] : 0.0f;
-    }
-    std::vector<int> ids((size_t)std::max<int64_t>(n, 1));
-    std::iota(ids.begin(), ids.end(), 0);
+    if (c->nscenes != 1) return mpm_set_error(MPM_E_INVALID, "mpm_upload_particles: batch handle, use mpm_batch_upload");
+    // a new particle set: the buffer starts over in its unbinned layout
     HIPCHECK(hipStreamSynchronize(c->stream));
     c->cur = 0;
-    HIPCHECK(hipMemcpy(c->pbuf[0], h.data(), h.size() * sizeof(float), hipMemcpyHostToDevice));
-    if (n > 0) HIPCHECK(hipMemcpy(c->idbuf[0], ids.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
-    HIPCHECK(hipMemset(c->keybuf[0], 0, (size_t)c->pcap * sizeof(int)));
+    HIPCHECK(hipMemset(c->keybuf[0], 0xff, (size_t)c->pcap * sizeof(int)));
     mpm::Counters hc{};
-    hc.n_live = (int)n;
-    hc.n_prev = (int)n;
+    hc.n_live = (int)c->pcap;
+    hc.n_prev = (int)c->pcap;
     HIPCHECK(hipMemcpy(c->ctr, &hc, sizeof(hc), hipMemcpyHostToDevice));
-    mpm::KP p = make_kp(c, 0.0f);
-    if (D == 3) mpm::launch_max_speed<3>(p, c->stream, c->grid_particle);
-    else mpm::launch_max_speed<2>(p, c->stream, c->grid_particle);
-    c->have_particles = true;
-    c->waiting = volume == nullptr;
-    c->initialized = false;
-    c->binned = false;
-    return check_launch("mpm_upload_particles");
+    c->initialized = c->binned = c->ever_binned = c->stepped = false;
+    int rc = upload_scene(c, 0, n, x, v, F, Jp, mass, volume);
+    if (rc) return rc;
+    return upload_max_speed(c);
 }
 
 int mpm_download_particles(mpm_handle c, int64_t cap, float* x, float* v, float* F, float* Jp, float* mass,
                            float* volume, int32_t* ids, int64_t* n_out) {
     if (!c) return mpm_set_error(MPM_E_INVALID, "mpm_download_particles: null handle");
-    mpm::Counters hc{};
-    int rc = read_counters(c, hc);
-    if (rc) return rc;
-    const int D = c->D, NF = nfields(D);
-    const int64_t n = hc.n_live;
-    std::vector<float> h((size_t)NF * std::max<int64_t>(n, 1));
-    std::vector<int> id((size_t)std::max<int64_t>(n, 1)), key((size_t)std::max<int64_t>(n, 1));
-    for (int fld = 0; fld < NF && n > 0; ++fld)
-        HIPCHECK(hipMemcpy(&h[(size_t)fld * n], c->pbuf[c->cur] + (size_t)fld * c->pcap, (size_t)n * sizeof(float), hipMemcpyDeviceToHost));
-    if (n > 0) {
-        HIPCHECK(hipMemcpy(id.data(), c->idbuf[c->cur], (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
-        HIPCHECK(hipMemcpy(key.data(), c->keybuf[c->cur], (size_t)n * sizeof(int), hipMemcpyDeviceToHost));
-    }
-    std::vector<int64_t> order;
-    order.reserve((size_t)n);
-    for (int64_t i = 0; i < n; ++i) if (key[(size_t)i] >= 0) order.push_back(i);
-    std::sort(order.begin(), order.end(), [&](int64_t a, int64_t b) { return id[(size_t)a] < id[(size_t)b]; });
-    const int64_t live = (int64_t)order.size();
-    if (n_out) *n_out = live;
-    if (live > cap) return mpm_set_error(MPM_E_CAPACITY, "mpm_download_particles: cap too small");
-    auto g = [&](int fld, int64_t i) { return h[(size_t)fld * n + i]; };
-    for (int64_t k = 0; k < live; ++k) {
-        const int64_t i = order[(size_t)k];
-        for (int a = 0; a < D; ++a) {
-            if (x) x[k * D + a] = g(a, i);
-            if (v) v[k * D + a] = g(D + a, i);
-        }
-        if (F) for (int a = 0; a < D * D; ++a) F[k * D * D + a] = g(2 * D + a, i);
-        if (Jp) Jp[k] = g(2 * D + D * D, i);
-        if (mass) mass[k] = g(2 * D + D * D + 1, i);
-        if (volume) volume[k] = g(2 * D + D * D + 2, i);
-        if (ids) ids[k] = id[(size_t)i];
-    }
-    return MPM_OK;
+    return download_scene(c, 0, cap, x, v, F, Jp, mass, volume, ids, n_out);
 }
 
 int mpm_initialize(mpm_handle c) {
     if (!c) return mpm_set_error(MPM_E_INVALID, "mpm_initialize: null handle");
-    if (!c->have_particles) return mpm_set_error(MPM_E_NOT_READY, "mpm_initialize: no particles uploaded");
+    for (const SceneHost& s : c->sc)
+        if (!s.uploaded) return mpm_set_error(MPM_E_NOT_READY, "mpm_initialize: no particles uploaded");
     int rc = c->D == 3 ? initialize<3>(c) : initialize<2>(c);
     if (rc) return rc;
     c->initialized = true;
@@ -565,33 +801,26 @@ int mpm_substep(mpm_handle c, float dt, mpm_step_stats* stats) {
     if (!c) return mpm_set_error(MPM_E_INVALID, "mpm_substep: null handle");
     if (!c->initialized) return mpm_set_error(MPM_E_NOT_READY, "mpm_substep: call mpm_initialize first");
     if (!(dt > 0.0f) || !std::isfinite(dt)) return mpm_set_error(MPM_E_INVALID, "mpm_substep: dt must be positive");
-    int rc = c->D == 3 ? substep<3>(c, dt) : substep<2>(c, dt);
+    std::vector<float> dts((size_t)c->nscenes, dt);
+    int rc = do_substep(c, dts.data());
     if (rc) return rc;
-    c->last_dt = dt;
-    rc = check_launch("mpm_substep");
-    if (rc) return rc;
-    rc = collect_timing(c);
-    if (rc) return rc;
-    if (stats) return fill_stats(c, stats);
+    if (stats) return fill_stats(c, 0, stats);
     return MPM_OK;
 }
 
 int mpm_max_particle_velocity(mpm_handle c, float* out) {
     if (!c || !out) return mpm_set_error(MPM_E_INVALID, "mpm_max_particle_velocity: null argument");
-    mpm::Counters hc{};
-    int rc = read_counters(c, hc);
+    std::vector<float> v((size_t)c->nscenes);
+    int rc = max_velocities(c, v.data());
     if (rc) return rc;
-    float v2;
-    std::memcpy(&v2, &hc.max_v2, sizeof(v2));
-    *out = std::sqrt(v2);
+    *out = v[0];
     return MPM_OK;
 }
 
 int mpm_advance_to_time(mpm_handle c, double time_in, double target_in, int32_t* substeps_out, double* time_out,
                         mpm_step_stats* stats) {
     if (!c) return mpm_set_error(MPM_E_INVALID, "mpm_advance_to_time: null handle");
-    const float min_dt = c->desc.min_dt, max_dt = c->desc.max_dt, cfl = c->desc.cfl;
-    const float dx_min = 1.0f / (float)c->desc.counts;
+    if (c->nscenes != 1) return mpm_set_error(MPM_E_INVALID, "mpm_advance_to_time: batch handle, use mpm_batch_advance_to_time");
     float time = (float)time_in;
     const float target_time = (float)target_in;
     bool done = false;
@@ -600,13 +829,7 @@ int mpm_advance_to_time(mpm_handle c, double time_in, double target_in, int32_t*
         float max_v = 0.0f;
         int rc = mpm_max_particle_velocity(c, &max_v);
         if (rc) return rc;
-        float dt = std::max(min_dt, std::min(max_dt, cfl * dx_min / std::max(max_v, 1e-2f)));
-        if (target_time - time < dt * 1.001f) {
-            dt = target_time - time;
-            done = true;
-        } else if (target_time - time < 2.0f * dt) {
-            dt = (target_time - time) * 0.5f;
-        }
+        const float dt = next_dt(c->sc[0].desc, max_v, time, target_time, done);
         if (!(dt > 0.0f)) break;
         rc = mpm_substep(c, dt, nullptr);
         if (rc) return rc;
@@ -615,7 +838,7 @@ int mpm_advance_to_time(mpm_handle c, double time_in, double target_in, int32_t*
     }
     if (substeps_out) *substeps_out = n;
     if (time_out) *time_out = (double)time;
-    if (stats) return fill_stats(c, stats);
+    if (stats) return fill_stats(c, 0, stats);
     return MPM_OK;
 }
 
@@ -627,12 +850,17 @@ int mpm_synchronize(mpm_handle c) {
 
 int mpm_run_stage(mpm_handle c, int32_t stage, float dt) {
     if (!c) return mpm_set_error(MPM_E_INVALID, "mpm_run_stage: null handle");
-    if (!c->have_particles) return mpm_set_error(MPM_E_NOT_READY, "mpm_run_stage: no particles uploaded");
+    for (const SceneHost& s : c->sc)
+        if (!s.uploaded) return mpm_set_error(MPM_E_NOT_READY, "mpm_run_stage: no particles uploaded");
     if (stage != MPM_STAGE_BIN && stage != MPM_STAGE_CONSTITUTIVE && !c->binned)
         return mpm_set_error(MPM_E_NOT_READY, "mpm_run_stage: particles moved since the last MPM_STAGE_BIN");
     int rc = c->D == 3 ? run_stage<3>(c, stage, dt) : run_stage<2>(c, stage, dt);
     if (rc) return rc;
     if (stage == MPM_STAGE_WAITING || stage == MPM_STAGE_P2G) c->initialized = true;
+    if (stage == MPM_STAGE_G2P && dt > 0.0f) {
+        for (SceneHost& s : c->sc) s.last_dt = dt;
+        c->stepped = true;
+    }
     rc = check_launch("mpm_run_stage");
     if (rc) return rc;
     return collect_timing(c);
@@ -640,7 +868,7 @@ int mpm_run_stage(mpm_handle c, int32_t stage, float dt) {
 
 int mpm_read_grid(mpm_handle c, int32_t channel, float* out, int64_t out_len) {
     if (!c || !out) return mpm_set_error(MPM_E_INVALID, "mpm_read_grid: null argument");
-    const int D = c->D, B = D == 3 ? 4 : 8, n = c->desc.counts;
+    const int D = c->D, n = c->sc[0].desc.counts;
     const float* src = nullptr;
     int width = D;
     switch (channel) {
@@ -656,75 +884,50 @@ int mpm_read_grid(mpm_handle c, int32_t channel, float* out, int64_t out_len) {
     int64_t total = 1;
     for (int a = 0; a < D; ++a) total *= n + 2;
     if (out_len < total * width) return mpm_set_error(MPM_E_INVALID, "mpm_read_grid: output too small");
-    mpm::Counters hc{};
-    int rc = read_counters(c, hc);
+    mpm::SceneState st{};
+    int rc = read_scene(c, 0, st, nullptr);
     if (rc) return rc;
-    const int nbk = std::min(hc.nblocks, c->bcap);
+    const int b0 = st.block_begin, nbk = std::max(st.block_end - st.block_begin, 0);
     std::vector<int> slots((size_t)std::max(nbk, 1));
     std::vector<float> vals((size_t)std::max(nbk, 1) * 64 * width);
     if (nbk > 0) {
-        HIPCHECK(hipMemcpy(slots.data(), c->block_slot, (size_t)nbk * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(slots.data(), c->block_slot + b0, (size_t)nbk * sizeof(int), hipMemcpyDeviceToHost));
         for (int w = 0; w < width; ++w)
-            HIPCHECK(hipMemcpy(&vals[(size_t)w * nbk * 64], src + (size_t)w * c->gcap, (size_t)nbk * 64 * sizeof(float), hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(&vals[(size_t)w * nbk * 64], src + (size_t)w * c->gcap + (size_t)b0 * 64,
+                               (size_t)nbk * 64 * sizeof(float), hipMemcpyDeviceToHost));
     }
     std::fill(out, out + total * width, 0.0f);
-    for (int b = 0; b < nbk; ++b) {
-        int bb[3];
-        int s = slots[(size_t)b];
-        for (int a = D - 1; a >= 0; --a) { bb[a] = s % c->nb; s /= c->nb; }
+    for (int b = 0; b < nbk; ++b)
         for (int l = 0; l < 64; ++l) {
-            int lc[3];
-            if (D == 3) { lc[0] = l >> 4; lc[1] = (l >> 2) & 3; lc[2] = l & 3; }
-            else { lc[0] = l >> 3; lc[1] = l & 7; }
-            int64_t o = 0;
-            bool ok = true;
-            for (int a = 0; a < D; ++a) {
-                const int i = B * bb[a] + lc[a] - B;
-                ok = ok && i >= -1 && i <= n;
-                o = o * (n + 2) + (i + 1);
-            }
-            if (!ok) continue;
+            int64_t o;
+            if (!dense_index(c, slots[(size_t)b], l, o)) continue;
             for (int w = 0; w < width; ++w) out[o * width + w] = vals[(size_t)w * nbk * 64 + (size_t)b * 64 + l];
         }
-    }
     return MPM_OK;
 }
 
 int mpm_read_flags(mpm_handle c, uint32_t* out, int64_t out_len) {
     if (!c || !out) return mpm_set_error(MPM_E_INVALID, "mpm_read_flags: null argument");
-    const int D = c->D, B = D == 3 ? 4 : 8, n = c->desc.counts;
+    const int D = c->D, n = c->sc[0].desc.counts;
     int64_t total = 1;
     for (int a = 0; a < D; ++a) total *= n + 2;
     if (out_len < total) return mpm_set_error(MPM_E_INVALID, "mpm_read_flags: output too small");
-    mpm::Counters hc{};
-    int rc = read_counters(c, hc);
+    mpm::SceneState st{};
+    int rc = read_scene(c, 0, st, nullptr);
     if (rc) return rc;
-    const int nbk = std::min(hc.nblocks, c->bcap);
+    const int b0 = st.block_begin, nbk = std::max(st.block_end - st.block_begin, 0);
     std::vector<int> slots((size_t)std::max(nbk, 1));
     std::vector<unsigned> fl((size_t)std::max(nbk, 1) * 64);
     if (nbk > 0) {
-        HIPCHECK(hipMemcpy(slots.data(), c->block_slot, (size_t)nbk * sizeof(int), hipMemcpyDeviceToHost));
-        HIPCHECK(hipMemcpy(fl.data(), c->g_flags, (size_t)nbk * 64 * sizeof(unsigned), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(slots.data(), c->block_slot + b0, (size_t)nbk * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(fl.data(), c->g_flags + (size_t)b0 * 64, (size_t)nbk * 64 * sizeof(unsigned), hipMemcpyDeviceToHost));
     }
     std::fill(out, out + total, 0u);
-    for (int b = 0; b < nbk; ++b) {
-        int bb[3];
-        int s = slots[(size_t)b];
-        for (int a = D - 1; a >= 0; --a) { bb[a] = s % c->nb; s /= c->nb; }
+    for (int b = 0; b < nbk; ++b)
         for (int l = 0; l < 64; ++l) {
-            int lc[3];
-            if (D == 3) { lc[0] = l >> 4; lc[1] = (l >> 2) & 3; lc[2] = l & 3; }
-            else { lc[0] = l >> 3; lc[1] = l & 7; }
-            int64_t o = 0;
-            bool ok = true;
-            for (int a = 0; a < D; ++a) {
-                const int i = B * bb[a] + lc[a] - B;
-                ok = ok && i >= -1 && i <= n;
-                o = o * (n + 2) + (i + 1);
-            }
-            if (ok) out[o] = fl[(size_t)b * 64 + l];
+            int64_t o;
+            if (dense_index(c, slots[(size_t)b], l, o)) out[o] = fl[(size_t)b * 64 + l];
         }
-    }
     return MPM_OK;
 }
 
@@ -732,27 +935,28 @@ int mpm_read_bins(mpm_handle c, int64_t cap_blocks, int32_t* block_coords, int32
                   int64_t cap_particles, int32_t* particle_block, int32_t* particle_ids, int32_t* nblocks,
                   int64_t* nparticles) {
     if (!c) return mpm_set_error(MPM_E_INVALID, "mpm_read_bins: null handle");
-    mpm::Counters hc{};
-    int rc = read_counters(c, hc);
+    mpm::SceneState st{};
+    int rc = read_scene(c, 0, st, nullptr);
     if (rc) return rc;
     const int D = c->D;
-    const int nbk = std::min(hc.nblocks, c->bcap);
-    const int64_t np = hc.n_live;
+    const int b0 = st.block_begin, nbk = std::max(st.block_end - st.block_begin, 0);
+    const int64_t p0 = st.pbegin, np = std::max(st.pend - st.pbegin, 0);
     if (nblocks) *nblocks = nbk;
     if (nparticles) *nparticles = np;
     std::vector<int> slots((size_t)std::max(nbk, 1)), pstart((size_t)nbk + 1);
     if (nbk > 0) {
-        HIPCHECK(hipMemcpy(slots.data(), c->block_slot, (size_t)nbk * sizeof(int), hipMemcpyDeviceToHost));
-        HIPCHECK(hipMemcpy(pstart.data(), c->block_pstart, ((size_t)nbk + 1) * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(slots.data(), c->block_slot + b0, (size_t)nbk * sizeof(int), hipMemcpyDeviceToHost));
+        HIPCHECK(hipMemcpy(pstart.data(), c->block_pstart + b0, ((size_t)nbk + 1) * sizeof(int), hipMemcpyDeviceToHost));
     } else {
-        pstart[0] = 0;
+        pstart[0] = (int)p0;
     }
+    const int nb = c->table[0].nb;
     if (block_coords || block_particle_counts) {
         if (cap_blocks < nbk) return mpm_set_error(MPM_E_CAPACITY, "mpm_read_bins: cap_blocks too small");
         for (int b = 0; b < nbk; ++b) {
-            int s = slots[(size_t)b];
+            int s = slots[(size_t)b] - c->table[0].slot_base;
             int bb[3];
-            for (int a = D - 1; a >= 0; --a) { bb[a] = s % c->nb; s /= c->nb; }
+            for (int a = D - 1; a >= 0; --a) { bb[a] = s % nb; s /= nb; }
             if (block_coords) for (int a = 0; a < D; ++a) block_coords[(int64_t)b * D + a] = bb[a];
             if (block_particle_counts) block_particle_counts[b] = pstart[(size_t)b + 1] - pstart[(size_t)b];
         }
@@ -760,17 +964,17 @@ int mpm_read_bins(mpm_handle c, int64_t cap_blocks, int32_t* block_coords, int32
     if (particle_block || particle_ids) {
         if (cap_particles < np) return mpm_set_error(MPM_E_CAPACITY, "mpm_read_bins: cap_particles too small");
         if (particle_ids && np > 0)
-            HIPCHECK(hipMemcpy(particle_ids, c->idbuf[c->cur], (size_t)np * sizeof(int), hipMemcpyDeviceToHost));
+            HIPCHECK(hipMemcpy(particle_ids, c->idbuf[c->cur] + p0, (size_t)np * sizeof(int), hipMemcpyDeviceToHost));
         if (particle_block)
             for (int b = 0; b < nbk; ++b)
-                for (int i = pstart[(size_t)b]; i < pstart[(size_t)b + 1]; ++i) particle_block[i] = b;
+                for (int i = pstart[(size_t)b]; i < pstart[(size_t)b + 1]; ++i) particle_block[i - p0] = b;
     }
     return MPM_OK;
 }
 
 int mpm_cg_stats(mpm_handle c, mpm_step_stats* stats) {
     if (!c || !stats) return mpm_set_error(MPM_E_INVALID, "mpm_cg_stats: null argument");
-    return fill_stats(c, stats);
+    return fill_stats(c, 0, stats);
 }
 
 int mpm_enable_timing(mpm_handle c, int32_t on) {
@@ -795,21 +999,32 @@ int mpm_kernel_times(mpm_handle c, int32_t nclasses, double* ms, int64_t* launch
 
 int mpm_time_cg_operator(mpm_handle c, int32_t reps, float* avg_ms) {
     if (!c || !avg_ms || reps <= 0) return mpm_set_error(MPM_E_INVALID, "mpm_time_cg_operator: bad argument");
-    if (c->last_dt <= 0.0f) return mpm_set_error(MPM_E_NOT_READY, "mpm_time_cg_operator: run a substep first");
-    // the operator kernel returns at once after a finished solve: clear the
-    // done flag for the timed launches and restore it afterwards
-    int done = 0;
-    HIPCHECK(hipMemcpyAsync(&done, &c->cg->done, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    if (!c->stepped) return mpm_set_error(MPM_E_NOT_READY, "mpm_time_cg_operator: run a substep first");
+    // the operator kernel skips finished solves: reopen them for the timed
+    // launches (same dt, same search direction) and restore the state after
+    const size_t n = (size_t)c->nscenes;
+    std::vector<mpm::CGState> saved(n), open(n);
+    mpm::Counters hc{};
     HIPCHECK(hipStreamSynchronize(c->stream));
-    HIPCHECK(hipMemsetAsync(&c->cg->done, 0, sizeof(int), c->stream));
+    HIPCHECK(hipMemcpy(saved.data(), c->cg, sizeof(mpm::CGState) * n, hipMemcpyDeviceToHost));
+    HIPCHECK(hipMemcpy(&hc, c->ctr, sizeof(hc), hipMemcpyDeviceToHost));
+    open = saved;
+    for (auto& g : open) g.done = 0;
+    mpm::Counters hopen = hc;
+    hopen.cg_ndone = 0;
+    HIPCHECK(hipMemcpy(c->cg, open.data(), sizeof(mpm::CGState) * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->ctr, &hopen, sizeof(hopen), hipMemcpyHostToDevice));
+    std::vector<float> dts(n);
+    for (size_t s = 0; s < n; ++s) dts[s] = c->sc[s].last_dt;
+    set_step(c, dts.data(), 0.0f, false, false);
     hipEvent_t e0, e1;
     HIPCHECK(hipEventCreate(&e0));
     HIPCHECK(hipEventCreate(&e1));
-    mpm::KP p = make_kp(c, c->last_dt);
+    mpm::KP p = make_kp(c);
     HIPCHECK(hipEventRecord(e0, c->stream));
     for (int r = 0; r < reps; ++r) {
-        if (c->D == 3) mpm::launch_cg_force<3>(p, c->stream, c->grid_block, false);
-        else mpm::launch_cg_force<2>(p, c->stream, c->grid_block, false);
+        if (c->D == 3) mpm::launch_cg_force<3>(p, c->stream, c->grid_block, false, 1);
+        else mpm::launch_cg_force<2>(p, c->stream, c->grid_block, false, 1);
     }
     HIPCHECK(hipEventRecord(e1, c->stream));
     HIPCHECK(hipEventSynchronize(e1));
@@ -817,8 +1032,8 @@ int mpm_time_cg_operator(mpm_handle c, int32_t reps, float* avg_ms) {
     HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
     HIPCHECK(hipEventDestroy(e0));
     HIPCHECK(hipEventDestroy(e1));
-    HIPCHECK(hipMemcpyAsync(&c->cg->done, &done, sizeof(int), hipMemcpyHostToDevice, c->stream));
-    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipMemcpy(c->cg, saved.data(), sizeof(mpm::CGState) * n, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->ctr, &hc, sizeof(hc), hipMemcpyHostToDevice));
     int rc = check_launch("mpm_time_cg_operator");
     if (rc) return rc;
     *avg_ms = ms / (float)reps;
@@ -827,5 +1042,92 @@ int mpm_time_cg_operator(mpm_handle c, int32_t reps, float* avg_ms) {
 
 const char* mpm_kernel_class_name(int32_t cls) { return (cls >= 0 && cls < KC_COUNT) ? kClassNames[cls] : ""; }
 int32_t mpm_kernel_class_count(void) { return KC_COUNT; }
+
+// ---- batches ----------------------------------------------------------------
+int mpm_batch_create(const mpm_scene_desc* descs, const int64_t* capacities, int32_t nscenes, mpm_handle* out) {
+    return create_batch(descs, capacities, nscenes, out);
+}
+
+int32_t mpm_batch_size(mpm_handle c) { return c ? c->nscenes : 0; }
+
+int mpm_batch_upload(mpm_handle c, int32_t scene, int64_t n, const float* x, const float* v, const float* F,
+                     const float* Jp, const float* mass, const float* volume) {
+    if (!c || !x || !v || !mass) return mpm_set_error(MPM_E_INVALID, "mpm_batch_upload: null argument");
+    if (scene < 0 || scene >= c->nscenes) return mpm_set_error(MPM_E_INVALID, "mpm_batch_upload: no such scene");
+    if (c->ever_binned) return mpm_set_error(MPM_E_NOT_READY, "mpm_batch_upload: the batch is already initialized");
+    int rc = upload_scene(c, scene, n, x, v, F, Jp, mass, volume);
+    if (rc) return rc;
+    return upload_max_speed(c);
+}
+
+int mpm_batch_substep(mpm_handle c, const float* dt) {
+    if (!c || !dt) return mpm_set_error(MPM_E_INVALID, "mpm_batch_substep: null argument");
+    if (!c->initialized) return mpm_set_error(MPM_E_NOT_READY, "mpm_batch_substep: call mpm_initialize first");
+    for (int s = 0; s < c->nscenes; ++s)
+        if (!std::isfinite(dt[s])) return mpm_set_error(MPM_E_INVALID, "mpm_batch_substep: dt must be finite");
+    return do_substep(c, dt);
+}
+
+int mpm_batch_max_velocity(mpm_handle c, float* out) {
+    if (!c || !out) return mpm_set_error(MPM_E_INVALID, "mpm_batch_max_velocity: null argument");
+    return max_velocities(c, out);
+}
+
+int mpm_batch_stats(mpm_handle c, int32_t scene, mpm_step_stats* stats) {
+    if (!c || !stats) return mpm_set_error(MPM_E_INVALID, "mpm_batch_stats: null argument");
+    if (scene < 0 || scene >= c->nscenes) return mpm_set_error(MPM_E_INVALID, "mpm_batch_stats: no such scene");
+    return fill_stats(c, scene, stats);
+}
+
+int mpm_batch_download(mpm_handle c, int32_t scene, int64_t cap, float* x, float* v, float* F, float* Jp,
+                       float* mass, float* volume, int32_t* ids, int64_t* n_out) {
+    if (!c) return mpm_set_error(MPM_E_INVALID, "mpm_batch_download: null handle");
+    if (scene < 0 || scene >= c->nscenes) return mpm_set_error(MPM_E_INVALID, "mpm_batch_download: no such scene");
+    return download_scene(c, scene, cap, x, v, F, Jp, mass, volume, ids, n_out);
+}
+
+int mpm_batch_advance_to_time(mpm_handle c, const double* time, const double* target, double* time_out,
+                              int32_t* substeps_out) {
+    if (!c || !time || !target) return mpm_set_error(MPM_E_INVALID, "mpm_batch_advance_to_time: null argument");
+    if (!c->initialized) return mpm_set_error(MPM_E_NOT_READY, "mpm_batch_advance_to_time: call mpm_initialize first");
+    const size_t n = (size_t)c->nscenes;
+    std::vector<float> t(n), tgt(n), vmax(n), dts(n);
+    std::vector<int> steps(n, 0);
+    std::vector<char> done(n, 0), last(n, 0);
+    for (size_t s = 0; s < n; ++s) {
+        t[s] = (float)time[s];
+        tgt[s] = (float)target[s];
+        done[s] = !(tgt[s] > t[s]);
+    }
+    for (;;) {
+        bool any = false;
+        int rc = max_velocities(c, vmax.data());
+        if (rc) return rc;
+        for (size_t s = 0; s < n; ++s) {
+            dts[s] = 0.0f;
+            if (done[s]) continue;
+            bool l = false;
+            const float dt = next_dt(c->sc[s].desc, vmax[s], t[s], tgt[s], l);
+            if (!(dt > 0.0f)) { done[s] = 1; continue; }
+            dts[s] = dt;
+            last[s] = l ? 1 : 0;
+            any = true;
+        }
+        if (!any) break;
+        rc = do_substep(c, dts.data());
+        if (rc) return rc;
+        for (size_t s = 0; s < n; ++s)
+            if (dts[s] > 0.0f) {
+                t[s] += dts[s];
+                ++steps[s];
+                if (last[s]) done[s] = 1;
+            }
+    }
+    for (size_t s = 0; s < n; ++s) {
+        if (time_out) time_out[s] = (double)t[s];
+        if (substeps_out) substeps_out[s] = steps[s];
+    }
+    return MPM_OK;
+}
 
 }  // extern "C"

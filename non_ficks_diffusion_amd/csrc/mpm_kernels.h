@@ -1,4 +1,10 @@
 // Kernel parameter block and launch entry points of the gfx950 substep.
+//
+// One launch advances a *batch* of independent scenes (a single scene is a
+// batch of one).  Scenes are concatenated: their block slots occupy
+// consecutive ranges of one slot table, their particles share one buffer
+// (grouped by scene after binning, because slots are scene-major), and
+// every block or particle looks its scene's parameters up in a scene table.
 #pragma once
 #include <hip/hip_runtime.h>
 #include <cstdint>
@@ -14,28 +20,10 @@ constexpr int SCAN_TILE = WG * SCAN_PER_THREAD;
 constexpr int MAX_PLANES = 8;
 constexpr unsigned MPM_FLAG_INTERIOR_DEV = 1u;  // Cell_Type_Interior (Flag_Setup_Helper.h:32)
 
-// Device counters of one scene.
-struct Counters {
-    int nblocks;        // active blocks after binning
-    int n_live;         // live particles (after the last binning / upload)
-    int n_prev;         // particles in the buffer before the last binning
-    int overflow;       // block capacity exceeded
-    int nonfinite;      // NaN/Inf or out-of-domain particle met in binning
-    unsigned max_v2;    // bits of max |V|^2 (non-negative float) after G2P
-    int n_removed;      // particles removed by the last G2P
-    int pad;
-};
-
-// CG scalars.  rho of iteration k lives in rho_hist[k & 1] so a kernel never
-// reads a slot that another work-group of the same kernel writes.
-struct CGState {
-    double rho_hist[2], alpha, beta, conv, rr, sq;
-    int iter, done, converged, multiplies;
-};
-
-struct KP {
-    // scene
-    int counts, nb, nslots;
+// Read-only parameters of one scene during a substep.
+struct SceneParams {
+    int counts, nb;            // cells per axis; block slots per axis (padded)
+    int slot_base, nslots;     // the scene's slots: [slot_base, slot_base + nslots)
     float dx, inv_dx;
     Material mat;
     float gravity[3];
@@ -44,30 +32,73 @@ struct KP {
     float pn[MAX_PLANES][3], ps[MAX_PLANES][3], pmu[MAX_PLANES];
     int crop_ghost_x, project_mode;
     double cg_tol;
-    int cg_max, cg_min;
+    int cg_min, cg_max;
+    float dt;                  // dt of this substep
+    int active;                // 0: the scene keeps its state this substep
+    int waiting;               // particles still need Process_Waiting_Particles
+    int pad;
+};
+
+// Per-scene results written by the kernels.
+struct SceneState {
+    int block_begin, block_end;  // active blocks of the scene after binning
+    int pbegin, pend;            // its particles after binning
+    unsigned max_v2;             // bits of max |V|^2 (non-negative float) after G2P
+    int n_removed;               // particles removed by the last G2P
+    int nonfinite;               // NaN/Inf particle met in binning
+    int pad;
+};
+
+// Batch counters.
+struct Counters {
+    int nblocks;        // active blocks after binning (all scenes)
+    int n_live;         // particles in the buffer after the last binning / upload
+    int n_prev;         // particles in the buffer before the last binning
+    int overflow;       // block capacity exceeded
+    int cg_ndone;       // scenes whose CG solve is finished
+    int pad[3];
+};
+
+// CG scalars of one scene.  rho of iteration k lives in rho_hist[k & 1].
+struct CGState {
+    double rho_hist[2], alpha, beta, conv, rr, sq;
+    int iter, done, converged, multiplies;
+};
+
+struct KP {
+    // scenes
+    SceneParams* scn;
+    SceneState* sst;
+    CGState* cg;
+    const int* slot_base;  // nscenes + 1 entries: scene of a slot by bisection
+    int nscenes;
     // particles: field-major arrays of capacity pcap
     float* pcur;
     float* palt;
     int* idcur;
     int* idalt;
-    int* keycur;     // home slot per particle, -1 = removed
+    int* keycur;     // global key slot*64 + cell per particle, -1 = removed or empty
     int* keyalt;
+    int* scur;       // scene of each particle
+    int* salt;
     int* prank;
     float* rcache;   // rotation R per particle (sorted order), field-major
     float* pop;      // per particle (sorted order) op_record_floats(D) floats:
                      // X and the linearised CG force operator
     int64_t pcap;
     // slots and blocks
+    int nslots;        // all scenes
     int* slot_count;
     int* slot_touched;
     int* slot_active;
     int* block_slot;
+    int* block_scene;
     int* block_pstart;
     int* block_nbr;
     int* cell_count;   // per slot, 64 cell counters (zero between substeps)
     int* cell_start;   // per active block, 65 particle offsets (cell ranges)
     int bcap;
-    int* scan_part;  // 2 * nparts
+    int* scan_part;    // 2 * nparts
     int nparts;
     // grid channels, capacity gcap nodes each; vector channels hold D arrays
     float* g_m;
@@ -78,34 +109,42 @@ struct KP {
     float* g_r;
     unsigned* g_flags;
     int64_t gcap;
-    float* tiles;     // per active block, (D+1) * (B+2)^D floats
+    float* tiles;    // per active block, (D+1) * (B+2)^D floats
     // control
     Counters* ctr;
-    CGState* cg;
-    double* red;      // 3 * red_stride partials: s.q | r.r | max |r|^2
-    int red_stride;   // node-kernel work-groups
-    int red_count;
-    unsigned* wg_part;  // 2 * wg_part_count: per-work-group max |v|^2 | removals of G2P
-    int wg_part_count;
-    float dt;
+    double* bsum;    // per active block partial sums (s.q or r.r)
+    double* bmax;    // per active block partial max |r|^2
+    unsigned* bu;    // 2 * bcap: per block max |V|^2 bits | removals (G2P)
+};
+
+// Per-substep scene settings, passed by value (kernel arguments are copied
+// at launch, so consecutive substeps never race on a host buffer).
+constexpr int STEP_ARGS_MAX = 480;
+struct StepArgs {
+    int first, n;          // scenes [first, first + n)
+    int force_active;      // 1: every scene active (grid build, stage-by-stage calls)
+    int clear_waiting;     // 1: volumes are set, clear the waiting flags
+    float dt[STEP_ARGS_MAX];
 };
 
 // launchers (host)
+void launch_set_step(const KP& p, hipStream_t s, const StepArgs& a);
 template <int D> void launch_bin(const KP& p, hipStream_t s, int grid);
-template <int D> void launch_scan(const KP& p, hipStream_t s);
+void launch_scan(const KP& p, hipStream_t s);
 template <int D> void launch_block_setup(const KP& p, hipStream_t s, int grid);
+void launch_scene_ranges(const KP& p, hipStream_t s);
 template <int D> void launch_reorder(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_p2g(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_normalize(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_init_volume(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_particle_force(const KP& p, hipStream_t s, int grid, bool constitutive, bool force);
 template <int D> void launch_grid_update(const KP& p, hipStream_t s, int grid, bool force, bool collision, bool cg_init, bool write_f);
-template <int D> void launch_cg_force(const KP& p, hipStream_t s, int grid, bool restart);
+template <int D> void launch_cg_force(const KP& p, hipStream_t s, int grid, bool restart, int k);
 template <int D> void launch_cg_restart_node(const KP& p, hipStream_t s, int grid);
-template <int D> void launch_cg_update_s(const KP& p, hipStream_t s, int grid, int k);
-template <int D> void launch_cg_node1(const KP& p, hipStream_t s, int grid);
-template <int D> void launch_cg_node2(const KP& p, hipStream_t s, int grid, int k);
-void launch_cg_finalize(const KP& p, hipStream_t s, int k_end);
+template <int D> void launch_cg_node1(const KP& p, hipStream_t s, int grid, int k);
+template <int D> void launch_cg_node2(const KP& p, hipStream_t s, int grid);
+void launch_cg_reduce_sq(const KP& p, hipStream_t s, int k);
+void launch_cg_reduce_rr(const KP& p, hipStream_t s, int k);
 template <int D> void launch_g2p(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_max_speed(const KP& p, hipStream_t s, int grid);
 

@@ -1,26 +1,29 @@
-// gfx950 kernels of the MPM snow substep.
+// gfx950 kernels of the MPM snow substep, for a batch of scenes.
 //
-// Layout in HBM (one scene):
+// Layout in HBM:
 //   particles  field-major FP32 arrays (X[D], V[D], F[D*D], Jp, m, V0) of
-//              capacity pcap, double buffered; after binning they are
-//              grouped by home block so every block's particles are one
+//              capacity pcap, double buffered, with a scene id per particle;
+//              after binning they are grouped by home block (so by scene,
+//              then block, then cell) and every block's particles are one
 //              contiguous range.
-//   grid       block-sparse: the padded domain is cut into 4^3 (3D) or
-//              8^2 (2D) node blocks; a dense slot table maps block
-//              coordinates to a compacted ordinal; every channel is an
-//              array of ordinal*64 + local-node entries.
+//   grid       block-sparse: each scene's padded domain is cut into 4^3
+//              (3D) or 8^2 (2D) node blocks; the scenes' dense slot tables
+//              are concatenated and map block coordinates to a compacted
+//              ordinal; every node channel is an array of ordinal*64 + local
+//              node entries.  Scene parameters are looked up per block.
 //   tiles      per active block, the (B+2)^D node tile (block plus a
 //              one-node halo) of each scattered channel.
 // Block kernels (P2G, force, CG operator, G2P) run one work-group per
 // active block.  A block's particles touch only its nodes and the one-node
-// halo, so scatters accumulate into an LDS tile and the work-group writes
+// halo, so scatters accumulate into LDS tiles and the work-group writes
 // the whole tile to the tile buffer with plain stores; the next node kernel
 // sums, for every node, the (at most 2^D) tiles that cover it in a fixed
 // order.  There are no global atomics on the scatter path, and the
 // cross-block sum is deterministic.  Gathers load the tile once from the
-// 3^D neighbour blocks.  Reductions (CG dot products and norms) write one
-// FP64 partial per work-group; the next kernel reduces the partials in a
-// fixed order in every work-group, so no fences or atomics are needed.
+// 3^D neighbour blocks.  Node kernels run one wave per block; reductions
+// (CG dot products and norms, G2P's max speed) write one partial per block,
+// and a one-work-group-per-scene kernel sums a scene's block partials in a
+// fixed order.
 #include "mpm_kernels.h"
 
 namespace mpm {
@@ -43,12 +46,25 @@ __device__ inline XcdRange xcd_blocks(int n) {
     const int lo = (int)(((int64_t)n * x) >> 3), hi = (int)(((int64_t)n * (x + 1)) >> 3);
     return {lo + k, hi, (int)(gridDim.x >> 3)};
 }
-// Node loops (one thread per node): the nodes of the same per-XCD block
-// range, so a node kernel reads the tiles its XCD's block kernel wrote.
-__device__ inline XcdRange xcd_nodes(int nblocks) {
-    const XcdRange r = xcd_blocks(nblocks);
-    const int lo = r.first - (int)(blockIdx.x >> 3);
-    return {lo * 64 + (int)(blockIdx.x >> 3) * WG + (int)threadIdx.x, r.end * 64, r.step * WG};
+// Node loops: one wave per block (lane = node), over the same per-XCD block
+// range as the block kernels, so a node kernel reads the tiles its XCD's
+// block kernel wrote.
+__device__ inline XcdRange xcd_wave_blocks(int n) {
+    constexpr int W = WG / 64;
+    const int x = (int)(blockIdx.x & 7), k = (int)(blockIdx.x >> 3);
+    const int lo = (int)(((int64_t)n * x) >> 3), hi = (int)(((int64_t)n * (x + 1)) >> 3);
+    return {lo + k * W + (int)(threadIdx.x >> 6), hi, (int)(gridDim.x >> 3) * W};
+}
+
+// Scene of a global slot: bisection over the scenes' first slots.
+__device__ inline int scene_of_slot(const KP& p, int slot) {
+    int lo = 0, hi = p.nscenes;  // slot_base[lo] <= slot < slot_base[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (p.slot_base[mid] <= slot) lo = mid;
+        else hi = mid;
+    }
+    return lo;
 }
 
 template <int D>
@@ -144,15 +160,15 @@ __device__ inline void stencil_grad(const float (*w)[3], const float (*dw)[3], c
 // which can only happen if particles moved after the last binning; no tile
 // index outside [0, B+1]^D is ever formed.
 template <int D>
-__device__ inline bool particle_weights(const KP& p, int i, const int* bb, float (*w)[3], float (*dw)[3], int* l,
-                                        float* X) {
+__device__ inline bool particle_weights(const KP& p, const SceneParams& sc, int i, const int* bb, float (*w)[3],
+                                        float (*dw)[3], int* l, float* X) {
     constexpr int B = Dims<D>::B;
     bool ok = true;
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         X[a] = p.pcur[(PF<D>::X + a) * p.pcap + i];
         int c;
-        bspline_weights(X[a], p.dx, p.inv_dx, c, w[a], dw[a]);
+        bspline_weights(X[a], sc.dx, sc.inv_dx, c, w[a], dw[a]);
         l[a] = c + B - B * bb[a];
         ok = ok && l[a] >= 0 && l[a] < B;
     }
@@ -164,7 +180,7 @@ __device__ inline bool particle_weights(const KP& p, int i, const int* bb, float
 // receive nothing (the reference's slab crop,
 // Tools/Cropped_Influence_Iterator.h:31-42).
 template <int D, int C>
-__device__ inline void gather_node_tiles(const KP& p, int n, float* out) {
+__device__ inline void gather_node_tiles(const KP& p, const SceneParams& sc, int n, float* out) {
     using Dm = Dims<D>;
     constexpr int B = Dm::B, TE = Dm::TE, TN = Dm::TN, NBR = Dm::NBR;
     const int b = n >> 6;
@@ -172,11 +188,11 @@ __device__ inline void gather_node_tiles(const KP& p, int n, float* out) {
     local_coords<D>(n & 63, lc);
 #pragma unroll
     for (int c = 0; c < C; ++c) out[c] = 0.0f;
-    if (p.crop_ghost_x) {
+    if (sc.crop_ghost_x) {
         int bb[D];
-        slot_coords<D>(p.block_slot[b], p.nb, bb);
+        slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
         const int ix = B * bb[0] + lc[0] - B;
-        if (ix < 0 || ix > p.counts - 1) return;
+        if (ix < 0 || ix > sc.counts - 1) return;
     }
     int od[D][2], td[D][2], nopt[D];
 #pragma unroll
@@ -213,64 +229,29 @@ __device__ inline void gather_node_tiles(const KP& p, int n, float* out) {
     }
 }
 
-// sum and max over the work-group; result valid in thread 0
-__device__ inline void wg_reduce2(double& sum, double& mx) {
-#pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        sum += __shfl_xor(sum, off);
-        mx = fmax(mx, __shfl_xor(mx, off));
-    }
-    __shared__ double ss[WG / 64], sm[WG / 64];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    __syncthreads();
-    if (lane == 0) { ss[wave] = sum; sm[wave] = mx; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double s = 0.0, m = 0.0;
-#pragma unroll
-        for (int k = 0; k < WG / 64; ++k) { s += ss[k]; m = fmax(m, sm[k]); }
-        sum = s;
-        mx = m;
-    }
-}
-
-// Reduces `count` partials (sum array, max array) in a fixed order; the
-// result is broadcast to every thread of the work-group.
-__device__ inline void reduce_partials(const double* sums, const double* maxs, int count, double& sum, double& mx) {
-    double s = 0.0, m = 0.0;
-    for (int k = threadIdx.x; k < count; k += WG) {
-        s += sums[k];
-        if (maxs) m = fmax(m, maxs[k]);
-    }
-    wg_reduce2(s, m);
-    __shared__ double bs, bm;
-    if (threadIdx.x == 0) { bs = s; bm = m; }
-    __syncthreads();
-    sum = bs;
-    mx = bm;
-}
-
-__device__ inline bool inside_plane(const KP& p, int id, const float* x, int D) {
+__device__ inline bool inside_plane(const SceneParams& sc, int id, const float* x, int D) {
     float s = 0.0f;
-    for (int a = 0; a < D; ++a) s += (x[a] - p.ps[id][a]) * p.pn[id][a];
+    for (int a = 0; a < D; ++a) s += (x[a] - sc.ps[id][a]) * sc.pn[id][a];
     return s < 0.0f;
 }
 
 template <int D>
-__device__ inline void node_center(const KP& p, int n, float* x) {
+__device__ inline void node_center(const KP& p, const SceneParams& sc, int n, float* x) {
     constexpr int B = Dims<D>::B;
     int bb[D], lc[D];
-    slot_coords<D>(p.block_slot[n >> 6], p.nb, bb);
+    slot_coords<D>(p.block_slot[n >> 6] - sc.slot_base, sc.nb, bb);
     local_coords<D>(n & 63, lc);
 #pragma unroll
     for (int a = 0; a < D; ++a) {
         const int i = B * bb[a] + lc[a] - B;
-        x[a] = ((float)i + 0.5f) * p.dx;
+        x[a] = ((float)i + 0.5f) * sc.dx;
     }
 }
 
 // ============================================================================
-// Binning: key = home block slot, rank within the block, activation marks
+// Binning: key = global slot*64 + cell of the home cell, rank within the
+// cell, activation marks of the home block's 3^D neighbourhood.  Empty and
+// removed entries carry key -1.
 // ============================================================================
 template <int D>
 __global__ __launch_bounds__(WG) void k_bin(KP p) {
@@ -282,29 +263,33 @@ __global__ __launch_bounds__(WG) void k_bin(KP p) {
         const int i = base + lane;
         bool valid = i < n && p.keycur[i] >= 0;
         int c[D];
-        bool finite = true;
+        int si = 0;
         if (valid) {
+            si = p.scur[i];
+            const SceneParams& sc = p.scn[si];
+            bool finite = true;
 #pragma unroll
             for (int a = 0; a < D; ++a) {
                 const float X = p.pcur[(PF<D>::X + a) * p.pcap + i];
                 finite = finite && isfinite(X);
-                c[a] = finite ? cell_of(X, p.inv_dx) : -1;
-                valid = valid && finite && c[a] >= 0 && c[a] <= p.counts - 1;
+                c[a] = finite ? cell_of(X, sc.inv_dx) : -1;
+                valid = valid && finite && c[a] >= 0 && c[a] <= sc.counts - 1;
             }
             if (!valid) {
                 p.keycur[i] = -1;
-                if (!finite) atomicOr(&p.ctr->nonfinite, 1);
+                if (!finite) atomicOr(&p.sst[si].nonfinite, 1);
             }
         }
         int slot = -1, key = -1;
         if (valid) {
-            slot = 0;
-            int cell = 0;
+            const SceneParams& sc = p.scn[si];
+            int local = 0, cell = 0;
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                slot = slot * p.nb + (c[a] + B) / B;
+                local = local * sc.nb + (c[a] + B) / B;
                 cell = cell * B + (c[a] + B) % B;
             }
+            slot = sc.slot_base + local;
             key = slot * 64 + cell;
         }
         // wave-aggregated atomics: one per distinct (block, cell) for the rank in
@@ -330,18 +315,20 @@ __global__ __launch_bounds__(WG) void k_bin(KP p) {
             todo &= ~grp;
         }
         if (valid) {
+            const SceneParams& sc = p.scn[si];
             p.prank[i] = rank;
             p.keycur[i] = key;
             int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
 #pragma unroll
             for (int a = 0; a < D; ++a) { lo[a] = (c[a] + B - 1) / B; hi[a] = (c[a] + B + 1) / B; }
+            const int nb = sc.nb, sb = sc.slot_base;
             if constexpr (D == 3) {
                 for (int x = lo[0]; x <= hi[0]; ++x)
                     for (int y = lo[1]; y <= hi[1]; ++y)
-                        for (int z = lo[2]; z <= hi[2]; ++z) p.slot_touched[(x * p.nb + y) * p.nb + z] = 1;
+                        for (int z = lo[2]; z <= hi[2]; ++z) p.slot_touched[sb + (x * nb + y) * nb + z] = 1;
             } else {
                 for (int x = lo[0]; x <= hi[0]; ++x)
-                    for (int y = lo[1]; y <= hi[1]; ++y) p.slot_touched[x * p.nb + y] = 1;
+                    for (int y = lo[1]; y <= hi[1]; ++y) p.slot_touched[sb + x * nb + y] = 1;
             }
         }
     }
@@ -415,8 +402,6 @@ __global__ __launch_bounds__(WG) void k_scan_top(KP p) {
     }
     if (threadIdx.x == 0) {
         p.ctr->n_prev = p.ctr->n_live;
-        p.ctr->max_v2 = 0u;
-        p.ctr->n_removed = 0;
         if (ta > p.bcap) {
             // the substep becomes a no-op: particles keep their order and state
             p.ctr->overflow = 1;
@@ -464,9 +449,10 @@ __global__ __launch_bounds__(WG) void k_scan_apply(KP p) {
     }
 }
 
-// Per active block (one wave, lane = cell): neighbour table, and the cell
-// ranges -- an exclusive scan of the block's 64 cell counts -- after which
-// the cell counters are cleared for the next binning.
+// Per active block (one wave, lane = cell): its scene, the neighbour table
+// (inside the scene's slot range), and the cell ranges -- an exclusive scan
+// of the block's 64 cell counts -- after which the cell counters are
+// cleared for the next binning.
 template <int D>
 __global__ __launch_bounds__(WG) void k_block_setup(KP p) {
     using Dm = Dims<D>;
@@ -474,9 +460,12 @@ __global__ __launch_bounds__(WG) void k_block_setup(KP p) {
     const int group = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int b = blockIdx.x * (WG / 64) + group; b < nblocks; b += gridDim.x * (WG / 64)) {
         const int slot = p.block_slot[b];
+        const int si = scene_of_slot(p, slot);
+        const SceneParams& sc = p.scn[si];
+        if (lane == 0) p.block_scene[b] = si;
         if (lane < Dm::NBR) {
             int bb[D];
-            slot_coords<D>(slot, p.nb, bb);
+            slot_coords<D>(slot - sc.slot_base, sc.nb, bb);
             int o[3];
             if constexpr (D == 3) { o[0] = lane / 9; o[1] = (lane / 3) % 3; o[2] = lane % 3; }
             else { o[0] = lane / 3; o[1] = lane % 3; }
@@ -485,10 +474,10 @@ __global__ __launch_bounds__(WG) void k_block_setup(KP p) {
 #pragma unroll
             for (int a = 0; a < D; ++a) {
                 const int c = bb[a] + o[a] - 1;
-                in = in && c >= 0 && c < p.nb;
-                s = s * p.nb + c;
+                in = in && c >= 0 && c < sc.nb;
+                s = s * sc.nb + c;
             }
-            p.block_nbr[b * Dm::NBR + lane] = in ? p.slot_active[s] : -1;
+            p.block_nbr[b * Dm::NBR + lane] = in ? p.slot_active[sc.slot_base + s] : -1;
         }
         const int cnt = p.cell_count[slot * 64 + lane];
         int x = cnt;
@@ -502,6 +491,30 @@ __global__ __launch_bounds__(WG) void k_block_setup(KP p) {
         if (lane == 63) p.cell_start[b * 65 + 64] = start + cnt;
         p.cell_count[slot * 64 + lane] = 0;
     }
+}
+
+// Per scene: its active block range and particle range (active blocks are
+// in slot order, so each scene's blocks are contiguous).
+__global__ __launch_bounds__(WG) void k_scene_ranges(KP p) {
+    const int si = blockIdx.x * WG + threadIdx.x;
+    if (si >= p.nscenes) return;
+    const int nb = p.ctr->nblocks;
+    auto lower = [&](int slot) {
+        int lo = 0, hi = nb;
+        while (lo < hi) {
+            const int mid = (lo + hi) >> 1;
+            if (p.block_slot[mid] < slot) lo = mid + 1;
+            else hi = mid;
+        }
+        return lo;
+    };
+    const SceneParams& sc = p.scn[si];
+    const int b0 = lower(sc.slot_base), b1 = lower(sc.slot_base + sc.nslots);
+    SceneState& st = p.sst[si];
+    st.block_begin = b0;
+    st.block_end = b1;
+    st.pbegin = nb > 0 ? p.block_pstart[b0] : 0;
+    st.pend = nb > 0 ? p.block_pstart[b1] : 0;
 }
 
 // scatter into (block, cell) order
@@ -518,6 +531,7 @@ __global__ __launch_bounds__(WG) void k_reorder(KP p) {
 #pragma unroll
         for (int f = 0; f < PF<D>::N; ++f) p.palt[f * p.pcap + dest] = p.pcur[f * p.pcap + i];
         p.idalt[dest] = p.idcur[i];
+        p.salt[dest] = p.scur[i];
         p.keyalt[dest] = overflow ? key : 0;
     }
 }
@@ -561,10 +575,10 @@ __device__ inline void store_wave_tiles(const KP& p, int b, const float* lds) {
 // Weights of particle i; false if it is not in cell l of block bb (only after
 // particles moved without re-binning: the particle is skipped).
 template <int D>
-__device__ inline bool cell_particle(const KP& p, int i, const int* bb, const int* l, float (*w)[3], float (*dw)[3],
-                                     float* X) {
+__device__ inline bool cell_particle(const KP& p, const SceneParams& sc, int i, const int* bb, const int* l,
+                                     float (*w)[3], float (*dw)[3], float* X) {
     int lp[D];
-    if (!particle_weights<D>(p, i, bb, w, dw, lp, X)) return false;
+    if (!particle_weights<D>(p, sc, i, bb, w, dw, lp, X)) return false;
     bool same = true;
 #pragma unroll
     for (int a = 0; a < D; ++a) same = same && lp[a] == l[a];
@@ -915,14 +929,16 @@ __global__ __launch_bounds__(BWG) void k_p2g(KP p) {
     local_coords<D>(c, l);
     const XcdRange xr = xcd_blocks(nblocks);
     for (int b = xr.first; b < xr.end; b += xr.step) {
+        const SceneParams& sc = p.scn[p.block_scene[b]];
+        if (!sc.active) continue;
         for (int t = threadIdx.x; t < NW * C * LTN; t += BWG) tile[t] = 0.0f;
         int bb[D];
-        slot_coords<D>(p.block_slot[b], p.nb, bb);
+        slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
         __syncthreads();
         const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
         for (int i = i0 + wv; i < i1; i += NW) {
             float w[D][3], dw[D][3], X[D];
-            if (!cell_particle<D>(p, i, bb, l, w, dw, X)) continue;
+            if (!cell_particle<D>(p, sc, i, bb, l, w, dw, X)) continue;
             const float m = p.pcur[PF<D>::M * p.pcap + i];
             float mv[D];
 #pragma unroll
@@ -948,11 +964,14 @@ __global__ __launch_bounds__(BWG) void k_p2g(KP p) {
 // Velocity_Normalization_Helper (Velocity_Normalization_Helper.h:39-52)
 template <int D>
 __global__ __launch_bounds__(WG) void k_normalize(KP p) {
-    const int nn = p.ctr->nblocks * 64;
-    const XcdRange nr = xcd_nodes(nn >> 6);
-    for (int n = nr.first; n < nr.end; n += nr.step) {
+    const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
+    const int lane = threadIdx.x & 63;
+    for (int b = r.first; b < r.end; b += r.step) {
+        const SceneParams& sc = p.scn[p.block_scene[b]];
+        if (!sc.active) continue;
+        const int n = b * 64 + lane;
         float v[D + 1];
-        gather_node_tiles<D, D + 1>(p, n, v);
+        gather_node_tiles<D, D + 1>(p, sc, n, v);
         const float m = v[0];
         const bool interior = m > 0.0f;
         p.g_m[n] = m;
@@ -985,14 +1004,16 @@ __global__ __launch_bounds__(BWG) void k_init_volume(KP p) {
     __shared__ float tile[1][LTN];
     __shared__ int nbr[NBR];
     const int nblocks = p.ctr->nblocks;
-    float vol = 1.0f;
-    for (int a = 0; a < D; ++a) vol *= p.dx;
-    const float inv_vol = 1.0f / vol;
     const XcdRange xr = xcd_blocks(nblocks);
     for (int b = xr.first; b < xr.end; b += xr.step) {
+        const SceneParams& sc = p.scn[p.block_scene[b]];
+        if (!sc.waiting) continue;
+        float vol = 1.0f;
+        for (int a = 0; a < D; ++a) vol *= sc.dx;
+        const float inv_vol = 1.0f / vol;
         if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
-        slot_coords<D>(p.block_slot[b], p.nb, bb);
+        slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
         __syncthreads();
         const float* src[1] = {p.g_m};
         gather_tile<D, 1>(p, nbr, tile, src);
@@ -1001,7 +1022,7 @@ __global__ __launch_bounds__(BWG) void k_init_volume(KP p) {
         for (int i = p.block_pstart[b] + (int)threadIdx.x; i < i1; i += BWG) {
             float w[D][3], dw[D][3], X[D];
             int l[D];
-            if (!particle_weights<D>(p, i, bb, w, dw, l, X)) continue;
+            if (!particle_weights<D>(p, sc, i, bb, w, dw, l, X)) continue;
             float rho = 0.0f;
 #pragma unroll
             for (int s = 0; s < Dm::ST; ++s) {
@@ -1032,8 +1053,10 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
     local_coords<D>(c, l);
     const XcdRange xr = xcd_blocks(nblocks);
     for (int b = xr.first; b < xr.end; b += xr.step) {
+        const SceneParams& sc = p.scn[p.block_scene[b]];
+        if (!sc.active) continue;
         int bb[D];
-        slot_coords<D>(p.block_slot[b], p.nb, bb);
+        slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
         if (FORCE) {
             for (int t = threadIdx.x; t < NW * D * LTN; t += BWG) tile[t] = 0.0f;
             __syncthreads();
@@ -1046,7 +1069,7 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
             float Jp = p.pcur[PF<D>::J * p.pcap + i];
             if (CONST) {
                 float U[DD];
-                plasticity<D>(F, Jp, R, p.mat, U);
+                plasticity<D>(F, Jp, R, sc.mat, U);
 #pragma unroll
                 for (int k = 0; k < DD; ++k) {
                     p.pcur[(PF<D>::F + k) * p.pcap + i] = F[k];
@@ -1054,7 +1077,7 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
                 }
                 p.pcur[PF<D>::J * p.pcap + i] = Jp;
                 StressState<D> so;
-                stress_state<D>(F, R, Jp, p.mat, so);
+                stress_state<D>(F, R, Jp, sc.mat, so);
                 float Xo[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) Xo[a] = p.pcur[(PF<D>::X + a) * p.pcap + i];
@@ -1065,9 +1088,9 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
             }
             if (FORCE) {
                 float w[D][3], dw[D][3], X[D];
-                if (!cell_particle<D>(p, i, bb, l, w, dw, X)) continue;
+                if (!cell_particle<D>(p, sc, i, bb, l, w, dw, X)) continue;
                 StressState<D> st;
-                stress_state<D>(F, R, Jp, p.mat, st);
+                stress_state<D>(F, R, Jp, sc.mat, st);
                 float P[DD];
                 first_pk<D>(st, P);
                 const float V0 = p.pcur[PF<D>::VOL * p.pcap + i];
@@ -1084,7 +1107,7 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
                     }
                 float gm[D];
 #pragma unroll
-                for (int a = 0; a < D; ++a) gm[a] = p.gravity[a] * m;
+                for (int a = 0; a < D; ++a) gm[a] = sc.gravity[a] * m;
 #pragma unroll
                 for (int s = 0; s < ST; ++s) {
                     int off[D];
@@ -1119,23 +1142,31 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
 // ============================================================================
 template <int D, bool FORCE, bool COLL, bool CGINIT, bool WRITE_F>
 __global__ __launch_bounds__(WG) void k_grid_update(KP p) {
-    if (CGINIT && blockIdx.x == 0 && threadIdx.x == 0) {
-        p.cg->iter = 0;
-        p.cg->done = 0;
-        p.cg->converged = 0;
-        p.cg->multiplies = 0;
-        p.cg->rho_hist[0] = p.cg->rho_hist[1] = 0.0;
-        p.cg->alpha = p.cg->beta = p.cg->conv = p.cg->rr = p.cg->sq = 0.0;
+    if (CGINIT) {
+        // a new solve for every scene (inactive scenes finish at the first test)
+        for (int si = blockIdx.x * WG + threadIdx.x; si < p.nscenes; si += gridDim.x * WG) {
+            CGState& cg = p.cg[si];
+            cg.iter = 0;
+            cg.done = 0;
+            cg.converged = 0;
+            cg.multiplies = 0;
+            cg.rho_hist[0] = cg.rho_hist[1] = 0.0;
+            cg.alpha = cg.beta = cg.conv = cg.rr = cg.sq = 0.0;
+        }
+        if (blockIdx.x == 0 && threadIdx.x == 0) p.ctr->cg_ndone = 0;
     }
-    const int nn = p.ctr->nblocks * 64;
-    const float dt = p.dt;
-    const XcdRange nr = xcd_nodes(nn >> 6);
-    for (int n = nr.first; n < nr.end; n += nr.step) {
+    const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
+    const int lane = threadIdx.x & 63;
+    for (int b = r.first; b < r.end; b += r.step) {
+        const SceneParams& sc = p.scn[p.block_scene[b]];
+        if (!sc.active) continue;
+        const int n = b * 64 + lane;
+        const float dt = sc.dt;
         const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
         float vs[D];
         if (FORCE) {
             float f[D];
-            gather_node_tiles<D, D>(p, n, f);
+            gather_node_tiles<D, D>(p, sc, n, f);
             if (WRITE_F) {
 #pragma unroll
                 for (int a = 0; a < D; ++a) p.g_q[a * p.gcap + n] = f[a];
@@ -1154,20 +1185,20 @@ __global__ __launch_bounds__(WG) void k_grid_update(KP p) {
         }
         if (COLL && interior) {
             float x[D];
-            node_center<D>(p, n, x);
-            for (int id = 0; id < p.nplanes; ++id) {
-                if (!inside_plane(p, id, x, D)) continue;
+            node_center<D>(p, sc, n, x);
+            for (int id = 0; id < sc.nplanes; ++id) {
+                if (!inside_plane(sc, id, x, D)) continue;
                 float proj = 0.0f;
 #pragma unroll
-                for (int a = 0; a < D; ++a) proj += vs[a] * p.pn[id][a];
+                for (int a = 0; a < D; ++a) proj += vs[a] * sc.pn[id][a];
                 if (proj < 0.0f) {
 #pragma unroll
-                    for (int a = 0; a < D; ++a) vs[a] -= p.pn[id][a] * proj;
+                    for (int a = 0; a < D; ++a) vs[a] -= sc.pn[id][a] * proj;
                     float nrm = 0.0f;
 #pragma unroll
                     for (int a = 0; a < D; ++a) nrm += vs[a] * vs[a];
                     nrm = sqrtf(nrm);
-                    const float mu = p.pmu[id];
+                    const float mu = sc.pmu[id];
                     if (-proj * mu < nrm) {
 #pragma unroll
                         for (int a = 0; a < D; ++a) vs[a] += vs[a] / nrm * proj * mu;
@@ -1193,19 +1224,37 @@ __global__ __launch_bounds__(WG) void k_grid_update(KP p) {
 }
 
 // ============================================================================
-// CG operator: MPM_CG_System::Force (MPM_CG_System.h:150-215)
-//   T = sum_i grad w_ip (x) u_i;  dF = T^T F;  A = V0 F dP(dF)^T;  q_i += A^T grad w_ip
+// CG (Apply_Force's solve, MPM_Example.cpp:700-711) for every scene of the
+// batch at once: Hestenes-Stiefel CG in the M inner product with a restart
+// at iteration 0.  Launch order per solve:
+//   grid_update<CGINIT>, cg_force<RESTART>, cg_restart_node, cg_reduce_rr(0),
+//   then for k = 0, 1, ...: cg_force(k), cg_node1(k), cg_reduce_sq(k),
+//   cg_node2, cg_reduce_rr(k + 1).
+// A scene whose solve is finished is skipped by every kernel; when all are,
+// the kernels return at once.
 // ============================================================================
+
+// s = r (k = 0) or r + beta s: the one formula shared by the operator's
+// on-the-fly gather and node1's store, so both see identical bits
+__device__ inline float cg_direction(float r, float s, float beta, int k) {
+    return k == 0 ? r : fmaf(beta, s, r);
+}
+
+__device__ inline bool cg_all_done(const KP& p) { return p.ctr->cg_ndone >= p.nscenes; }
+
+// MPM_CG_System::Force (MPM_CG_System.h:150-215) with the linearised operator
+// record: T = sum_i grad w_ip (x) u_i;  A = L T;  q_i += A^T grad w_ip.
+// RESTART applies it to v* (the initial guess); otherwise to the search
+// direction of iteration k, s = r + beta s, formed while gathering.
 template <int D, bool RESTART>
-__global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
+__global__ __launch_bounds__(BWG) void k_cg_force(KP p, int k) {
     using Dm = Dims<D>;
     constexpr int LTN = Dm::LTN, NBR = Dm::NBR, DD = D * D;
     __shared__ float ut[D][LTN];
     __shared__ float qt[NW * D * LTN];
     __shared__ int nbr[NBR];
-    if (p.cg->done) return;
+    if (cg_all_done(p)) return;
     const int nblocks = p.ctr->nblocks;
-    const float* u = RESTART ? p.g_vs : p.g_s;
     const int c = threadIdx.x & 63, wv = threadIdx.x >> 6;
     float* mine = qt + wv * D * LTN;
     int l[D];
@@ -1217,15 +1266,30 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
     }
     const XcdRange xr = xcd_blocks(nblocks);
     for (int b = xr.first; b < xr.end; b += xr.step) {
+        const int si = p.block_scene[b];
+        const SceneParams& sc = p.scn[si];
+        if (!sc.active || p.cg[si].done) continue;
+        const float beta = RESTART ? 0.0f : (float)p.cg[si].beta;
         if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
-        slot_coords<D>(p.block_slot[b], p.nb, bb);
+        slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
         for (int t = threadIdx.x; t < NW * D * LTN; t += BWG) qt[t] = 0.0f;
         __syncthreads();
-        const float* src[D];
+        for (int t = threadIdx.x; t < Dm::TN; t += BWG) {
+            int nb, local, tc[D];
+            tile_map<D>(t, nb, local, tc);
+            const int blk = nbr[nb];
+            const int li = lds_index<D>(tc);
+            const int64_t n = (int64_t)blk * 64 + local;
 #pragma unroll
-        for (int a = 0; a < D; ++a) src[a] = u + a * p.gcap;
-        gather_tile<D, D>(p, nbr, ut, src);
+            for (int a = 0; a < D; ++a) {
+                float u = 0.0f;
+                if (blk >= 0)
+                    u = RESTART ? p.g_vs[a * p.gcap + n]
+                                : cg_direction(p.g_r[a * p.gcap + n], p.g_s[a * p.gcap + n], beta, k);
+                ut[a][li] = u;
+            }
+        }
         __syncthreads();
         const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
         for (int i = i0 + wv; i < i1; i += NW) {
@@ -1238,7 +1302,7 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
 #pragma unroll
                 for (int a = 0; a < D; ++a) {
                     int cl;
-                    bspline_weights(X[a], p.dx, p.inv_dx, cl, w[a], dw[a]);
+                    bspline_weights(X[a], sc.dx, sc.inv_dx, cl, w[a], dw[a]);
                     same = same && cl + Dm::B - Dm::B * bb[a] == l[a];
                 }
             }
@@ -1257,198 +1321,212 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p) {
 
 // Project (Project_Helper.h:30-45): zero at nodes inside a plane
 template <int D>
-__device__ inline bool project_zero(const KP& p, int n) {
-    if (p.project_mode != 0 || p.nplanes == 0) return false;
+__device__ inline bool project_zero(const KP& p, const SceneParams& sc, int n) {
+    if (sc.project_mode != 0 || sc.nplanes == 0) return false;
     float x[D];
-    node_center<D>(p, n, x);
-    for (int id = 0; id < p.nplanes; ++id)
-        if (inside_plane(p, id, x, D)) return true;
+    node_center<D>(p, sc, n, x);
+    for (int id = 0; id < sc.nplanes; ++id)
+        if (inside_plane(sc, id, x, D)) return true;
     return false;
 }
 
+// sum and max over a wave; result valid in every lane
+__device__ inline void wave_reduce2(double& sum, double& mx) {
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) {
+        sum += __shfl_xor(sum, off);
+        mx = fmax(mx, __shfl_xor(mx, off));
+    }
+}
+
 // CG restart: q = dt^2/m q + x (Multiply_Helper), r = b - q, Project(r);
-// partials of r.r (M-weighted) and max |r|^2.
+// per-block partials of r.r (M-weighted) and max |r|^2.
 template <int D>
 __global__ __launch_bounds__(WG) void k_cg_restart_node(KP p) {
-    const int nn = p.ctr->nblocks * 64;
-    const float sds = p.dt * p.dt;
-    if (blockIdx.x == 0 && threadIdx.x == 0) p.cg->multiplies = 1;
-    double sum = 0.0, mx = 0.0;
-    const XcdRange nr = xcd_nodes(nn >> 6);
-    for (int n = nr.first; n < nr.end; n += nr.step) {
+    const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
+    const int lane = threadIdx.x & 63;
+    for (int b = r.first; b < r.end; b += r.step) {
+        const SceneParams& sc = p.scn[p.block_scene[b]];
+        if (!sc.active) continue;
+        const int n = b * 64 + lane;
+        const float sds = sc.dt * sc.dt;
         const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
-        float q[D], r[D];
-        gather_node_tiles<D, D>(p, n, q);
+        float q[D], rv[D];
+        gather_node_tiles<D, D>(p, sc, n, q);
 #pragma unroll
-        for (int a = 0; a < D; ++a) r[a] = p.g_r[a * p.gcap + n];
+        for (int a = 0; a < D; ++a) rv[a] = p.g_r[a * p.gcap + n];
         if (interior) {
-            const float c = sds / p.g_m[n];
+            const float cm = sds / p.g_m[n];
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                q[a] = c * q[a] + p.g_vs[a * p.gcap + n];
-                r[a] = r[a] - q[a];
+                q[a] = cm * q[a] + p.g_vs[a * p.gcap + n];
+                rv[a] = rv[a] - q[a];
             }
         }
-        if (project_zero<D>(p, n)) {
+        if (project_zero<D>(p, sc, n)) {
 #pragma unroll
-            for (int a = 0; a < D; ++a) r[a] = 0.0f;
+            for (int a = 0; a < D; ++a) rv[a] = 0.0f;
         }
 #pragma unroll
         for (int a = 0; a < D; ++a) {
-            p.g_r[a * p.gcap + n] = r[a];
+            p.g_r[a * p.gcap + n] = rv[a];
             p.g_q[a * p.gcap + n] = q[a];
         }
+        double sum = 0.0, mx = 0.0;
         if (interior) {
             const double m = (double)p.g_m[n];
-            double rr = 0.0, r2 = 0.0;
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                rr += (double)r[a] * ((double)r[a] * m);
-                r2 += (double)r[a] * (double)r[a];
+                sum += (double)rv[a] * ((double)rv[a] * m);
+                mx += (double)rv[a] * (double)rv[a];
             }
-            sum += rr;
-            mx = fmax(mx, r2);
         }
-    }
-    wg_reduce2(sum, mx);
-    if (threadIdx.x == 0) {
-        p.red[p.red_stride + blockIdx.x] = sum;
-        p.red[2 * p.red_stride + blockIdx.x] = mx;
-    }
-}
-
-// Convergence test of iteration k and s = r (k = 0) or s = (rho/rho_old) s + r.
-// Every work-group reduces the r.r / max|r|^2 partials of the previous
-// kernel itself (same order, same result); work-group 0 records the state.
-template <int D>
-__global__ __launch_bounds__(WG) void k_cg_update_s(KP p, int k) {
-    CGState* cg = p.cg;
-    if (cg->done) return;
-    double rr, mx;
-    reduce_partials(p.red + p.red_stride, p.red + 2 * p.red_stride, p.red_count, rr, mx);
-    const double conv = sqrt(mx);
-    bool done = false, converged = false;
-    if (conv <= p.cg_tol && (k >= p.cg_min || conv < (double)FLT_EPSILON)) {
-        done = converged = true;
-    }
-    const double rho_old = cg->rho_hist[(k + 1) & 1];
-    const double beta = k == 0 ? 0.0 : rr / rho_old;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        cg->iter = k;
-        cg->conv = conv;
-        cg->rr = rr;
-        if (done) {
-            cg->done = 1;
-            cg->converged = converged ? 1 : 0;
-        } else {
-            cg->rho_hist[k & 1] = rr;
-            cg->beta = beta;
-        }
-    }
-    if (done) return;
-    const int nn = p.ctr->nblocks * 64;
-    const float fb = (float)beta;
-    const XcdRange nr = xcd_nodes(nn >> 6);
-    for (int n = nr.first; n < nr.end; n += nr.step) {
-        if (!(p.g_flags[n] & MPM_FLAG_INTERIOR_DEV)) continue;
-#pragma unroll
-        for (int a = 0; a < D; ++a) {
-            const float r = p.g_r[a * p.gcap + n];
-            p.g_s[a * p.gcap + n] = k == 0 ? r : fb * p.g_s[a * p.gcap + n] + r;
+        wave_reduce2(sum, mx);
+        if (lane == 0) {
+            p.bsum[b] = sum;
+            p.bmax[b] = mx;
         }
     }
 }
 
-// tile sum of the operator, q = dt^2/m q + s, Project(q); partials of s.q
+// s = r + beta s stored for this iteration (interior nodes), tile sum of
+// the operator, q = dt^2/m q + s, Project(q); per-block partials of s.q
 template <int D>
-__global__ __launch_bounds__(WG) void k_cg_node1(KP p) {
-    if (p.cg->done) return;
-    const int nn = p.ctr->nblocks * 64;
-    const float sds = p.dt * p.dt;
-    double sum = 0.0, mx = 0.0;
-    const XcdRange nr = xcd_nodes(nn >> 6);
-    for (int n = nr.first; n < nr.end; n += nr.step) {
+__global__ __launch_bounds__(WG) void k_cg_node1(KP p, int k) {
+    if (cg_all_done(p)) return;
+    const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
+    const int lane = threadIdx.x & 63;
+    for (int b = r.first; b < r.end; b += r.step) {
+        const int si = p.block_scene[b];
+        const SceneParams& sc = p.scn[si];
+        if (!sc.active || p.cg[si].done) continue;
+        const float beta = (float)p.cg[si].beta;
+        const float sds = sc.dt * sc.dt;
+        const int n = b * 64 + lane;
         const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
-        float q[D];
-        gather_node_tiles<D, D>(p, n, q);
+        float q[D], sv[D];
+        gather_node_tiles<D, D>(p, sc, n, q);
         if (interior) {
-            const float c = sds / p.g_m[n];
+            const float cm = sds / p.g_m[n];
 #pragma unroll
-            for (int a = 0; a < D; ++a) q[a] = c * q[a] + p.g_s[a * p.gcap + n];
+            for (int a = 0; a < D; ++a) {
+                sv[a] = cg_direction(p.g_r[a * p.gcap + n], p.g_s[a * p.gcap + n], beta, k);
+                p.g_s[a * p.gcap + n] = sv[a];
+                q[a] = cm * q[a] + sv[a];
+            }
         }
-        if (project_zero<D>(p, n)) {
+        if (project_zero<D>(p, sc, n)) {
 #pragma unroll
             for (int a = 0; a < D; ++a) q[a] = 0.0f;
         }
 #pragma unroll
         for (int a = 0; a < D; ++a) p.g_q[a * p.gcap + n] = q[a];
+        double sum = 0.0, mx = 0.0;
         if (interior) {
             const double m = (double)p.g_m[n];
-            double sq = 0.0;
 #pragma unroll
-            for (int a = 0; a < D; ++a) sq += (double)p.g_s[a * p.gcap + n] * ((double)q[a] * m);
-            sum += sq;
+            for (int a = 0; a < D; ++a) sum += (double)sv[a] * ((double)q[a] * m);
         }
+        wave_reduce2(sum, mx);
+        if (lane == 0) p.bsum[b] = sum;
     }
-    wg_reduce2(sum, mx);
-    if (threadIdx.x == 0) p.red[blockIdx.x] = sum;
 }
 
-// alpha = rho / s.q;  x += alpha s, r -= alpha q; partials of r.r and max|r|^2
+// x += alpha s, r -= alpha q; per-block partials of r.r and max |r|^2
 template <int D>
-__global__ __launch_bounds__(WG) void k_cg_node2(KP p, int k) {
-    CGState* cg = p.cg;
-    if (cg->done) return;
-    double sq, unused;
-    reduce_partials(p.red, nullptr, p.red_count, sq, unused);
-    const double alpha_d = sq != 0.0 ? cg->rho_hist[k & 1] / sq : (double)FLT_MAX;
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        cg->sq = sq;
-        cg->alpha = alpha_d;
-        cg->multiplies += 1;
-    }
-    const float alpha = (float)alpha_d;
-    const int nn = p.ctr->nblocks * 64;
-    double sum = 0.0, mx = 0.0;
-    const XcdRange nr = xcd_nodes(nn >> 6);
-    for (int n = nr.first; n < nr.end; n += nr.step) {
+__global__ __launch_bounds__(WG) void k_cg_node2(KP p) {
+    if (cg_all_done(p)) return;
+    const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
+    const int lane = threadIdx.x & 63;
+    for (int b = r.first; b < r.end; b += r.step) {
+        const int si = p.block_scene[b];
+        const SceneParams& sc = p.scn[si];
+        if (!sc.active || p.cg[si].done) continue;
+        const float alpha = (float)p.cg[si].alpha;
+        const int n = b * 64 + lane;
         const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
-        if (!interior) continue;
-        const double m = (double)p.g_m[n];
-        double rr = 0.0, r2 = 0.0;
+        double sum = 0.0, mx = 0.0;
+        if (interior) {
+            const double m = (double)p.g_m[n];
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            const float s = p.g_s[a * p.gcap + n], q = p.g_q[a * p.gcap + n];
-            p.g_vs[a * p.gcap + n] = alpha * s + p.g_vs[a * p.gcap + n];
-            const float r = -alpha * q + p.g_r[a * p.gcap + n];
-            p.g_r[a * p.gcap + n] = r;
-            rr += (double)r * ((double)r * m);
-            r2 += (double)r * (double)r;
+            for (int a = 0; a < D; ++a) {
+                const float sv = p.g_s[a * p.gcap + n], q = p.g_q[a * p.gcap + n];
+                p.g_vs[a * p.gcap + n] = alpha * sv + p.g_vs[a * p.gcap + n];
+                const float rv = -alpha * q + p.g_r[a * p.gcap + n];
+                p.g_r[a * p.gcap + n] = rv;
+                sum += (double)rv * ((double)rv * m);
+                mx += (double)rv * (double)rv;
+            }
         }
-        sum += rr;
-        mx = fmax(mx, r2);
-    }
-    wg_reduce2(sum, mx);
-    if (threadIdx.x == 0) {
-        p.red[p.red_stride + blockIdx.x] = sum;
-        p.red[2 * p.red_stride + blockIdx.x] = mx;
+        wave_reduce2(sum, mx);
+        if (lane == 0) {
+            p.bsum[b] = sum;
+            p.bmax[b] = mx;
+        }
     }
 }
 
-// After the last launched iteration: convergence test of iteration k_end
-// (one work-group).
-__global__ __launch_bounds__(WG) void k_cg_finalize(KP p, int k_end) {
-    CGState* cg = p.cg;
-    if (cg->done) return;
-    double rr, mx;
-    reduce_partials(p.red + p.red_stride, p.red + 2 * p.red_stride, p.red_count, rr, mx);
+// Sum / max of a scene's block partials in a fixed order (one work-group per
+// scene); the result is broadcast to every thread.
+__device__ inline void scene_reduce(const KP& p, int si, bool with_max, double& sum, double& mx) {
+    const int b0 = p.sst[si].block_begin, b1 = p.sst[si].block_end;
+    double s = 0.0, m = 0.0;
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += WG) {
+        s += p.bsum[b];
+        if (with_max) m = fmax(m, p.bmax[b]);
+    }
+    wave_reduce2(s, m);
+    __shared__ double ss[WG / 64], sm[WG / 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { ss[wave] = s; sm[wave] = m; }
+    __syncthreads();
+    s = 0.0;
+    m = 0.0;
+#pragma unroll
+    for (int w = 0; w < WG / 64; ++w) { s += ss[w]; m = fmax(m, sm[w]); }
+    sum = s;
+    mx = m;
+}
+
+// alpha = rho_k / s.q for every scene still iterating
+__global__ __launch_bounds__(WG) void k_cg_reduce_sq(KP p, int k) {
+    const int si = blockIdx.x;
+    CGState& cg = p.cg[si];
+    if (cg.done) return;
+    double sq, unused;
+    scene_reduce(p, si, false, sq, unused);
     if (threadIdx.x == 0) {
-        const double conv = sqrt(mx);
-        cg->iter = k_end;
-        cg->conv = conv;
-        cg->rr = rr;
-        cg->converged = (conv <= p.cg_tol && (k_end >= p.cg_min || conv < (double)FLT_EPSILON)) ? 1 : 0;
-        cg->done = 1;
+        cg.sq = sq;
+        cg.alpha = sq != 0.0 ? cg.rho_hist[k & 1] / sq : (double)FLT_MAX;
+        cg.multiplies += 1;
+    }
+}
+
+// Convergence test of iteration k (Convergence_Norm, MPM_CG_System.h:241-249:
+// L-inf over nodes of |r|) and beta = rho_k / rho_{k-1}.  A scene is done
+// when the test passes (after cg_min iterations) or at its cap cg_max.
+__global__ __launch_bounds__(WG) void k_cg_reduce_rr(KP p, int k) {
+    const int si = blockIdx.x;
+    CGState& cg = p.cg[si];
+    if (cg.done) return;
+    const SceneParams& sc = p.scn[si];
+    double rr, mx;
+    scene_reduce(p, si, true, rr, mx);
+    if (threadIdx.x != 0) return;
+    if (k == 0) cg.multiplies = 1;  // the restart Multiply
+    const double conv = sqrt(mx);
+    const bool pass = conv <= sc.cg_tol && (k >= sc.cg_min || conv < (double)FLT_EPSILON);
+    cg.iter = k;
+    cg.conv = conv;
+    cg.rr = rr;
+    const bool empty = !sc.active || p.sst[si].block_end <= p.sst[si].block_begin;
+    if (pass || k >= sc.cg_max || empty) {
+        cg.converged = (pass || empty) ? 1 : 0;
+        cg.done = 1;
+        atomicAdd(&p.ctr->cg_ndone, 1);
+    } else {
+        cg.beta = k == 0 ? 0.0 : rr / cg.rho_hist[(k + 1) & 1];
+        cg.rho_hist[k & 1] = rr;
     }
 }
 
@@ -1461,16 +1539,20 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
     constexpr int LTN = Dm::LTN, NBR = Dm::NBR, DD = D * D;
     __shared__ float vt[2 * D][LTN];
     __shared__ int nbr[NBR];
+    __shared__ unsigned wbest[NW];
+    __shared__ int wrem[NW];
     const int nblocks = p.ctr->nblocks;
-    const float dt = p.dt, flip = p.flip;
-    const float lo = 0.0f + 0.5f * p.dx, hi = 1.0f - 0.5f * p.dx;
-    unsigned best = 0u;
-    int removed = 0;
     const XcdRange xr = xcd_blocks(nblocks);
     for (int b = xr.first; b < xr.end; b += xr.step) {
+        const SceneParams& sc = p.scn[p.block_scene[b]];
+        if (!sc.active) continue;
+        const float dt = sc.dt, flip = sc.flip;
+        const float lo = 0.0f + 0.5f * sc.dx, hi = 1.0f - 0.5f * sc.dx;
+        unsigned best = 0u;
+        int removed = 0;
         if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
-        slot_coords<D>(p.block_slot[b], p.nb, bb);
+        slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
         __syncthreads();
         const float* src[2 * D];
 #pragma unroll
@@ -1484,7 +1566,7 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
         for (int i = p.block_pstart[b] + (int)threadIdx.x; i < i1; i += BWG) {
             float w[D][3], dw[D][3], X[D];
             int l[D];
-            if (!particle_weights<D>(p, i, bb, w, dw, l, X)) continue;
+            if (!particle_weights<D>(p, sc, i, bb, w, dw, l, X)) continue;
             float vpic[D], vflip[D], G[DD];
 #pragma unroll
             for (int a = 0; a < D; ++a) {
@@ -1541,42 +1623,43 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
                 best = max(best, __float_as_uint(v2));
             }
         }
-        __syncthreads();
-    }
-    // per-work-group partials (plain stores, no same-address atomics);
-    // k_g2p_finish folds them into the counters
+        // per-block partials (plain stores); k_g2p_reduce folds each scene's
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-        best = max(best, (unsigned)__shfl_xor((int)best, off));
-        removed += __shfl_xor(removed, off);
-    }
-    __shared__ unsigned wbest[NW];
-    __shared__ int wrem[NW];
-    if ((threadIdx.x & 63) == 0) {
-        wbest[threadIdx.x >> 6] = best;
-        wrem[threadIdx.x >> 6] = removed;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        unsigned bb = 0u;
-        int rr = 0;
-#pragma unroll
-        for (int w = 0; w < NW; ++w) {
-            bb = max(bb, wbest[w]);
-            rr += wrem[w];
+        for (int off = 32; off > 0; off >>= 1) {
+            best = max(best, (unsigned)__shfl_xor((int)best, off));
+            removed += __shfl_xor(removed, off);
         }
-        p.wg_part[blockIdx.x] = bb;
-        p.wg_part[p.wg_part_count + blockIdx.x] = (unsigned)rr;
+        if ((threadIdx.x & 63) == 0) {
+            wbest[threadIdx.x >> 6] = best;
+            wrem[threadIdx.x >> 6] = removed;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            unsigned bm = 0u;
+            int rm = 0;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) {
+                bm = max(bm, wbest[w]);
+                rm += wrem[w];
+            }
+            p.bu[b] = bm;
+            p.bu[p.bcap + b] = (unsigned)rm;
+        }
+        __syncthreads();
     }
 }
 
-// One work-group: max speed and removal count of the last G2P, in a fixed
-// order, into the counters read by the host and the next binning.
-__global__ __launch_bounds__(WG) void k_g2p_finish(KP p, int nwg) {
+// One work-group per scene: max speed and removal count of the last G2P, in
+// a fixed order (MPM_Example::Max_Particle_Velocity, MPM_Example.cpp:375-387).
+// Scenes that did not advance keep their values.
+__global__ __launch_bounds__(WG) void k_g2p_reduce(KP p) {
+    const int si = blockIdx.x;
+    if (!p.scn[si].active) return;
+    const int b0 = p.sst[si].block_begin, b1 = p.sst[si].block_end;
     unsigned best = 0u, removed = 0u;
-    for (int i = threadIdx.x; i < nwg; i += WG) {
-        best = max(best, p.wg_part[i]);
-        removed += p.wg_part[p.wg_part_count + i];
+    for (int b = b0 + (int)threadIdx.x; b < b1; b += WG) {
+        best = max(best, p.bu[b]);
+        removed += p.bu[p.bcap + b];
     }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
@@ -1594,40 +1677,73 @@ __global__ __launch_bounds__(WG) void k_g2p_finish(KP p, int nwg) {
             best = max(best, sb[w]);
             removed += sr[w];
         }
-        p.ctr->max_v2 = best;
-        p.ctr->n_removed = (int)removed;
+        p.sst[si].max_v2 = best;
+        p.sst[si].n_removed = (int)removed;
     }
 }
 
+// Max |V|^2 per scene over the particle buffer (before the first binning):
+// one atomic per distinct scene in a wave.
 template <int D>
 __global__ __launch_bounds__(WG) void k_max_speed(KP p) {
     const int n = p.ctr->n_live;
-    unsigned best = 0u;
-    for (int i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) {
-        if (p.keycur[i] < 0) continue;
-        float v2 = 0.0f;
+    const int lane = threadIdx.x & 63;
+    for (int base = blockIdx.x * WG + (threadIdx.x & ~63); base < n; base += gridDim.x * WG) {
+        const int i = base + lane;
+        const bool valid = i < n && p.keycur[i] >= 0;
+        unsigned v2b = 0u;
+        int si = -1;
+        if (valid) {
+            si = p.scur[i];
+            float v2 = 0.0f;
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            const float v = p.pcur[(PF<D>::V + a) * p.pcap + i];
-            v2 += v * v;
+            for (int a = 0; a < D; ++a) {
+                const float v = p.pcur[(PF<D>::V + a) * p.pcap + i];
+                v2 += v * v;
+            }
+            v2b = __float_as_uint(v2);
         }
-        best = max(best, __float_as_uint(v2));
-    }
+        unsigned long long todo = __ballot(valid);
+        while (todo) {
+            const int leader = __ffsll((long long)todo) - 1;
+            const int ls = __shfl(si, leader);
+            const unsigned long long grp = __ballot(valid && si == ls);
+            unsigned m = (valid && si == ls) ? v2b : 0u;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) best = max(best, (unsigned)__shfl_xor((int)best, off));
-    if ((threadIdx.x & 63) == 0 && best) atomicMax(&p.ctr->max_v2, best);
+            for (int off = 32; off > 0; off >>= 1) m = max(m, (unsigned)__shfl_xor((int)m, off));
+            if (lane == leader && m) atomicMax(&p.sst[ls].max_v2, m);
+            todo &= ~grp;
+        }
+    }
+}
+
+// dt and activity of scenes [first, first + n) for the next substep
+__global__ __launch_bounds__(WG) void k_set_step(KP p, StepArgs a) {
+    const int i = blockIdx.x * WG + threadIdx.x;
+    if (i >= a.n) return;
+    SceneParams& sc = p.scn[a.first + i];
+    const float dt = a.dt[i];
+    sc.dt = dt > 0.0f ? dt : 0.0f;
+    sc.active = (a.force_active || dt > 0.0f) ? 1 : 0;
+    if (a.clear_waiting) sc.waiting = 0;
 }
 
 // ============================================================================
 // launchers
 // ============================================================================
+void launch_set_step(const KP& p, hipStream_t s, const StepArgs& a) {
+    hipLaunchKernelGGL(k_set_step, dim3((a.n + WG - 1) / WG), dim3(WG), 0, s, p, a);
+}
 template <int D> void launch_bin(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_bin<D>, dim3(grid), dim3(WG), 0, s, p); }
-template <int D> void launch_scan(const KP& p, hipStream_t s) {
+void launch_scan(const KP& p, hipStream_t s) {
     hipLaunchKernelGGL(k_scan_reduce, dim3(p.nparts), dim3(WG), 0, s, p);
     hipLaunchKernelGGL(k_scan_top, dim3(1), dim3(WG), 0, s, p);
     hipLaunchKernelGGL(k_scan_apply, dim3(p.nparts), dim3(WG), 0, s, p);
 }
 template <int D> void launch_block_setup(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_block_setup<D>, dim3(grid), dim3(WG), 0, s, p); }
+void launch_scene_ranges(const KP& p, hipStream_t s) {
+    hipLaunchKernelGGL(k_scene_ranges, dim3((p.nscenes + WG - 1) / WG), dim3(WG), 0, s, p);
+}
 template <int D> void launch_reorder(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_reorder<D>, dim3(grid), dim3(WG), 0, s, p); }
 template <int D> void launch_p2g(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_p2g<D>, dim3(grid), dim3(BWG), 0, s, p); }
 template <int D> void launch_normalize(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_normalize<D>, dim3(grid), dim3(WG), 0, s, p); }
@@ -1640,41 +1756,40 @@ template <int D> void launch_particle_force(const KP& p, hipStream_t s, int grid
 template <int D> void launch_grid_update(const KP& p, hipStream_t s, int grid, bool force, bool collision, bool cg_init, bool write_f) {
     if (force && collision && !cg_init && !write_f) hipLaunchKernelGGL((k_grid_update<D, true, true, false, false>), dim3(grid), dim3(WG), 0, s, p);
     else if (force && !collision && !cg_init && write_f) hipLaunchKernelGGL((k_grid_update<D, true, false, false, true>), dim3(grid), dim3(WG), 0, s, p);
+    else if (force && !collision && !cg_init) hipLaunchKernelGGL((k_grid_update<D, true, false, false, false>), dim3(grid), dim3(WG), 0, s, p);
     else if (!force && collision && !cg_init) hipLaunchKernelGGL((k_grid_update<D, false, true, false, false>), dim3(grid), dim3(WG), 0, s, p);
     else if (!force && !collision && cg_init) hipLaunchKernelGGL((k_grid_update<D, false, false, true, false>), dim3(grid), dim3(WG), 0, s, p);
     else hipLaunchKernelGGL((k_grid_update<D, true, true, true, false>), dim3(grid), dim3(WG), 0, s, p);
 }
-template <int D> void launch_cg_force(const KP& p, hipStream_t s, int grid, bool restart) {
-    if (restart) hipLaunchKernelGGL((k_cg_force<D, true>), dim3(grid), dim3(BWG), 0, s, p);
-    else hipLaunchKernelGGL((k_cg_force<D, false>), dim3(grid), dim3(BWG), 0, s, p);
+template <int D> void launch_cg_force(const KP& p, hipStream_t s, int grid, bool restart, int k) {
+    if (restart) hipLaunchKernelGGL((k_cg_force<D, true>), dim3(grid), dim3(BWG), 0, s, p, 0);
+    else hipLaunchKernelGGL((k_cg_force<D, false>), dim3(grid), dim3(BWG), 0, s, p, k);
 }
 template <int D> void launch_cg_restart_node(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_cg_restart_node<D>, dim3(grid), dim3(WG), 0, s, p); }
-template <int D> void launch_cg_update_s(const KP& p, hipStream_t s, int grid, int k) { hipLaunchKernelGGL(k_cg_update_s<D>, dim3(grid), dim3(WG), 0, s, p, k); }
-template <int D> void launch_cg_node1(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_cg_node1<D>, dim3(grid), dim3(WG), 0, s, p); }
-template <int D> void launch_cg_node2(const KP& p, hipStream_t s, int grid, int k) { hipLaunchKernelGGL(k_cg_node2<D>, dim3(grid), dim3(WG), 0, s, p, k); }
-void launch_cg_finalize(const KP& p, hipStream_t s, int k_end) { hipLaunchKernelGGL(k_cg_finalize, dim3(1), dim3(WG), 0, s, p, k_end); }
+template <int D> void launch_cg_node1(const KP& p, hipStream_t s, int grid, int k) { hipLaunchKernelGGL(k_cg_node1<D>, dim3(grid), dim3(WG), 0, s, p, k); }
+template <int D> void launch_cg_node2(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_cg_node2<D>, dim3(grid), dim3(WG), 0, s, p); }
+void launch_cg_reduce_sq(const KP& p, hipStream_t s, int k) { hipLaunchKernelGGL(k_cg_reduce_sq, dim3(p.nscenes), dim3(WG), 0, s, p, k); }
+void launch_cg_reduce_rr(const KP& p, hipStream_t s, int k) { hipLaunchKernelGGL(k_cg_reduce_rr, dim3(p.nscenes), dim3(WG), 0, s, p, k); }
 template <int D> void launch_g2p(const KP& p, hipStream_t s, int grid) {
     hipLaunchKernelGGL(k_g2p<D>, dim3(grid), dim3(BWG), 0, s, p);
-    hipLaunchKernelGGL(k_g2p_finish, dim3(1), dim3(WG), 0, s, p, grid);
+    hipLaunchKernelGGL(k_g2p_reduce, dim3(p.nscenes), dim3(WG), 0, s, p);
 }
 template <int D> void launch_max_speed(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_max_speed<D>, dim3(grid), dim3(WG), 0, s, p); }
 
-#define MPM_INSTANTIATE(D)                                                                   \
-    template void launch_bin<D>(const KP&, hipStream_t, int);                                \
-    template void launch_scan<D>(const KP&, hipStream_t);                                    \
-    template void launch_block_setup<D>(const KP&, hipStream_t, int);                        \
-    template void launch_reorder<D>(const KP&, hipStream_t, int);                            \
-    template void launch_p2g<D>(const KP&, hipStream_t, int);                                \
-    template void launch_normalize<D>(const KP&, hipStream_t, int);                          \
-    template void launch_init_volume<D>(const KP&, hipStream_t, int);                        \
-    template void launch_particle_force<D>(const KP&, hipStream_t, int, bool, bool);         \
+#define MPM_INSTANTIATE(D)                                                                    \
+    template void launch_bin<D>(const KP&, hipStream_t, int);                                 \
+    template void launch_block_setup<D>(const KP&, hipStream_t, int);                         \
+    template void launch_reorder<D>(const KP&, hipStream_t, int);                             \
+    template void launch_p2g<D>(const KP&, hipStream_t, int);                                 \
+    template void launch_normalize<D>(const KP&, hipStream_t, int);                           \
+    template void launch_init_volume<D>(const KP&, hipStream_t, int);                         \
+    template void launch_particle_force<D>(const KP&, hipStream_t, int, bool, bool);          \
     template void launch_grid_update<D>(const KP&, hipStream_t, int, bool, bool, bool, bool); \
-    template void launch_cg_force<D>(const KP&, hipStream_t, int, bool);                     \
-    template void launch_cg_restart_node<D>(const KP&, hipStream_t, int);                    \
-    template void launch_cg_update_s<D>(const KP&, hipStream_t, int, int);                   \
-    template void launch_cg_node1<D>(const KP&, hipStream_t, int);                           \
-    template void launch_cg_node2<D>(const KP&, hipStream_t, int, int);                           \
-    template void launch_g2p<D>(const KP&, hipStream_t, int);                                \
+    template void launch_cg_force<D>(const KP&, hipStream_t, int, bool, int);                 \
+    template void launch_cg_restart_node<D>(const KP&, hipStream_t, int);                     \
+    template void launch_cg_node1<D>(const KP&, hipStream_t, int, int);                       \
+    template void launch_cg_node2<D>(const KP&, hipStream_t, int);                            \
+    template void launch_g2p<D>(const KP&, hipStream_t, int);                                 \
     template void launch_max_speed<D>(const KP&, hipStream_t, int);
 MPM_INSTANTIATE(2)
 MPM_INSTANTIATE(3)
