@@ -81,7 +81,7 @@ struct mpm_context {
     mpm::SceneState* sst = nullptr;
     mpm::CGState* cg = nullptr;
     int* slot_base = nullptr;
-    double *bsum = nullptr, *bmax = nullptr;
+    double *bsum = nullptr, *bmax = nullptr, *bsq = nullptr;
     unsigned* bu = nullptr;
     int cg_predict = 0;    // iterations launched before the done count was seen, last solve
     int* h_done = nullptr;  // pinned
@@ -200,6 +200,7 @@ mpm::KP make_kp(const mpm_context* c) {
     p.ctr = c->ctr;
     p.bsum = c->bsum;
     p.bmax = c->bmax;
+    p.bsq = c->bsq;
     p.bu = c->bu;
     return p;
 }
@@ -292,25 +293,30 @@ int max_cg_iterations(const mpm_context* c) {
 
 // Apply_Force's implicit solve (MPM_Example.cpp:700-711) for every active
 // scene: CG set-up, restart (r = b - A x, Project) and iterations until every
-// scene's solve is done (converged or at its cap).
+// scene's solve is done (converged or at its cap).  Batches reduce each
+// scene's block partials in per-scene kernels (five launches per iteration);
+// one scene needs no reducer launches (four per iteration).
 template <int D>
 int stage_cg(mpm_context* c, bool do_init) {
     mpm::KP p = make_kp(c);
+    const bool one = c->nscenes == 1;
+    const int gn = c->grid_node;
     if (do_init)
-        timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, false, false, true, false); });
-    timed(c, KC_CG_FORCE, [&] { mpm::launch_cg_force<D>(p, c->stream, c->grid_block, true, 0); });
+        timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, gn, false, false, true, false); });
+    timed(c, KC_CG_FORCE, [&] { mpm::launch_cg_force<D>(p, c->stream, c->grid_block, true, 0, one); });
     timed(c, KC_CG_NODE, [&] {
-        mpm::launch_cg_restart_node<D>(p, c->stream, c->grid_node);
-        mpm::launch_cg_reduce_rr(p, c->stream, 0);
+        mpm::launch_cg_restart_node<D>(p, c->stream, gn, one);
+        if (!one) mpm::launch_cg_reduce_rr(p, c->stream, 0);
     });
     const int max_it = max_cg_iterations(c);
     auto iteration = [&](int k) {
-        timed(c, KC_CG_FORCE, [&] { mpm::launch_cg_force<D>(p, c->stream, c->grid_block, false, k); });
+        if (one) timed(c, KC_CG_NODE, [&] { mpm::launch_cg_update_s<D>(p, c->stream, gn, k); });
+        timed(c, KC_CG_FORCE, [&] { mpm::launch_cg_force<D>(p, c->stream, c->grid_block, false, k, one); });
         timed(c, KC_CG_NODE, [&] {
-            mpm::launch_cg_node1<D>(p, c->stream, c->grid_node, k);
-            mpm::launch_cg_reduce_sq(p, c->stream, k);
-            mpm::launch_cg_node2<D>(p, c->stream, c->grid_node);
-            mpm::launch_cg_reduce_rr(p, c->stream, k + 1);
+            mpm::launch_cg_node1<D>(p, c->stream, gn, k, one);
+            if (!one) mpm::launch_cg_reduce_sq(p, c->stream, k);
+            mpm::launch_cg_node2<D>(p, c->stream, gn, k, one);
+            if (!one) mpm::launch_cg_reduce_rr(p, c->stream, k + 1);
         });
     };
     // Iterations after every solve is done return at once, so launching a few
@@ -322,6 +328,7 @@ int stage_cg(mpm_context* c, bool do_init) {
     int slot = 0, launched = 0;
     int boundary[2] = {0, 0};
     bool pending[2] = {false, false};
+    bool finished = false;
     while (launched < max_it) {
         const int n = launched == 0 ? first : std::min(chunk, max_it - launched);
         for (int j = 0; j < n; ++j) iteration(launched + j);
@@ -337,11 +344,16 @@ int stage_cg(mpm_context* c, bool do_init) {
             if (c->h_done[slot] >= c->nscenes) {
                 // done by the first boundary: try one fewer next time
                 c->cg_predict = boundary[slot] == first ? first - 1 : boundary[slot];
+                finished = true;
                 break;
             }
         }
     }
-    if (launched >= max_it) c->cg_predict = max_it;
+    if (!finished) {
+        c->cg_predict = max_it;
+        // one scene: the convergence test after the last iteration (batches ran it in reduce_rr)
+        if (one) timed(c, KC_CG_NODE, [&] { mpm::launch_cg_update_s<D>(p, c->stream, gn, max_it); });
+    }
     return MPM_OK;
 }
 
@@ -529,13 +541,16 @@ int create_batch(const mpm_scene_desc* descs, const int64_t* capacities, int nsc
     A(dalloc(c, &c->sst, (size_t)nscenes));
     A(dalloc(c, &c->cg, (size_t)nscenes));
     A(dalloc(c, &c->slot_base, (size_t)nscenes + 1));
-    A(dalloc(c, &c->bsum, (size_t)c->bcap));
-    A(dalloc(c, &c->bmax, (size_t)c->bcap));
-    A(dalloc(c, &c->bu, (size_t)2 * c->bcap));
     c->grid_particle = (int)std::min<int64_t>(std::max<int64_t>((c->pcap + mpm::WG - 1) / mpm::WG, 8), 4096);
     c->grid_block = (int)std::min<int64_t>(((int64_t)c->bcap + 7) / 8 * 8, 8192);
     // node kernels: one wave per block, up to 1024 work-groups (a multiple of 8 for the XCD ranges)
     c->grid_node = (int)std::min<int64_t>(((int64_t)c->bcap / (mpm::WG / 64) + 7) / 8 * 8 + 8, 1024);
+    // reduction partials: per block (batches) or per node work-group (one scene)
+    const size_t nred = (size_t)std::max(c->bcap, c->grid_node);
+    A(dalloc(c, &c->bsum, nred));
+    A(dalloc(c, &c->bmax, nred));
+    A(dalloc(c, &c->bsq, nred));
+    A(dalloc(c, &c->bu, (size_t)2 * c->bcap));
     if (rc == MPM_OK) {
         if (hipMemcpy(c->scn, c->table.data(), sizeof(mpm::SceneParams) * (size_t)nscenes, hipMemcpyHostToDevice) != hipSuccess ||
             hipMemcpy(c->slot_base, slot_base.data(), sizeof(int) * slot_base.size(), hipMemcpyHostToDevice) != hipSuccess ||
@@ -1023,8 +1038,8 @@ int mpm_time_cg_operator(mpm_handle c, int32_t reps, float* avg_ms) {
     mpm::KP p = make_kp(c);
     HIPCHECK(hipEventRecord(e0, c->stream));
     for (int r = 0; r < reps; ++r) {
-        if (c->D == 3) mpm::launch_cg_force<3>(p, c->stream, c->grid_block, false, 1);
-        else mpm::launch_cg_force<2>(p, c->stream, c->grid_block, false, 1);
+        if (c->D == 3) mpm::launch_cg_force<3>(p, c->stream, c->grid_block, false, 1, c->nscenes == 1);
+        else mpm::launch_cg_force<2>(p, c->stream, c->grid_block, false, 1, c->nscenes == 1);
     }
     HIPCHECK(hipEventRecord(e1, c->stream));
     HIPCHECK(hipEventSynchronize(e1));

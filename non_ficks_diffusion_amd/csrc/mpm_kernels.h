@@ -112,8 +112,9 @@ struct KP {
     float* tiles;    // per active block, (D+1) * (B+2)^D floats
     // control
     Counters* ctr;
-    double* bsum;    // per active block partial sums (s.q or r.r)
-    double* bmax;    // per active block partial max |r|^2
+    double* bsum;    // partial sums of r.r, per active block (per work-group for one scene)
+    double* bmax;    // partial max |r|^2, same layout
+    double* bsq;     // partial sums of s.q, same layout
     unsigned* bu;    // 2 * bcap: per block max |V|^2 bits | removals (G2P)
 };
 
@@ -139,10 +140,12 @@ template <int D> void launch_normalize(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_init_volume(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_particle_force(const KP& p, hipStream_t s, int grid, bool constitutive, bool force);
 template <int D> void launch_grid_update(const KP& p, hipStream_t s, int grid, bool force, bool collision, bool cg_init, bool write_f);
-template <int D> void launch_cg_force(const KP& p, hipStream_t s, int grid, bool restart, int k);
-template <int D> void launch_cg_restart_node(const KP& p, hipStream_t s, int grid);
-template <int D> void launch_cg_node1(const KP& p, hipStream_t s, int grid, int k);
-template <int D> void launch_cg_node2(const KP& p, hipStream_t s, int grid);
+template <int D> void launch_cg_force(const KP& p, hipStream_t s, int grid, bool restart, int k, bool one);
+// `one`: a batch of one scene (per-work-group partials, see k_cg_update_s)
+template <int D> void launch_cg_restart_node(const KP& p, hipStream_t s, int grid, bool one);
+template <int D> void launch_cg_update_s(const KP& p, hipStream_t s, int grid, int k);
+template <int D> void launch_cg_node1(const KP& p, hipStream_t s, int grid, int k, bool one);
+template <int D> void launch_cg_node2(const KP& p, hipStream_t s, int grid, int k, bool one);
 void launch_cg_reduce_sq(const KP& p, hipStream_t s, int k);
 void launch_cg_reduce_rr(const KP& p, hipStream_t s, int k);
 template <int D> void launch_g2p(const KP& p, hipStream_t s, int grid);

@@ -1246,7 +1246,7 @@ __device__ inline bool cg_all_done(const KP& p) { return p.ctr->cg_ndone >= p.ns
 // record: T = sum_i grad w_ip (x) u_i;  A = L T;  q_i += A^T grad w_ip.
 // RESTART applies it to v* (the initial guess); otherwise to the search
 // direction of iteration k, s = r + beta s, formed while gathering.
-template <int D, bool RESTART>
+template <int D, bool RESTART, bool ONE>
 __global__ __launch_bounds__(BWG) void k_cg_force(KP p, int k) {
     using Dm = Dims<D>;
     constexpr int LTN = Dm::LTN, NBR = Dm::NBR, DD = D * D;
@@ -1266,10 +1266,11 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p, int k) {
     }
     const XcdRange xr = xcd_blocks(nblocks);
     for (int b = xr.first; b < xr.end; b += xr.step) {
-        const int si = p.block_scene[b];
+        // one scene: no per-block lookups (its solve is live: cg_all_done above)
+        const int si = ONE ? 0 : p.block_scene[b];
         const SceneParams& sc = p.scn[si];
-        if (!sc.active || p.cg[si].done) continue;
-        const float beta = RESTART ? 0.0f : (float)p.cg[si].beta;
+        if (!sc.active || (!ONE && p.cg[si].done)) continue;
+        const float beta = (RESTART || ONE) ? 0.0f : (float)p.cg[si].beta;
         if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
         slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
@@ -1286,6 +1287,7 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p, int k) {
                 float u = 0.0f;
                 if (blk >= 0)
                     u = RESTART ? p.g_vs[a * p.gcap + n]
+                        : ONE   ? p.g_s[a * p.gcap + n]
                                 : cg_direction(p.g_r[a * p.gcap + n], p.g_s[a * p.gcap + n], beta, k);
                 ut[a][li] = u;
             }
@@ -1339,14 +1341,103 @@ __device__ inline void wave_reduce2(double& sum, double& mx) {
     }
 }
 
+// Two reduction layouts.  Batches (ONE = false): node kernels write one
+// partial per block and a one-work-group-per-scene kernel reduces each
+// scene's partials.  A batch of one scene (ONE = true): node kernels write
+// one partial per work-group and every work-group of the consuming kernel
+// reduces all of them itself (same values, same order, same result), which
+// saves the two reducer launches per CG iteration.
+
+// Writes this wave's partials (ONE: accumulated over its blocks) -- per block
+// in batches, per work-group for one scene.
+__device__ inline void wg_partials_store(double* dsum, double* dmax, double sum, double mx) {
+    __shared__ double ws[WG / 64], wm[WG / 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    if (lane == 0) { ws[wave] = sum; wm[wave] = mx; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double s = 0.0, m = 0.0;
+#pragma unroll
+        for (int w = 0; w < WG / 64; ++w) { s += ws[w]; m = fmax(m, wm[w]); }
+        dsum[blockIdx.x] = s;
+        if (dmax) dmax[blockIdx.x] = m;
+    }
+}
+
+// Sum / max of `count` partials starting at `first`, in a fixed order
+// relative to `first`; the result is broadcast to every thread.
+__device__ inline void reduce_range(const double* sums, const double* maxs, int first, int count, double& sum,
+                                    double& mx) {
+    constexpr int U = 8;  // eight independent loads in flight per thread
+    double sa[U], ma[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) { sa[u] = 0.0; ma[u] = 0.0; }
+    for (int base = (int)threadIdx.x; base < count; base += U * WG) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int j = base + u * WG;
+            if (j < count) {
+                sa[u] += sums[first + j];
+                if (maxs) ma[u] = fmax(ma[u], maxs[first + j]);
+            }
+        }
+    }
+    double s = 0.0, m = 0.0;
+#pragma unroll
+    for (int u = 0; u < U; ++u) { s += sa[u]; m = fmax(m, ma[u]); }
+    wave_reduce2(s, m);
+    __shared__ double ss[WG / 64], sm[WG / 64];
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    __syncthreads();
+    if (lane == 0) { ss[wave] = s; sm[wave] = m; }
+    __syncthreads();
+    s = 0.0;
+    m = 0.0;
+#pragma unroll
+    for (int w = 0; w < WG / 64; ++w) { s += ss[w]; m = fmax(m, sm[w]); }
+    sum = s;
+    mx = m;
+}
+
+// Convergence test of iteration k (Convergence_Norm, MPM_CG_System.h:241-249:
+// L-inf over nodes of |r|) on r.r and max |r|^2, and beta = rho_k / rho_{k-1}.
+// A scene is done when the test passes (after cg_min iterations) or at its
+// cap cg_max.  `record`: this thread writes the scene's state.
+__device__ inline bool cg_test(const KP& p, int si, int k, double rr, double mx, bool record, float& beta_f) {
+    CGState& cg = p.cg[si];
+    const SceneParams& sc = p.scn[si];
+    const double conv = sqrt(mx);
+    const bool pass = conv <= sc.cg_tol && (k >= sc.cg_min || conv < (double)FLT_EPSILON);
+    const bool empty = !sc.active || p.sst[si].block_end <= p.sst[si].block_begin;
+    const bool done = pass || k >= sc.cg_max || empty;
+    const double beta = k == 0 ? 0.0 : rr / cg.rho_hist[(k + 1) & 1];
+    if (record) {
+        if (k == 0) cg.multiplies = 1;  // the restart Multiply
+        cg.iter = k;
+        cg.conv = conv;
+        cg.rr = rr;
+        if (done) {
+            cg.converged = (pass || empty) ? 1 : 0;
+            cg.done = 1;
+            atomicAdd(&p.ctr->cg_ndone, 1);
+        } else {
+            cg.beta = beta;
+            cg.rho_hist[k & 1] = rr;
+        }
+    }
+    beta_f = (float)beta;
+    return done;
+}
+
 // CG restart: q = dt^2/m q + x (Multiply_Helper), r = b - q, Project(r);
-// per-block partials of r.r (M-weighted) and max |r|^2.
-template <int D>
+// partials of r.r (M-weighted) and max |r|^2.
+template <int D, bool ONE>
 __global__ __launch_bounds__(WG) void k_cg_restart_node(KP p) {
     const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
     const int lane = threadIdx.x & 63;
+    double wsum = 0.0, wmx = 0.0;
     for (int b = r.first; b < r.end; b += r.step) {
-        const SceneParams& sc = p.scn[p.block_scene[b]];
+        const SceneParams& sc = p.scn[ONE ? 0 : p.block_scene[b]];
         if (!sc.active) continue;
         const int n = b * 64 + lane;
         const float sds = sc.dt * sc.dt;
@@ -1382,25 +1473,52 @@ __global__ __launch_bounds__(WG) void k_cg_restart_node(KP p) {
             }
         }
         wave_reduce2(sum, mx);
-        if (lane == 0) {
+        if (ONE) {
+            wsum += sum;
+            wmx = fmax(wmx, mx);
+        } else if (lane == 0) {
             p.bsum[b] = sum;
             p.bmax[b] = mx;
         }
     }
+    if (ONE) wg_partials_store(p.bsum, p.bmax, wsum, wmx);
 }
 
-// s = r + beta s stored for this iteration (interior nodes), tile sum of
-// the operator, q = dt^2/m q + s, Project(q); per-block partials of s.q
+// One scene: convergence test of iteration k on the r.r partials (every
+// work-group reduces them; work-group 0 records the state), then
+// s = r (k = 0) or r + beta s on the interior nodes.
 template <int D>
+__global__ __launch_bounds__(WG) void k_cg_update_s(KP p, int k, int nparts) {
+    if (cg_all_done(p)) return;
+    double rr, mx;
+    reduce_range(p.bsum, p.bmax, 0, nparts, rr, mx);
+    float beta;
+    if (cg_test(p, 0, k, rr, mx, blockIdx.x == 0 && threadIdx.x == 0, beta)) return;
+    const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
+    const int lane = threadIdx.x & 63;
+    for (int b = r.first; b < r.end; b += r.step) {
+        const int n = b * 64 + lane;
+        if (!(p.g_flags[n] & MPM_FLAG_INTERIOR_DEV)) continue;
+#pragma unroll
+        for (int a = 0; a < D; ++a)
+            p.g_s[a * p.gcap + n] = cg_direction(p.g_r[a * p.gcap + n], p.g_s[a * p.gcap + n], beta, k);
+    }
+}
+
+// Tile sum of the operator, q = dt^2/m q + s, Project(q); partials of s.q.
+// Batches form and store this iteration's s = r + beta s (interior nodes);
+// for one scene k_cg_update_s stored it already.
+template <int D, bool ONE>
 __global__ __launch_bounds__(WG) void k_cg_node1(KP p, int k) {
     if (cg_all_done(p)) return;
     const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
     const int lane = threadIdx.x & 63;
+    double wsum = 0.0;
     for (int b = r.first; b < r.end; b += r.step) {
-        const int si = p.block_scene[b];
+        const int si = ONE ? 0 : p.block_scene[b];
         const SceneParams& sc = p.scn[si];
-        if (!sc.active || p.cg[si].done) continue;
-        const float beta = (float)p.cg[si].beta;
+        if (!sc.active || (!ONE && p.cg[si].done)) continue;
+        const float beta = ONE ? 0.0f : (float)p.cg[si].beta;
         const float sds = sc.dt * sc.dt;
         const int n = b * 64 + lane;
         const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
@@ -1410,8 +1528,12 @@ __global__ __launch_bounds__(WG) void k_cg_node1(KP p, int k) {
             const float cm = sds / p.g_m[n];
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                sv[a] = cg_direction(p.g_r[a * p.gcap + n], p.g_s[a * p.gcap + n], beta, k);
-                p.g_s[a * p.gcap + n] = sv[a];
+                if (ONE) {
+                    sv[a] = p.g_s[a * p.gcap + n];
+                } else {
+                    sv[a] = cg_direction(p.g_r[a * p.gcap + n], p.g_s[a * p.gcap + n], beta, k);
+                    p.g_s[a * p.gcap + n] = sv[a];
+                }
                 q[a] = cm * q[a] + sv[a];
             }
         }
@@ -1428,21 +1550,40 @@ __global__ __launch_bounds__(WG) void k_cg_node1(KP p, int k) {
             for (int a = 0; a < D; ++a) sum += (double)sv[a] * ((double)q[a] * m);
         }
         wave_reduce2(sum, mx);
-        if (lane == 0) p.bsum[b] = sum;
+        if (ONE) wsum += sum;
+        else if (lane == 0) p.bsq[b] = sum;
     }
+    if (ONE) wg_partials_store(p.bsq, nullptr, wsum, 0.0);
 }
 
-// x += alpha s, r -= alpha q; per-block partials of r.r and max |r|^2
-template <int D>
-__global__ __launch_bounds__(WG) void k_cg_node2(KP p) {
+// alpha = rho_k / s.q;  x += alpha s, r -= alpha q; partials of r.r and
+// max |r|^2.  One scene: every work-group reduces the s.q partials itself
+// (work-group 0 records alpha); batches read alpha from k_cg_reduce_sq.
+template <int D, bool ONE>
+__global__ __launch_bounds__(WG) void k_cg_node2(KP p, int k, int nparts) {
     if (cg_all_done(p)) return;
+    float alpha1 = 0.0f;
+    if (ONE) {
+        CGState& cg = p.cg[0];
+        if (cg.done) return;
+        double sq, unused;
+        reduce_range(p.bsq, nullptr, 0, nparts, sq, unused);
+        const double alpha_d = sq != 0.0 ? cg.rho_hist[k & 1] / sq : (double)FLT_MAX;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            cg.sq = sq;
+            cg.alpha = alpha_d;
+            cg.multiplies += 1;
+        }
+        alpha1 = (float)alpha_d;
+    }
     const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
     const int lane = threadIdx.x & 63;
+    double wsum = 0.0, wmx = 0.0;
     for (int b = r.first; b < r.end; b += r.step) {
-        const int si = p.block_scene[b];
+        const int si = ONE ? 0 : p.block_scene[b];
         const SceneParams& sc = p.scn[si];
-        if (!sc.active || p.cg[si].done) continue;
-        const float alpha = (float)p.cg[si].alpha;
+        if (!sc.active || (!ONE && p.cg[si].done)) continue;
+        const float alpha = ONE ? alpha1 : (float)p.cg[si].alpha;
         const int n = b * 64 + lane;
         const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
         double sum = 0.0, mx = 0.0;
@@ -1459,42 +1600,26 @@ __global__ __launch_bounds__(WG) void k_cg_node2(KP p) {
             }
         }
         wave_reduce2(sum, mx);
-        if (lane == 0) {
+        if (ONE) {
+            wsum += sum;
+            wmx = fmax(wmx, mx);
+        } else if (lane == 0) {
             p.bsum[b] = sum;
             p.bmax[b] = mx;
         }
     }
+    if (ONE) wg_partials_store(p.bsum, p.bmax, wsum, wmx);
 }
 
-// Sum / max of a scene's block partials in a fixed order (one work-group per
-// scene); the result is broadcast to every thread.
-__device__ inline void scene_reduce(const KP& p, int si, bool with_max, double& sum, double& mx) {
-    const int b0 = p.sst[si].block_begin, b1 = p.sst[si].block_end;
-    double s = 0.0, m = 0.0;
-    for (int b = b0 + (int)threadIdx.x; b < b1; b += WG) {
-        s += p.bsum[b];
-        if (with_max) m = fmax(m, p.bmax[b]);
-    }
-    wave_reduce2(s, m);
-    __shared__ double ss[WG / 64], sm[WG / 64];
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    if (lane == 0) { ss[wave] = s; sm[wave] = m; }
-    __syncthreads();
-    s = 0.0;
-    m = 0.0;
-#pragma unroll
-    for (int w = 0; w < WG / 64; ++w) { s += ss[w]; m = fmax(m, sm[w]); }
-    sum = s;
-    mx = m;
-}
-
-// alpha = rho_k / s.q for every scene still iterating
+// Batches: alpha = rho_k / s.q for every scene still iterating (one
+// work-group per scene over the scene's block partials)
 __global__ __launch_bounds__(WG) void k_cg_reduce_sq(KP p, int k) {
     const int si = blockIdx.x;
     CGState& cg = p.cg[si];
     if (cg.done) return;
+    const int b0 = p.sst[si].block_begin, b1 = p.sst[si].block_end;
     double sq, unused;
-    scene_reduce(p, si, false, sq, unused);
+    reduce_range(p.bsq, nullptr, b0, b1 - b0, sq, unused);
     if (threadIdx.x == 0) {
         cg.sq = sq;
         cg.alpha = sq != 0.0 ? cg.rho_hist[k & 1] / sq : (double)FLT_MAX;
@@ -1502,32 +1627,15 @@ __global__ __launch_bounds__(WG) void k_cg_reduce_sq(KP p, int k) {
     }
 }
 
-// Convergence test of iteration k (Convergence_Norm, MPM_CG_System.h:241-249:
-// L-inf over nodes of |r|) and beta = rho_k / rho_{k-1}.  A scene is done
-// when the test passes (after cg_min iterations) or at its cap cg_max.
+// Batches: convergence test of iteration k and beta per scene
 __global__ __launch_bounds__(WG) void k_cg_reduce_rr(KP p, int k) {
     const int si = blockIdx.x;
-    CGState& cg = p.cg[si];
-    if (cg.done) return;
-    const SceneParams& sc = p.scn[si];
+    if (p.cg[si].done) return;
+    const int b0 = p.sst[si].block_begin, b1 = p.sst[si].block_end;
     double rr, mx;
-    scene_reduce(p, si, true, rr, mx);
-    if (threadIdx.x != 0) return;
-    if (k == 0) cg.multiplies = 1;  // the restart Multiply
-    const double conv = sqrt(mx);
-    const bool pass = conv <= sc.cg_tol && (k >= sc.cg_min || conv < (double)FLT_EPSILON);
-    cg.iter = k;
-    cg.conv = conv;
-    cg.rr = rr;
-    const bool empty = !sc.active || p.sst[si].block_end <= p.sst[si].block_begin;
-    if (pass || k >= sc.cg_max || empty) {
-        cg.converged = (pass || empty) ? 1 : 0;
-        cg.done = 1;
-        atomicAdd(&p.ctr->cg_ndone, 1);
-    } else {
-        cg.beta = k == 0 ? 0.0 : rr / cg.rho_hist[(k + 1) & 1];
-        cg.rho_hist[k & 1] = rr;
-    }
+    reduce_range(p.bsum, p.bmax, b0, b1 - b0, rr, mx);
+    float beta;
+    (void)cg_test(p, si, k, rr, mx, threadIdx.x == 0, beta);
 }
 
 // ============================================================================
@@ -1656,11 +1764,23 @@ __global__ __launch_bounds__(WG) void k_g2p_reduce(KP p) {
     const int si = blockIdx.x;
     if (!p.scn[si].active) return;
     const int b0 = p.sst[si].block_begin, b1 = p.sst[si].block_end;
-    unsigned best = 0u, removed = 0u;
-    for (int b = b0 + (int)threadIdx.x; b < b1; b += WG) {
-        best = max(best, p.bu[b]);
-        removed += p.bu[p.bcap + b];
+    constexpr int U = 8;
+    unsigned ba[U], ra[U];
+#pragma unroll
+    for (int u = 0; u < U; ++u) { ba[u] = 0u; ra[u] = 0u; }
+    for (int base = b0 + (int)threadIdx.x; base < b1; base += U * WG) {
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int b = base + u * WG;
+            if (b < b1) {
+                ba[u] = max(ba[u], p.bu[b]);
+                ra[u] += p.bu[p.bcap + b];
+            }
+        }
     }
+    unsigned best = 0u, removed = 0u;
+#pragma unroll
+    for (int u = 0; u < U; ++u) { best = max(best, ba[u]); removed += ra[u]; }
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) {
         best = max(best, (unsigned)__shfl_xor((int)best, off));
@@ -1761,13 +1881,26 @@ template <int D> void launch_grid_update(const KP& p, hipStream_t s, int grid, b
     else if (!force && !collision && cg_init) hipLaunchKernelGGL((k_grid_update<D, false, false, true, false>), dim3(grid), dim3(WG), 0, s, p);
     else hipLaunchKernelGGL((k_grid_update<D, true, true, true, false>), dim3(grid), dim3(WG), 0, s, p);
 }
-template <int D> void launch_cg_force(const KP& p, hipStream_t s, int grid, bool restart, int k) {
-    if (restart) hipLaunchKernelGGL((k_cg_force<D, true>), dim3(grid), dim3(BWG), 0, s, p, 0);
-    else hipLaunchKernelGGL((k_cg_force<D, false>), dim3(grid), dim3(BWG), 0, s, p, k);
+template <int D> void launch_cg_force(const KP& p, hipStream_t s, int grid, bool restart, int k, bool one) {
+    if (restart) hipLaunchKernelGGL((k_cg_force<D, true, false>), dim3(grid), dim3(BWG), 0, s, p, 0);
+    else if (one) hipLaunchKernelGGL((k_cg_force<D, false, true>), dim3(grid), dim3(BWG), 0, s, p, k);
+    else hipLaunchKernelGGL((k_cg_force<D, false, false>), dim3(grid), dim3(BWG), 0, s, p, k);
 }
-template <int D> void launch_cg_restart_node(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_cg_restart_node<D>, dim3(grid), dim3(WG), 0, s, p); }
-template <int D> void launch_cg_node1(const KP& p, hipStream_t s, int grid, int k) { hipLaunchKernelGGL(k_cg_node1<D>, dim3(grid), dim3(WG), 0, s, p, k); }
-template <int D> void launch_cg_node2(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_cg_node2<D>, dim3(grid), dim3(WG), 0, s, p); }
+template <int D> void launch_cg_restart_node(const KP& p, hipStream_t s, int grid, bool one) {
+    if (one) hipLaunchKernelGGL((k_cg_restart_node<D, true>), dim3(grid), dim3(WG), 0, s, p);
+    else hipLaunchKernelGGL((k_cg_restart_node<D, false>), dim3(grid), dim3(WG), 0, s, p);
+}
+template <int D> void launch_cg_update_s(const KP& p, hipStream_t s, int grid, int k) {
+    hipLaunchKernelGGL(k_cg_update_s<D>, dim3(grid), dim3(WG), 0, s, p, k, grid);
+}
+template <int D> void launch_cg_node1(const KP& p, hipStream_t s, int grid, int k, bool one) {
+    if (one) hipLaunchKernelGGL((k_cg_node1<D, true>), dim3(grid), dim3(WG), 0, s, p, k);
+    else hipLaunchKernelGGL((k_cg_node1<D, false>), dim3(grid), dim3(WG), 0, s, p, k);
+}
+template <int D> void launch_cg_node2(const KP& p, hipStream_t s, int grid, int k, bool one) {
+    if (one) hipLaunchKernelGGL((k_cg_node2<D, true>), dim3(grid), dim3(WG), 0, s, p, k, grid);
+    else hipLaunchKernelGGL((k_cg_node2<D, false>), dim3(grid), dim3(WG), 0, s, p, k, grid);
+}
 void launch_cg_reduce_sq(const KP& p, hipStream_t s, int k) { hipLaunchKernelGGL(k_cg_reduce_sq, dim3(p.nscenes), dim3(WG), 0, s, p, k); }
 void launch_cg_reduce_rr(const KP& p, hipStream_t s, int k) { hipLaunchKernelGGL(k_cg_reduce_rr, dim3(p.nscenes), dim3(WG), 0, s, p, k); }
 template <int D> void launch_g2p(const KP& p, hipStream_t s, int grid) {
@@ -1785,10 +1918,11 @@ template <int D> void launch_max_speed(const KP& p, hipStream_t s, int grid) { h
     template void launch_init_volume<D>(const KP&, hipStream_t, int);                         \
     template void launch_particle_force<D>(const KP&, hipStream_t, int, bool, bool);          \
     template void launch_grid_update<D>(const KP&, hipStream_t, int, bool, bool, bool, bool); \
-    template void launch_cg_force<D>(const KP&, hipStream_t, int, bool, int);                 \
-    template void launch_cg_restart_node<D>(const KP&, hipStream_t, int);                     \
-    template void launch_cg_node1<D>(const KP&, hipStream_t, int, int);                       \
-    template void launch_cg_node2<D>(const KP&, hipStream_t, int);                            \
+    template void launch_cg_force<D>(const KP&, hipStream_t, int, bool, int, bool);           \
+    template void launch_cg_restart_node<D>(const KP&, hipStream_t, int, bool);               \
+    template void launch_cg_update_s<D>(const KP&, hipStream_t, int, int);                    \
+    template void launch_cg_node1<D>(const KP&, hipStream_t, int, int, bool);                 \
+    template void launch_cg_node2<D>(const KP&, hipStream_t, int, int, bool);                 \
     template void launch_g2p<D>(const KP&, hipStream_t, int);                                 \
     template void launch_max_speed<D>(const KP&, hipStream_t, int);
 MPM_INSTANTIATE(2)
