@@ -260,6 +260,16 @@ int mpm_scene_preset(const char* name, int32_t counts, uint64_t seed, mpm_scene_
  * arrays must hold n_particles entries. */
 int mpm_scene_generate(const char* name, int32_t counts, uint64_t seed, int64_t cap, float* x,
                        float* v, float* mass, int64_t* n_out);
+/* Sized variants for batched and served scenes (no reference counterpart:
+ * the reference's scenes fix their own particle counts).  target_n > 0
+ * scales the preset's shapes toward target_n particles at 8 per cell of the
+ * given grid, clamped to the domain, and draws exactly target_n uniform
+ * samples; target_n == 0 is the plain preset.  Presets with a fixed count
+ * (the test 18 scenes) reject target_n > 0. */
+int mpm_scene_preset_sized(const char* name, int32_t counts, uint64_t seed, int64_t target_n,
+                           mpm_scene_desc* desc, int64_t* n_particles);
+int mpm_scene_generate_sized(const char* name, int32_t counts, uint64_t seed, int64_t target_n,
+                             int64_t cap, float* x, float* v, float* mass, int64_t* n_out);
 
 #ifdef __cplusplus
 }

@@ -119,6 +119,10 @@ SIGNATURES = {
                                         ctypes.POINTER(SceneDesc), _LP]),
     "mpm_scene_generate": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64,
                                           _FP, _FP, _FP, _LP]),
+    "mpm_scene_preset_sized": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64,
+                                              ctypes.POINTER(SceneDesc), _LP]),
+    "mpm_scene_generate_sized": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64,
+                                                ctypes.c_int64, _FP, _FP, _FP, _LP]),
 }
 
 _lib: Optional[ctypes.CDLL] = None
@@ -158,22 +162,26 @@ def device_info() -> str:
     return buf.value.decode()
 
 
-def scene_preset(name: str, counts: int = 0, seed: int = 0) -> Tuple[SceneDesc, int]:
+def scene_preset(name: str, counts: int = 0, seed: int = 0, n: int = 0) -> Tuple[SceneDesc, int]:
+    """Scene description and particle count of a named scene; n > 0 sizes
+    the scene to exactly n particles (mpm_scene_preset_sized)."""
     d = SceneDesc()
-    n = ctypes.c_int64()
-    _check(lib().mpm_scene_preset(name.encode(), counts, seed, ctypes.byref(d), ctypes.byref(n)))
-    return d, n.value
-
-
-def scene_generate(name: str, counts: int = 0, seed: int = 0):
-    """Returns (desc, x[n,dim], v[n,dim], mass[n]) of a named scene (scene hook
-    Initialize_Particles, mpm_3d/Standard_Tests/Standard_Tests.h:47-154)."""
-    d, n = scene_preset(name, counts, seed)
-    x = np.zeros((n, d.dim), np.float32)
-    v = np.zeros((n, d.dim), np.float32)
-    m = np.zeros(n, np.float32)
     out = ctypes.c_int64()
-    _check(lib().mpm_scene_generate(name.encode(), counts, seed, n, _fp(x), _fp(v), _fp(m), ctypes.byref(out)))
+    _check(lib().mpm_scene_preset_sized(name.encode(), counts, seed, n, ctypes.byref(d), ctypes.byref(out)))
+    return d, out.value
+
+
+def scene_generate(name: str, counts: int = 0, seed: int = 0, n: int = 0):
+    """Returns (desc, x[n,dim], v[n,dim], mass[n]) of a named scene (scene hook
+    Initialize_Particles, mpm_3d/Standard_Tests/Standard_Tests.h:47-154);
+    n > 0 sizes the scene to exactly n particles."""
+    d, count = scene_preset(name, counts, seed, n)
+    x = np.zeros((count, d.dim), np.float32)
+    v = np.zeros((count, d.dim), np.float32)
+    m = np.zeros(count, np.float32)
+    out = ctypes.c_int64()
+    _check(lib().mpm_scene_generate_sized(name.encode(), counts, seed, n, count, _fp(x), _fp(v), _fp(m),
+                                          ctypes.byref(out)))
     return d, x, v, m
 
 

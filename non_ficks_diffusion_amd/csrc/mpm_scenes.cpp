@@ -13,6 +13,13 @@
 // Get_Vector_In_Sphere); those are reproduced by rejection sampling from
 // the same counter-based generator (Nova's generator is not available, so
 // the positions are new, the statistics are the same).
+//
+// Sized scenes (target_n > 0): the preset's shapes are scaled toward the
+// volume that holds target_n particles at 8 per cell (clamped so the shapes
+// stay inside the domain), and exactly target_n uniform samples are drawn
+// inside them, split over the shapes by volume -- the same fixed-count
+// sampling as test 18.  Batched and served scenes draw grid size and
+// particle count independently, so their particles-per-cell varies.
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -79,7 +86,22 @@ struct Spec {
     double density = 400.0;
     int fixed_count = 0;     // >0: uniform samples in each sphere (reference test 18 style)
     double fixed_mass_density = 2.0;
+    int64_t target_n = 0;    // >0: exactly this many uniform samples over all shapes
 };
+
+double shape_volume(const Shape& s, int dim) {
+    if (s.kind == SPHERE) return dim == 3 ? 4.0 / 3.0 * M_PI * s.r * s.r * s.r : M_PI * s.r * s.r;
+    double v = 1.0;
+    for (int k = 0; k < dim; ++k) v *= 2.0 * s.half[k];
+    return v;
+}
+
+// Volume that holds n particles at 8 per cell of a counts^3 grid.
+double volume_for(int64_t n, int counts) { return (double)n / (8.0 * (double)counts * counts * counts); }
+
+double clampd(double x, double lo, double hi) { return std::max(lo, std::min(hi, x)); }
+// x in [lo, hi] at fraction u; the midpoint if the interval is empty.
+double place(double lo, double hi, double u) { return hi >= lo ? lo + (hi - lo) * u : 0.5 * (lo + hi); }
 
 void snow_material(mpm_material& m, double E, double nu, double theta_c, double theta_s, double xi) {
     m.mu0 = (float)(E / (2.0 * (1.0 + nu)));
@@ -143,8 +165,9 @@ Shape sphere(double x, double y, double z, double r, float vx, float vy, float v
     return s;
 }
 
-bool build_spec(const std::string& name, int counts, uint64_t seed, Spec& sp) {
+bool build_spec(const std::string& name, int counts, uint64_t seed, int64_t target_n, Spec& sp) {
     mpm_scene_desc& d = sp.desc;
+    sp.target_n = target_n;
     if (name == "disk2d") {
         // 2D snow disk r=0.1 at (0.5,0.7), V=(0,-2), ppc 4, ground y=0.05.
         base_desc(d, 2, counts > 0 ? counts : 64);
@@ -154,8 +177,10 @@ bool build_spec(const std::string& name, int counts, uint64_t seed, Spec& sp) {
         // Two snowballs r=0.12 at x=0.3 and x=0.7 moving head-on at 3 m/s each, ground y=0.05.
         base_desc(d, 3, counts > 0 ? counts : 128);
         add_plane(d, 0, 1, 0, 0, 0.05, 0);
-        sp.shapes.push_back(sphere(0.3, 0.5, 0.5, 0.12, 3.f, 0.f, 0.f));
-        sp.shapes.push_back(sphere(0.7, 0.5, 0.5, 0.12, -3.f, 0.f, 0.f));
+        double r = 0.12;
+        if (target_n > 0) r = clampd(std::cbrt(3.0 * volume_for(target_n, d.counts) / (8.0 * M_PI)), 0.03, 0.18);
+        sp.shapes.push_back(sphere(0.3, 0.5, 0.5, r, 3.f, 0.f, 0.f));
+        sp.shapes.push_back(sphere(0.7, 0.5, 0.5, r, -3.f, 0.f, 0.f));
     } else if (name == "avalanche") {
         // Snow slab 0.6 x 0.1 x 0.125 resting on a 30-degree incline; ground y=0.05 and
         // a wall at x=0.05 catch the slide.
@@ -168,13 +193,16 @@ bool build_spec(const std::string& name, int counts, uint64_t seed, Spec& sp) {
         Shape s{};
         s.kind = OBOX;
         const double t[3] = {c30, s30, 0}, n[3] = {-s30, c30, 0}, z[3] = {0, 0, 1};
+        // sized: the slab scaled by f, its lower end 0.1 up the incline from p0
+        const double f = target_n > 0 ? clampd(std::cbrt(volume_for(target_n, d.counts) / 0.0075), 0.3, 1.3) : 1.0;
+        const double along = target_n > 0 ? 0.1 + 0.3 * f : 0.6;
         for (int a = 0; a < 3; ++a) {
-            s.c[a] = p0[a] + 0.6 * t[a] + 0.05 * n[a];
+            s.c[a] = p0[a] + along * t[a] + 0.05 * f * n[a];
             s.axis[0][a] = t[a];
             s.axis[1][a] = n[a];
             s.axis[2][a] = z[a];
         }
-        s.half[0] = 0.3; s.half[1] = 0.05; s.half[2] = 0.0625;
+        s.half[0] = 0.3 * f; s.half[1] = 0.05 * f; s.half[2] = 0.0625 * f;
         sp.shapes.push_back(s);
     } else if (name == "ref3d_test18" || name == "ref2d_test18") {
         // Reference test 18: two balls r=0.1 at x=0.8 / 0.2 with V=(-2,0.1) / (2,0.1),
@@ -203,9 +231,16 @@ bool build_spec(const std::string& name, int counts, uint64_t seed, Spec& sp) {
         const double u0 = unit_float(key3(seed, 11, 0)), u1 = unit_float(key3(seed, 11, 1));
         const double u2 = unit_float(key3(seed, 11, 2)), u3 = unit_float(key3(seed, 11, 3));
         if (name == "blob") {
-            const double r = 0.08 + 0.10 * u0;
-            sp.shapes.push_back(sphere(0.3 + 0.4 * u1, 0.35 + 0.3 * u2, 0.3 + 0.4 * u3, r,
-                                       (float)(2.0 * (u1 - 0.5)), (float)(-1.0 - u2), 0.f));
+            if (target_n > 0) {
+                const double r = clampd(std::cbrt(3.0 * volume_for(target_n, d.counts) / (4.0 * M_PI)), 0.03, 0.3);
+                sp.shapes.push_back(sphere(place(0.06 + r, 0.94 - r, u1), place(0.07 + r, 0.94 - r, u2),
+                                           place(0.06 + r, 0.94 - r, u3), r,
+                                           (float)(2.0 * (u1 - 0.5)), (float)(-1.0 - u2), 0.f));
+            } else {
+                const double r = 0.08 + 0.10 * u0;
+                sp.shapes.push_back(sphere(0.3 + 0.4 * u1, 0.35 + 0.3 * u2, 0.3 + 0.4 * u3, r,
+                                           (float)(2.0 * (u1 - 0.5)), (float)(-1.0 - u2), 0.f));
+            }
         } else {
             Shape s{};
             s.kind = OBOX;
@@ -214,7 +249,13 @@ bool build_spec(const std::string& name, int counts, uint64_t seed, Spec& sp) {
             s.half[0] = 0.12 + 0.15 * u0;
             s.half[1] = 0.03 + 0.04 * u1;
             s.half[2] = 0.08 + 0.1 * u2;
-            s.c[0] = 0.5; s.c[1] = 0.06 + s.half[1] + 0.2 * u3; s.c[2] = 0.5;
+            if (target_n > 0) {
+                const double f = std::cbrt(volume_for(target_n, d.counts) / shape_volume(s, 3));
+                s.half[0] = clampd(f * s.half[0], 0.01, 0.44);
+                s.half[1] = clampd(f * s.half[1], 0.01, 0.3);
+                s.half[2] = clampd(f * s.half[2], 0.01, 0.44);
+            }
+            s.c[0] = 0.5; s.c[1] = std::min(0.06 + s.half[1] + 0.2 * u3, 0.94 - s.half[1]); s.c[2] = 0.5;
             s.v[0] = 0.f; s.v[1] = -0.5f; s.v[2] = 0.f;
             sp.shapes.push_back(s);
         }
@@ -242,6 +283,34 @@ int64_t generate(const Spec& sp, uint64_t seed, int64_t cap, float* x, float* v,
         }
         ++count;
     };
+    if (sp.target_n > 0) {
+        // exactly target_n uniform samples, split over the shapes by volume
+        double vtot = 0.0;
+        for (const Shape& s : sp.shapes) vtot += shape_volume(s, dim);
+        int64_t given = 0;
+        for (size_t si = 0; si < sp.shapes.size(); ++si) {
+            const Shape& s = sp.shapes[si];
+            const double vs = shape_volume(s, dim);
+            const int64_t ns = si + 1 == sp.shapes.size() ? sp.target_n - given
+                                                          : (int64_t)std::llround((double)sp.target_n * vs / vtot);
+            given += ns;
+            if (ns <= 0) continue;
+            const float m = (float)(sp.density * vs / (double)ns);
+            double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+            s.bounds(dim, lo, hi);
+            uint64_t attempt = 0;
+            for (int64_t i = 0; i < ns;) {
+                double p[3] = {0, 0, 0};
+                for (int a = 0; a < dim; ++a)
+                    p[a] = lo[a] + (hi[a] - lo[a]) * unit_float(key3(seed ^ (si * 0x1000193ull), attempt, (uint64_t)a + 7));
+                ++attempt;
+                if (!s.inside(p, dim)) continue;
+                emit(p, s.v, m);
+                ++i;
+            }
+        }
+        return count;
+    }
     if (sp.fixed_count > 0) {
         for (size_t si = 0; si < sp.shapes.size(); ++si) {
             const Shape& s = sp.shapes[si];
@@ -296,25 +365,41 @@ int64_t generate(const Spec& sp, uint64_t seed, int64_t cap, float* x, float* v,
 
 }  // namespace
 
-extern "C" int mpm_scene_preset(const char* name, int32_t counts, uint64_t seed,
-                                mpm_scene_desc* desc, int64_t* n_particles) {
+extern "C" int mpm_scene_preset_sized(const char* name, int32_t counts, uint64_t seed, int64_t target_n,
+                                      mpm_scene_desc* desc, int64_t* n_particles) {
     if (!name || !desc) return mpm_set_error(MPM_E_INVALID, "mpm_scene_preset: null argument");
+    if (target_n < 0 || target_n > ((int64_t)1 << 30)) return mpm_set_error(MPM_E_INVALID, "mpm_scene_preset: bad target_n");
     Spec sp;
-    if (!build_spec(name, counts, seed, sp))
+    if (!build_spec(name, counts, seed, target_n, sp))
         return mpm_set_error(MPM_E_INVALID, std::string("unknown scene ") + name);
+    if (target_n > 0 && sp.fixed_count > 0)
+        return mpm_set_error(MPM_E_INVALID, std::string("scene ") + name + " has a fixed particle count");
     *desc = sp.desc;
-    if (n_particles) *n_particles = generate(sp, seed, 0, nullptr, nullptr, nullptr);
+    if (n_particles) *n_particles = target_n > 0 ? target_n : generate(sp, seed, 0, nullptr, nullptr, nullptr);
     return MPM_OK;
 }
 
-extern "C" int mpm_scene_generate(const char* name, int32_t counts, uint64_t seed, int64_t cap,
-                                  float* x, float* v, float* mass, int64_t* n_out) {
+extern "C" int mpm_scene_generate_sized(const char* name, int32_t counts, uint64_t seed, int64_t target_n,
+                                        int64_t cap, float* x, float* v, float* mass, int64_t* n_out) {
     if (!name || !x || !v || !mass) return mpm_set_error(MPM_E_INVALID, "mpm_scene_generate: null argument");
-    Spec sp;
-    if (!build_spec(name, counts, seed, sp))
-        return mpm_set_error(MPM_E_INVALID, std::string("unknown scene ") + name);
-    const int64_t n = generate(sp, seed, cap, x, v, mass);
+    mpm_scene_desc d;
+    int64_t n = 0;
+    int rc = mpm_scene_preset_sized(name, counts, seed, target_n, &d, &n);
+    if (rc) return rc;
     if (n_out) *n_out = n;
     if (n > cap) return mpm_set_error(MPM_E_CAPACITY, "mpm_scene_generate: capacity too small");
+    Spec sp;
+    build_spec(name, counts, seed, target_n, sp);
+    generate(sp, seed, cap, x, v, mass);
     return MPM_OK;
+}
+
+extern "C" int mpm_scene_preset(const char* name, int32_t counts, uint64_t seed, mpm_scene_desc* desc,
+                                int64_t* n_particles) {
+    return mpm_scene_preset_sized(name, counts, seed, 0, desc, n_particles);
+}
+
+extern "C" int mpm_scene_generate(const char* name, int32_t counts, uint64_t seed, int64_t cap, float* x, float* v,
+                                  float* mass, int64_t* n_out) {
+    return mpm_scene_generate_sized(name, counts, seed, 0, cap, x, v, mass, n_out);
 }
