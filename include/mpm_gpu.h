@@ -112,6 +112,20 @@ typedef struct mpm_step_stats {
 
 typedef struct mpm_context* mpm_handle;
 
+/* Launch shapes of a handle's kernels (no counterpart in the reference,
+ * whose loops run on CPU threads).  Every count is a multiple of 8 (the
+ * XCDs of an MI355X). */
+typedef struct mpm_launch_config {
+    int32_t block_groups;     /* work-groups of the per-block kernels (P2G, forces, CG operator, G2P);
+                                 each takes about active_blocks / block_groups 4^3 blocks */
+    int32_t node_groups;      /* work-groups of the per-node kernels (normalize, grid update, CG vector
+                                 updates), at most 2048 */
+    int32_t particle_groups;  /* work-groups of the per-particle kernels (binning, reorder); particles
+                                 per thread = live particles / (256 particle_groups) */
+    int32_t from_cache;       /* mpm_autotune: 1 if taken from the size-bucket cache */
+    float block_ms, node_ms, particle_ms;  /* mpm_autotune: time of the chosen shape's timing sequence */
+} mpm_launch_config;
+
 /* ---- library ---------------------------------------------------------- */
 const char* mpm_last_error(void);
 /* Writes "<device name>;<CUs>;<arch>" into buf. */
@@ -219,6 +233,30 @@ int32_t mpm_kernel_class_count(void);
  * average milliseconds per launch.  Needs a completed substep; leaves the
  * state of that substep unchanged except the operator's tile buffer. */
 int mpm_time_cg_operator(mpm_handle h, int32_t reps, float* avg_ms);
+
+/* ---- launch shapes, autotuner, captured substeps ------------------------ */
+/* (No counterpart in the reference.)  Current launch shapes of a handle. */
+int mpm_get_launch_config(mpm_handle h, mpm_launch_config* cfg);
+/* Sets the launch shapes (results do not depend on them). */
+int mpm_set_launch_config(mpm_handle h, const mpm_launch_config* cfg);
+/* Autotuner: times candidate work-group counts for the per-block,
+ * per-node and per-particle kernels on the handle's current particles
+ * (re-binned, with the grid and CG set up for dt; nscenes dts, or one dt for
+ * a single scene) and keeps the fastest of each.  Particle state is not
+ * advanced; grid and CG buffers are left as scratch, so call it between
+ * substeps.  flags & MPM_TUNE_CACHED: reuse the result of an earlier tune of
+ * a handle in the same size bucket (dimension, one scene or a batch,
+ * log2 of particle capacity and of block slots) without timing.  Results
+ * go to *out (may be NULL). */
+#define MPM_TUNE_CACHED 1
+int mpm_autotune(mpm_handle h, const float* dt, int32_t flags, mpm_launch_config* out);
+/* Captured substeps: with on != 0 every substep replays a HIP graph of its
+ * whole launch sequence (captured at the first such substep, one graph per
+ * particle-buffer parity), the CG with a fixed count of max(cg_max_iterations)
+ * iterations whose kernels return at once for finished solves, and dts read
+ * from device memory.  Results equal the launched substep's.  Needs CG caps
+ * of at most 256 iterations. */
+int mpm_set_graph(mpm_handle h, int32_t on);
 
 /* ---- batches of scenes ---------------------------------------------------
  * No counterpart in the reference, which runs one MPM_Example per process.

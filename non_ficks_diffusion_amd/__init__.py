@@ -76,6 +76,17 @@ class StepStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class LaunchConfig(ctypes.Structure):
+    _fields_ = [("block_groups", ctypes.c_int32), ("node_groups", ctypes.c_int32),
+                ("particle_groups", ctypes.c_int32), ("from_cache", ctypes.c_int32),
+                ("block_ms", ctypes.c_float), ("node_ms", ctypes.c_float), ("particle_ms", ctypes.c_float)]
+
+    def as_dict(self) -> Dict[str, float]:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+MPM_TUNE_CACHED = 1
+
 _FP = ctypes.POINTER(ctypes.c_float)
 _IP = ctypes.POINTER(ctypes.c_int32)
 _UP = ctypes.POINTER(ctypes.c_uint32)
@@ -106,6 +117,10 @@ SIGNATURES = {
     "mpm_kernel_class_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "mpm_kernel_class_count": (ctypes.c_int32, []),
     "mpm_time_cg_operator": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
+    "mpm_get_launch_config": (ctypes.c_int, [_H, ctypes.POINTER(LaunchConfig)]),
+    "mpm_set_launch_config": (ctypes.c_int, [_H, ctypes.POINTER(LaunchConfig)]),
+    "mpm_autotune": (ctypes.c_int, [_H, _FP, ctypes.c_int32, ctypes.POINTER(LaunchConfig)]),
+    "mpm_set_graph": (ctypes.c_int, [_H, ctypes.c_int32]),
     "mpm_batch_create": (ctypes.c_int, [ctypes.POINTER(SceneDesc), _LP, ctypes.c_int32, ctypes.POINTER(_H)]),
     "mpm_batch_size": (ctypes.c_int32, [_H]),
     "mpm_batch_upload": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP]),
@@ -190,8 +205,37 @@ def kernel_class_names():
     return [L.mpm_kernel_class_name(k).decode() for k in range(L.mpm_kernel_class_count())]
 
 
-class Scene:
+class _LaunchControl:
+    """Launch shapes, autotuner and captured substeps of a handle."""
+
+    def launch_config(self) -> Dict[str, float]:
+        g = LaunchConfig()
+        _check(lib().mpm_get_launch_config(self._h, ctypes.byref(g)))
+        return g.as_dict()
+
+    def set_launch_config(self, block_groups: int, node_groups: int, particle_groups: int) -> None:
+        g = LaunchConfig(block_groups, node_groups, particle_groups, 0, 0.0, 0.0, 0.0)
+        _check(lib().mpm_set_launch_config(self._h, ctypes.byref(g)))
+
+    def autotune(self, dt, cached: bool = False) -> Dict[str, float]:
+        """Times candidate launch shapes on the current particles (state is
+        not advanced) and keeps the fastest; cached: reuse an earlier result
+        of the same size bucket."""
+        d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float32), (self._nscenes(),)), dtype=np.float32)
+        g = LaunchConfig()
+        _check(lib().mpm_autotune(self._h, _fp(d), MPM_TUNE_CACHED if cached else 0, ctypes.byref(g)))
+        return g.as_dict()
+
+    def use_graph(self, on: bool = True) -> None:
+        """Each substep replays a captured HIP graph of its launch sequence."""
+        _check(lib().mpm_set_graph(self._h, 1 if on else 0))
+
+
+class Scene(_LaunchControl):
     """One device-resident simulation (MPM_Example + MPM_Driver on the GPU)."""
+
+    def _nscenes(self) -> int:
+        return 1
 
     def __init__(self, desc: SceneDesc, capacity: int):
         self.desc = desc.copy()
@@ -324,9 +368,12 @@ class Scene:
         return {names[k]: (ms[k], n[k]) for k in range(len(names))}
 
 
-class Batch:
+class Batch(_LaunchControl):
     """Many independent scenes of one dimension advanced together: every
     kernel of a substep covers all of them, and each runs its own CG."""
+
+    def _nscenes(self) -> int:
+        return self.n
 
     def __init__(self, descs, capacities):
         self.descs = [d.copy() for d in descs]

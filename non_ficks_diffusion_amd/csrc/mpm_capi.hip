@@ -5,6 +5,9 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <map>
+#include <mutex>
+#include <tuple>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -87,6 +90,14 @@ struct mpm_context {
     int* h_done = nullptr;  // pinned
     hipEvent_t done_ev[2] = {nullptr, nullptr};
     int grid_particle = 0, grid_block = 0, grid_node = 0;
+    size_t nred = 0;           // entries of the reduction partial arrays
+    // captured substeps: one graph per particle-buffer parity
+    bool graph = false;
+    hipGraphExec_t gexec[2] = {nullptr, nullptr};
+    float* d_dt = nullptr;     // per-scene dt read by a replayed graph
+    float* h_dt = nullptr;     // pinned staging of d_dt
+    hipEvent_t dt_ev = nullptr;
+    bool dt_pending = false;
     bool initialized = false;
     bool binned = false;       // particles sit in the block ranges of the last binning
     bool ever_binned = false;  // particle ranges are the binned ones (not the upload ranges)
@@ -117,6 +128,8 @@ int dalloc(mpm_context* c, T** p, size_t count) {
 }
 
 int nfields(int D) { return 2 * D + D * D + 3; }
+constexpr int MAX_NODE_GROUPS = 2048;
+constexpr int MAX_GRAPH_CG_ITERATIONS = 256;
 int block_edge(int D) { return D == 3 ? 4 : 8; }
 int slots_per_axis(int D, int counts) { const int B = block_edge(D); return (counts + 2 * B) / B + 1; }
 
@@ -297,7 +310,7 @@ int max_cg_iterations(const mpm_context* c) {
 // scene's block partials in per-scene kernels (five launches per iteration);
 // one scene needs no reducer launches (four per iteration).
 template <int D>
-int stage_cg(mpm_context* c, bool do_init) {
+int stage_cg(mpm_context* c, bool do_init, bool fixed = false) {
     mpm::KP p = make_kp(c);
     const bool one = c->nscenes == 1;
     const int gn = c->grid_node;
@@ -319,6 +332,12 @@ int stage_cg(mpm_context* c, bool do_init) {
             if (!one) mpm::launch_cg_reduce_rr(p, c->stream, k + 1);
         });
     };
+    if (fixed) {
+        // captured substeps: every iteration up to the largest cap, no polling
+        for (int k = 0; k < max_it; ++k) iteration(k);
+        if (one) mpm::launch_cg_update_s<D>(p, c->stream, gn, max_it);
+        return MPM_OK;
+    }
     // Iterations after every solve is done return at once, so launching a few
     // too many is harmless.  The first batch is the previous solve's iteration
     // count; after it the host polls the done count one chunk behind the GPU
@@ -395,15 +414,17 @@ int run_stage(mpm_context* c, int stage, float dt) {
 // MPM_Driver::Advance_Step (MPM_Driver.cpp:103-116) for every scene with
 // dt > 0, with the constitutive update, explicit force, collision and CG
 // set-up fused.
+// capture: the launch sequence of a graph (dts from d_dt, fixed CG count).
 template <int D>
-int substep(mpm_context* c, const float* dts) {
-    set_step(c, dts, 0.0f, false, false);
+int substep(mpm_context* c, const float* dts, bool capture = false) {
+    if (capture) mpm::launch_set_step_dev(make_kp(c), c->stream, c->d_dt);
+    else set_step(c, dts, 0.0f, false, false);
     stage_bin<D>(c);
     stage_p2g<D>(c);
     mpm::KP p = make_kp(c);
     timed(c, KC_PFORCE, [&] { mpm::launch_particle_force<D>(p, c->stream, c->grid_block, true, true); });
     timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, true, true, true, false); });
-    int rc = stage_cg<D>(c, false);
+    int rc = stage_cg<D>(c, false, capture);
     if (rc != MPM_OK) return rc;
     stage_g2p<D>(c);
     return MPM_OK;
@@ -546,10 +567,11 @@ int create_batch(const mpm_scene_desc* descs, const int64_t* capacities, int nsc
     // node kernels: one wave per block, up to 1024 work-groups (a multiple of 8 for the XCD ranges)
     c->grid_node = (int)std::min<int64_t>(((int64_t)c->bcap / (mpm::WG / 64) + 7) / 8 * 8 + 8, 1024);
     // reduction partials: per block (batches) or per node work-group (one scene)
-    const size_t nred = (size_t)std::max(c->bcap, c->grid_node);
-    A(dalloc(c, &c->bsum, nred));
-    A(dalloc(c, &c->bmax, nred));
-    A(dalloc(c, &c->bsq, nred));
+    c->nred = (size_t)std::max(c->bcap, MAX_NODE_GROUPS);
+    A(dalloc(c, &c->bsum, c->nred));
+    A(dalloc(c, &c->bmax, c->nred));
+    A(dalloc(c, &c->bsq, c->nred));
+    A(dalloc(c, &c->d_dt, (size_t)nscenes));
     A(dalloc(c, &c->bu, (size_t)2 * c->bcap));
     if (rc == MPM_OK) {
         if (hipMemcpy(c->scn, c->table.data(), sizeof(mpm::SceneParams) * (size_t)nscenes, hipMemcpyHostToDevice) != hipSuccess ||
@@ -570,6 +592,12 @@ int create_batch(const mpm_scene_desc* descs, const int64_t* capacities, int nsc
         c->h_done = (int*)hp;
         for (int k = 0; k < 2 && rc == MPM_OK; ++k)
             if (hipEventCreateWithFlags(&c->done_ev[k], hipEventDisableTiming) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipEventCreate failed");
+        hp = nullptr;
+        if (rc == MPM_OK && hipHostMalloc(&hp, sizeof(float) * (size_t)nscenes, hipHostMallocDefault) != hipSuccess)
+            rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");
+        c->h_dt = (float*)hp;
+        if (rc == MPM_OK && hipEventCreateWithFlags(&c->dt_ev, hipEventDisableTiming) != hipSuccess)
+            rc = mpm_set_error(MPM_E_HIP, "hipEventCreate failed");
     }
     if (rc != MPM_OK) {
         std::string msg = g_err;
@@ -687,8 +715,55 @@ int download_scene(mpm_context* c, int s, int64_t cap, float* x, float* v, float
     return MPM_OK;
 }
 
+void invalidate_graphs(mpm_context* c) {
+    for (int k = 0; k < 2; ++k)
+        if (c->gexec[k]) {
+            (void)hipGraphExecDestroy(c->gexec[k]);
+            c->gexec[k] = nullptr;
+        }
+}
+
+// One substep as a graph replay: the dts go to d_dt through pinned staging,
+// then the graph of the current buffer parity (captured on first use) runs.
+int graph_substep(mpm_context* c, const float* dts) {
+    if (c->dt_pending) {  // the staging buffer is free once the last copy ran
+        HIPCHECK(hipEventSynchronize(c->dt_ev));
+        c->dt_pending = false;
+    }
+    std::memcpy(c->h_dt, dts, sizeof(float) * (size_t)c->nscenes);
+    HIPCHECK(hipMemcpyAsync(c->d_dt, c->h_dt, sizeof(float) * (size_t)c->nscenes, hipMemcpyHostToDevice, c->stream));
+    HIPCHECK(hipEventRecord(c->dt_ev, c->stream));
+    c->dt_pending = true;
+    const int parity = c->cur;
+    if (!c->gexec[parity]) {
+        const bool timing = c->timing;
+        c->timing = false;
+        HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = c->D == 3 ? substep<3>(c, nullptr, true) : substep<2>(c, nullptr, true);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        c->timing = timing;
+        c->cur = parity;  // capture records launches only: the buffers have not moved
+        if (rc != MPM_OK || e != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc != MPM_OK ? rc : mpm_set_error(MPM_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+        }
+        const hipError_t ei = hipGraphInstantiate(&c->gexec[parity], g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ei != hipSuccess) {
+            c->gexec[parity] = nullptr;
+            return mpm_set_error(MPM_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+        }
+    }
+    HIPCHECK(hipGraphLaunch(c->gexec[parity], c->stream));
+    c->cur = parity ^ 1;  // the replay binned once
+    c->binned = false;
+    c->ever_binned = true;
+    return MPM_OK;
+}
+
 int do_substep(mpm_context* c, const float* dts) {
-    int rc = c->D == 3 ? substep<3>(c, dts) : substep<2>(c, dts);
+    int rc = c->graph ? graph_substep(c, dts) : c->D == 3 ? substep<3>(c, dts) : substep<2>(c, dts);
     if (rc) return rc;
     for (int s = 0; s < c->nscenes; ++s)
         if (dts[s] > 0.0f) c->sc[(size_t)s].last_dt = dts[s];
@@ -707,6 +782,176 @@ int max_velocities(mpm_context* c, float* out) {
         std::memcpy(&v2, &st[(size_t)s].max_v2, sizeof(v2));
         out[s] = std::sqrt(v2);
     }
+    return MPM_OK;
+}
+
+// ---- launch-shape autotuner ----------------------------------------------
+// Size bucket: dimension, batch of one or more, log2 of particle capacity
+// and of block slots.
+using TuneKey = std::tuple<int, int, int, int>;
+std::mutex g_tune_mutex;
+std::map<TuneKey, mpm_launch_config> g_tune_cache;
+
+int ilog2(int64_t v) {
+    int r = 0;
+    while (((int64_t)1 << (r + 1)) <= v) ++r;
+    return r;
+}
+TuneKey tune_key(const mpm_context* c) { return TuneKey(c->D, c->nscenes == 1 ? 1 : 0, ilog2(c->pcap), ilog2(c->nslots)); }
+int round8(int64_t v) { return (int)std::max<int64_t>(8, (v + 7) / 8 * 8); }
+
+int check_config(const mpm_context* c, const mpm_launch_config& g) {
+    auto ok = [](int v, int hi) { return v >= 8 && v <= hi && v % 8 == 0; };
+    if (!ok(g.block_groups, 1 << 20) || !ok(g.node_groups, (int)std::min<size_t>(c->nred, MAX_NODE_GROUPS)) ||
+        !ok(g.particle_groups, 1 << 20))
+        return mpm_set_error(MPM_E_INVALID, "launch config: work-group counts must be multiples of 8 in range "
+                                            "(node_groups at most 2048)");
+    return MPM_OK;
+}
+
+void apply_config(mpm_context* c, const mpm_launch_config& g) {
+    if (g.block_groups != c->grid_block || g.node_groups != c->grid_node || g.particle_groups != c->grid_particle)
+        invalidate_graphs(c);
+    c->grid_block = g.block_groups;
+    c->grid_node = g.node_groups;
+    c->grid_particle = g.particle_groups;
+}
+
+// Re-opens every scene's CG solve so that operator launches do work.
+int reopen_cg(mpm_context* c) {
+    std::vector<mpm::CGState> g((size_t)c->nscenes);
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipMemcpy(g.data(), c->cg, sizeof(mpm::CGState) * g.size(), hipMemcpyDeviceToHost));
+    for (auto& x : g) x.done = 0;
+    HIPCHECK(hipMemcpy(c->cg, g.data(), sizeof(mpm::CGState) * g.size(), hipMemcpyHostToDevice));
+    const int zero = 0;
+    HIPCHECK(hipMemcpy(&c->ctr->cg_ndone, &zero, sizeof(int), hipMemcpyHostToDevice));
+    return MPM_OK;
+}
+
+// Median time of five runs of seq (after one untimed run); prep runs
+// before each, outside the timed span.
+template <class Prep, class Seq>
+int time_median(mpm_context* c, Prep&& prep, Seq&& seq, float& ms_out) {
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    float t[5];
+    int rc = MPM_OK;
+    for (int r = -1; r < 5 && rc == MPM_OK; ++r) {
+        rc = prep();
+        if (rc) break;
+        if (hipEventRecord(e0, c->stream) != hipSuccess) { rc = mpm_set_error(MPM_E_HIP, "hipEventRecord"); break; }
+        seq();
+        if (hipEventRecord(e1, c->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess) {
+            rc = mpm_set_error(MPM_E_HIP, "autotune timing failed");
+            break;
+        }
+        float ms = 0.0f;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        if (r >= 0) t[r] = ms;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) return rc;
+    std::sort(t, t + 5);
+    ms_out = t[2];
+    return check_launch("autotune");
+}
+
+// Times each candidate work-group count on a sequence of the kernels it
+// shapes, weighted as in a substep of a few CG iterations:
+//   per-block:    P2G + particle force + 8 CG operator applications;
+//   per-node:     normalize + grid update + 8 CG restart node passes;
+//   per-particle: binning (bin, scan, block set-up, reorder) twice.
+// Every kernel of a sequence is idempotent on the binned particles, so the
+// particle state does not change.
+template <int D>
+int autotune(mpm_context* c, const float* dts, mpm_launch_config& best) {
+    struct TimingOff {  // no per-class events while tuning
+        mpm_context* c;
+        bool saved;
+        ~TimingOff() { c->timing = saved; }
+    } timing_off{c, c->timing};
+    c->timing = false;
+    set_step(c, dts, 0.0f, false, false);
+    stage_bin<D>(c);
+    mpm::Counters h{};
+    int rc = read_counters(c, h);
+    if (rc) return rc;
+    if (h.overflow) return mpm_set_error(MPM_E_CAPACITY, "mpm_autotune: active blocks exceed the block capacity");
+    const bool one = c->nscenes == 1;
+    mpm_launch_config cur{c->grid_block, c->grid_node, c->grid_particle, 0, 0.0f, 0.0f, 0.0f};
+    best = cur;
+    // a grid and CG set-up on the binned particles
+    {
+        mpm::KP p = make_kp(c);
+        mpm::launch_p2g<D>(p, c->stream, c->grid_block);
+        mpm::launch_normalize<D>(p, c->stream, c->grid_node);
+        mpm::launch_particle_force<D>(p, c->stream, c->grid_block, false, true);
+        mpm::launch_grid_update<D>(p, c->stream, c->grid_node, true, true, true, false);
+        mpm::launch_cg_force<D>(p, c->stream, c->grid_block, true, 0, one);
+        mpm::launch_cg_restart_node<D>(p, c->stream, c->grid_node, one);
+    }
+    auto no_prep = [] { return MPM_OK; };
+    auto reopen = [&] { return reopen_cg(c); };
+    // per-block kernels
+    std::vector<int> cand;
+    for (int g : {1024, 2048, 4096, 8192, 16384, 32768})
+        if (g <= round8(h.nblocks) || g == 1024) cand.push_back(g);
+    if (std::find(cand.begin(), cand.end(), cur.block_groups) == cand.end()) cand.push_back(cur.block_groups);
+    float best_ms = FLT_MAX;
+    for (int g : cand) {
+        const mpm::KP p = make_kp(c);
+        float ms = 0.0f;
+        rc = time_median(c, reopen, [&] {
+            mpm::launch_p2g<D>(p, c->stream, g);
+            mpm::launch_particle_force<D>(p, c->stream, g, false, true);
+            for (int k = 0; k < 8; ++k) mpm::launch_cg_force<D>(p, c->stream, g, false, 1, one);
+        }, ms);
+        if (rc) return rc;
+        if (ms < best_ms) { best_ms = ms; best.block_groups = g; best.block_ms = ms; }
+    }
+    // per-node kernels
+    cand.clear();
+    const int node_hi = (int)std::min<size_t>(c->nred, MAX_NODE_GROUPS);
+    for (int g : {128, 256, 512, 1024, 2048})
+        if (g <= node_hi && (g <= round8(h.nblocks / 4 + 8) || g == 128)) cand.push_back(g);
+    if (std::find(cand.begin(), cand.end(), cur.node_groups) == cand.end()) cand.push_back(cur.node_groups);
+    best_ms = FLT_MAX;
+    for (int g : cand) {
+        const mpm::KP p = make_kp(c);
+        float ms = 0.0f;
+        rc = time_median(c, no_prep, [&] {
+            mpm::launch_normalize<D>(p, c->stream, g);
+            mpm::launch_grid_update<D>(p, c->stream, g, true, true, true, false);
+            for (int k = 0; k < 8; ++k) mpm::launch_cg_restart_node<D>(p, c->stream, g, one);
+        }, ms);
+        if (rc) return rc;
+        if (ms < best_ms) { best_ms = ms; best.node_groups = g; best.node_ms = ms; }
+    }
+    // per-particle kernels: 1 to 16 particles per thread
+    cand.clear();
+    for (int ppt : {1, 2, 4, 8, 16}) {
+        const int g = round8(((int64_t)h.n_live + (int64_t)mpm::WG * ppt - 1) / ((int64_t)mpm::WG * ppt));
+        if (g <= (1 << 20) && std::find(cand.begin(), cand.end(), g) == cand.end()) cand.push_back(g);
+    }
+    if (std::find(cand.begin(), cand.end(), cur.particle_groups) == cand.end()) cand.push_back(cur.particle_groups);
+    best_ms = FLT_MAX;
+    const int keep_particle = c->grid_particle;
+    for (int g : cand) {
+        float ms = 0.0f;
+        c->grid_particle = g;
+        rc = time_median(c, no_prep, [&] {
+            stage_bin<D>(c);
+            stage_bin<D>(c);
+        }, ms);
+        if (rc) { c->grid_particle = keep_particle; return rc; }
+        if (ms < best_ms) { best_ms = ms; best.particle_groups = g; best.particle_ms = ms; }
+    }
+    c->grid_particle = keep_particle;
+    // the grid and CG buffers hold tuning scratch; the particles stay binned
+    c->binned = true;
     return MPM_OK;
 }
 
@@ -771,6 +1016,9 @@ int mpm_destroy(mpm_handle c) {
     for (hipEvent_t e : c->ev) (void)hipEventDestroy(e);
     for (int k = 0; k < 2; ++k) if (c->done_ev[k]) (void)hipEventDestroy(c->done_ev[k]);
     if (c->h_done) (void)hipHostFree(c->h_done);
+    invalidate_graphs(c);
+    if (c->dt_ev) (void)hipEventDestroy(c->dt_ev);
+    if (c->h_dt) (void)hipHostFree(c->h_dt);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MPM_OK;
@@ -1052,6 +1300,62 @@ int mpm_time_cg_operator(mpm_handle c, int32_t reps, float* avg_ms) {
     int rc = check_launch("mpm_time_cg_operator");
     if (rc) return rc;
     *avg_ms = ms / (float)reps;
+    return MPM_OK;
+}
+
+int mpm_get_launch_config(mpm_handle c, mpm_launch_config* cfg) {
+    if (!c || !cfg) return mpm_set_error(MPM_E_INVALID, "mpm_get_launch_config: null argument");
+    *cfg = mpm_launch_config{c->grid_block, c->grid_node, c->grid_particle, 0, 0.0f, 0.0f, 0.0f};
+    return MPM_OK;
+}
+
+int mpm_set_launch_config(mpm_handle c, const mpm_launch_config* cfg) {
+    if (!c || !cfg) return mpm_set_error(MPM_E_INVALID, "mpm_set_launch_config: null argument");
+    int rc = check_config(c, *cfg);
+    if (rc) return rc;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    apply_config(c, *cfg);
+    return MPM_OK;
+}
+
+int mpm_autotune(mpm_handle c, const float* dt, int32_t flags, mpm_launch_config* out) {
+    if (!c || !dt) return mpm_set_error(MPM_E_INVALID, "mpm_autotune: null argument");
+    if (!c->initialized) return mpm_set_error(MPM_E_NOT_READY, "mpm_autotune: call mpm_initialize first");
+    const TuneKey key = tune_key(c);
+    mpm_launch_config best{};
+    bool cached = false;
+    if (flags & MPM_TUNE_CACHED) {
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        auto it = g_tune_cache.find(key);
+        if (it != g_tune_cache.end()) {
+            best = it->second;
+            cached = true;
+        }
+    }
+    if (!cached) {
+        std::vector<float> dts((size_t)c->nscenes, dt[0]);
+        if (c->nscenes > 1) std::copy(dt, dt + c->nscenes, dts.begin());
+        int rc = c->D == 3 ? autotune<3>(c, dts.data(), best) : autotune<2>(c, dts.data(), best);
+        if (rc) return rc;
+        std::lock_guard<std::mutex> lock(g_tune_mutex);
+        g_tune_cache[key] = best;
+    }
+    int rc = check_config(c, best);
+    if (rc) return rc;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    apply_config(c, best);
+    best.from_cache = cached ? 1 : 0;
+    if (out) *out = best;
+    return MPM_OK;
+}
+
+int mpm_set_graph(mpm_handle c, int32_t on) {
+    if (!c) return mpm_set_error(MPM_E_INVALID, "mpm_set_graph: null handle");
+    if (on && max_cg_iterations(c) > MAX_GRAPH_CG_ITERATIONS)
+        return mpm_set_error(MPM_E_INVALID, "mpm_set_graph: captured substeps need CG caps of at most 256 iterations");
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    if (!on) invalidate_graphs(c);
+    c->graph = on != 0;
     return MPM_OK;
 }
 

@@ -130,6 +130,7 @@ struct StepArgs {
 
 // launchers (host)
 void launch_set_step(const KP& p, hipStream_t s, const StepArgs& a);
+void launch_set_step_dev(const KP& p, hipStream_t s, const float* dt);  // nscenes dts in device memory
 template <int D> void launch_bin(const KP& p, hipStream_t s, int grid);
 void launch_scan(const KP& p, hipStream_t s);
 template <int D> void launch_block_setup(const KP& p, hipStream_t s, int grid);

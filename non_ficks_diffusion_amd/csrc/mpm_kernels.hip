@@ -1848,11 +1848,25 @@ __global__ __launch_bounds__(WG) void k_set_step(KP p, StepArgs a) {
     if (a.clear_waiting) sc.waiting = 0;
 }
 
+// The same from a device array of nscenes dts (replayed graphs: the host
+// copies the substep's dts into `dt` before each replay).
+__global__ __launch_bounds__(WG) void k_set_step_dev(KP p, const float* dt) {
+    const int i = blockIdx.x * WG + threadIdx.x;
+    if (i >= p.nscenes) return;
+    SceneParams& sc = p.scn[i];
+    const float d = dt[i];
+    sc.dt = d > 0.0f ? d : 0.0f;
+    sc.active = d > 0.0f ? 1 : 0;
+}
+
 // ============================================================================
 // launchers
 // ============================================================================
 void launch_set_step(const KP& p, hipStream_t s, const StepArgs& a) {
     hipLaunchKernelGGL(k_set_step, dim3((a.n + WG - 1) / WG), dim3(WG), 0, s, p, a);
+}
+void launch_set_step_dev(const KP& p, hipStream_t s, const float* dt) {
+    hipLaunchKernelGGL(k_set_step_dev, dim3((p.nscenes + WG - 1) / WG), dim3(WG), 0, s, p, dt);
 }
 template <int D> void launch_bin(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_bin<D>, dim3(grid), dim3(WG), 0, s, p); }
 void launch_scan(const KP& p, hipStream_t s) {

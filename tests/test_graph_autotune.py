@@ -1,0 +1,131 @@
+"""Captured substeps and the launch-shape autotuner.
+
+A replayed graph must give what the launched substep gives: the graph runs
+the CG to the largest cap with finished solves returning at once, so every
+substep matches to the FP32 bar of one substep chain (1e-5 relative L2; only
+the order of particles inside a cell, decided by binning's atomics, differs
+between runs).  Each replay is checked against the launched run, with
+downloads (and so synchronisations) between replays.  Launch shapes change
+no result, and tuning leaves the particle state as it was.
+"""
+import numpy as np
+import pytest
+
+import non_ficks_diffusion_amd as M
+
+pytestmark = pytest.mark.gpu
+
+SCENES = [("two_balls", 24, 0, 0), ("blob", 32, 1001, 20000), ("slab", 40, 1002, 8000)]
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def make_batch(data):
+    b = M.Batch([d for d, _, _, _ in data], [len(m) for _, _, _, m in data])
+    for k, (_, x, v, m) in enumerate(data):
+        b.upload(k, x, v, m)
+    b.initialize()
+    return b
+
+
+def load():
+    return [M.scene_generate(n, c, s, k) for n, c, s, k in SCENES]
+
+
+def assert_same(got, ref, what):
+    assert np.array_equal(got["ids"], ref["ids"]), what
+    for f in ("x", "v", "F", "Jp"):
+        assert rel(got[f], ref[f]) < 1e-5, (what, f, rel(got[f], ref[f]))
+
+
+def test_batch_graph_replays_match_launched_substeps():
+    data = load()
+    eager, graph = make_batch(data), make_batch(data)
+    graph.use_graph(True)
+    dts = [[1e-3, 1e-3, 5e-4], [1e-3, 5e-4, 5e-4], [5e-4, 0.0, 1e-3], [1e-3, 1e-3, 1e-3], [5e-4, 5e-4, 0.0]]
+    for step, dt in enumerate(dts):
+        eager.substep(dt)
+        graph.substep(dt)
+        for k in range(len(data)):
+            assert_same(graph.download(k), eager.download(k), (step, k))
+            se, sg = eager.stats(k), graph.stats(k)
+            # a residual at the tolerance may pass one iteration apart (sum order)
+            assert abs(se["cg_iterations"] - sg["cg_iterations"]) <= 1, (step, k)
+            assert se["active_blocks"] == sg["active_blocks"], (step, k)
+    eager.close()
+    graph.close()
+
+
+def test_single_scene_graph_replays_match_launched_substeps():
+    d, x, v, m = M.scene_generate("two_balls", 32, 0)
+    runs = []
+    for g in (False, True):
+        s = M.Scene(d, len(m))
+        s.upload(x, v, m)
+        s.initialize()
+        s.use_graph(g)
+        runs.append(s)
+    for step in range(4):
+        for s in runs:
+            s.substep(1e-3)
+        assert_same(runs[1].download(), runs[0].download(), step)
+        assert abs(runs[1].stats()["cg_iterations"] - runs[0].stats()["cg_iterations"]) <= 1
+    for s in runs:
+        s.close()
+
+
+def test_graph_rejects_uncapped_cg():
+    d, x, v, m = M.scene_generate("ref3d_test18", 20, 0)  # CG cap 10000
+    s = M.Scene(d, len(m))
+    with pytest.raises(M.MPMError):
+        s.use_graph(True)
+    s.close()
+
+
+def test_autotune_keeps_particles_and_results():
+    data = load()
+    tuned, plain = make_batch(data), make_batch(data)
+    dt = [1e-3, 1e-3, 1e-3]
+    before = [tuned.download(k) for k in range(len(data))]
+    cfg = tuned.autotune(dt)
+    for k in range(len(data)):
+        after = tuned.download(k)
+        for f in ("x", "v", "F", "Jp", "volume"):
+            assert np.array_equal(after[f], before[k][f]), (k, f)
+    for key in ("block_groups", "node_groups", "particle_groups"):
+        assert cfg[key] % 8 == 0 and cfg[key] >= 8
+    assert tuned.launch_config()["block_groups"] == cfg["block_groups"]
+    for _ in range(3):
+        tuned.substep(dt)
+        plain.substep(dt)
+    for k in range(len(data)):
+        assert_same(tuned.download(k), plain.download(k), k)
+    # same size bucket: the cached result, no timing
+    again = make_batch(data)
+    cached = again.autotune(dt, cached=True)
+    assert cached["from_cache"] == 1
+    assert cached["block_groups"] == cfg["block_groups"]
+    for b in (tuned, plain, again):
+        b.close()
+
+
+def test_launch_config_shapes_do_not_change_results():
+    data = load()
+    a, b = make_batch(data), make_batch(data)
+    b.set_launch_config(64, 128, 16)
+    b.use_graph(True)
+    for _ in range(3):
+        a.substep(1e-3)
+        b.substep(1e-3)
+    for k in range(len(data)):
+        assert_same(b.download(k), a.download(k), k)
+    with pytest.raises(M.MPMError):
+        b.set_launch_config(100, 128, 16)  # not a multiple of 8
+    with pytest.raises(M.MPMError):
+        b.set_launch_config(64, 4096, 16)  # node work-groups above 2048
+    a.close()
+    b.close()
