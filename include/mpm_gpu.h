@@ -288,6 +288,66 @@ int mpm_batch_download(mpm_handle h, int32_t scene, int64_t cap, float* x, float
 int mpm_batch_advance_to_time(mpm_handle h, const double* time, const double* target,
                               double* time_out, int32_t* substeps_out);
 
+/* ---- serving: continuous batching on one device arena ---------------- */
+/* (No counterpart in the reference, which runs one scene per process.)  A
+ * server owns an arena sized from an HBM budget.  Requests are scenes with
+ * a frame count; mpm_serve_tick admits queued requests in arrival order
+ * while the arena has room (particle space, a free slot range, blocks of
+ * storage by estimate, the byte budget), then advances every resident scene
+ * by one frame (MPM_Driver::Advance_To_Target_Time, MPM_Driver.cpp:55-80,
+ * with each scene's own dt) in batched substeps.  Finished requests leave the
+ * arena at the end of the tick.  A handle is not thread-safe. */
+typedef struct mpm_server* mpm_server_handle;
+
+typedef struct mpm_serve_config {
+    int32_t dim;                 /* 3 (or 2) */
+    int32_t max_scenes;          /* resident scenes at most */
+    int32_t max_counts;          /* largest grid of a request, cells per axis */
+    int32_t cg_max_iterations;   /* CG cap every request must stay within (<= 256) */
+    int64_t hbm_budget_bytes;    /* the arena is sized from it; admission stays within it */
+    int32_t graph;               /* replay captured substeps */
+    int32_t keep_results;        /* keep finished requests' particles for mpm_serve_download */
+} mpm_serve_config;
+
+enum { MPM_REQ_QUEUED = 0, MPM_REQ_RUNNING = 1, MPM_REQ_DONE = 2 };
+
+typedef struct mpm_request_status {
+    int32_t state;               /* MPM_REQ_* */
+    int32_t frames_done, frames_total;
+    int32_t substeps;
+    int64_t particles;
+    int64_t bytes;               /* admission estimate */
+    double sim_time;
+    double submitted, admitted, finished;  /* server clock in seconds, -1 until then */
+} mpm_request_status;
+
+typedef struct mpm_tick_stats {
+    int32_t admitted, finished, resident, queued;
+    int32_t substeps;            /* batched substeps of the tick */
+    int32_t frames;              /* frames completed (one per resident scene) */
+    int64_t particle_substeps;   /* particles of the advancing scenes, summed over substeps */
+    int64_t bytes_in_use;        /* admission estimates of the resident scenes */
+    double ms;                   /* wall time of the tick, device work included */
+} mpm_tick_stats;
+
+int mpm_serve_create(const mpm_serve_config* cfg, mpm_server_handle* out);
+int mpm_serve_destroy(mpm_server_handle s);
+/* Arena capacities and device bytes. */
+int mpm_serve_arena(mpm_server_handle s, int64_t* particles, int64_t* slots, int64_t* blocks, int64_t* bytes);
+/* Queues a request (particles copied; F = I, Jp = 1, volumes from the grid
+ * at admission).  MPM_E_CAPACITY if it could never fit the arena. */
+int mpm_serve_submit(mpm_server_handle s, const mpm_scene_desc* desc, int64_t n, const float* x,
+                     const float* v, const float* mass, int32_t frames, int64_t* request_id);
+int mpm_serve_tick(mpm_server_handle s, mpm_tick_stats* out);
+int mpm_serve_poll(mpm_server_handle s, int64_t request_id, mpm_request_status* out);
+/* Particles of a running request (current state) or of a finished one
+ * (final state, with keep_results), sorted by id as mpm_download_particles. */
+int mpm_serve_download(mpm_server_handle s, int64_t request_id, int64_t cap, float* x, float* v,
+                       float* F, float* Jp, float* mass, float* volume, int32_t* ids,
+                       int64_t* n_out);
+/* Forgets a finished request. */
+int mpm_serve_release(mpm_server_handle s, int64_t request_id);
+
 /* ---- scenes (scene hook Initialize_Particles) ------------------------- */
 /* Fills desc with the preset of a named scene and returns the particle
  * count it generates.  Known names: "disk2d", "two_balls",

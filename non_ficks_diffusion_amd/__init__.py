@@ -87,6 +87,34 @@ class LaunchConfig(ctypes.Structure):
 
 MPM_TUNE_CACHED = 1
 
+
+class ServeConfig(ctypes.Structure):
+    _fields_ = [("dim", ctypes.c_int32), ("max_scenes", ctypes.c_int32), ("max_counts", ctypes.c_int32),
+                ("cg_max_iterations", ctypes.c_int32), ("hbm_budget_bytes", ctypes.c_int64),
+                ("graph", ctypes.c_int32), ("keep_results", ctypes.c_int32)]
+
+
+class RequestStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int32), ("frames_done", ctypes.c_int32), ("frames_total", ctypes.c_int32),
+                ("substeps", ctypes.c_int32), ("particles", ctypes.c_int64), ("bytes", ctypes.c_int64),
+                ("sim_time", ctypes.c_double), ("submitted", ctypes.c_double), ("admitted", ctypes.c_double),
+                ("finished", ctypes.c_double)]
+
+    def as_dict(self) -> Dict[str, float]:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TickStats(ctypes.Structure):
+    _fields_ = [("admitted", ctypes.c_int32), ("finished", ctypes.c_int32), ("resident", ctypes.c_int32),
+                ("queued", ctypes.c_int32), ("substeps", ctypes.c_int32), ("frames", ctypes.c_int32),
+                ("particle_substeps", ctypes.c_int64), ("bytes_in_use", ctypes.c_int64), ("ms", ctypes.c_double)]
+
+    def as_dict(self) -> Dict[str, float]:
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
+
 _FP = ctypes.POINTER(ctypes.c_float)
 _IP = ctypes.POINTER(ctypes.c_int32)
 _UP = ctypes.POINTER(ctypes.c_uint32)
@@ -121,6 +149,16 @@ SIGNATURES = {
     "mpm_set_launch_config": (ctypes.c_int, [_H, ctypes.POINTER(LaunchConfig)]),
     "mpm_autotune": (ctypes.c_int, [_H, _FP, ctypes.c_int32, ctypes.POINTER(LaunchConfig)]),
     "mpm_set_graph": (ctypes.c_int, [_H, ctypes.c_int32]),
+    "mpm_serve_create": (ctypes.c_int, [ctypes.POINTER(ServeConfig), ctypes.POINTER(_H)]),
+    "mpm_serve_destroy": (ctypes.c_int, [_H]),
+    "mpm_serve_arena": (ctypes.c_int, [_H, _LP, _LP, _LP, _LP]),
+    "mpm_serve_submit": (ctypes.c_int, [_H, ctypes.POINTER(SceneDesc), ctypes.c_int64, _FP, _FP, _FP, ctypes.c_int32,
+                                        _LP]),
+    "mpm_serve_tick": (ctypes.c_int, [_H, ctypes.POINTER(TickStats)]),
+    "mpm_serve_poll": (ctypes.c_int, [_H, ctypes.c_int64, ctypes.POINTER(RequestStatus)]),
+    "mpm_serve_download": (ctypes.c_int, [_H, ctypes.c_int64, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _IP,
+                                          _LP]),
+    "mpm_serve_release": (ctypes.c_int, [_H, ctypes.c_int64]),
     "mpm_batch_create": (ctypes.c_int, [ctypes.POINTER(SceneDesc), _LP, ctypes.c_int32, ctypes.POINTER(_H)]),
     "mpm_batch_size": (ctypes.c_int32, [_H]),
     "mpm_batch_upload": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP]),
@@ -458,3 +496,72 @@ class Batch(_LaunchControl):
         n = (ctypes.c_int64 * len(names))()
         _check(lib().mpm_kernel_times(self._h, len(names), ms, n))
         return {names[k]: (ms[k], n[k]) for k in range(len(names))}
+
+
+class Server:
+    """Continuous batching of scene requests on one device arena: submit()
+    queues a scene with a frame count, tick() admits what fits and advances
+    every resident scene by one frame (mpm_serve_*)."""
+
+    def __init__(self, hbm_budget_bytes: int, max_scenes: int = 64, max_counts: int = 256, dim: int = 3,
+                 cg_max_iterations: int = 50, graph: bool = False, keep_results: bool = False):
+        self.dim = dim
+        self.keep_results = keep_results
+        cfg = ServeConfig(dim, max_scenes, max_counts, cg_max_iterations, int(hbm_budget_bytes),
+                          1 if graph else 0, 1 if keep_results else 0)
+        h = _H()
+        _check(lib().mpm_serve_create(ctypes.byref(cfg), ctypes.byref(h)))
+        self._h = h
+        self._n = {}
+
+    def close(self) -> None:
+        if getattr(self, "_h", None):
+            lib().mpm_serve_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def arena(self) -> Dict[str, int]:
+        v = [ctypes.c_int64() for _ in range(4)]
+        _check(lib().mpm_serve_arena(self._h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("particles", "slots", "blocks", "bytes"), (x.value for x in v)))
+
+    def submit(self, desc: SceneDesc, x, v, mass, frames: int) -> int:
+        x, v, mass = _f32(x), _f32(v), _f32(mass)
+        d = desc.copy()
+        rid = ctypes.c_int64()
+        _check(lib().mpm_serve_submit(self._h, ctypes.byref(d), mass.shape[0], _fp(x), _fp(v), _fp(mass), int(frames),
+                                      ctypes.byref(rid)))
+        self._n[rid.value] = mass.shape[0]
+        return rid.value
+
+    def tick(self) -> Dict[str, float]:
+        st = TickStats()
+        _check(lib().mpm_serve_tick(self._h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def poll(self, rid: int) -> Dict[str, float]:
+        st = RequestStatus()
+        _check(lib().mpm_serve_poll(self._h, int(rid), ctypes.byref(st)))
+        return st.as_dict()
+
+    def download(self, rid: int) -> Dict[str, np.ndarray]:
+        d = self.dim
+        cap = self._n[rid]
+        out = {"x": np.zeros((cap, d), np.float32), "v": np.zeros((cap, d), np.float32),
+               "F": np.zeros((cap, d, d), np.float32), "Jp": np.zeros(cap, np.float32),
+               "mass": np.zeros(cap, np.float32), "volume": np.zeros(cap, np.float32),
+               "ids": np.zeros(cap, np.int32)}
+        n = ctypes.c_int64()
+        _check(lib().mpm_serve_download(self._h, int(rid), cap, _fp(out["x"]), _fp(out["v"]), _fp(out["F"]),
+                                        _fp(out["Jp"]), _fp(out["mass"]), _fp(out["volume"]),
+                                        out["ids"].ctypes.data_as(_IP), ctypes.byref(n)))
+        return {k: a[:n.value] for k, a in out.items()}
+
+    def release(self, rid: int) -> None:
+        _check(lib().mpm_serve_release(self._h, int(rid)))
+        self._n.pop(rid, None)

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "../../include/mpm_gpu.h"
+#include "mpm_context.h"
 #include "mpm_internal.h"
 #include "mpm_kernels.h"
 
@@ -36,81 +37,11 @@ int mpm_set_error(int code, const std::string& msg) {
 
 namespace {
 
-enum KClass {
-    KC_BIN = 0, KC_SCAN, KC_SETUP, KC_REORDER, KC_P2G, KC_NORMALIZE, KC_INIT_VOLUME, KC_PFORCE, KC_GRID,
-    KC_CG_FORCE, KC_CG_NODE, KC_G2P, KC_MAXSPEED, KC_COUNT
-};
 const char* kClassNames[KC_COUNT] = {"bin",          "scan",         "block_setup", "reorder", "p2g",
                                      "normalize",    "init_volume",  "constitutive_force", "grid_update",
                                      "cg_force",     "cg_node",      "g2p",         "max_speed"};
 
-// Host view of one scene of a batch.
-struct SceneHost {
-    mpm_scene_desc desc{};
-    int64_t capacity = 0;  // particle entries reserved for the scene
-    int64_t pbase = 0;     // its range in the particle buffer before the first binning
-    int64_t n_uploaded = 0;
-    bool uploaded = false;
-    float last_dt = 0.0f;
-};
-
 }  // namespace
-
-struct mpm_context {
-    int D = 3;
-    int nscenes = 0;
-    std::vector<SceneHost> sc;
-    std::vector<mpm::SceneParams> table;  // host copy of the scene table
-    int64_t pcap = 0;
-    int bcap = 0, nslots = 0, nparts = 0;
-    int64_t gcap = 0;
-    hipStream_t stream = nullptr;
-    float* pbuf[2] = {nullptr, nullptr};
-    int* idbuf[2] = {nullptr, nullptr};
-    int* keybuf[2] = {nullptr, nullptr};
-    int* scbuf[2] = {nullptr, nullptr};
-    int cur = 0;
-    int* prank = nullptr;
-    float* rcache = nullptr;
-    float* pop = nullptr;
-    int *slot_count = nullptr, *slot_touched = nullptr, *slot_active = nullptr;
-    int *block_slot = nullptr, *block_scene = nullptr, *block_pstart = nullptr, *block_nbr = nullptr;
-    int *scan_part = nullptr, *cell_count = nullptr, *cell_start = nullptr;
-    float *g_m = nullptr, *g_v = nullptr, *g_vs = nullptr, *g_q = nullptr, *g_s = nullptr, *g_r = nullptr;
-    unsigned* g_flags = nullptr;
-    float* tiles = nullptr;
-    mpm::Counters* ctr = nullptr;
-    mpm::SceneParams* scn = nullptr;
-    mpm::SceneState* sst = nullptr;
-    mpm::CGState* cg = nullptr;
-    int* slot_base = nullptr;
-    double *bsum = nullptr, *bmax = nullptr, *bsq = nullptr;
-    unsigned* bu = nullptr;
-    int cg_predict = 0;    // iterations launched before the done count was seen, last solve
-    int* h_done = nullptr;  // pinned
-    hipEvent_t done_ev[2] = {nullptr, nullptr};
-    int grid_particle = 0, grid_block = 0, grid_node = 0;
-    size_t nred = 0;           // entries of the reduction partial arrays
-    // captured substeps: one graph per particle-buffer parity
-    bool graph = false;
-    hipGraphExec_t gexec[2] = {nullptr, nullptr};
-    float* d_dt = nullptr;     // per-scene dt read by a replayed graph
-    float* h_dt = nullptr;     // pinned staging of d_dt
-    hipEvent_t dt_ev = nullptr;
-    bool dt_pending = false;
-    bool initialized = false;
-    bool binned = false;       // particles sit in the block ranges of the last binning
-    bool ever_binned = false;  // particle ranges are the binned ones (not the upload ranges)
-    bool stepped = false;      // a substep has run (operator timing needs its state)
-    std::vector<void*> allocations;
-    // timing
-    bool timing = false;
-    std::vector<hipEvent_t> ev;
-    std::vector<int> ev_class;
-    size_t ev_used = 0;
-    double ms[KC_COUNT] = {};
-    int64_t launches[KC_COUNT] = {};
-};
 
 namespace {
 
@@ -174,7 +105,7 @@ mpm::KP make_kp(const mpm_context* c) {
     p.scn = c->scn;
     p.sst = c->sst;
     p.cg = c->cg;
-    p.slot_base = c->slot_base;
+    p.slot_scene = c->slot_scene;
     p.nscenes = c->nscenes;
     p.pcur = c->pbuf[c->cur];
     p.palt = c->pbuf[c->cur ^ 1];
@@ -299,6 +230,7 @@ void stage_waiting(mpm_context* c) {
 }
 
 int max_cg_iterations(const mpm_context* c) {
+    if (c->pool) return c->pool_cg_max;
     int m = 0;
     for (const SceneHost& s : c->sc) m = std::max(m, s.desc.cg_max_iterations);
     return m;
@@ -422,6 +354,12 @@ int substep(mpm_context* c, const float* dts, bool capture = false) {
     stage_bin<D>(c);
     stage_p2g<D>(c);
     mpm::KP p = make_kp(c);
+    if (c->pool) {
+        // scenes attached since the last substep: Process_Waiting_Particles
+        // (every other scene's blocks return at once)
+        timed(c, KC_INIT_VOLUME, [&] { mpm::launch_init_volume<D>(p, c->stream, c->grid_block); });
+        mpm::launch_clear_waiting(p, c->stream);
+    }
     timed(c, KC_PFORCE, [&] { mpm::launch_particle_force<D>(p, c->stream, c->grid_block, true, true); });
     timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, true, true, true, false); });
     int rc = stage_cg<D>(c, false, capture);
@@ -481,50 +419,10 @@ int fill_stats(mpm_context* c, int s, mpm_step_stats* out) {
     return MPM_OK;
 }
 
-int create_batch(const mpm_scene_desc* descs, const int64_t* capacities, int nscenes, mpm_handle* out) {
-    if (!descs || !capacities || !out || nscenes < 1) return mpm_set_error(MPM_E_INVALID, "mpm_batch_create: bad argument");
-    const int D = descs[0].dim;
-    int64_t pcap = 0, slots = 0;
-    for (int s = 0; s < nscenes; ++s) {
-        const mpm_scene_desc& d = descs[s];
-        if (d.dim != 2 && d.dim != 3) return mpm_set_error(MPM_E_INVALID, "mpm_create: dim must be 2 or 3");
-        if (d.dim != D) return mpm_set_error(MPM_E_INVALID, "mpm_batch_create: scenes of one batch share dim");
-        if (d.counts < 4 || d.counts > 4096) return mpm_set_error(MPM_E_INVALID, "mpm_create: counts out of range");
-        if (d.nplanes < 0 || d.nplanes > MPM_MAX_PLANES) return mpm_set_error(MPM_E_INVALID, "mpm_create: too many planes");
-        if (capacities[s] < 1 || capacities[s] > (int64_t)1 << 30) return mpm_set_error(MPM_E_INVALID, "mpm_create: bad capacity");
-        const int64_t nb = slots_per_axis(D, d.counts);
-        slots += D == 3 ? nb * nb * nb : nb * nb;
-        pcap += capacities[s];
-    }
-    // keys are slot*64 + cell in 32 bits
-    if (slots * 64 > ((int64_t)1 << 31) - 1) return mpm_set_error(MPM_E_INVALID, "mpm_create: grid too large");
-    if (pcap > ((int64_t)1 << 31) - 1) return mpm_set_error(MPM_E_INVALID, "mpm_create: too many particles");
-    mpm_context* c = new mpm_context;
-    c->D = D;
-    c->nscenes = nscenes;
-    c->sc.resize((size_t)nscenes);
-    c->table.resize((size_t)nscenes);
-    std::vector<int> slot_base((size_t)nscenes + 1);
-    int64_t sb = 0, pb = 0;
-    for (int s = 0; s < nscenes; ++s) {
-        const mpm_scene_desc& d = descs[s];
-        const int nb = slots_per_axis(D, d.counts);
-        const int ns = D == 3 ? nb * nb * nb : nb * nb;
-        c->sc[(size_t)s].desc = d;
-        c->sc[(size_t)s].capacity = capacities[s];
-        c->sc[(size_t)s].pbase = pb;
-        c->table[(size_t)s] = scene_params(d, (int)sb, ns);
-        slot_base[(size_t)s] = (int)sb;
-        sb += ns;
-        pb += capacities[s];
-    }
-    slot_base[(size_t)nscenes] = (int)sb;
-    c->nslots = (int)sb;
-    c->pcap = pcap;
-    // Block capacity: every slot of small grids, else pcap/16 blocks -- about
-    // 8x the active blocks (home blocks plus halo) of scenes packed at 8
-    // particles per cell.  Overflow makes the substep a no-op and is reported.
-    c->bcap = (int)std::min<int64_t>(c->nslots, std::max<int64_t>(4096, pcap / 16));
+// Device buffers, stream, events and launch shapes of a context whose D,
+// nscenes, pcap, nslots and bcap are set.
+int allocate_context(mpm_context* c) {
+    const int D = c->D, nscenes = c->nscenes;
     c->gcap = (int64_t)c->bcap * 64;
     c->nparts = (c->nslots + mpm::SCAN_TILE - 1) / mpm::SCAN_TILE;
     int rc = MPM_OK;
@@ -542,6 +440,7 @@ int create_batch(const mpm_scene_desc* descs, const int64_t* capacities, int nsc
     A(dalloc(c, &c->slot_count, (size_t)c->nslots));
     A(dalloc(c, &c->slot_touched, (size_t)c->nslots));
     A(dalloc(c, &c->slot_active, (size_t)c->nslots));
+    A(dalloc(c, &c->slot_scene, (size_t)c->nslots));
     A(dalloc(c, &c->block_slot, (size_t)c->bcap));
     A(dalloc(c, &c->block_scene, (size_t)c->bcap));
     A(dalloc(c, &c->block_pstart, (size_t)c->bcap + 1));
@@ -561,7 +460,6 @@ int create_batch(const mpm_scene_desc* descs, const int64_t* capacities, int nsc
     A(dalloc(c, &c->scn, (size_t)nscenes));
     A(dalloc(c, &c->sst, (size_t)nscenes));
     A(dalloc(c, &c->cg, (size_t)nscenes));
-    A(dalloc(c, &c->slot_base, (size_t)nscenes + 1));
     c->grid_particle = (int)std::min<int64_t>(std::max<int64_t>((c->pcap + mpm::WG - 1) / mpm::WG, 8), 4096);
     c->grid_block = (int)std::min<int64_t>(((int64_t)c->bcap + 7) / 8 * 8, 8192);
     // node kernels: one wave per block, up to 1024 work-groups (a multiple of 8 for the XCD ranges)
@@ -571,21 +469,8 @@ int create_batch(const mpm_scene_desc* descs, const int64_t* capacities, int nsc
     A(dalloc(c, &c->bsum, c->nred));
     A(dalloc(c, &c->bmax, c->nred));
     A(dalloc(c, &c->bsq, c->nred));
-    A(dalloc(c, &c->d_dt, (size_t)nscenes));
     A(dalloc(c, &c->bu, (size_t)2 * c->bcap));
-    if (rc == MPM_OK) {
-        if (hipMemcpy(c->scn, c->table.data(), sizeof(mpm::SceneParams) * (size_t)nscenes, hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemcpy(c->slot_base, slot_base.data(), sizeof(int) * slot_base.size(), hipMemcpyHostToDevice) != hipSuccess ||
-            hipMemset(c->keybuf[0], 0xff, (size_t)c->pcap * sizeof(int)) != hipSuccess)
-            rc = mpm_set_error(MPM_E_HIP, "scene table upload failed");
-    }
-    if (rc == MPM_OK) {
-        // the particle buffer spans every scene's range; empty entries carry key -1
-        mpm::Counters hc{};
-        hc.n_live = (int)c->pcap;
-        hc.n_prev = (int)c->pcap;
-        if (hipMemcpy(c->ctr, &hc, sizeof(hc), hipMemcpyHostToDevice) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "counter upload failed");
-    }
+    A(dalloc(c, &c->d_dt, (size_t)nscenes));
     if (rc == MPM_OK) {
         void* hp = nullptr;
         if (hipHostMalloc(&hp, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");
@@ -599,12 +484,129 @@ int create_batch(const mpm_scene_desc* descs, const int64_t* capacities, int nsc
         if (rc == MPM_OK && hipEventCreateWithFlags(&c->dt_ev, hipEventDisableTiming) != hipSuccess)
             rc = mpm_set_error(MPM_E_HIP, "hipEventCreate failed");
     }
+    if (rc == MPM_OK && hipMemset(c->keybuf[0], 0xff, (size_t)c->pcap * sizeof(int)) != hipSuccess)
+        rc = mpm_set_error(MPM_E_HIP, "key buffer initialisation failed");
+    return rc;
+}
+
+// Scene table entry of a free arena entry: no slots, no particles, inactive;
+// its CG counts as done at once (cap 0).
+mpm::SceneParams free_entry_params() {
+    mpm::SceneParams p{};
+    p.counts = 4;
+    p.nb = 1;
+    p.dx = 0.25f;
+    p.inv_dx = 4.0f;
+    return p;
+}
+
+int upload_tables(mpm_context* c, const std::vector<int>& slot_scene, int n_live) {
+    HIPCHECK(hipMemcpy(c->scn, c->table.data(), sizeof(mpm::SceneParams) * (size_t)c->nscenes, hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->slot_scene, slot_scene.data(), sizeof(int) * slot_scene.size(), hipMemcpyHostToDevice));
+    mpm::Counters hc{};
+    hc.n_live = n_live;
+    hc.n_prev = n_live;
+    HIPCHECK(hipMemcpy(c->ctr, &hc, sizeof(hc), hipMemcpyHostToDevice));
+    return MPM_OK;
+}
+
+int check_desc(const mpm_scene_desc& d, const char* who) {
+    if (d.dim != 2 && d.dim != 3) return mpm_set_error(MPM_E_INVALID, std::string(who) + ": dim must be 2 or 3");
+    if (d.counts < 4 || d.counts > 4096) return mpm_set_error(MPM_E_INVALID, std::string(who) + ": counts out of range");
+    if (d.nplanes < 0 || d.nplanes > MPM_MAX_PLANES) return mpm_set_error(MPM_E_INVALID, std::string(who) + ": too many planes");
+    return MPM_OK;
+}
+
+int slots_of_desc(const mpm_scene_desc& d) {
+    const int nb = slots_per_axis(d.dim, d.counts);
+    return d.dim == 3 ? nb * nb * nb : nb * nb;
+}
+
+int create_batch(const mpm_scene_desc* descs, const int64_t* capacities, int nscenes, mpm_handle* out) {
+    if (!descs || !capacities || !out || nscenes < 1) return mpm_set_error(MPM_E_INVALID, "mpm_batch_create: bad argument");
+    const int D = descs[0].dim;
+    int64_t pcap = 0, slots = 0;
+    for (int s = 0; s < nscenes; ++s) {
+        const mpm_scene_desc& d = descs[s];
+        int rc = check_desc(d, "mpm_create");
+        if (rc) return rc;
+        if (d.dim != D) return mpm_set_error(MPM_E_INVALID, "mpm_batch_create: scenes of one batch share dim");
+        if (capacities[s] < 1 || capacities[s] > (int64_t)1 << 30) return mpm_set_error(MPM_E_INVALID, "mpm_create: bad capacity");
+        slots += slots_of_desc(d);
+        pcap += capacities[s];
+    }
+    // keys are slot*64 + cell in 32 bits
+    if (slots * 64 > ((int64_t)1 << 31) - 1) return mpm_set_error(MPM_E_INVALID, "mpm_create: grid too large");
+    if (pcap > ((int64_t)1 << 31) - 1) return mpm_set_error(MPM_E_INVALID, "mpm_create: too many particles");
+    mpm_context* c = new mpm_context;
+    c->D = D;
+    c->nscenes = nscenes;
+    c->sc.resize((size_t)nscenes);
+    c->table.resize((size_t)nscenes);
+    std::vector<int> slot_scene((size_t)slots);
+    int64_t sb = 0, pb = 0;
+    for (int s = 0; s < nscenes; ++s) {
+        const mpm_scene_desc& d = descs[s];
+        const int ns = slots_of_desc(d);
+        c->sc[(size_t)s].desc = d;
+        c->sc[(size_t)s].capacity = capacities[s];
+        c->sc[(size_t)s].pbase = pb;
+        c->table[(size_t)s] = scene_params(d, (int)sb, ns);
+        std::fill(slot_scene.begin() + sb, slot_scene.begin() + sb + ns, s);
+        sb += ns;
+        pb += capacities[s];
+    }
+    c->nslots = (int)sb;
+    c->pcap = pcap;
+    // Block capacity: every slot of small grids, else pcap/16 blocks -- about
+    // 8x the active blocks (home blocks plus halo) of scenes packed at 8
+    // particles per cell.  Overflow makes the substep a no-op and is reported.
+    c->bcap = (int)std::min<int64_t>(c->nslots, std::max<int64_t>(4096, pcap / 16));
+    int rc = allocate_context(c);
+    // the particle buffer spans every scene's range; empty entries carry key -1
+    if (rc == MPM_OK) rc = upload_tables(c, slot_scene, (int)c->pcap);
     if (rc != MPM_OK) {
         std::string msg = g_err;
         mpm_destroy(c);
         return mpm_set_error(rc, msg);
     }
     *out = c;
+    return MPM_OK;
+}
+
+// ---- serving arena ------------------------------------------------------------
+int alloc_slots(mpm_context* c, int count) {
+    for (size_t k = 0; k < c->free_slots.size(); ++k) {
+        auto& r = c->free_slots[k];
+        if (r.second >= count) {
+            const int first = r.first;
+            r.first += count;
+            r.second -= count;
+            if (r.second == 0) c->free_slots.erase(c->free_slots.begin() + (long)k);
+            return first;
+        }
+    }
+    return -1;
+}
+
+void free_slot_range(mpm_context* c, int first, int count) {
+    auto& f = c->free_slots;
+    auto it = std::lower_bound(f.begin(), f.end(), std::make_pair(first, 0));
+    it = f.insert(it, {first, count});
+    size_t k = (size_t)(it - f.begin());
+    if (k + 1 < f.size() && f[k].first + f[k].second == f[k + 1].first) {  // merge with the next
+        f[k].second += f[k + 1].second;
+        f.erase(f.begin() + (long)k + 1);
+    }
+    if (k > 0 && f[k - 1].first + f[k - 1].second == f[k].first) {  // merge with the previous
+        f[k - 1].second += f[k].second;
+        f.erase(f.begin() + (long)k);
+    }
+}
+
+int fill_slot_scene(mpm_context* c, int first, int count, int value) {
+    std::vector<int> h((size_t)count, value);
+    HIPCHECK(hipMemcpy(c->slot_scene + first, h.data(), sizeof(int) * h.size(), hipMemcpyHostToDevice));
     return MPM_OK;
 }
 
@@ -989,6 +991,172 @@ int dense_index(const mpm_context* c, int slot, int l, int64_t& o) {
 }
 
 }  // namespace
+
+// ---- internal entry points of the request server -----------------------------
+namespace mpm_host {
+
+UnitBytes unit_bytes(int D) {
+    const int64_t nf = nfields(D), DD = (int64_t)D * D;
+    UnitBytes u;
+    // two particle buffers (fields, id, key, scene) + rank, rotation cache, operator record
+    u.particle = 2 * (nf + 3) * 4 + 4 + DD * 4 + (int64_t)mpm::op_record_floats(D) * 4;
+    // count, touched, active, owner + 64 cell counters
+    u.slot = 4 * 4 + 64 * 4;
+    // block tables, neighbours, cell ranges, tiles, 64 nodes of grid channels, partials
+    const int64_t nbr = D == 3 ? 27 : 9, tile = D == 3 ? 216 : 100;
+    u.block = 3 * 4 + nbr * 4 + 65 * 4 + (D + 1) * tile * 4 + 64 * (1 + 5 * D + 1) * 4 + 3 * 8 + 2 * 4;
+    return u;
+}
+
+int pool_create(int D, int max_scenes, int64_t pcap, int64_t nslots, int64_t bcap, int cg_max, mpm_context** out) {
+    if (D != 2 && D != 3) return mpm_set_error(MPM_E_INVALID, "arena: dim must be 2 or 3");
+    if (max_scenes < 1 || max_scenes > (1 << 20)) return mpm_set_error(MPM_E_INVALID, "arena: bad scene count");
+    if (pcap < 1 || pcap > ((int64_t)1 << 31) - 1) return mpm_set_error(MPM_E_INVALID, "arena: bad particle capacity");
+    if (nslots < 1 || nslots * 64 > ((int64_t)1 << 31) - 1) return mpm_set_error(MPM_E_INVALID, "arena: bad slot capacity");
+    if (bcap < 1 || bcap > nslots) return mpm_set_error(MPM_E_INVALID, "arena: block capacity must be in [1, slots]");
+    if (cg_max < 1 || cg_max > MAX_GRAPH_CG_ITERATIONS) return mpm_set_error(MPM_E_INVALID, "arena: CG cap must be in [1, 256]");
+    mpm_context* c = new mpm_context;
+    c->D = D;
+    c->nscenes = max_scenes;
+    c->pool = true;
+    c->pool_cg_max = cg_max;
+    c->sc.resize((size_t)max_scenes);
+    c->table.assign((size_t)max_scenes, free_entry_params());
+    c->entry_used.assign((size_t)max_scenes, 0);
+    c->pcap = pcap;
+    c->nslots = (int)nslots;
+    c->bcap = (int)bcap;
+    c->free_slots.push_back({0, (int)nslots});
+    int rc = allocate_context(c);
+    if (rc == MPM_OK) rc = upload_tables(c, std::vector<int>((size_t)nslots, -1), 0);
+    if (rc != MPM_OK) {
+        std::string msg = g_err;
+        mpm_destroy(c);
+        return mpm_set_error(rc, msg);
+    }
+    c->initialized = true;
+    *out = c;
+    return MPM_OK;
+}
+
+int slots_of(const mpm_scene_desc& d) { return slots_of_desc(d); }
+
+int pool_largest_free_slots(const mpm_context* c) {
+    int m = 0;
+    for (const auto& r : c->free_slots) m = std::max(m, r.second);
+    return m;
+}
+
+int live_particles(mpm_context* c, int64_t* n) {
+    mpm::Counters h{};
+    int rc = read_counters(c, h);
+    if (rc) return rc;
+    *n = h.n_live;
+    if (h.overflow) return mpm_set_error(MPM_E_CAPACITY, "active blocks exceed the block capacity");
+    return MPM_OK;
+}
+
+int pool_attach(mpm_context* c, const mpm_scene_desc& d, int64_t n, const float* x, const float* v,
+                const float* mass, int* entry) {
+    if (!c->pool) return mpm_set_error(MPM_E_INVALID, "arena: not an arena");
+    int rc = check_desc(d, "arena attach");
+    if (rc) return rc;
+    if (d.dim != c->D) return mpm_set_error(MPM_E_INVALID, "arena attach: dimension differs from the arena's");
+    if (d.cg_max_iterations < 1 || d.cg_max_iterations > c->pool_cg_max)
+        return mpm_set_error(MPM_E_INVALID, "arena attach: CG cap above the arena's");
+    if (n < 1 || !x || !v || !mass) return mpm_set_error(MPM_E_INVALID, "arena attach: no particles");
+    int e = -1;
+    for (int k = 0; k < c->nscenes; ++k)
+        if (!c->entry_used[(size_t)k]) { e = k; break; }
+    if (e < 0) return mpm_set_error(MPM_E_CAPACITY, "arena attach: no free scene entry");
+    int64_t n_live = 0;
+    rc = live_particles(c, &n_live);
+    if (rc) return rc;
+    if (n_live + n > c->pcap) return mpm_set_error(MPM_E_CAPACITY, "arena attach: particle space exhausted");
+    const int ns = slots_of_desc(d);
+    const int first = alloc_slots(c, ns);
+    if (first < 0) return mpm_set_error(MPM_E_CAPACITY, "arena attach: no free slot range");
+    mpm::SceneParams sp = scene_params(d, first, ns);
+    sp.waiting = 1;
+    c->table[(size_t)e] = sp;
+    SceneHost& sh = c->sc[(size_t)e];
+    sh = SceneHost{};
+    sh.desc = d;
+    sh.capacity = n;
+    sh.pbase = n_live;
+    sh.n_uploaded = n;
+    sh.uploaded = true;
+    // particles appended after the live ones in the current buffer
+    const int D = c->D, NF = nfields(D);
+    std::vector<float> h((size_t)NF * (size_t)n);
+    auto f = [&](int field, int64_t i) -> float& { return h[(size_t)field * (size_t)n + (size_t)i]; };
+    for (int64_t i = 0; i < n; ++i) {
+        for (int a = 0; a < D; ++a) {
+            f(a, i) = x[i * D + a];
+            f(D + a, i) = v[i * D + a];
+        }
+        for (int a = 0; a < D; ++a)
+            for (int b = 0; b < D; ++b) f(2 * D + a * D + b, i) = a == b ? 1.0f : 0.0f;
+        f(2 * D + D * D, i) = 1.0f;
+        f(2 * D + D * D + 1, i) = mass[i];
+        f(2 * D + D * D + 2, i) = 0.0f;
+    }
+    std::vector<int> ids((size_t)n), keys((size_t)n, 0), scene((size_t)n, e);
+    std::iota(ids.begin(), ids.end(), 0);
+    const int cur = c->cur;
+    for (int fld = 0; fld < NF; ++fld)
+        HIPCHECK(hipMemcpy(c->pbuf[cur] + (size_t)fld * c->pcap + n_live, &h[(size_t)fld * n], (size_t)n * sizeof(float),
+                           hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->idbuf[cur] + n_live, ids.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->keybuf[cur] + n_live, keys.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->scbuf[cur] + n_live, scene.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->scn + e, &sp, sizeof(sp), hipMemcpyHostToDevice));
+    mpm::SceneState st{};
+    HIPCHECK(hipMemcpy(c->sst + e, &st, sizeof(st), hipMemcpyHostToDevice));
+    rc = fill_slot_scene(c, first, ns, e);
+    if (rc) return rc;
+    const int nl = (int)(n_live + n);
+    HIPCHECK(hipMemcpy(&c->ctr->n_live, &nl, sizeof(int), hipMemcpyHostToDevice));
+    c->entry_used[(size_t)e] = 1;
+    c->binned = false;
+    // its max |V| for the first dt (the other scenes' maxima do not change)
+    rc = upload_max_speed(c);
+    if (rc) return rc;
+    *entry = e;
+    return MPM_OK;
+}
+
+int pool_detach(mpm_context* c, int e) {
+    if (!c->pool || e < 0 || e >= c->nscenes || !c->entry_used[(size_t)e])
+        return mpm_set_error(MPM_E_INVALID, "arena detach: no such scene");
+    mpm::launch_remove_scene(make_kp(c), c->stream, c->grid_particle, e);
+    int rc = check_launch("arena detach");
+    if (rc) return rc;
+    const mpm::SceneParams old = c->table[(size_t)e];
+    rc = fill_slot_scene(c, old.slot_base, old.nslots, -1);
+    if (rc) return rc;
+    free_slot_range(c, old.slot_base, old.nslots);
+    c->table[(size_t)e] = free_entry_params();
+    HIPCHECK(hipMemcpy(c->scn + e, &c->table[(size_t)e], sizeof(mpm::SceneParams), hipMemcpyHostToDevice));
+    mpm::SceneState st{};
+    HIPCHECK(hipMemcpy(c->sst + e, &st, sizeof(st), hipMemcpyHostToDevice));
+    c->sc[(size_t)e] = SceneHost{};
+    c->entry_used[(size_t)e] = 0;
+    return MPM_OK;
+}
+
+int substep(mpm_context* c, const float* dts) { return do_substep(c, dts); }
+int max_velocities(mpm_context* c, float* out) { return ::max_velocities(c, out); }
+int download(mpm_context* c, int scene, int64_t cap, float* x, float* v, float* F, float* Jp, float* mass,
+             float* volume, int32_t* ids, int64_t* n_out) {
+    return download_scene(c, scene, cap, x, v, F, Jp, mass, volume, ids, n_out);
+}
+int stats(mpm_context* c, int scene, mpm_step_stats* out) { return fill_stats(c, scene, out); }
+float next_dt(const mpm_scene_desc& d, float max_v, float time, float target, bool& last) {
+    return ::next_dt(d, max_v, time, target, last);
+}
+
+}  // namespace mpm_host
 
 extern "C" {
 

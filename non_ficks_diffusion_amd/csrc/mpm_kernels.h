@@ -70,7 +70,7 @@ struct KP {
     SceneParams* scn;
     SceneState* sst;
     CGState* cg;
-    const int* slot_base;  // nscenes + 1 entries: scene of a slot by bisection
+    const int* slot_scene;  // per global slot: the scene that owns it, -1 if none
     int nscenes;
     // particles: field-major arrays of capacity pcap
     float* pcur;
@@ -131,6 +131,8 @@ struct StepArgs {
 // launchers (host)
 void launch_set_step(const KP& p, hipStream_t s, const StepArgs& a);
 void launch_set_step_dev(const KP& p, hipStream_t s, const float* dt);  // nscenes dts in device memory
+void launch_clear_waiting(const KP& p, hipStream_t s);
+void launch_remove_scene(const KP& p, hipStream_t s, int grid, int scene);
 template <int D> void launch_bin(const KP& p, hipStream_t s, int grid);
 void launch_scan(const KP& p, hipStream_t s);
 template <int D> void launch_block_setup(const KP& p, hipStream_t s, int grid);

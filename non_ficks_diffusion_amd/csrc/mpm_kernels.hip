@@ -56,16 +56,8 @@ __device__ inline XcdRange xcd_wave_blocks(int n) {
     return {lo + k * W + (int)(threadIdx.x >> 6), hi, (int)(gridDim.x >> 3) * W};
 }
 
-// Scene of a global slot: bisection over the scenes' first slots.
-__device__ inline int scene_of_slot(const KP& p, int slot) {
-    int lo = 0, hi = p.nscenes;  // slot_base[lo] <= slot < slot_base[hi]
-    while (hi - lo > 1) {
-        const int mid = (lo + hi) >> 1;
-        if (p.slot_base[mid] <= slot) lo = mid;
-        else hi = mid;
-    }
-    return lo;
-}
+// Scene of a global slot (slot ranges of an arena are not in scene order).
+__device__ inline int scene_of_slot(const KP& p, int slot) { return p.slot_scene[slot]; }
 
 template <int D>
 __device__ inline void slot_coords(int slot, int nb, int* b) {
@@ -1848,6 +1840,19 @@ __global__ __launch_bounds__(WG) void k_set_step(KP p, StepArgs a) {
     if (a.clear_waiting) sc.waiting = 0;
 }
 
+// Process_Waiting_Particles ran for every waiting scene: clear the flags.
+__global__ __launch_bounds__(WG) void k_clear_waiting(KP p) {
+    const int i = blockIdx.x * WG + threadIdx.x;
+    if (i < p.nscenes) p.scn[i].waiting = 0;
+}
+
+// Removes every particle of scene `scene` (key -1: the next binning drops it).
+__global__ __launch_bounds__(WG) void k_remove_scene(KP p, int scene) {
+    const int n = p.ctr->n_live;
+    for (int i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG)
+        if (p.scur[i] == scene) p.keycur[i] = -1;
+}
+
 // The same from a device array of nscenes dts (replayed graphs: the host
 // copies the substep's dts into `dt` before each replay).
 __global__ __launch_bounds__(WG) void k_set_step_dev(KP p, const float* dt) {
@@ -1867,6 +1872,12 @@ void launch_set_step(const KP& p, hipStream_t s, const StepArgs& a) {
 }
 void launch_set_step_dev(const KP& p, hipStream_t s, const float* dt) {
     hipLaunchKernelGGL(k_set_step_dev, dim3((p.nscenes + WG - 1) / WG), dim3(WG), 0, s, p, dt);
+}
+void launch_clear_waiting(const KP& p, hipStream_t s) {
+    hipLaunchKernelGGL(k_clear_waiting, dim3((p.nscenes + WG - 1) / WG), dim3(WG), 0, s, p);
+}
+void launch_remove_scene(const KP& p, hipStream_t s, int grid, int scene) {
+    hipLaunchKernelGGL(k_remove_scene, dim3(grid), dim3(WG), 0, s, p, scene);
 }
 template <int D> void launch_bin(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_bin<D>, dim3(grid), dim3(WG), 0, s, p); }
 void launch_scan(const KP& p, hipStream_t s) {

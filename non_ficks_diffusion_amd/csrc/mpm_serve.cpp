@@ -1,0 +1,345 @@
+// Request server: continuous batching of snow scenes on one device arena.
+//
+// The arena is one batch context whose scene entries come and go (see
+// mpm_context.h): an admitted request attaches its scene to a free entry, a
+// contiguous range of block slots and particle space, and its particle,
+// grid and CG state stay on the device until it finishes.  One tick admits
+// queued requests in arrival order while the arena has room, then advances
+// every resident scene by one frame: batched substeps, each scene with its
+// own dt from the reference's rule (MPM_Driver::Advance_To_Target_Time,
+// MPM_Driver.cpp:55-80) toward its own frame end, scenes that reached it
+// idling until the slowest is done.  Finished requests leave the arena and
+// their space goes to the next ones.  No reduction mixes scenes, so a scene
+// evolves in the mix as it does alone.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/mpm_gpu.h"
+#include "mpm_context.h"
+#include "mpm_internal.h"
+
+namespace {
+
+struct Request {
+    mpm_scene_desc desc{};
+    int64_t n = 0;
+    std::vector<float> x, v, m;  // host copy until admitted
+    int frames_total = 0, frames_done = 0;
+    int state = MPM_REQ_QUEUED;
+    int entry = -1;
+    int slots = 0;
+    int64_t blocks = 0, bytes = 0;  // admission estimate
+    float time = 0.0f;              // simulated time
+    int substeps = 0;
+    double submitted = -1.0, admitted = -1.0, finished = -1.0;
+    // final particles (keep_results)
+    std::vector<float> rx, rv, rF, rJp, rm, rvol;
+    std::vector<int32_t> rids;
+    int64_t rn = 0;
+};
+
+}  // namespace
+
+struct mpm_server {
+    mpm_serve_config cfg{};
+    mpm_context* c = nullptr;
+    mpm_host::UnitBytes ub{};
+    int64_t pcap = 0, nslots = 0, bcap = 0, arena_bytes = 0;
+    int64_t particles_in_use = 0, blocks_in_use = 0, bytes_in_use = 0;
+    std::map<int64_t, Request> req;
+    std::deque<int64_t> queue;
+    std::vector<int64_t> by_entry;  // arena entry -> request id, -1 if free
+    int64_t next_id = 1;
+    std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+    double now() const {
+        return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+    }
+};
+
+namespace {
+
+// Blocks of storage a scene of n particles may need: home blocks plus
+// their halo at low particle density, at most every slot of its grid.
+int64_t block_estimate(int64_t n, int slots) { return std::min<int64_t>(slots, n / 8 + 64); }
+
+bool fits(const mpm_server* s, const Request& r) {
+    return s->particles_in_use + r.n <= s->pcap && s->blocks_in_use + r.blocks <= s->bcap &&
+           s->bytes_in_use + r.bytes <= s->cfg.hbm_budget_bytes && mpm_host::pool_largest_free_slots(s->c) >= r.slots &&
+           std::find(s->by_entry.begin(), s->by_entry.end(), (int64_t)-1) != s->by_entry.end();
+}
+
+int finish(mpm_server* s, int64_t id, Request& r) {
+    if (s->cfg.keep_results) {
+        const int D = s->cfg.dim;
+        const int64_t cap = r.n;
+        r.rx.resize((size_t)cap * D);
+        r.rv.resize((size_t)cap * D);
+        r.rF.resize((size_t)cap * D * D);
+        r.rJp.resize((size_t)cap);
+        r.rm.resize((size_t)cap);
+        r.rvol.resize((size_t)cap);
+        r.rids.resize((size_t)cap);
+        int rc = mpm_host::download(s->c, r.entry, cap, r.rx.data(), r.rv.data(), r.rF.data(), r.rJp.data(),
+                                    r.rm.data(), r.rvol.data(), r.rids.data(), &r.rn);
+        if (rc) return rc;
+    }
+    int rc = mpm_host::pool_detach(s->c, r.entry);
+    if (rc) return rc;
+    s->by_entry[(size_t)r.entry] = -1;
+    s->particles_in_use -= r.n;
+    s->blocks_in_use -= r.blocks;
+    s->bytes_in_use -= r.bytes;
+    r.entry = -1;
+    r.state = MPM_REQ_DONE;
+    r.finished = s->now();
+    (void)id;
+    return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int mpm_serve_create(const mpm_serve_config* cfg, mpm_server_handle* out) {
+    if (!cfg || !out) return mpm_set_error(MPM_E_INVALID, "mpm_serve_create: null argument");
+    const mpm_serve_config& g = *cfg;
+    if (g.dim != 2 && g.dim != 3) return mpm_set_error(MPM_E_INVALID, "mpm_serve_create: dim must be 2 or 3");
+    if (g.max_scenes < 1 || g.max_counts < 4 || g.max_counts > 4096 || g.hbm_budget_bytes <= 0 ||
+        g.cg_max_iterations < 1 || g.cg_max_iterations > 256)
+        return mpm_set_error(MPM_E_INVALID, "mpm_serve_create: bad configuration");
+    mpm_server* s = new mpm_server;
+    s->cfg = g;
+    s->ub = mpm_host::unit_bytes(g.dim);
+    // Arena: the slots of max_scenes largest grids (block keys are 32-bit),
+    // the rest of the budget split between particles and blocks of storage
+    // as a scene at 8 particles per cell uses them (n/8 blocks per n).
+    mpm_scene_desc big{};
+    big.dim = g.dim;
+    big.counts = g.max_counts;
+    const int64_t key_slots = (((int64_t)1 << 31) - 1) / 64;
+    s->nslots = std::min<int64_t>(key_slots, (int64_t)g.max_scenes * mpm_host::slots_of(big));
+    s->nslots = std::min<int64_t>(s->nslots, std::max<int64_t>(1, g.hbm_budget_bytes / 4 / s->ub.slot));
+    const int64_t rest = g.hbm_budget_bytes - s->nslots * s->ub.slot;
+    const double per_particle = (double)s->ub.particle + (double)s->ub.block / 8.0;
+    const int64_t np = (int64_t)((double)rest / per_particle);
+    s->pcap = std::max<int64_t>(1, std::min<int64_t>(np, ((int64_t)1 << 31) - 1));
+    // block indices * 65 stay below 2^31
+    s->bcap = std::max<int64_t>(1, std::min<int64_t>({np / 8, s->nslots, (((int64_t)1 << 31) - 1) / 65}));
+    s->arena_bytes = s->pcap * s->ub.particle + s->nslots * s->ub.slot + s->bcap * s->ub.block;
+    if (rest <= 0 || np < 1)
+        return delete s, mpm_set_error(MPM_E_INVALID, "mpm_serve_create: budget too small for the slot table");
+    int rc = mpm_host::pool_create(g.dim, g.max_scenes, s->pcap, s->nslots, s->bcap, g.cg_max_iterations, &s->c);
+    if (rc) {
+        delete s;
+        return rc;
+    }
+    if (g.graph) {
+        rc = mpm_set_graph(s->c, 1);
+        if (rc) {
+            mpm_destroy(s->c);
+            delete s;
+            return rc;
+        }
+    }
+    s->by_entry.assign((size_t)g.max_scenes, -1);
+    *out = s;
+    return MPM_OK;
+}
+
+int mpm_serve_destroy(mpm_server_handle s) {
+    if (!s) return MPM_OK;
+    mpm_destroy(s->c);
+    delete s;
+    return MPM_OK;
+}
+
+int mpm_serve_arena(mpm_server_handle s, int64_t* particles, int64_t* slots, int64_t* blocks, int64_t* bytes) {
+    if (!s) return mpm_set_error(MPM_E_INVALID, "mpm_serve_arena: null handle");
+    if (particles) *particles = s->pcap;
+    if (slots) *slots = s->nslots;
+    if (blocks) *blocks = s->bcap;
+    if (bytes) *bytes = s->arena_bytes;
+    return MPM_OK;
+}
+
+int mpm_serve_submit(mpm_server_handle s, const mpm_scene_desc* desc, int64_t n, const float* x, const float* v,
+                     const float* mass, int32_t frames, int64_t* id) {
+    if (!s || !desc || !x || !v || !mass || !id) return mpm_set_error(MPM_E_INVALID, "mpm_serve_submit: null argument");
+    if (desc->dim != s->cfg.dim) return mpm_set_error(MPM_E_INVALID, "mpm_serve_submit: dimension differs from the server's");
+    if (desc->counts < 4 || desc->counts > s->cfg.max_counts)
+        return mpm_set_error(MPM_E_INVALID, "mpm_serve_submit: grid larger than the server's max_counts");
+    if (desc->cg_max_iterations < 1 || desc->cg_max_iterations > s->cfg.cg_max_iterations)
+        return mpm_set_error(MPM_E_INVALID, "mpm_serve_submit: CG cap above the server's");
+    if (n < 1 || frames < 1) return mpm_set_error(MPM_E_INVALID, "mpm_serve_submit: no particles or frames");
+    if (!(desc->frame_rate > 0.0f)) return mpm_set_error(MPM_E_INVALID, "mpm_serve_submit: frame rate must be positive");
+    Request r;
+    r.desc = *desc;
+    r.n = n;
+    r.frames_total = frames;
+    r.slots = mpm_host::slots_of(*desc);
+    r.blocks = block_estimate(n, r.slots);
+    r.bytes = n * s->ub.particle + (int64_t)r.slots * s->ub.slot + r.blocks * s->ub.block;
+    if (r.n > s->pcap || r.blocks > s->bcap || r.slots > s->nslots || r.bytes > s->cfg.hbm_budget_bytes)
+        return mpm_set_error(MPM_E_CAPACITY, "mpm_serve_submit: the request exceeds the whole arena");
+    const int D = desc->dim;
+    r.x.assign(x, x + n * D);
+    r.v.assign(v, v + n * D);
+    r.m.assign(mass, mass + n);
+    r.submitted = s->now();
+    const int64_t k = s->next_id++;
+    s->req.emplace(k, std::move(r));
+    s->queue.push_back(k);
+    *id = k;
+    return MPM_OK;
+}
+
+int mpm_serve_tick(mpm_server_handle s, mpm_tick_stats* out) {
+    if (!s) return mpm_set_error(MPM_E_INVALID, "mpm_serve_tick: null handle");
+    const double t_start = s->now();
+    mpm_tick_stats st{};
+    // admission, in arrival order, while the arena has room
+    while (!s->queue.empty()) {
+        Request& r = s->req[s->queue.front()];
+        if (!fits(s, r)) break;
+        int e = -1;
+        int rc = mpm_host::pool_attach(s->c, r.desc, r.n, r.x.data(), r.v.data(), r.m.data(), &e);
+        if (rc == MPM_E_CAPACITY) break;  // fragmented slots or dead particles not yet compacted
+        if (rc) return rc;
+        r.entry = e;
+        r.state = MPM_REQ_RUNNING;
+        r.admitted = s->now();
+        std::vector<float>().swap(r.x);
+        std::vector<float>().swap(r.v);
+        std::vector<float>().swap(r.m);
+        s->by_entry[(size_t)e] = s->queue.front();
+        s->particles_in_use += r.n;
+        s->blocks_in_use += r.blocks;
+        s->bytes_in_use += r.bytes;
+        s->queue.pop_front();
+        ++st.admitted;
+    }
+    // one frame of every resident scene
+    const int ne = s->cfg.max_scenes;
+    std::vector<float> target((size_t)ne, 0.0f), vmax((size_t)ne), dts((size_t)ne);
+    std::vector<char> running((size_t)ne, 0), last((size_t)ne, 0);
+    for (int e = 0; e < ne; ++e) {
+        const int64_t id = s->by_entry[(size_t)e];
+        if (id < 0) continue;
+        Request& r = s->req[id];
+        target[(size_t)e] = (float)(r.frames_done + 1) / r.desc.frame_rate;
+        running[(size_t)e] = r.time < target[(size_t)e];
+        ++st.resident;
+    }
+    for (;;) {
+        int rc = mpm_host::max_velocities(s->c, vmax.data());
+        if (rc) return rc;
+        bool any = false;
+        int64_t work = 0;
+        for (int e = 0; e < ne; ++e) {
+            dts[(size_t)e] = 0.0f;
+            if (!running[(size_t)e]) continue;
+            Request& r = s->req[s->by_entry[(size_t)e]];
+            bool l = false;
+            const float dt = mpm_host::next_dt(r.desc, vmax[(size_t)e], r.time, target[(size_t)e], l);
+            if (!(dt > 0.0f)) {
+                running[(size_t)e] = 0;
+                continue;
+            }
+            dts[(size_t)e] = dt;
+            last[(size_t)e] = l ? 1 : 0;
+            work += r.n;
+            any = true;
+        }
+        if (!any) break;
+        rc = mpm_host::substep(s->c, dts.data());
+        if (rc) return rc;
+        ++st.substeps;
+        st.particle_substeps += work;
+        for (int e = 0; e < ne; ++e) {
+            if (!(dts[(size_t)e] > 0.0f)) continue;
+            Request& r = s->req[s->by_entry[(size_t)e]];
+            r.time += dts[(size_t)e];
+            ++r.substeps;
+            if (last[(size_t)e]) running[(size_t)e] = 0;
+        }
+    }
+    int64_t n_live = 0;
+    int rc = mpm_host::live_particles(s->c, &n_live);  // synchronises; reports block overflow
+    if (rc) return rc;
+    for (int e = 0; e < ne; ++e) {
+        const int64_t id = s->by_entry[(size_t)e];
+        if (id < 0) continue;
+        Request& r = s->req[id];
+        ++r.frames_done;
+        ++st.frames;
+        if (r.frames_done >= r.frames_total) {
+            rc = finish(s, id, r);
+            if (rc) return rc;
+            ++st.finished;
+        }
+    }
+    st.queued = (int32_t)s->queue.size();
+    st.bytes_in_use = s->bytes_in_use;
+    st.ms = (s->now() - t_start) * 1e3;
+    if (out) *out = st;
+    return MPM_OK;
+}
+
+int mpm_serve_poll(mpm_server_handle s, int64_t id, mpm_request_status* out) {
+    if (!s || !out) return mpm_set_error(MPM_E_INVALID, "mpm_serve_poll: null argument");
+    auto it = s->req.find(id);
+    if (it == s->req.end()) return mpm_set_error(MPM_E_INVALID, "mpm_serve_poll: unknown request");
+    const Request& r = it->second;
+    out->state = r.state;
+    out->frames_done = r.frames_done;
+    out->frames_total = r.frames_total;
+    out->substeps = r.substeps;
+    out->particles = r.n;
+    out->bytes = r.bytes;
+    out->sim_time = r.time;
+    out->submitted = r.submitted;
+    out->admitted = r.admitted;
+    out->finished = r.finished;
+    return MPM_OK;
+}
+
+int mpm_serve_download(mpm_server_handle s, int64_t id, int64_t cap, float* x, float* v, float* F, float* Jp,
+                       float* mass, float* volume, int32_t* ids, int64_t* n_out) {
+    if (!s) return mpm_set_error(MPM_E_INVALID, "mpm_serve_download: null handle");
+    auto it = s->req.find(id);
+    if (it == s->req.end()) return mpm_set_error(MPM_E_INVALID, "mpm_serve_download: unknown request");
+    Request& r = it->second;
+    if (r.state == MPM_REQ_RUNNING)
+        return mpm_host::download(s->c, r.entry, cap, x, v, F, Jp, mass, volume, ids, n_out);
+    if (r.state != MPM_REQ_DONE || !s->cfg.keep_results)
+        return mpm_set_error(MPM_E_NOT_READY, "mpm_serve_download: no particles kept for this request");
+    if (n_out) *n_out = r.rn;
+    if (r.rn > cap) return mpm_set_error(MPM_E_CAPACITY, "mpm_serve_download: cap too small");
+    const int D = s->cfg.dim;
+    const size_t n = (size_t)r.rn;
+    if (x) std::memcpy(x, r.rx.data(), n * D * sizeof(float));
+    if (v) std::memcpy(v, r.rv.data(), n * D * sizeof(float));
+    if (F) std::memcpy(F, r.rF.data(), n * D * D * sizeof(float));
+    if (Jp) std::memcpy(Jp, r.rJp.data(), n * sizeof(float));
+    if (mass) std::memcpy(mass, r.rm.data(), n * sizeof(float));
+    if (volume) std::memcpy(volume, r.rvol.data(), n * sizeof(float));
+    if (ids) std::memcpy(ids, r.rids.data(), n * sizeof(int32_t));
+    return MPM_OK;
+}
+
+int mpm_serve_release(mpm_server_handle s, int64_t id) {
+    if (!s) return mpm_set_error(MPM_E_INVALID, "mpm_serve_release: null handle");
+    auto it = s->req.find(id);
+    if (it == s->req.end()) return mpm_set_error(MPM_E_INVALID, "mpm_serve_release: unknown request");
+    if (it->second.state != MPM_REQ_DONE) return mpm_set_error(MPM_E_NOT_READY, "mpm_serve_release: request not finished");
+    s->req.erase(it);
+    return MPM_OK;
+}
+
+}  // extern "C"

@@ -33,7 +33,9 @@ def test_struct_layouts_match_header(tmp_path):
     import subprocess
     src = tmp_path / "layout.c"
     checks = {"mpm_plane": M.Plane, "mpm_material": M.Material, "mpm_scene_desc": M.SceneDesc,
-              "mpm_step_stats": M.StepStats}
+              "mpm_step_stats": M.StepStats, "mpm_launch_config": M.LaunchConfig,
+              "mpm_serve_config": M.ServeConfig, "mpm_request_status": M.RequestStatus,
+              "mpm_tick_stats": M.TickStats}
     lines = ['#include <stdio.h>', '#include <stddef.h>', '#include "mpm_gpu.h"', "int main(void){"]
     for cname, py in checks.items():
         lines.append(f'printf("{cname} %zu\\n", sizeof({cname}));')
