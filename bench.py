@@ -196,6 +196,140 @@ def bench_batch(args):
     print(json.dumps(out), flush=True)
 
 
+def pct(a, q):
+    return float(np.percentile(np.asarray(a, np.float64), q)) if len(a) else None
+
+
+def latency_summary(first, inter):
+    """Milliseconds: time to first frame (arrival to first frame, queueing
+    included) and time between later frames of a request."""
+    return {"first_frame_p50_ms": pct(first, 50), "first_frame_p99_ms": pct(first, 99),
+            "frame_p50_ms": pct(inter, 50), "frame_p99_ms": pct(inter, 99),
+            "frames": len(first) + len(inter)}
+
+
+def bench_serve(args):
+    """Serving form: a Poisson stream of snow-impact / avalanche requests,
+    served by the request server (continuous batching on one device arena)
+    and by the plain loop (one scene at a time, in arrival order), each
+    against the same arrival clock."""
+    import non_ficks_diffusion_amd as M
+    from non_ficks_diffusion_amd import workloads as W
+
+    reqs = W.serve_requests(args.requests, args.rate, frames_min=args.frames_min, frames_max=args.frames_max)
+    data = [r.spec.generate() for r in reqs]
+    check = sorted({int(np.argmin([r.spec.n for r in reqs])), int(np.argmax([r.spec.n for r in reqs]))})
+
+    def work_of(k, substeps):
+        return reqs[k].spec.n * substeps
+
+    # batched form
+    srv = M.Server(int(args.budget_gb * 1e9), max_scenes=args.max_scenes, max_counts=256, graph=args.graph,
+                   keep_results=True, max_tick_substeps=args.tick_substeps)
+    arena = srv.arena()
+    ids = [None] * len(reqs)
+    offset = [0.0] * len(reqs)  # server clock minus this clock, at each submission
+    nxt = 0
+    work = 0
+    ticks = 0
+    t0 = time.perf_counter()
+    while True:
+        now = time.perf_counter() - t0
+        while nxt < len(reqs) and reqs[nxt].arrival <= now:
+            d, x, v, m = data[nxt]
+            ids[nxt] = srv.submit(d, x, v, m, reqs[nxt].frames)
+            offset[nxt] = srv.poll(ids[nxt])["submitted"] - (time.perf_counter() - t0)
+            nxt += 1
+        if nxt == len(reqs) and all(srv.poll(i)["state"] == M.REQ_DONE for i in ids):
+            break
+        if all(srv.poll(ids[k])["state"] == M.REQ_DONE for k in range(nxt)):
+            time.sleep(max(0.0, reqs[nxt].arrival - now))  # idle until the next arrival
+            continue
+        st = srv.tick()
+        ticks += 1
+        work += st["particle_substeps"]
+    span_b = time.perf_counter() - t0 - reqs[0].arrival
+    first, inter = [], []
+    for k, i in enumerate(ids):
+        ft = srv.frame_times(i)
+        arrival_srv = reqs[k].arrival + offset[k]
+        first.append((ft[0] - arrival_srv) * 1e3)
+        inter.extend((np.diff(ft) * 1e3).tolist())
+    served = {k: srv.download(ids[k]) for k in check}
+    steps_b = [srv.poll(i)["substeps"] for i in ids]
+    srv.close()
+    lat_b = latency_summary(first, inter)
+
+    # plain form: one scene at a time, in arrival order
+    first, inter = [], []
+    work_p = 0
+    plain = {}
+    steps_p = []
+    t0 = time.perf_counter()
+    for k, r in enumerate(reqs):
+        now = time.perf_counter() - t0
+        if r.arrival > now:
+            time.sleep(r.arrival - now)
+        d, x, v, m = data[k]
+        s = M.Scene(d, len(m))
+        s.upload(x, v, m)
+        s.initialize()
+        t, n_steps = 0.0, 0
+        prev = r.arrival
+        for f in range(r.frames):
+            target = float(np.float32(f + 1) / np.float32(d.frame_rate))
+            t, n, _ = s.advance_to_time(t, target)
+            n_steps += n
+            s.synchronize()
+            t_end = time.perf_counter() - t0
+            (first if f == 0 else inter).append((t_end - prev) * 1e3)
+            prev = t_end
+        work_p += work_of(k, n_steps)
+        steps_p.append(n_steps)
+        if k in check:
+            plain[k] = s.download()
+        s.close()
+    span_p = time.perf_counter() - t0 - reqs[0].arrival
+    lat_p = latency_summary(first, inter)
+    err = {"x": 0.0, "v": 0.0, "F": 0.0}
+    for k in check:
+        for f in err:
+            err[f] = max(err[f], rel_l2(served[k][f], plain[k][f]))
+    value, value_p = work / span_b, work_p / span_p
+    ns = [r.spec.n for r in reqs]
+    out = {
+        "metric": "particle-substeps/s and p50/p99 frame latency, mixed request stream",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": int(sum(steps_b)),
+        "warmup": 0,
+        "ms_per_step": span_b * 1e3 / max(1, sum(steps_b)),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(reqs)} Poisson requests at {args.rate} Hz, snow-impact (two_balls) or "
+                               f"avalanche, N log-uniform 10k-2M, grids 64^3/128^3/256^3, "
+                               f"{args.frames_min}-{args.frames_max} frames at 24 fps",
+                   "particles_min": int(min(ns)), "particles_max": int(max(ns)),
+                   "frames_total": int(sum(r.frames for r in reqs)), "hbm_budget_gb": args.budget_gb,
+                   "arena": arena, "max_scenes": args.max_scenes, "graph": bool(args.graph),
+                   "tick_substeps": args.tick_substeps, "ticks": ticks,
+                   "precision": "FP32 state, FP64 CG reductions and SVD", "parallelism": "single GPU"},
+        "latency": lat_b,
+        "makespan_s": span_b,
+        "plain_form": {"value": value_p, "latency": lat_p, "makespan_s": span_p,
+                       "form": "one scene at a time in arrival order, a handle per request"},
+        "speedup_vs_plain": value / value_p,
+        "substeps_equal": steps_b == steps_p,
+        "served_vs_plain_rel_l2": {k: float(f"{v:.3e}") for k, v in err.items()},
+        "checked_requests": check,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -204,9 +338,18 @@ def main():
     ap.add_argument("--counts", type=int, default=COUNTS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-substeps", type=int, default=2)
-    ap.add_argument("--workload", choices=["avalanche", "batch"], default="avalanche",
+    ap.add_argument("--workload", choices=["avalanche", "batch", "serve"], default="avalanche",
                     help="avalanche: the 3D 1M-particle scene (default); batch: 256 scenes of uneven size, "
-                         "batched against one scene per handle")
+                         "batched against one scene per handle; serve: a Poisson request stream served with "
+                         "continuous batching against the one-scene-at-a-time loop")
+    ap.add_argument("--requests", type=int, default=24, help="serve: number of requests")
+    ap.add_argument("--rate", type=float, default=4.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--frames-min", type=int, default=60, help="serve: frames per request, lower bound")
+    ap.add_argument("--frames-max", type=int, default=240, help="serve: frames per request, upper bound")
+    ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget of the arena")
+    ap.add_argument("--max-scenes", type=int, default=32, help="serve: resident scenes at most")
+    ap.add_argument("--tick-substeps", type=int, default=32,
+                    help="serve: a tick also ends after this many substeps (new arrivals join at tick ends)")
     ap.add_argument("--scenes", type=int, default=256, help="batch: number of scenes")
     ap.add_argument("--graph", action="store_true", help="batch: replay a captured graph per substep")
     ap.add_argument("--autotune", action="store_true", help="batch: pick launch shapes by the autotuner first")
@@ -224,6 +367,9 @@ def main():
         return
     if args.workload == "batch":
         bench_batch(args)
+        return
+    if args.workload == "serve":
+        bench_serve(args)
         return
 
     import non_ficks_diffusion_amd as M

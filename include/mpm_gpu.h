@@ -293,10 +293,13 @@ int mpm_batch_advance_to_time(mpm_handle h, const double* time, const double* ta
  * server owns an arena sized from an HBM budget.  Requests are scenes with
  * a frame count; mpm_serve_tick admits queued requests in arrival order
  * while the arena has room (particle space, a free slot range, blocks of
- * storage by estimate, the byte budget), then advances every resident scene
- * by one frame (MPM_Driver::Advance_To_Target_Time, MPM_Driver.cpp:55-80,
- * with each scene's own dt) in batched substeps.  Finished requests leave the
- * arena at the end of the tick.  A handle is not thread-safe. */
+ * storage by estimate, the byte budget), then runs batched substeps until
+ * every scene resident at its start has completed a frame
+ * (MPM_Driver::Advance_To_Target_Time, MPM_Driver.cpp:55-80, each scene with
+ * its own dt toward its own next frame end).  Scenes with shorter frames go
+ * on with their next ones meanwhile; between substeps, finished requests
+ * leave the arena and queued ones are admitted into the room they free.  A
+ * handle is not thread-safe. */
 typedef struct mpm_server* mpm_server_handle;
 
 typedef struct mpm_serve_config {
@@ -307,6 +310,8 @@ typedef struct mpm_serve_config {
     int64_t hbm_budget_bytes;    /* the arena is sized from it; admission stays within it */
     int32_t graph;               /* replay captured substeps */
     int32_t keep_results;        /* keep finished requests' particles for mpm_serve_download */
+    int32_t max_tick_substeps;   /* > 0: a tick also ends after this many substeps */
+    int32_t pad;
 } mpm_serve_config;
 
 enum { MPM_REQ_QUEUED = 0, MPM_REQ_RUNNING = 1, MPM_REQ_DONE = 2 };
@@ -324,7 +329,7 @@ typedef struct mpm_request_status {
 typedef struct mpm_tick_stats {
     int32_t admitted, finished, resident, queued;
     int32_t substeps;            /* batched substeps of the tick */
-    int32_t frames;              /* frames completed (one per resident scene) */
+    int32_t frames;              /* frames completed */
     int64_t particle_substeps;   /* particles of the advancing scenes, summed over substeps */
     int64_t bytes_in_use;        /* admission estimates of the resident scenes */
     double ms;                   /* wall time of the tick, device work included */
@@ -345,6 +350,10 @@ int mpm_serve_poll(mpm_server_handle s, int64_t request_id, mpm_request_status* 
 int mpm_serve_download(mpm_server_handle s, int64_t request_id, int64_t cap, float* x, float* v,
                        float* F, float* Jp, float* mass, float* volume, int32_t* ids,
                        int64_t* n_out);
+/* Completion times (server clock, seconds) of a request's frames so far;
+ * *n = frames completed, at most cap times written. */
+int mpm_serve_frame_times(mpm_server_handle s, int64_t request_id, int32_t cap, double* times,
+                          int32_t* n);
 /* Forgets a finished request. */
 int mpm_serve_release(mpm_server_handle s, int64_t request_id);
 

@@ -91,7 +91,8 @@ MPM_TUNE_CACHED = 1
 class ServeConfig(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int32), ("max_scenes", ctypes.c_int32), ("max_counts", ctypes.c_int32),
                 ("cg_max_iterations", ctypes.c_int32), ("hbm_budget_bytes", ctypes.c_int64),
-                ("graph", ctypes.c_int32), ("keep_results", ctypes.c_int32)]
+                ("graph", ctypes.c_int32), ("keep_results", ctypes.c_int32),
+                ("max_tick_substeps", ctypes.c_int32), ("pad", ctypes.c_int32)]
 
 
 class RequestStatus(ctypes.Structure):
@@ -159,6 +160,8 @@ SIGNATURES = {
     "mpm_serve_download": (ctypes.c_int, [_H, ctypes.c_int64, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _IP,
                                           _LP]),
     "mpm_serve_release": (ctypes.c_int, [_H, ctypes.c_int64]),
+    "mpm_serve_frame_times": (ctypes.c_int, [_H, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
+                                             _IP]),
     "mpm_batch_create": (ctypes.c_int, [ctypes.POINTER(SceneDesc), _LP, ctypes.c_int32, ctypes.POINTER(_H)]),
     "mpm_batch_size": (ctypes.c_int32, [_H]),
     "mpm_batch_upload": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP]),
@@ -504,11 +507,12 @@ class Server:
     every resident scene by one frame (mpm_serve_*)."""
 
     def __init__(self, hbm_budget_bytes: int, max_scenes: int = 64, max_counts: int = 256, dim: int = 3,
-                 cg_max_iterations: int = 50, graph: bool = False, keep_results: bool = False):
+                 cg_max_iterations: int = 50, graph: bool = False, keep_results: bool = False,
+                 max_tick_substeps: int = 0):
         self.dim = dim
         self.keep_results = keep_results
         cfg = ServeConfig(dim, max_scenes, max_counts, cg_max_iterations, int(hbm_budget_bytes),
-                          1 if graph else 0, 1 if keep_results else 0)
+                          1 if graph else 0, 1 if keep_results else 0, int(max_tick_substeps), 0)
         h = _H()
         _check(lib().mpm_serve_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h
@@ -561,6 +565,15 @@ class Server:
                                         _fp(out["Jp"]), _fp(out["mass"]), _fp(out["volume"]),
                                         out["ids"].ctypes.data_as(_IP), ctypes.byref(n)))
         return {k: a[:n.value] for k, a in out.items()}
+
+    def frame_times(self, rid: int) -> np.ndarray:
+        """Completion time of each finished frame (server clock, seconds)."""
+        n = ctypes.c_int32()
+        _check(lib().mpm_serve_frame_times(self._h, int(rid), 0, None, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1), np.float64)
+        _check(lib().mpm_serve_frame_times(self._h, int(rid), n.value,
+                                           out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)), ctypes.byref(n)))
+        return out[:n.value]
 
     def release(self, rid: int) -> None:
         _check(lib().mpm_serve_release(self._h, int(rid)))

@@ -500,12 +500,13 @@ mpm::SceneParams free_entry_params() {
     return p;
 }
 
-int upload_tables(mpm_context* c, const std::vector<int>& slot_scene, int n_live) {
+int upload_tables(mpm_context* c, const std::vector<int>& slot_scene, int n_live, int slot_hw) {
     HIPCHECK(hipMemcpy(c->scn, c->table.data(), sizeof(mpm::SceneParams) * (size_t)c->nscenes, hipMemcpyHostToDevice));
     HIPCHECK(hipMemcpy(c->slot_scene, slot_scene.data(), sizeof(int) * slot_scene.size(), hipMemcpyHostToDevice));
     mpm::Counters hc{};
     hc.n_live = n_live;
     hc.n_prev = n_live;
+    hc.slot_hw = slot_hw;
     HIPCHECK(hipMemcpy(c->ctr, &hc, sizeof(hc), hipMemcpyHostToDevice));
     return MPM_OK;
 }
@@ -564,7 +565,7 @@ int create_batch(const mpm_scene_desc* descs, const int64_t* capacities, int nsc
     c->bcap = (int)std::min<int64_t>(c->nslots, std::max<int64_t>(4096, pcap / 16));
     int rc = allocate_context(c);
     // the particle buffer spans every scene's range; empty entries carry key -1
-    if (rc == MPM_OK) rc = upload_tables(c, slot_scene, (int)c->pcap);
+    if (rc == MPM_OK) rc = upload_tables(c, slot_scene, (int)c->pcap, c->nslots);
     if (rc != MPM_OK) {
         std::string msg = g_err;
         mpm_destroy(c);
@@ -602,6 +603,15 @@ void free_slot_range(mpm_context* c, int first, int count) {
         f[k - 1].second += f[k].second;
         f.erase(f.begin() + (long)k);
     }
+}
+
+// End of the highest allocated slot range of an arena, for the binning's scan.
+int update_slot_hw(mpm_context* c) {
+    int hw = 0;
+    for (int e = 0; e < c->nscenes; ++e)
+        if (c->entry_used[(size_t)e]) hw = std::max(hw, c->table[(size_t)e].slot_base + c->table[(size_t)e].nslots);
+    HIPCHECK(hipMemcpy(&c->ctr->slot_hw, &hw, sizeof(int), hipMemcpyHostToDevice));
+    return MPM_OK;
 }
 
 int fill_slot_scene(mpm_context* c, int first, int count, int value) {
@@ -1028,7 +1038,7 @@ int pool_create(int D, int max_scenes, int64_t pcap, int64_t nslots, int64_t bca
     c->bcap = (int)bcap;
     c->free_slots.push_back({0, (int)nslots});
     int rc = allocate_context(c);
-    if (rc == MPM_OK) rc = upload_tables(c, std::vector<int>((size_t)nslots, -1), 0);
+    if (rc == MPM_OK) rc = upload_tables(c, std::vector<int>((size_t)nslots, -1), 0, 0);
     if (rc != MPM_OK) {
         std::string msg = g_err;
         mpm_destroy(c);
@@ -1119,6 +1129,8 @@ int pool_attach(mpm_context* c, const mpm_scene_desc& d, int64_t n, const float*
     HIPCHECK(hipMemcpy(&c->ctr->n_live, &nl, sizeof(int), hipMemcpyHostToDevice));
     c->entry_used[(size_t)e] = 1;
     c->binned = false;
+    rc = update_slot_hw(c);
+    if (rc) return rc;
     // its max |V| for the first dt (the other scenes' maxima do not change)
     rc = upload_max_speed(c);
     if (rc) return rc;
@@ -1142,7 +1154,7 @@ int pool_detach(mpm_context* c, int e) {
     HIPCHECK(hipMemcpy(c->sst + e, &st, sizeof(st), hipMemcpyHostToDevice));
     c->sc[(size_t)e] = SceneHost{};
     c->entry_used[(size_t)e] = 0;
-    return MPM_OK;
+    return update_slot_hw(c);
 }
 
 int substep(mpm_context* c, const float* dts) { return do_substep(c, dts); }
@@ -1201,6 +1213,7 @@ int mpm_upload_particles(mpm_handle c, int64_t n, const float* x, const float* v
     c->cur = 0;
     HIPCHECK(hipMemset(c->keybuf[0], 0xff, (size_t)c->pcap * sizeof(int)));
     mpm::Counters hc{};
+    hc.slot_hw = c->nslots;
     hc.n_live = (int)c->pcap;
     hc.n_prev = (int)c->pcap;
     HIPCHECK(hipMemcpy(c->ctr, &hc, sizeof(hc), hipMemcpyHostToDevice));

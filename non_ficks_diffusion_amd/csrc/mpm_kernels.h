@@ -56,7 +56,8 @@ struct Counters {
     int n_prev;         // particles in the buffer before the last binning
     int overflow;       // block capacity exceeded
     int cg_ndone;       // scenes whose CG solve is finished
-    int pad[3];
+    int slot_hw;        // slots [0, slot_hw) hold every scene's range (the binning scans them)
+    int pad[2];
 };
 
 // CG scalars of one scene.  rho of iteration k lives in rho_hist[k & 1].

@@ -349,13 +349,24 @@ __device__ inline int wg_exclusive_scan(int v, int& total) {
     return before + x - v;
 }
 
+// Slots scanned: [0, slot_hw) -- an arena keeps its scenes' ranges low.
+__device__ inline int scanned_slots(const KP& p) { return min(p.nslots, p.ctr->slot_hw); }
+
 __global__ __launch_bounds__(WG) void k_scan_reduce(KP p) {
+    const int ns = scanned_slots(p);
+    if ((int64_t)blockIdx.x * SCAN_TILE >= ns) {  // beyond the scanned slots: empty part
+        if (threadIdx.x == 0) {
+            p.scan_part[blockIdx.x] = 0;
+            p.scan_part[p.nparts + blockIdx.x] = 0;
+        }
+        return;
+    }
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
     int act = 0, cnt = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_PER_THREAD; ++k) {
         const int s = base + k;
-        if (s < p.nslots) {
+        if (s < ns) {
             act += p.slot_touched[s] != 0;
             cnt += p.slot_count[s];
         }
@@ -407,14 +418,16 @@ __global__ __launch_bounds__(WG) void k_scan_top(KP p) {
 }
 
 __global__ __launch_bounds__(WG) void k_scan_apply(KP p) {
+    const int ns = scanned_slots(p);
+    if ((int64_t)blockIdx.x * SCAN_TILE >= ns) return;
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
     int act[SCAN_PER_THREAD], cnt[SCAN_PER_THREAD];
     int sa = 0, sc = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_PER_THREAD; ++k) {
         const int s = base + k;
-        act[k] = (s < p.nslots) ? (p.slot_touched[s] != 0) : 0;
-        cnt[k] = (s < p.nslots) ? p.slot_count[s] : 0;
+        act[k] = (s < ns) ? (p.slot_touched[s] != 0) : 0;
+        cnt[k] = (s < ns) ? p.slot_count[s] : 0;
         sa += act[k];
         sc += cnt[k];
     }
@@ -424,7 +437,7 @@ __global__ __launch_bounds__(WG) void k_scan_apply(KP p) {
 #pragma unroll
     for (int k = 0; k < SCAN_PER_THREAD; ++k) {
         const int s = base + k;
-        if (s >= p.nslots) break;
+        if (s >= ns) break;
         if (act[k]) {
             p.slot_active[s] = ra;
             if (ra < p.bcap) {

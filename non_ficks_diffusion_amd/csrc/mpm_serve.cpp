@@ -4,13 +4,15 @@
 // mpm_context.h): an admitted request attaches its scene to a free entry, a
 // contiguous range of block slots and particle space, and its particle,
 // grid and CG state stay on the device until it finishes.  One tick admits
-// queued requests in arrival order while the arena has room, then advances
-// every resident scene by one frame: batched substeps, each scene with its
-// own dt from the reference's rule (MPM_Driver::Advance_To_Target_Time,
-// MPM_Driver.cpp:55-80) toward its own frame end, scenes that reached it
-// idling until the slowest is done.  Finished requests leave the arena and
-// their space goes to the next ones.  No reduction mixes scenes, so a scene
-// evolves in the mix as it does alone.
+// queued requests in arrival order while the arena has room, then runs
+// batched substeps until every scene resident at its start has completed a
+// frame, each scene with its own dt from the reference's rule (MPM_Driver::Advance_To_Target_Time,
+// MPM_Driver.cpp:55-80) toward its own next frame end.  A scene whose frames
+// are short goes on with its next frame instead of waiting for the slowest
+// one, so every resident particle advances in every substep.  Between
+// substeps, finished requests leave the arena and queued ones take the room
+// they free.  No reduction mixes scenes, so a scene evolves in the mix as it
+// does alone.
 #include <algorithm>
 #include <chrono>
 #include <cmath>
@@ -38,6 +40,7 @@ struct Request {
     float time = 0.0f;              // simulated time
     int substeps = 0;
     double submitted = -1.0, admitted = -1.0, finished = -1.0;
+    std::vector<double> frame_t;    // completion time of each frame (server clock)
     // final particles (keep_results)
     std::vector<float> rx, rv, rF, rJp, rm, rvol;
     std::vector<int32_t> rids;
@@ -99,6 +102,31 @@ int finish(mpm_server* s, int64_t id, Request& r) {
     r.state = MPM_REQ_DONE;
     r.finished = s->now();
     (void)id;
+    return MPM_OK;
+}
+
+// Admits queued requests in arrival order while the arena has room.
+int admit(mpm_server* s, mpm_tick_stats& st) {
+    while (!s->queue.empty()) {
+        Request& r = s->req[s->queue.front()];
+        if (!fits(s, r)) break;
+        int e = -1;
+        int rc = mpm_host::pool_attach(s->c, r.desc, r.n, r.x.data(), r.v.data(), r.m.data(), &e);
+        if (rc == MPM_E_CAPACITY) break;  // fragmented slots or dead particles not yet compacted
+        if (rc) return rc;
+        r.entry = e;
+        r.state = MPM_REQ_RUNNING;
+        r.admitted = s->now();
+        std::vector<float>().swap(r.x);
+        std::vector<float>().swap(r.v);
+        std::vector<float>().swap(r.m);
+        s->by_entry[(size_t)e] = s->queue.front();
+        s->particles_in_use += r.n;
+        s->blocks_in_use += r.blocks;
+        s->bytes_in_use += r.bytes;
+        s->queue.pop_front();
+        ++st.admitted;
+    }
     return MPM_OK;
 }
 
@@ -203,52 +231,69 @@ int mpm_serve_tick(mpm_server_handle s, mpm_tick_stats* out) {
     if (!s) return mpm_set_error(MPM_E_INVALID, "mpm_serve_tick: null handle");
     const double t_start = s->now();
     mpm_tick_stats st{};
-    // admission, in arrival order, while the arena has room
-    while (!s->queue.empty()) {
-        Request& r = s->req[s->queue.front()];
-        if (!fits(s, r)) break;
-        int e = -1;
-        int rc = mpm_host::pool_attach(s->c, r.desc, r.n, r.x.data(), r.v.data(), r.m.data(), &e);
-        if (rc == MPM_E_CAPACITY) break;  // fragmented slots or dead particles not yet compacted
-        if (rc) return rc;
-        r.entry = e;
-        r.state = MPM_REQ_RUNNING;
-        r.admitted = s->now();
-        std::vector<float>().swap(r.x);
-        std::vector<float>().swap(r.v);
-        std::vector<float>().swap(r.m);
-        s->by_entry[(size_t)e] = s->queue.front();
-        s->particles_in_use += r.n;
-        s->blocks_in_use += r.blocks;
-        s->bytes_in_use += r.bytes;
-        s->queue.pop_front();
-        ++st.admitted;
-    }
-    // one frame of every resident scene
+    int rc = admit(s, st);
+    if (rc) return rc;
     const int ne = s->cfg.max_scenes;
-    std::vector<float> target((size_t)ne, 0.0f), vmax((size_t)ne), dts((size_t)ne);
-    std::vector<char> running((size_t)ne, 0), last((size_t)ne, 0);
+    // every scene resident at the start owes one frame to this tick
+    std::vector<char> owe((size_t)ne, 0), last((size_t)ne, 0), completed((size_t)ne, 0);
     for (int e = 0; e < ne; ++e) {
-        const int64_t id = s->by_entry[(size_t)e];
-        if (id < 0) continue;
-        Request& r = s->req[id];
-        target[(size_t)e] = (float)(r.frames_done + 1) / r.desc.frame_rate;
-        running[(size_t)e] = r.time < target[(size_t)e];
+        if (s->by_entry[(size_t)e] < 0) continue;
+        owe[(size_t)e] = 1;
         ++st.resident;
     }
+    std::vector<float> vmax((size_t)ne), dts((size_t)ne);
+    bool any_completed = false;
     for (;;) {
-        int rc = mpm_host::max_velocities(s->c, vmax.data());
+        rc = mpm_host::max_velocities(s->c, vmax.data());  // waits for the last substep
         if (rc) return rc;
+        if (any_completed) {
+            // frames ended by the last substep: time stamps, retirement, admission
+            const double t = s->now();
+            bool freed = false;
+            for (int e = 0; e < ne; ++e) {
+                if (!completed[(size_t)e]) continue;
+                completed[(size_t)e] = 0;
+                const int64_t id = s->by_entry[(size_t)e];
+                Request& r = s->req[id];
+                r.frame_t.push_back(t);
+                if (r.frames_done >= r.frames_total) {
+                    rc = finish(s, id, r);
+                    if (rc) return rc;
+                    ++st.finished;
+                    owe[(size_t)e] = 0;
+                    freed = true;
+                }
+            }
+            any_completed = false;
+            if (freed && !s->queue.empty()) {
+                const int before = st.admitted;
+                rc = admit(s, st);
+                if (rc) return rc;
+                if (st.admitted > before) {  // their max |V| for the first dt
+                    rc = mpm_host::max_velocities(s->c, vmax.data());
+                    if (rc) return rc;
+                }
+            }
+        }
+        bool owing = false;
+        for (int e = 0; e < ne; ++e) owing = owing || owe[(size_t)e];
+        if (!owing || (s->cfg.max_tick_substeps > 0 && st.substeps >= s->cfg.max_tick_substeps)) break;
         bool any = false;
         int64_t work = 0;
         for (int e = 0; e < ne; ++e) {
             dts[(size_t)e] = 0.0f;
-            if (!running[(size_t)e]) continue;
-            Request& r = s->req[s->by_entry[(size_t)e]];
+            const int64_t id = s->by_entry[(size_t)e];
+            if (id < 0) continue;
+            Request& r = s->req[id];
+            const float target = (float)(r.frames_done + 1) / r.desc.frame_rate;
             bool l = false;
-            const float dt = mpm_host::next_dt(r.desc, vmax[(size_t)e], r.time, target[(size_t)e], l);
-            if (!(dt > 0.0f)) {
-                running[(size_t)e] = 0;
+            const float dt = mpm_host::next_dt(r.desc, vmax[(size_t)e], r.time, target, l);
+            if (!(dt > 0.0f)) {  // already at the frame end
+                ++r.frames_done;
+                ++st.frames;
+                owe[(size_t)e] = 0;
+                completed[(size_t)e] = 1;
+                any_completed = true;
                 continue;
             }
             dts[(size_t)e] = dt;
@@ -256,7 +301,7 @@ int mpm_serve_tick(mpm_server_handle s, mpm_tick_stats* out) {
             work += r.n;
             any = true;
         }
-        if (!any) break;
+        if (!any) continue;
         rc = mpm_host::substep(s->c, dts.data());
         if (rc) return rc;
         ++st.substeps;
@@ -266,28 +311,32 @@ int mpm_serve_tick(mpm_server_handle s, mpm_tick_stats* out) {
             Request& r = s->req[s->by_entry[(size_t)e]];
             r.time += dts[(size_t)e];
             ++r.substeps;
-            if (last[(size_t)e]) running[(size_t)e] = 0;
+            if (last[(size_t)e]) {
+                ++r.frames_done;
+                ++st.frames;
+                owe[(size_t)e] = 0;
+                completed[(size_t)e] = 1;
+                any_completed = true;
+            }
         }
     }
     int64_t n_live = 0;
-    int rc = mpm_host::live_particles(s->c, &n_live);  // synchronises; reports block overflow
+    rc = mpm_host::live_particles(s->c, &n_live);  // reports a block-capacity overflow
     if (rc) return rc;
-    for (int e = 0; e < ne; ++e) {
-        const int64_t id = s->by_entry[(size_t)e];
-        if (id < 0) continue;
-        Request& r = s->req[id];
-        ++r.frames_done;
-        ++st.frames;
-        if (r.frames_done >= r.frames_total) {
-            rc = finish(s, id, r);
-            if (rc) return rc;
-            ++st.finished;
-        }
-    }
     st.queued = (int32_t)s->queue.size();
     st.bytes_in_use = s->bytes_in_use;
     st.ms = (s->now() - t_start) * 1e3;
     if (out) *out = st;
+    return MPM_OK;
+}
+
+int mpm_serve_frame_times(mpm_server_handle s, int64_t id, int32_t cap, double* times, int32_t* n) {
+    if (!s) return mpm_set_error(MPM_E_INVALID, "mpm_serve_frame_times: null handle");
+    auto it = s->req.find(id);
+    if (it == s->req.end()) return mpm_set_error(MPM_E_INVALID, "mpm_serve_frame_times: unknown request");
+    const std::vector<double>& t = it->second.frame_t;
+    if (n) *n = (int32_t)t.size();
+    if (times) std::copy(t.begin(), t.begin() + std::min<size_t>(t.size(), (size_t)std::max(cap, 0)), times);
     return MPM_OK;
 }
 

@@ -100,10 +100,12 @@ def test_admission_waits_for_a_free_entry_and_reuses_it():
     t = srv.tick()
     assert t["admitted"] == 1 and t["queued"] == 1 and t["resident"] == 1
     assert srv.poll(a)["state"] == M.REQ_RUNNING and srv.poll(b)["state"] == M.REQ_QUEUED
-    t = srv.tick()  # a finishes and leaves the arena
-    assert t["finished"] == 1 and srv.poll(a)["state"] == M.REQ_DONE
-    t = srv.tick()  # b takes the freed entry and slots
-    assert t["admitted"] == 1 and srv.poll(b)["state"] == M.REQ_DONE
+    t = srv.tick()  # a finishes and leaves the arena; b takes the freed entry and slots at once
+    assert t["finished"] == 1 and t["admitted"] == 1 and srv.poll(a)["state"] == M.REQ_DONE
+    assert srv.poll(b)["state"] == M.REQ_RUNNING
+    t = srv.tick()
+    assert t["finished"] == 1 and srv.poll(b)["state"] == M.REQ_DONE
+    assert len(srv.frame_times(a)) == 2 and len(srv.frame_times(b)) == 1
     alone, steps = run_alone(d1, x1, v1, m1, 1)
     got = srv.download(b)
     assert srv.poll(b)["substeps"] == steps
@@ -111,6 +113,28 @@ def test_admission_waits_for_a_free_entry_and_reuses_it():
     srv.release(a)
     with pytest.raises(M.MPMError):
         srv.poll(a)
+    srv.close()
+
+
+def test_short_frames_go_on_within_a_tick():
+    """A scene with short frames (coarse grid, large dt) is not held at the
+    slowest scene's pace: it completes several frames in one tick."""
+    data = load()
+    srv = M.Server(2 << 30, max_scenes=4, max_counts=64, keep_results=True)
+    (d0, x0, v0, m0), _ = data[1]  # avalanche 48^3: many substeps per frame
+    d1, x1, v1, m1 = M.scene_generate("two_balls", 16, 0, 1500)  # 16^3: few substeps per frame
+    slow = srv.submit(d0, x0, v0, m0, 1)
+    fast = srv.submit(d1, x1, v1, m1, 3)
+    srv.tick()
+    assert srv.poll(slow)["frames_done"] == 1
+    assert srv.poll(fast)["frames_done"] >= 2
+    t = srv.frame_times(fast)
+    assert np.all(np.diff(t) >= 0)
+    while srv.poll(fast)["state"] != M.REQ_DONE:
+        srv.tick()
+    alone, steps = run_alone(d1, x1, v1, m1, 3)
+    assert srv.poll(fast)["substeps"] == steps
+    assert rel(srv.download(fast)["x"], alone["x"]) < 1e-4
     srv.close()
 
 
