@@ -134,6 +134,33 @@ def bench_batch(args):
     live = sum(b.stats(k)["live_particles"] for k in range(len(specs)))
     value = live * args.steps / el_b
 
+    its = [b.stats(k)["cg_iterations"] for k in range(len(specs))]
+    n_arr = np.array(caps)
+    out = {
+        "metric": "particle-substeps/s, 256-scene batch",
+        "value": value,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el_b * 1e3 / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"batch of {len(specs)} 3D snow blobs/slabs, seeds 1000+s, grids 32^3-128^3, "
+                               f"N log-uniform 2k-200k",
+                   "particles": int(n_arr.sum()), "n_min": int(n_arr.min()), "n_max": int(n_arr.max()),
+                   "cg_iterations_max": int(max(its)), "cg_iterations_median": float(np.median(its)),
+                   "graph": bool(args.graph), "autotune": tune,
+                   "precision": "FP32 state, FP64 CG reductions and SVD", "parallelism": "single GPU"},
+    }
+    if args.no_plain:
+        b.close()
+        print(json.dumps(out), flush=True)
+        return
+
     # plain form: one handle per scene, each substep its own launches
     scenes = []
     for (d, x, v, m) in data:
@@ -163,36 +190,16 @@ def bench_batch(args):
         got, ref = b.download(k), scenes[k].download()
         for f in err:
             err[f] = max(err[f], rel_l2(got[f], ref[f]))
-    its = [b.stats(k)["cg_iterations"] for k in range(len(specs))]
     for s in scenes:
         s.close()
     b.close()
-    n_arr = np.array(caps)
-    out = {
-        "metric": "particle-substeps/s, 256-scene batch",
-        "value": value,
-        "unit": "particle-substeps/s",
-        "n_gpus": 1,
-        "steps": args.steps,
-        "warmup": args.warmup,
-        "ms_per_step": el_b * 1e3 / args.steps,
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"batch of {len(specs)} 3D snow blobs/slabs, seeds 1000+s, grids 32^3-128^3, "
-                               f"N log-uniform 2k-200k",
-                   "particles": int(n_arr.sum()), "n_min": int(n_arr.min()), "n_max": int(n_arr.max()),
-                   "cg_iterations_max": int(max(its)), "cg_iterations_median": float(np.median(its)),
-                   "graph": bool(args.graph), "autotune": tune,
-                   "precision": "FP32 state, FP64 CG reductions and SVD", "parallelism": "single GPU"},
+    out.update({
         "plain_form": {"value": plain, "ms_per_step": el_p * 1e3 / args.steps,
                        "form": "one handle per scene, scenes advanced one after another"},
         "speedup_vs_plain": value / plain,
         "batch_vs_plain_rel_l2": {k: float(f"{v:.3e}") for k, v in err.items()},
         "checked_scenes": check,
-    }
+    })
     print(json.dumps(out), flush=True)
 
 
@@ -232,6 +239,7 @@ def bench_serve(args):
     nxt = 0
     work = 0
     ticks = 0
+    bsteps = 0
     t0 = time.perf_counter()
     while True:
         now = time.perf_counter() - t0
@@ -247,6 +255,7 @@ def bench_serve(args):
             continue
         st = srv.tick()
         ticks += 1
+        bsteps += st["substeps"]
         work += st["particle_substeps"]
     span_b = time.perf_counter() - t0 - reqs[0].arrival
     first, inter = [], []
@@ -259,6 +268,35 @@ def bench_serve(args):
     steps_b = [srv.poll(i)["substeps"] for i in ids]
     srv.close()
     lat_b = latency_summary(first, inter)
+    ns = [r.spec.n for r in reqs]
+    out = {
+        "metric": "particle-substeps/s and p50/p99 frame latency, mixed request stream",
+        "value": work / span_b,
+        "unit": "particle-substeps/s",
+        "n_gpus": 1,
+        "steps": bsteps,
+        "warmup": 0,
+        "ms_per_step": span_b * 1e3 / max(1, bsteps),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"{len(reqs)} Poisson requests at {args.rate} Hz, snow-impact (two_balls) or "
+                               f"avalanche, N log-uniform 10k-2M, grids 64^3/128^3/256^3, "
+                               f"{args.frames_min}-{args.frames_max} frames at 24 fps",
+                   "particles_min": int(min(ns)), "particles_max": int(max(ns)),
+                   "frames_total": int(sum(r.frames for r in reqs)), "hbm_budget_gb": args.budget_gb,
+                   "arena": arena, "max_scenes": args.max_scenes, "graph": bool(args.graph),
+                   "tick_substeps": args.tick_substeps, "ticks": ticks,
+                   "request_substeps": int(sum(steps_b)),
+                   "precision": "FP32 state, FP64 CG reductions and SVD", "parallelism": "single GPU"},
+        "latency": lat_b,
+        "makespan_s": span_b,
+    }
+    if args.no_plain:
+        print(json.dumps(out), flush=True)
+        return
 
     # plain form: one scene at a time, in arrival order
     first, inter = [], []
@@ -296,37 +334,14 @@ def bench_serve(args):
         for f in err:
             err[f] = max(err[f], rel_l2(served[k][f], plain[k][f]))
     value, value_p = work / span_b, work_p / span_p
-    ns = [r.spec.n for r in reqs]
-    out = {
-        "metric": "particle-substeps/s and p50/p99 frame latency, mixed request stream",
-        "value": value,
-        "unit": "particle-substeps/s",
-        "n_gpus": 1,
-        "steps": int(sum(steps_b)),
-        "warmup": 0,
-        "ms_per_step": span_b * 1e3 / max(1, sum(steps_b)),
-        "higher_is_better": True,
-        "scaling": "weak",
-        "vs_baseline": None,
-        "dtype": "f32",
-        "data": "synthetic",
-        "config": {"workload": f"{len(reqs)} Poisson requests at {args.rate} Hz, snow-impact (two_balls) or "
-                               f"avalanche, N log-uniform 10k-2M, grids 64^3/128^3/256^3, "
-                               f"{args.frames_min}-{args.frames_max} frames at 24 fps",
-                   "particles_min": int(min(ns)), "particles_max": int(max(ns)),
-                   "frames_total": int(sum(r.frames for r in reqs)), "hbm_budget_gb": args.budget_gb,
-                   "arena": arena, "max_scenes": args.max_scenes, "graph": bool(args.graph),
-                   "tick_substeps": args.tick_substeps, "ticks": ticks,
-                   "precision": "FP32 state, FP64 CG reductions and SVD", "parallelism": "single GPU"},
-        "latency": lat_b,
-        "makespan_s": span_b,
+    out.update({
         "plain_form": {"value": value_p, "latency": lat_p, "makespan_s": span_p,
                        "form": "one scene at a time in arrival order, a handle per request"},
         "speedup_vs_plain": value / value_p,
         "substeps_equal": steps_b == steps_p,
         "served_vs_plain_rel_l2": {k: float(f"{v:.3e}") for k, v in err.items()},
         "checked_requests": check,
-    }
+    })
     print(json.dumps(out), flush=True)
 
 
@@ -353,6 +368,7 @@ def main():
     ap.add_argument("--scenes", type=int, default=256, help="batch: number of scenes")
     ap.add_argument("--graph", action="store_true", help="batch: replay a captured graph per substep")
     ap.add_argument("--autotune", action="store_true", help="batch: pick launch shapes by the autotuner first")
+    ap.add_argument("--no-plain", action="store_true", help="batch, serve: skip the plain-form comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (3 times), the reference kernel "
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
