@@ -122,8 +122,11 @@ typedef struct mpm_launch_config {
                                  updates), at most 2048 */
     int32_t particle_groups;  /* work-groups of the per-particle kernels (binning, reorder); particles
                                  per thread = live particles / (256 particle_groups) */
+    int32_t operator_mode;    /* CG operator: 0 = four waves per block, a cell's particles split over
+                                 them; 1 = one wave per block, a lane runs every particle of its cell */
+    int32_t operator_groups;  /* work-groups of the CG operator (four waves each) */
     int32_t from_cache;       /* mpm_autotune: 1 if taken from the size-bucket cache */
-    float block_ms, node_ms, particle_ms;  /* mpm_autotune: time of the chosen shape's timing sequence */
+    float block_ms, node_ms, particle_ms, operator_ms;  /* mpm_autotune: the chosen shapes' timing sequences */
 } mpm_launch_config;
 
 /* ---- library ---------------------------------------------------------- */

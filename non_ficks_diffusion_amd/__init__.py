@@ -78,8 +78,10 @@ class StepStats(ctypes.Structure):
 
 class LaunchConfig(ctypes.Structure):
     _fields_ = [("block_groups", ctypes.c_int32), ("node_groups", ctypes.c_int32),
-                ("particle_groups", ctypes.c_int32), ("from_cache", ctypes.c_int32),
-                ("block_ms", ctypes.c_float), ("node_ms", ctypes.c_float), ("particle_ms", ctypes.c_float)]
+                ("particle_groups", ctypes.c_int32), ("operator_mode", ctypes.c_int32),
+                ("operator_groups", ctypes.c_int32), ("from_cache", ctypes.c_int32),
+                ("block_ms", ctypes.c_float), ("node_ms", ctypes.c_float), ("particle_ms", ctypes.c_float),
+                ("operator_ms", ctypes.c_float)]
 
     def as_dict(self) -> Dict[str, float]:
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -254,8 +256,13 @@ class _LaunchControl:
         _check(lib().mpm_get_launch_config(self._h, ctypes.byref(g)))
         return g.as_dict()
 
-    def set_launch_config(self, block_groups: int, node_groups: int, particle_groups: int) -> None:
-        g = LaunchConfig(block_groups, node_groups, particle_groups, 0, 0.0, 0.0, 0.0)
+    def set_launch_config(self, block_groups: int, node_groups: int, particle_groups: int,
+                          operator_mode: Optional[int] = None, operator_groups: Optional[int] = None) -> None:
+        cur = self.launch_config()
+        g = LaunchConfig(block_groups, node_groups, particle_groups,
+                         cur["operator_mode"] if operator_mode is None else operator_mode,
+                         cur["operator_groups"] if operator_groups is None else operator_groups, 0,
+                         0.0, 0.0, 0.0, 0.0)
         _check(lib().mpm_set_launch_config(self._h, ctypes.byref(g)))
 
     def autotune(self, dt, cached: bool = False) -> Dict[str, float]:

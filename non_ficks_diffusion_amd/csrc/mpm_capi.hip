@@ -200,6 +200,18 @@ void set_step(mpm_context* c, const float* dts, float dt_all, bool force_active,
     }
 }
 
+// The CG operator in the handle's mode: four waves per block (a cell's
+// particles split over them) or one wave per block (a lane runs its cell).
+template <int D>
+void launch_operator_as(const mpm_context* c, const mpm::KP& p, int mode, int grid, bool restart, int k, bool one) {
+    if (mode == 1) mpm::launch_cg_force_cell<D>(p, c->stream, grid, restart, k, one);
+    else mpm::launch_cg_force<D>(p, c->stream, grid, restart, k, one);
+}
+template <int D>
+void launch_operator(const mpm_context* c, const mpm::KP& p, bool restart, int k, bool one) {
+    launch_operator_as<D>(c, p, c->op_mode, c->grid_op, restart, k, one);
+}
+
 // ---- stage sequences --------------------------------------------------------
 template <int D>
 void stage_bin(mpm_context* c) {
@@ -248,7 +260,7 @@ int stage_cg(mpm_context* c, bool do_init, bool fixed = false) {
     const int gn = c->grid_node;
     if (do_init)
         timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, gn, false, false, true, false); });
-    timed(c, KC_CG_FORCE, [&] { mpm::launch_cg_force<D>(p, c->stream, c->grid_block, true, 0, one); });
+    timed(c, KC_CG_FORCE, [&] { launch_operator<D>(c, p, true, 0, one); });
     timed(c, KC_CG_NODE, [&] {
         mpm::launch_cg_restart_node<D>(p, c->stream, gn, one);
         if (!one) mpm::launch_cg_reduce_rr(p, c->stream, 0);
@@ -256,7 +268,7 @@ int stage_cg(mpm_context* c, bool do_init, bool fixed = false) {
     const int max_it = max_cg_iterations(c);
     auto iteration = [&](int k) {
         if (one) timed(c, KC_CG_NODE, [&] { mpm::launch_cg_update_s<D>(p, c->stream, gn, k); });
-        timed(c, KC_CG_FORCE, [&] { mpm::launch_cg_force<D>(p, c->stream, c->grid_block, false, k, one); });
+        timed(c, KC_CG_FORCE, [&] { launch_operator<D>(c, p, false, k, one); });
         timed(c, KC_CG_NODE, [&] {
             mpm::launch_cg_node1<D>(p, c->stream, gn, k, one);
             if (!one) mpm::launch_cg_reduce_sq(p, c->stream, k);
@@ -460,8 +472,10 @@ int allocate_context(mpm_context* c) {
     A(dalloc(c, &c->scn, (size_t)nscenes));
     A(dalloc(c, &c->sst, (size_t)nscenes));
     A(dalloc(c, &c->cg, (size_t)nscenes));
-    c->grid_particle = (int)std::min<int64_t>(std::max<int64_t>((c->pcap + mpm::WG - 1) / mpm::WG, 8), 4096);
+    c->grid_particle = (int)std::min<int64_t>(((c->pcap + mpm::WG - 1) / mpm::WG + 7) / 8 * 8, 4096);
     c->grid_block = (int)std::min<int64_t>(((int64_t)c->bcap + 7) / 8 * 8, 8192);
+    c->grid_op = c->grid_block;
+    c->op_mode = 0;
     // node kernels: one wave per block, up to 1024 work-groups (a multiple of 8 for the XCD ranges)
     c->grid_node = (int)std::min<int64_t>(((int64_t)c->bcap / (mpm::WG / 64) + 7) / 8 * 8 + 8, 1024);
     // reduction partials: per block (batches) or per node work-group (one scene)
@@ -815,18 +829,33 @@ int round8(int64_t v) { return (int)std::max<int64_t>(8, (v + 7) / 8 * 8); }
 int check_config(const mpm_context* c, const mpm_launch_config& g) {
     auto ok = [](int v, int hi) { return v >= 8 && v <= hi && v % 8 == 0; };
     if (!ok(g.block_groups, 1 << 20) || !ok(g.node_groups, (int)std::min<size_t>(c->nred, MAX_NODE_GROUPS)) ||
-        !ok(g.particle_groups, 1 << 20))
+        !ok(g.particle_groups, 1 << 20) || !ok(g.operator_groups, 1 << 20))
         return mpm_set_error(MPM_E_INVALID, "launch config: work-group counts must be multiples of 8 in range "
                                             "(node_groups at most 2048)");
+    if (g.operator_mode != 0 && g.operator_mode != 1)
+        return mpm_set_error(MPM_E_INVALID, "launch config: operator_mode must be 0 or 1");
     return MPM_OK;
 }
 
+mpm_launch_config current_config(const mpm_context* c) {
+    mpm_launch_config g{};
+    g.block_groups = c->grid_block;
+    g.node_groups = c->grid_node;
+    g.particle_groups = c->grid_particle;
+    g.operator_mode = c->op_mode;
+    g.operator_groups = c->grid_op;
+    return g;
+}
+
 void apply_config(mpm_context* c, const mpm_launch_config& g) {
-    if (g.block_groups != c->grid_block || g.node_groups != c->grid_node || g.particle_groups != c->grid_particle)
+    if (g.block_groups != c->grid_block || g.node_groups != c->grid_node || g.particle_groups != c->grid_particle ||
+        g.operator_mode != c->op_mode || g.operator_groups != c->grid_op)
         invalidate_graphs(c);
     c->grid_block = g.block_groups;
     c->grid_node = g.node_groups;
     c->grid_particle = g.particle_groups;
+    c->op_mode = g.operator_mode;
+    c->grid_op = g.operator_groups;
 }
 
 // Re-opens every scene's CG solve so that operator launches do work.
@@ -893,7 +922,7 @@ int autotune(mpm_context* c, const float* dts, mpm_launch_config& best) {
     if (rc) return rc;
     if (h.overflow) return mpm_set_error(MPM_E_CAPACITY, "mpm_autotune: active blocks exceed the block capacity");
     const bool one = c->nscenes == 1;
-    mpm_launch_config cur{c->grid_block, c->grid_node, c->grid_particle, 0, 0.0f, 0.0f, 0.0f};
+    mpm_launch_config cur = current_config(c);
     best = cur;
     // a grid and CG set-up on the binned particles
     {
@@ -902,24 +931,46 @@ int autotune(mpm_context* c, const float* dts, mpm_launch_config& best) {
         mpm::launch_normalize<D>(p, c->stream, c->grid_node);
         mpm::launch_particle_force<D>(p, c->stream, c->grid_block, false, true);
         mpm::launch_grid_update<D>(p, c->stream, c->grid_node, true, true, true, false);
-        mpm::launch_cg_force<D>(p, c->stream, c->grid_block, true, 0, one);
+        launch_operator<D>(c, p, true, 0, one);
         mpm::launch_cg_restart_node<D>(p, c->stream, c->grid_node, one);
     }
     auto no_prep = [] { return MPM_OK; };
     auto reopen = [&] { return reopen_cg(c); };
-    // per-block kernels
+    // the CG operator: four waves per block, or one wave per block with a
+    // lane's cell accumulated in registers
     std::vector<int> cand;
     for (int g : {1024, 2048, 4096, 8192, 16384, 32768})
         if (g <= round8(h.nblocks) || g == 1024) cand.push_back(g);
     if (std::find(cand.begin(), cand.end(), cur.block_groups) == cand.end()) cand.push_back(cur.block_groups);
+    std::vector<std::pair<int, int>> ops;
+    for (int g : cand) ops.push_back({0, g});
+    for (int g : {256, 512, 1024, 2048, 4096})
+        if (g <= round8(h.nblocks / 4 + 8) || g == 256) ops.push_back({1, g});
+    if (std::find(ops.begin(), ops.end(), std::make_pair(cur.operator_mode, cur.operator_groups)) == ops.end())
+        ops.push_back({cur.operator_mode, cur.operator_groups});
     float best_ms = FLT_MAX;
-    for (int g : cand) {
+    for (const auto& og : ops) {
         const mpm::KP p = make_kp(c);
         float ms = 0.0f;
         rc = time_median(c, reopen, [&] {
+            for (int k = 0; k < 8; ++k) launch_operator_as<D>(c, p, og.first, og.second, false, 1, one);
+        }, ms);
+        if (rc) return rc;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best.operator_mode = og.first;
+            best.operator_groups = og.second;
+            best.operator_ms = ms;
+        }
+    }
+    // per-block kernels
+    best_ms = FLT_MAX;
+    for (int g : cand) {
+        const mpm::KP p = make_kp(c);
+        float ms = 0.0f;
+        rc = time_median(c, no_prep, [&] {
             mpm::launch_p2g<D>(p, c->stream, g);
             mpm::launch_particle_force<D>(p, c->stream, g, false, true);
-            for (int k = 0; k < 8; ++k) mpm::launch_cg_force<D>(p, c->stream, g, false, 1, one);
         }, ms);
         if (rc) return rc;
         if (ms < best_ms) { best_ms = ms; best.block_groups = g; best.block_ms = ms; }
@@ -1467,8 +1518,8 @@ int mpm_time_cg_operator(mpm_handle c, int32_t reps, float* avg_ms) {
     mpm::KP p = make_kp(c);
     HIPCHECK(hipEventRecord(e0, c->stream));
     for (int r = 0; r < reps; ++r) {
-        if (c->D == 3) mpm::launch_cg_force<3>(p, c->stream, c->grid_block, false, 1, c->nscenes == 1);
-        else mpm::launch_cg_force<2>(p, c->stream, c->grid_block, false, 1, c->nscenes == 1);
+        if (c->D == 3) launch_operator<3>(c, p, false, 1, c->nscenes == 1);
+        else launch_operator<2>(c, p, false, 1, c->nscenes == 1);
     }
     HIPCHECK(hipEventRecord(e1, c->stream));
     HIPCHECK(hipEventSynchronize(e1));
@@ -1486,7 +1537,7 @@ int mpm_time_cg_operator(mpm_handle c, int32_t reps, float* avg_ms) {
 
 int mpm_get_launch_config(mpm_handle c, mpm_launch_config* cfg) {
     if (!c || !cfg) return mpm_set_error(MPM_E_INVALID, "mpm_get_launch_config: null argument");
-    *cfg = mpm_launch_config{c->grid_block, c->grid_node, c->grid_particle, 0, 0.0f, 0.0f, 0.0f};
+    *cfg = current_config(c);
     return MPM_OK;
 }
 

@@ -59,6 +59,7 @@ struct mpm_context {
     int* h_done = nullptr;  // pinned
     hipEvent_t done_ev[2] = {nullptr, nullptr};
     int grid_particle = 0, grid_block = 0, grid_node = 0;
+    int op_mode = 0, grid_op = 0;  // CG operator: 0 four waves per block, 1 one wave per block
     size_t nred = 0;           // entries of the reduction partial arrays
     // captured substeps: one graph per particle-buffer parity
     bool graph = false;

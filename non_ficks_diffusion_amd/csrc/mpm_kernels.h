@@ -145,6 +145,8 @@ template <int D> void launch_init_volume(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_particle_force(const KP& p, hipStream_t s, int grid, bool constitutive, bool force);
 template <int D> void launch_grid_update(const KP& p, hipStream_t s, int grid, bool force, bool collision, bool cg_init, bool write_f);
 template <int D> void launch_cg_force(const KP& p, hipStream_t s, int grid, bool restart, int k, bool one);
+// the same with one wave per block and a lane's cell accumulated in registers
+template <int D> void launch_cg_force_cell(const KP& p, hipStream_t s, int grid, bool restart, int k, bool one);
 // `one`: a batch of one scene (per-work-group partials, see k_cg_update_s)
 template <int D> void launch_cg_restart_node(const KP& p, hipStream_t s, int grid, bool one);
 template <int D> void launch_cg_update_s(const KP& p, hipStream_t s, int grid, int k);

@@ -1326,6 +1326,178 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p, int k) {
     }
 }
 
+// The operator's scatter for one particle, accumulated in registers over the
+// particles of a cell (they share the cell's stencil): acc[node * D + cc],
+// node = (i * 3 + j) * 3 + k.
+template <int D>
+__device__ inline void grad_accumulate(const float (*w)[3], const float (*dw)[3], const float* A, float* acc) {
+    if constexpr (D == 3) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            float x0[3], x1[3], x2[3];
+#pragma unroll
+            for (int cc = 0; cc < 3; ++cc) {
+                x0[cc] = A[0 * 3 + cc] * dw[0][i];
+                x1[cc] = A[1 * 3 + cc] * w[0][i];
+                x2[cc] = A[2 * 3 + cc] * w[0][i];
+            }
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                float ya[3], yb[3];
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc) {
+                    ya[cc] = fmaf(w[1][j], x0[cc], dw[1][j] * x1[cc]);
+                    yb[cc] = w[1][j] * x2[cc];
+                }
+#pragma unroll
+                for (int k = 0; k < 3; ++k)
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) {
+                        float& a = acc[((i * 3 + j) * 3 + k) * 3 + cc];
+                        a = fmaf(w[2][k], ya[cc], fmaf(dw[2][k], yb[cc], a));
+                    }
+            }
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            float x0[2], x1[2];
+#pragma unroll
+            for (int cc = 0; cc < 2; ++cc) {
+                x0[cc] = A[0 * 2 + cc] * dw[0][i];
+                x1[cc] = A[1 * 2 + cc] * w[0][i];
+            }
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc) {
+                    float& a = acc[(i * 3 + j) * 2 + cc];
+                    a = fmaf(w[1][j], x0[cc], fmaf(dw[1][j], x1[cc], a));
+                }
+        }
+    }
+}
+
+// Adds a lane's accumulated stencil to the wave's LDS tile: one ordered
+// read-add-write per stencil node (lanes hold distinct cells).
+template <int D>
+__device__ inline void flush_accumulated(int base, const float* acc, float* q) {
+    constexpr int LX = Dims<D>::LX, LY = Dims<D>::LY, TN = Dims<D>::LTN;
+    if constexpr (D == 3) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j)
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    const int t = base + i * LX + j * LY + k;
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) q[cc * TN + t] += acc[((i * 3 + j) * 3 + k) * 3 + cc];
+                    MPM_STEP_FENCE();
+                }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3; ++i)
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                const int t = base + i * LX + j;
+#pragma unroll
+                for (int cc = 0; cc < 2; ++cc) q[cc * TN + t] += acc[(i * 3 + j) * 2 + cc];
+                MPM_STEP_FENCE();
+            }
+    }
+}
+
+// The same operator with one wave per block: lane c runs every particle of
+// cell c and accumulates the cell's scatter in registers, so a block costs
+// one read-add-write chain per cell instead of one per particle.  The waves
+// of a work-group take different blocks and share nothing (no barriers).
+template <int D, bool RESTART, bool ONE>
+__global__ __launch_bounds__(WG) void k_cg_force_cell(KP p, int k) {
+    using Dm = Dims<D>;
+    constexpr int LTN = Dm::LTN, NBR = Dm::NBR, DD = D * D, TN = Dm::TN, W = WG / 64;
+    __shared__ float ut_all[W][D * LTN];
+    __shared__ float qt_all[W][D * LTN];
+    __shared__ int nbr_all[W][NBR];
+    if (cg_all_done(p)) return;
+    const int c = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    float* ut = ut_all[wv];
+    float* qt = qt_all[wv];
+    int* nbr = nbr_all[wv];
+    int l[D];
+    local_coords<D>(c, l);
+    int base;
+    {
+        int off0[D];
+        base = tile_index<D>(l, 0, off0);
+    }
+    const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
+    for (int b = r.first; b < r.end; b += r.step) {
+        const int si = ONE ? 0 : p.block_scene[b];
+        const SceneParams& sc = p.scn[si];
+        if (!sc.active || (!ONE && p.cg[si].done)) continue;
+        const float beta = (RESTART || ONE) ? 0.0f : (float)p.cg[si].beta;
+        if (c < NBR) nbr[c] = p.block_nbr[b * NBR + c];
+        int bb[D];
+        slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
+        for (int t = c; t < D * LTN; t += 64) qt[t] = 0.0f;
+        MPM_STEP_FENCE();
+        for (int t = c; t < TN; t += 64) {
+            int nb, local, tc[D];
+            tile_map<D>(t, nb, local, tc);
+            const int blk = nbr[nb];
+            const int li = lds_index<D>(tc);
+            const int64_t n = (int64_t)blk * 64 + local;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                float u = 0.0f;
+                if (blk >= 0)
+                    u = RESTART ? p.g_vs[a * p.gcap + n]
+                        : ONE   ? p.g_s[a * p.gcap + n]
+                                : cg_direction(p.g_r[a * p.gcap + n], p.g_s[a * p.gcap + n], beta, k);
+                ut[a * LTN + li] = u;
+            }
+        }
+        MPM_STEP_FENCE();
+        float acc[D == 3 ? 81 : 18];
+#pragma unroll
+        for (int t = 0; t < (D == 3 ? 81 : 18); ++t) acc[t] = 0.0f;
+        const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
+        const float (*utile)[LTN] = reinterpret_cast<const float (*)[LTN]>(ut);
+        for (int i = i0; i < i1; ++i) {
+            const float* rec = p.pop + (int64_t)i * OpRec<D>::W;
+            float w[D][3], dw[D][3];
+            bool same = true;
+            {
+                const float4 x4 = *reinterpret_cast<const float4*>(rec);
+                const float X[3] = {x4.x, x4.y, x4.z};
+#pragma unroll
+                for (int a = 0; a < D; ++a) {
+                    int cl;
+                    bspline_weights(X[a], sc.dx, sc.inv_dx, cl, w[a], dw[a]);
+                    same = same && cl + Dm::B - Dm::B * bb[a] == l[a];
+                }
+            }
+            if (!same) continue;
+            float T[DD];
+            grad_gather<D>(w, dw, base, utile, T);
+            float A[DD];
+            apply_operator<D>(rec, T, A);
+            grad_accumulate<D>(w, dw, A, acc);
+        }
+        flush_accumulated<D>(base, acc, qt);
+        MPM_STEP_FENCE();
+        float* dst = p.tiles + (int64_t)b * D * TN;
+        for (int t = c; t < D * TN; t += 64) {
+            const int cc = t / TN;
+            int nb, local, tc[D];
+            tile_map<D>(t - cc * TN, nb, local, tc);
+            dst[t] = qt[cc * LTN + lds_index<D>(tc)];
+        }
+        MPM_STEP_FENCE();
+    }
+}
+
 // Project (Project_Helper.h:30-45): zero at nodes inside a plane
 template <int D>
 __device__ inline bool project_zero(const KP& p, const SceneParams& sc, int n) {
@@ -1924,6 +2096,11 @@ template <int D> void launch_cg_force(const KP& p, hipStream_t s, int grid, bool
     else if (one) hipLaunchKernelGGL((k_cg_force<D, false, true>), dim3(grid), dim3(BWG), 0, s, p, k);
     else hipLaunchKernelGGL((k_cg_force<D, false, false>), dim3(grid), dim3(BWG), 0, s, p, k);
 }
+template <int D> void launch_cg_force_cell(const KP& p, hipStream_t s, int grid, bool restart, int k, bool one) {
+    if (restart) hipLaunchKernelGGL((k_cg_force_cell<D, true, false>), dim3(grid), dim3(WG), 0, s, p, 0);
+    else if (one) hipLaunchKernelGGL((k_cg_force_cell<D, false, true>), dim3(grid), dim3(WG), 0, s, p, k);
+    else hipLaunchKernelGGL((k_cg_force_cell<D, false, false>), dim3(grid), dim3(WG), 0, s, p, k);
+}
 template <int D> void launch_cg_restart_node(const KP& p, hipStream_t s, int grid, bool one) {
     if (one) hipLaunchKernelGGL((k_cg_restart_node<D, true>), dim3(grid), dim3(WG), 0, s, p);
     else hipLaunchKernelGGL((k_cg_restart_node<D, false>), dim3(grid), dim3(WG), 0, s, p);
@@ -1957,6 +2134,7 @@ template <int D> void launch_max_speed(const KP& p, hipStream_t s, int grid) { h
     template void launch_particle_force<D>(const KP&, hipStream_t, int, bool, bool);          \
     template void launch_grid_update<D>(const KP&, hipStream_t, int, bool, bool, bool, bool); \
     template void launch_cg_force<D>(const KP&, hipStream_t, int, bool, int, bool);           \
+    template void launch_cg_force_cell<D>(const KP&, hipStream_t, int, bool, int, bool);      \
     template void launch_cg_restart_node<D>(const KP&, hipStream_t, int, bool);               \
     template void launch_cg_update_s<D>(const KP&, hipStream_t, int, int);                    \
     template void launch_cg_node1<D>(const KP&, hipStream_t, int, int, bool);                 \

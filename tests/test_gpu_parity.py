@@ -6,6 +6,8 @@ P2G conserves grid mass and momentum to 1e-6 relative.  The oracle runs in
 the same precision mode as the GPU path (FP32 state, FP64 CG reductions and
 FP64 SVD); the error against the FP64 oracle is reported too.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -174,6 +176,34 @@ def test_one_substep(name, counts, dt):
     for k in ("x", "v", "F", "Jp"):
         e = rel(g[k], r[k])
         print(f"  {k}: {e:.2e}")
+        assert e < 1e-5, (k, e)
+
+
+def test_one_substep_full_avalanche_against_fp64_oracle():
+    """The 3D sparse avalanche at full size (256^3, 1,006,609 particles), one
+    substep at its first CFL dt, against the FP64 oracle (double state, FP64
+    SVD and CG): the per-field error of the FP32 GPU path."""
+    d, x, v, m = M.scene_generate("avalanche", 256, 0)
+    s = M.Scene(d, len(m))
+    s.upload(x, v, m)
+    s.initialize()
+    dt = float(np.float32(d.max_dt))
+    st = s.substep(dt, stats=True)
+    g = s.download()
+    s.close()
+    threads = min(16, len(os.sched_getaffinity(0)))
+    o = O.Oracle(d, 8, threads, O.DOT_FP64 | O.SVD_FP64)
+    o.set_particles(x, v, m)
+    o.initialize()
+    o.advance_step(dt)
+    r = o.particles()
+    oc = o.cg_stats()
+    assert np.array_equal(g["ids"], r["ids"])
+    print("avalanche 256^3: gpu cg", st["cg_iterations"], f"{st['cg_residual']:.2e}", "fp64 oracle cg",
+          oc["cg_iterations"], f"{oc['cg_residual']:.2e}")
+    for k in ("x", "v", "F", "Jp"):
+        e = rel(g[k], r[k])
+        print(f"  {k}: relative L2 against FP64 {e:.2e}")
         assert e < 1e-5, (k, e)
 
 

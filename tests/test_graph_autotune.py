@@ -113,10 +113,11 @@ def test_autotune_keeps_particles_and_results():
         b.close()
 
 
-def test_launch_config_shapes_do_not_change_results():
+@pytest.mark.parametrize("operator_mode", [0, 1])
+def test_launch_config_shapes_do_not_change_results(operator_mode):
     data = load()
     a, b = make_batch(data), make_batch(data)
-    b.set_launch_config(64, 128, 16)
+    b.set_launch_config(64, 128, 16, operator_mode=operator_mode, operator_groups=40)
     b.use_graph(True)
     for _ in range(3):
         a.substep(1e-3)
@@ -127,5 +128,27 @@ def test_launch_config_shapes_do_not_change_results():
         b.set_launch_config(100, 128, 16)  # not a multiple of 8
     with pytest.raises(M.MPMError):
         b.set_launch_config(64, 4096, 16)  # node work-groups above 2048
+    with pytest.raises(M.MPMError):
+        b.set_launch_config(64, 128, 16, operator_mode=2)
     a.close()
     b.close()
+
+
+def test_single_scene_operator_modes_agree():
+    """One wave per block (a lane runs its whole cell, scatter accumulated in
+    registers) against four waves per block, on the single-scene path."""
+    d, x, v, m = M.scene_generate("avalanche", 64, 0)
+    runs = []
+    for mode in (0, 1):
+        s = M.Scene(d, len(m))
+        s.upload(x, v, m)
+        s.initialize()
+        cfg = s.launch_config()
+        s.set_launch_config(cfg["block_groups"], cfg["node_groups"], cfg["particle_groups"], operator_mode=mode,
+                            operator_groups=cfg["operator_groups"] if mode == 0 else 256)
+        for _ in range(3):
+            s.substep(1e-3)
+        runs.append(s)
+    assert_same(runs[1].download(), runs[0].download(), "modes")
+    for s in runs:
+        s.close()
