@@ -210,6 +210,13 @@ enum mpm_channel {
 int mpm_read_grid(mpm_handle h, int32_t channel, float* out, int64_t out_len);
 /* Dense node flags over the same index range. */
 int mpm_read_flags(mpm_handle h, uint32_t* out, int64_t out_len);
+/* Sparse flags for the viewer's hierarchy files (MPM_Example::
+ * Write_Output_Files, MPM_Example.cpp:1061-1062, read back by
+ * MPM_Renderable.h:125-168): for every active block its first node index
+ * (dim ints, node index as in mpm_read_flags) and its flags, 4^3 (2D: 8^2)
+ * per block with the last axis fastest.  Pointers may be NULL. */
+int mpm_read_block_flags(mpm_handle h, int64_t cap_blocks, int32_t* block_origin, uint32_t* flags,
+                         int32_t* nblocks);
 /* Binning result: active block coordinates (dim ints each, block index =
  * (node index + B) / B with B = 4 in 3D, 8 in 2D), particles homed in each
  * block, and for every live particle its home block's ordinal and its id.

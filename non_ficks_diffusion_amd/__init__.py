@@ -141,6 +141,7 @@ SIGNATURES = {
     "mpm_run_stage": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_float]),
     "mpm_read_grid": (ctypes.c_int, [_H, ctypes.c_int32, _FP, ctypes.c_int64]),
     "mpm_read_flags": (ctypes.c_int, [_H, _UP, ctypes.c_int64]),
+    "mpm_read_block_flags": (ctypes.c_int, [_H, ctypes.c_int64, _IP, _UP, _IP]),
     "mpm_read_bins": (ctypes.c_int, [_H, ctypes.c_int64, _IP, _IP, ctypes.c_int64, _IP, _IP, _IP, _LP]),
     "mpm_cg_stats": (ctypes.c_int, [_H, ctypes.POINTER(StepStats)]),
     "mpm_enable_timing": (ctypes.c_int, [_H, ctypes.c_int32]),
@@ -384,6 +385,16 @@ class Scene(_LaunchControl):
         out = np.zeros(self._dense_nodes(), np.uint32)
         _check(lib().mpm_read_flags(self._h, out.ctypes.data_as(_UP), out.size))
         return out.reshape((self.counts + 2,) * self.dim)
+
+    def read_block_flags(self):
+        """(origins[nblocks, dim], flags[nblocks, 64]) of the active blocks."""
+        n = ctypes.c_int32()
+        _check(lib().mpm_read_block_flags(self._h, 0, None, None, ctypes.byref(n)))
+        origin = np.zeros((max(n.value, 1), self.dim), np.int32)
+        flags = np.zeros((max(n.value, 1), 64), np.uint32)
+        _check(lib().mpm_read_block_flags(self._h, n.value, origin.ctypes.data_as(_IP),
+                                          flags.ctypes.data_as(ctypes.POINTER(ctypes.c_uint32)), ctypes.byref(n)))
+        return origin[:n.value], flags[:n.value]
 
     def read_bins(self):
         nb = ctypes.c_int32()

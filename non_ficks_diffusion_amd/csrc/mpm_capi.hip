@@ -1426,6 +1426,32 @@ int mpm_read_flags(mpm_handle c, uint32_t* out, int64_t out_len) {
     return MPM_OK;
 }
 
+int mpm_read_block_flags(mpm_handle c, int64_t cap_blocks, int32_t* block_origin, uint32_t* flags,
+                         int32_t* nblocks) {
+    if (!c) return mpm_set_error(MPM_E_INVALID, "mpm_read_block_flags: null handle");
+    mpm::SceneState st{};
+    int rc = read_scene(c, 0, st, nullptr);
+    if (rc) return rc;
+    const int D = c->D, B = block_edge(D), nb = c->table[0].nb;
+    const int b0 = st.block_begin, nbk = std::max(st.block_end - st.block_begin, 0);
+    if (nblocks) *nblocks = nbk;
+    if (!block_origin && !flags) return MPM_OK;
+    if (cap_blocks < nbk) return mpm_set_error(MPM_E_CAPACITY, "mpm_read_block_flags: cap_blocks too small");
+    if (nbk == 0) return MPM_OK;
+    std::vector<int> slots((size_t)nbk);
+    HIPCHECK(hipMemcpy(slots.data(), c->block_slot + b0, (size_t)nbk * sizeof(int), hipMemcpyDeviceToHost));
+    if (flags)
+        HIPCHECK(hipMemcpy(flags, c->g_flags + (size_t)b0 * 64, (size_t)nbk * 64 * sizeof(uint32_t), hipMemcpyDeviceToHost));
+    if (block_origin)
+        for (int b = 0; b < nbk; ++b) {
+            int s = slots[(size_t)b] - c->table[0].slot_base;
+            int bb[3];
+            for (int a = D - 1; a >= 0; --a) { bb[a] = s % nb; s /= nb; }
+            for (int a = 0; a < D; ++a) block_origin[(int64_t)b * D + a] = B * bb[a] - B;
+        }
+    return MPM_OK;
+}
+
 int mpm_read_bins(mpm_handle c, int64_t cap_blocks, int32_t* block_coords, int32_t* block_particle_counts,
                   int64_t cap_particles, int32_t* particle_block, int32_t* particle_ids, int32_t* nblocks,
                   int64_t* nparticles) {

@@ -232,6 +232,57 @@ class MPM_Example {
             g.write((const char*)p.Fe, sizeof(T) * d * d);
             g.write((const char*)&p.Jp, sizeof(T));
         }
+        Write_Hierarchy(frame);
+    }
+    // Grid hierarchy for the viewer (MPM_Example.cpp:1061-1062, read back by
+    // plugin_Non_Ficks_Diffusion/MPM_Renderable.h:68-78 and 125-168): one
+    // level of 4^3 (2D: 8^2) blocks of the last binning.
+    //   common/hierarchy.struct3d|2d: int levels, unsigned elements per block,
+    //     unsigned block size per axis;
+    //   common/fine_grid: cell counts (int per axis), domain min and max (T per
+    //     axis);
+    //   <frame>/levels: the level count as text;
+    //   <frame>/block_offsets: per level an unsigned block count, then each
+    //     block's first cell index (int per axis, cells counted from 1);
+    //   <frame>/flags: per block its unsigned flags, last axis fastest.
+    void Write_Hierarchy(int frame) {
+        const std::string common = output_directory + "/common";
+        Make_Directory(common);
+        {
+            std::ofstream f(common + (d == 3 ? "/hierarchy.struct3d" : "/hierarchy.struct2d"), std::ios::binary);
+            const int levels = 1;
+            const unsigned B = d == 3 ? 4u : 8u, elements = 64u;  // 4^3 or 8^2
+            f.write((const char*)&levels, sizeof(levels));
+            f.write((const char*)&elements, sizeof(elements));
+            for (int a = 0; a < d; ++a) f.write((const char*)&B, sizeof(B));
+        }
+        {
+            std::ofstream f(common + "/fine_grid", std::ios::binary);
+            for (int a = 0; a < d; ++a) f.write((const char*)&counts, sizeof(counts));
+            const T lo = 0, hi = 1;
+            for (int a = 0; a < d; ++a) f.write((const char*)&lo, sizeof(lo));
+            for (int a = 0; a < d; ++a) f.write((const char*)&hi, sizeof(hi));
+        }
+        const std::string dir = output_directory + "/" + std::to_string(frame);
+        {
+            std::ofstream f(dir + "/levels");
+            f << 1;
+        }
+        int32_t nblocks = 0;
+        check(mpm_read_block_flags(handle, 0, nullptr, nullptr, &nblocks), "mpm_read_block_flags");
+        std::vector<int32_t> origin((size_t)std::max(nblocks, 1) * d);
+        std::vector<uint32_t> flags((size_t)std::max(nblocks, 1) * 64);
+        check(mpm_read_block_flags(handle, nblocks, origin.data(), flags.data(), &nblocks), "mpm_read_block_flags");
+        std::ofstream fo(dir + "/block_offsets", std::ios::binary);
+        const unsigned nb = (unsigned)nblocks;
+        fo.write((const char*)&nb, sizeof(nb));
+        for (int b = 0; b < nblocks; ++b)
+            for (int a = 0; a < d; ++a) {
+                const int32_t index = origin[(size_t)b * d + a] + 1;
+                fo.write((const char*)&index, sizeof(index));
+            }
+        std::ofstream ff(dir + "/flags", std::ios::binary);
+        ff.write((const char*)flags.data(), (std::streamsize)((size_t)nblocks * 64 * sizeof(uint32_t)));
     }
     // Read_Output_Files (MPM_Example.cpp:1075-1079); restores F and Jp too when
     // the state file exists.

@@ -106,3 +106,51 @@ def test_driver_restart_is_lossless(tmp_path):
     assert again["x"].shape == full["x"].shape
     for k in ("x", "v", "F", "Jp"):
         assert _rel(again[k], full[k]) < 1e-4, k
+
+
+@pytest.mark.gpu
+@needs_driver
+def test_driver_writes_viewer_hierarchy(tmp_path):
+    """Per frame, the viewer's hierarchy files (MPM_Example.cpp:1061-1062,
+    MPM_Renderable.h:68-78, 125-168): one level of 4^3 blocks, each block's
+    first cell index and its 64 flags, consistent with the dense flags of
+    the same state."""
+    out = tmp_path / "run"
+    r = run(["-d", 3, "-test_number", 100, "-size", 32, "-last_frame", 1, "-o", out])
+    assert r.returncode == 0, r.stderr
+    head = np.frombuffer(open(out / "common" / "hierarchy.struct3d", "rb").read(), np.int32)
+    assert list(head) == [1, 64, 4, 4, 4]
+    grid = open(out / "common" / "fine_grid", "rb").read()
+    assert list(np.frombuffer(grid[:12], np.int32)) == [32, 32, 32]
+    assert list(np.frombuffer(grid[12:], np.float32)) == [0, 0, 0, 1, 1, 1]
+    for frame in ("0", "1"):
+        assert open(out / frame / "levels").read().strip() == "1"
+        bo = open(out / frame / "block_offsets", "rb").read()
+        nb = int(np.frombuffer(bo[:4], np.uint32)[0])
+        idx = np.frombuffer(bo[4:], np.int32).reshape(nb, 3)
+        fl = np.frombuffer(open(out / frame / "flags", "rb").read(), np.uint32).reshape(nb, 64)
+        assert nb > 0 and (fl & 1).any()
+        assert idx.min() >= 1 - 4 and idx.max() <= 32 + 1 and np.all((idx - 1) % 4 == 0)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name,counts", [("two_balls", 32), ("disk2d", 64)])
+def test_block_flags_match_dense_flags(name, counts):
+    d, x, v, m = M.scene_generate(name, counts, 0)
+    s = M.Scene(d, len(m))
+    s.upload(x, v, m)
+    s.initialize()
+    s.substep(1e-3)
+    origin, flags = s.read_block_flags()
+    dense = s.read_flags()
+    B = 4 if d.dim == 3 else 8
+    rebuilt = np.zeros_like(dense)
+    for b in range(len(origin)):
+        for l in range(64):
+            lc = [l >> 4, (l >> 2) & 3, l & 3] if d.dim == 3 else [l >> 3, l & 7]
+            node = [origin[b][a] + lc[a] for a in range(d.dim)]
+            if all(-1 <= i <= counts for i in node):
+                rebuilt[tuple(i + 1 for i in node)] = flags[b, l]
+    assert np.array_equal(rebuilt, dense)
+    assert np.all(origin % B == 0)
+    s.close()
