@@ -127,6 +127,7 @@ mpm::KP make_kp(const mpm_context* c) {
     p.block_scene = c->block_scene;
     p.block_pstart = c->block_pstart;
     p.block_nbr = c->block_nbr;
+    p.block_tnbr = c->block_tnbr;
     p.cell_count = c->cell_count;
     p.cell_start = c->cell_start;
     p.bcap = c->bcap;
@@ -457,6 +458,7 @@ int allocate_context(mpm_context* c) {
     A(dalloc(c, &c->block_scene, (size_t)c->bcap));
     A(dalloc(c, &c->block_pstart, (size_t)c->bcap + 1));
     A(dalloc(c, &c->block_nbr, (size_t)c->bcap * (D == 3 ? 27 : 9)));
+    A(dalloc(c, &c->block_tnbr, (size_t)c->bcap * (D == 3 ? 27 : 9)));
     A(dalloc(c, &c->cell_count, (size_t)c->nslots * 64));
     A(dalloc(c, &c->cell_start, (size_t)c->bcap * 65 + 1));
     A(dalloc(c, &c->tiles, (size_t)c->bcap * (D + 1) * (D == 3 ? 216 : 100)));
@@ -1065,7 +1067,7 @@ UnitBytes unit_bytes(int D) {
     u.slot = 4 * 4 + 64 * 4;
     // block tables, neighbours, cell ranges, tiles, 64 nodes of grid channels, partials
     const int64_t nbr = D == 3 ? 27 : 9, tile = D == 3 ? 216 : 100;
-    u.block = 3 * 4 + nbr * 4 + 65 * 4 + (D + 1) * tile * 4 + 64 * (1 + 5 * D + 1) * 4 + 3 * 8 + 2 * 4;
+    u.block = 3 * 4 + 2 * nbr * 4 + 65 * 4 + (D + 1) * tile * 4 + 64 * (1 + 5 * D + 1) * 4 + 3 * 8 + 2 * 4;
     return u;
 }
 

@@ -56,6 +56,9 @@ __device__ inline XcdRange xcd_wave_blocks(int n) {
     return {lo + k * W + (int)(threadIdx.x >> 6), hi, (int)(gridDim.x >> 3) * W};
 }
 
+// Active block b holds no particles (a halo block of its neighbours).
+__device__ inline bool block_empty(const KP& p, int b) { return p.block_pstart[b + 1] == p.block_pstart[b]; }
+
 // Scene of a global slot (slot ranges of an arena are not in scene order).
 __device__ inline int scene_of_slot(const KP& p, int slot) { return p.slot_scene[slot]; }
 
@@ -168,7 +171,8 @@ __device__ inline bool particle_weights(const KP& p, const SceneParams& sc, int 
 }
 
 // Sum, for node n, of channel c over every tile covering it, in a fixed
-// order.  With crop_ghost_x, nodes whose x index is outside [0, counts-1]
+// order.  Blocks without particles write no tile (their contribution is
+// zero) and are absent from the tile neighbour table.  With crop_ghost_x, nodes whose x index is outside [0, counts-1]
 // receive nothing (the reference's slab crop,
 // Tools/Cropped_Influence_Iterator.h:31-42).
 template <int D, int C>
@@ -196,7 +200,7 @@ __device__ inline void gather_node_tiles(const KP& p, const SceneParams& sc, int
         else if (lc[a] == B - 1) { od[a][1] = 2; td[a][1] = 0; nopt[a] = 2; }
         else { od[a][1] = 1; td[a][1] = lc[a] + 1; }
     }
-    const int* nbr = p.block_nbr + (int64_t)b * NBR;
+    const int* nbr = p.block_tnbr + (int64_t)b * NBR;
     if constexpr (D == 3) {
         for (int i = 0; i < nopt[0]; ++i)
             for (int j = 0; j < nopt[1]; ++j)
@@ -482,7 +486,10 @@ __global__ __launch_bounds__(WG) void k_block_setup(KP p) {
                 in = in && c >= 0 && c < sc.nb;
                 s = s * sc.nb + c;
             }
-            p.block_nbr[b * Dm::NBR + lane] = in ? p.slot_active[sc.slot_base + s] : -1;
+            const int act = in ? p.slot_active[sc.slot_base + s] : -1;
+            p.block_nbr[b * Dm::NBR + lane] = act;
+            // tiles exist only for blocks that hold particles
+            p.block_tnbr[b * Dm::NBR + lane] = (act >= 0 && p.block_pstart[act + 1] > p.block_pstart[act]) ? act : -1;
         }
         const int cnt = p.cell_count[slot * 64 + lane];
         int x = cnt;
@@ -935,7 +942,7 @@ __global__ __launch_bounds__(BWG) void k_p2g(KP p) {
     const XcdRange xr = xcd_blocks(nblocks);
     for (int b = xr.first; b < xr.end; b += xr.step) {
         const SceneParams& sc = p.scn[p.block_scene[b]];
-        if (!sc.active) continue;
+        if (!sc.active || block_empty(p, b)) continue;
         for (int t = threadIdx.x; t < NW * C * LTN; t += BWG) tile[t] = 0.0f;
         int bb[D];
         slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
@@ -1012,7 +1019,7 @@ __global__ __launch_bounds__(BWG) void k_init_volume(KP p) {
     const XcdRange xr = xcd_blocks(nblocks);
     for (int b = xr.first; b < xr.end; b += xr.step) {
         const SceneParams& sc = p.scn[p.block_scene[b]];
-        if (!sc.waiting) continue;
+        if (!sc.waiting || block_empty(p, b)) continue;
         float vol = 1.0f;
         for (int a = 0; a < D; ++a) vol *= sc.dx;
         const float inv_vol = 1.0f / vol;
@@ -1059,7 +1066,7 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
     const XcdRange xr = xcd_blocks(nblocks);
     for (int b = xr.first; b < xr.end; b += xr.step) {
         const SceneParams& sc = p.scn[p.block_scene[b]];
-        if (!sc.active) continue;
+        if (!sc.active || block_empty(p, b)) continue;
         int bb[D];
         slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
         if (FORCE) {
@@ -1274,7 +1281,7 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p, int k) {
         // one scene: no per-block lookups (its solve is live: cg_all_done above)
         const int si = ONE ? 0 : p.block_scene[b];
         const SceneParams& sc = p.scn[si];
-        if (!sc.active || (!ONE && p.cg[si].done)) continue;
+        if (!sc.active || (!ONE && p.cg[si].done) || block_empty(p, b)) continue;
         const float beta = (RESTART || ONE) ? 0.0f : (float)p.cg[si].beta;
         if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
@@ -1435,7 +1442,7 @@ __global__ __launch_bounds__(WG) void k_cg_force_cell(KP p, int k) {
     for (int b = r.first; b < r.end; b += r.step) {
         const int si = ONE ? 0 : p.block_scene[b];
         const SceneParams& sc = p.scn[si];
-        if (!sc.active || (!ONE && p.cg[si].done)) continue;
+        if (!sc.active || (!ONE && p.cg[si].done) || block_empty(p, b)) continue;
         const float beta = (RESTART || ONE) ? 0.0f : (float)p.cg[si].beta;
         if (c < NBR) nbr[c] = p.block_nbr[b * NBR + c];
         int bb[D];
@@ -1831,6 +1838,13 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
     for (int b = xr.first; b < xr.end; b += xr.step) {
         const SceneParams& sc = p.scn[p.block_scene[b]];
         if (!sc.active) continue;
+        if (block_empty(p, b)) {  // no particles: empty partials for the scene's reduction
+            if (threadIdx.x == 0) {
+                p.bu[b] = 0u;
+                p.bu[p.bcap + b] = 0u;
+            }
+            continue;
+        }
         const float dt = sc.dt, flip = sc.flip;
         const float lo = 0.0f + 0.5f * sc.dx, hi = 1.0f - 0.5f * sc.dx;
         unsigned best = 0u;
