@@ -230,7 +230,8 @@ __device__ inline void svd2_fp64(const double* F, double* U, double* sig, double
 // Fe = U Sigma V^T, and return R = U V^T.
 // ---------------------------------------------------------------------------
 template <int D>
-__device__ inline void plasticity(float* Fe, float& Jp, float* R, const Material& mat, float* Uout = nullptr) {
+__device__ inline void plasticity(float* Fe, float& Jp, float* R, const Material& mat, float* Uout = nullptr,
+                                  double* Sout = nullptr) {
     constexpr int DD = D * D;
     double F[DD], U[DD], V[DD], s[D];
 #pragma unroll
@@ -269,6 +270,10 @@ __device__ inline void plasticity(float* Fe, float& Jp, float* R, const Material
     if (Uout) {
 #pragma unroll
         for (int i = 0; i < DD; ++i) Uout[i] = (float)U[i];
+    }
+    if (Sout) {  // singular values of the (clamped) Fe
+#pragma unroll
+        for (int i = 0; i < D; ++i) Sout[i] = s[i];
     }
 }
 

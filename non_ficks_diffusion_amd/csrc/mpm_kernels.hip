@@ -702,14 +702,22 @@ __device__ inline void rotate_out(const float* U, const float* Ah, float* out) {
         for (int j = 0; j < 3; ++j) out[i * 3 + j] = t[i * 3 + 0] * U[j * 3 + 0] + t[i * 3 + 1] * U[j * 3 + 1] + t[i * 3 + 2] * U[j * 3 + 2];
 }
 
-// The record of particle i.  3D: in the frame of U (F = U S V^T) the map
-// T^ = U^T T U -> A^ = U^T A U of an isotropic energy decouples into a
-// 3x3 block on the diagonal entries and one 2x2 block per off-diagonal
-// pair (a,b),(b,a); the blocks are read off the reference's force map
-// applied to the rotated unit inputs u_a u_b^T and symmetrised.
+// The record of particle i.  3D: in the frame of U (F = U S V^T, R = U V^T)
+// the reference's force map T -> A = V0 F dP(T^T F)^T of fixed-corotated
+// snow (MPM_CG_System.h:176-210, dP of MPM_Constitutive_Model.cpp) decouples:
+// with T^ = U^T T U, dF^ = T^T S, the rotation term W^_ij = (dF^_ij -
+// dF^_ji) / (s_i + s_j) and the derivative of cof(S), A^ = U^T A U = V0 S dP^T
+// is a 3x3 block on the diagonal entries,
+//   V0 (2 mu s_a^2 [c == a] + lambda J^2 + lambda J (J - 1) [c != a]),
+// plus per off-diagonal pair (lo, hi), with s = s_lo + s_hi and k the third
+// index, the symmetric block
+//   [[2 mu V0 s_lo^2 (s - 1) / s,  V0 s_lo s_hi (2 mu / s - lambda (J - 1) s_k)],
+//    [            ...,             2 mu V0 s_hi^2 (s - 1) / s]].
+// The entries are formed in FP64 from the SVD's singular values and stored
+// in FP32.
 template <int D>
 __device__ inline void store_operator(const KP& p, int i, const StressState<D>& st, float V0, const float* X,
-                                      const float* U) {
+                                      const float* U, const double* S) {
     constexpr int DD = D * D;
     float rec[OpRec<D>::W];
 #pragma unroll
@@ -718,38 +726,23 @@ __device__ inline void store_operator(const KP& p, int i, const StressState<D>& 
     for (int a = 0; a < D; ++a) rec[a] = X[a];
     if constexpr (D == 3) {
         quat_of(U, rec + 3);
-        float Ur[9];
-        rot_of(rec + 3, Ur);  // the rotation the kernels will rebuild
-        float M[3][3], Pr[3][2][2];
-#pragma unroll
-        for (int a = 0; a < 3; ++a)
-#pragma unroll
-            for (int b = 0; b < 3; ++b) {
-                float T[9], A[9], Ah[9];
-#pragma unroll
-                for (int r = 0; r < 3; ++r)
-#pragma unroll
-                    for (int cc = 0; cc < 3; ++cc) T[r * 3 + cc] = Ur[r * 3 + a] * Ur[cc * 3 + b];
-                force_map<3>(st, V0, T, A);
-                rotate_in(Ur, A, Ah);
-                if (a == b) {
-#pragma unroll
-                    for (int c = 0; c < 3; ++c) M[c][a] = Ah[c * 3 + c];
-                } else {
-                    const int pr = (a + b) - 1;         // (0,1)->0, (0,2)->1, (1,2)->2
-                    const int lo = a < b ? a : b, hi = a < b ? b : a;
-                    const int col = a < b ? 0 : 1;      // input (T^_lo,hi, T^_hi,lo)
-                    Pr[pr][0][col] = Ah[lo * 3 + hi];
-                    Pr[pr][1][col] = Ah[hi * 3 + lo];
-                }
-            }
-        rec[7] = M[0][0]; rec[8] = 0.5f * (M[0][1] + M[1][0]); rec[9] = 0.5f * (M[0][2] + M[2][0]);
-        rec[10] = M[1][1]; rec[11] = 0.5f * (M[1][2] + M[2][1]); rec[12] = M[2][2];
+        const double v0 = (double)V0, mu = (double)st.mu, lam = (double)st.lambda;
+        const double J = S[0] * S[1] * S[2];
+        const double same = lam * J * J, cross = lam * J * (J - 1.0) + same;
+        rec[7] = (float)(v0 * (2.0 * mu * S[0] * S[0] + same));
+        rec[8] = (float)(v0 * cross);
+        rec[9] = (float)(v0 * cross);
+        rec[10] = (float)(v0 * (2.0 * mu * S[1] * S[1] + same));
+        rec[11] = (float)(v0 * cross);
+        rec[12] = (float)(v0 * (2.0 * mu * S[2] * S[2] + same));
+        const int lo[3] = {0, 0, 1}, hi[3] = {1, 2, 2}, third[3] = {2, 1, 0};
 #pragma unroll
         for (int pr = 0; pr < 3; ++pr) {
-            rec[13 + 3 * pr] = Pr[pr][0][0];
-            rec[14 + 3 * pr] = 0.5f * (Pr[pr][0][1] + Pr[pr][1][0]);
-            rec[15 + 3 * pr] = Pr[pr][1][1];
+            const double sl = S[lo[pr]], sh = S[hi[pr]], sm = sl + sh;
+            const double g = 2.0 * mu * v0 * (sm - 1.0) / sm;
+            rec[13 + 3 * pr] = (float)(g * sl * sl);
+            rec[14 + 3 * pr] = (float)(v0 * sl * sh * (2.0 * mu / sm - lam * (J - 1.0) * S[third[pr]]));
+            rec[15 + 3 * pr] = (float)(g * sh * sh);
         }
     } else {
         float L[DD][DD];
@@ -1081,7 +1074,8 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
             float Jp = p.pcur[PF<D>::J * p.pcap + i];
             if (CONST) {
                 float U[DD];
-                plasticity<D>(F, Jp, R, sc.mat, U);
+                double Sv[D];
+                plasticity<D>(F, Jp, R, sc.mat, U, Sv);
 #pragma unroll
                 for (int k = 0; k < DD; ++k) {
                     p.pcur[(PF<D>::F + k) * p.pcap + i] = F[k];
@@ -1093,7 +1087,7 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
                 float Xo[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) Xo[a] = p.pcur[(PF<D>::X + a) * p.pcap + i];
-                store_operator<D>(p, i, so, p.pcur[PF<D>::VOL * p.pcap + i], Xo, U);
+                store_operator<D>(p, i, so, p.pcur[PF<D>::VOL * p.pcap + i], Xo, U, Sv);
             } else {
 #pragma unroll
                 for (int k = 0; k < DD; ++k) R[k] = p.rcache[k * p.pcap + i];

@@ -217,6 +217,60 @@ def test_force_map_blocks_in_svd_frame():
         assert np.abs(Lbad - L).max() > 1e-6 * scale
 
 
+def closed_form_blocks(S, lam, mu, V0, drop_J_term=False):
+    """The force map's U-frame blocks from the singular values alone: with
+    dF^ = T^T S, W^_ij = (dF^_ij - dF^_ji) / (s_i + s_j) and the derivative of
+    cof(S), an input e_a e_b^T maps to A^ = V0 S dP^T with
+      diagonal block  V0 (2 mu s_a^2 [c == a] + lam J^2 + lam J (J - 1) [c != a]),
+      pair (lo, hi)   [[2 mu V0 s_lo^2 (s - 1) / s, V0 s_lo s_hi (2 mu / s - lam (J - 1) s_k)],
+                       [   (same),                   2 mu V0 s_hi^2 (s - 1) / s]],
+    s = s_lo + s_hi, k the third index.  Returns the 9x9 map on row-major
+    3x3 inputs in the U frame."""
+    J = S[0] * S[1] * S[2]
+    Lh = np.zeros((9, 9))
+    for a in range(3):
+        for c in range(3):
+            Lh[4 * c, 4 * a] = V0 * (2 * mu * S[a] ** 2 * (c == a) + lam * J * J + lam * J * (J - 1) * (c != a))
+    for lo, hi, k in [(0, 1, 2), (0, 2, 1), (1, 2, 0)]:
+        sm = S[lo] + S[hi]
+        off = V0 * S[lo] * S[hi] * (2 * mu / sm - (0.0 if drop_J_term else lam * (J - 1) * S[k]))
+        i_lh, i_hl = 3 * lo + hi, 3 * hi + lo
+        Lh[i_lh, i_lh] = 2 * mu * V0 * S[lo] ** 2 * (sm - 1) / sm
+        Lh[i_hl, i_hl] = 2 * mu * V0 * S[hi] ** 2 * (sm - 1) / sm
+        Lh[i_lh, i_hl] = Lh[i_hl, i_lh] = off
+    return Lh
+
+
+def test_force_map_blocks_closed_form():
+    """The operator record's blocks in closed form (what the GPU stores) equal
+    the oracle's force map taken through its dP in the U frame, for snow
+    (plastically clamped), elastic, and inverted states; dropping the
+    lambda (J - 1) cofactor term, a plausible slip, is caught."""
+    for plastic in (1, 0):
+        mat = snow(plastic=plastic)
+        lam, mu = mat.lambda0, mat.mu0
+        for k in range(20):
+            F = np.eye(3) + 0.3 * RNG.standard_normal((3, 3))
+            if k % 5 == 0:
+                F[:, 0] *= -1.0
+            Fo, _, R, _, _ = O.constitutive(mat, F, 1.0)
+            U, S, V = O.svd(Fo)
+            V0 = 0.7
+
+            def fmap(T):
+                return V0 * Fo @ O.dP(Fo, R, lam, mu, T.T @ Fo).T
+
+            Lh = np.zeros((9, 9))
+            for j in range(9):
+                E = np.zeros(9)
+                E[j] = 1.0
+                Lh[:, j] = (U.T @ fmap(U @ E.reshape(3, 3) @ U.T) @ U).ravel()
+            scale = np.abs(Lh).max()
+            assert np.abs(closed_form_blocks(S, lam, mu, V0) - Lh).max() < 1e-10 * scale, (plastic, k)
+            if abs(np.prod(S) - 1.0) > 1e-3:
+                assert np.abs(closed_form_blocks(S, lam, mu, V0, drop_J_term=True) - Lh).max() > 1e-8 * scale
+
+
 # --------------------------------------------------------------------------- scenes used below
 def tiny_scene(dim=3, n=6, counts=16, planes=0, seed=0, v_scale=0.0, F_scale=0.0, E_scale=1.0):
     d, _ = M.scene_preset("avalanche" if dim == 3 else "disk2d", counts, 0)
