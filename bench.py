@@ -232,7 +232,7 @@ def bench_serve(args):
 
     # batched form
     srv = M.Server(int(args.budget_gb * 1e9), max_scenes=args.max_scenes, max_counts=256, graph=args.graph,
-                   keep_results=True, max_tick_substeps=args.tick_substeps)
+                   keep_results=True, max_tick_substeps=args.tick_substeps, tune=args.autotune)
     arena = srv.arena()
     ids = [None] * len(reqs)
     offset = [0.0] * len(reqs)  # server clock minus this clock, at each submission
@@ -288,7 +288,7 @@ def bench_serve(args):
                    "particles_min": int(min(ns)), "particles_max": int(max(ns)),
                    "frames_total": int(sum(r.frames for r in reqs)), "hbm_budget_gb": args.budget_gb,
                    "arena": arena, "max_scenes": args.max_scenes, "graph": bool(args.graph),
-                   "tick_substeps": args.tick_substeps, "ticks": ticks,
+                   "autotune": bool(args.autotune), "tick_substeps": args.tick_substeps, "ticks": ticks,
                    "request_substeps": int(sum(steps_b)),
                    "precision": "FP32 state, FP64 CG reductions and SVD", "parallelism": "single GPU"},
         "latency": lat_b,
@@ -367,7 +367,8 @@ def main():
                     help="serve: a tick also ends after this many substeps (new arrivals join at tick ends)")
     ap.add_argument("--scenes", type=int, default=256, help="batch: number of scenes")
     ap.add_argument("--graph", action="store_true", help="batch: replay a captured graph per substep")
-    ap.add_argument("--autotune", action="store_true", help="batch: pick launch shapes by the autotuner first")
+    ap.add_argument("--autotune", action="store_true",
+                    help="batch: pick launch shapes by the autotuner first; serve: re-tune as the resident set grows")
     ap.add_argument("--no-plain", action="store_true", help="batch, serve: skip the plain-form comparison")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 1 GiB device-to-device copy (3 times), the reference kernel "

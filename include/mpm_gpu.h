@@ -321,7 +321,8 @@ typedef struct mpm_serve_config {
     int32_t graph;               /* replay captured substeps */
     int32_t keep_results;        /* keep finished requests' particles for mpm_serve_download */
     int32_t max_tick_substeps;   /* > 0: a tick also ends after this many substeps */
-    int32_t pad;
+    int32_t tune;                /* run mpm_autotune on the resident scenes whenever their particle
+                                    count has doubled since the last tune (from 1M particles on) */
 } mpm_serve_config;
 
 enum { MPM_REQ_QUEUED = 0, MPM_REQ_RUNNING = 1, MPM_REQ_DONE = 2 };

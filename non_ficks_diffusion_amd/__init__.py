@@ -94,7 +94,7 @@ class ServeConfig(ctypes.Structure):
     _fields_ = [("dim", ctypes.c_int32), ("max_scenes", ctypes.c_int32), ("max_counts", ctypes.c_int32),
                 ("cg_max_iterations", ctypes.c_int32), ("hbm_budget_bytes", ctypes.c_int64),
                 ("graph", ctypes.c_int32), ("keep_results", ctypes.c_int32),
-                ("max_tick_substeps", ctypes.c_int32), ("pad", ctypes.c_int32)]
+                ("max_tick_substeps", ctypes.c_int32), ("tune", ctypes.c_int32)]
 
 
 class RequestStatus(ctypes.Structure):
@@ -526,11 +526,11 @@ class Server:
 
     def __init__(self, hbm_budget_bytes: int, max_scenes: int = 64, max_counts: int = 256, dim: int = 3,
                  cg_max_iterations: int = 50, graph: bool = False, keep_results: bool = False,
-                 max_tick_substeps: int = 0):
+                 max_tick_substeps: int = 0, tune: bool = False):
         self.dim = dim
         self.keep_results = keep_results
         cfg = ServeConfig(dim, max_scenes, max_counts, cg_max_iterations, int(hbm_budget_bytes),
-                          1 if graph else 0, 1 if keep_results else 0, int(max_tick_substeps), 0)
+                          1 if graph else 0, 1 if keep_results else 0, int(max_tick_substeps), 1 if tune else 0)
         h = _H()
         _check(lib().mpm_serve_create(ctypes.byref(cfg), ctypes.byref(h)))
         self._h = h

@@ -55,6 +55,7 @@ struct mpm_server {
     mpm_host::UnitBytes ub{};
     int64_t pcap = 0, nslots = 0, bcap = 0, arena_bytes = 0;
     int64_t particles_in_use = 0, blocks_in_use = 0, bytes_in_use = 0;
+    int64_t tuned_at = 0;  // resident particles at the last launch-shape tune
     std::map<int64_t, Request> req;
     std::deque<int64_t> queue;
     std::vector<int64_t> by_entry;  // arena entry -> request id, -1 if free
@@ -302,6 +303,11 @@ int mpm_serve_tick(mpm_server_handle s, mpm_tick_stats* out) {
             any = true;
         }
         if (!any) continue;
+        if (s->cfg.tune && s->particles_in_use >= std::max<int64_t>(1000000, 2 * s->tuned_at)) {
+            rc = mpm_autotune(s->c, dts.data(), 0, nullptr);  // times shapes; advances nothing
+            if (rc) return rc;
+            s->tuned_at = s->particles_in_use;
+        }
         rc = mpm_host::substep(s->c, dts.data());
         if (rc) return rc;
         ++st.substeps;
