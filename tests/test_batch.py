@@ -94,3 +94,23 @@ def test_batch_advance_to_time_matches_alone():
         for f in ("x", "v", "F"):
             assert rel(got[f], alone[f]) < 1e-4, (k, f)
         s.close()
+
+
+def test_batch_2d_matches_scenes_alone():
+    """2D scenes (8^2 blocks) through the same batched kernels."""
+    scenes = [("disk2d", 64, 0, 0), ("disk2d", 48, 3, 800), ("ref2d_test18", 64, 0, 0)]
+    data = [M.scene_generate(n, c, s, k) for n, c, s, k in scenes]
+    b = M.Batch([d for d, _, _, _ in data], [len(m) for _, _, _, m in data])
+    for k, (d, x, v, m) in enumerate(data):
+        b.upload(k, x, v, m)
+    b.initialize()
+    dts = [1e-3, 1e-3, 2e-3]
+    for _ in range(4):
+        b.substep(dts)
+    for k, (d, x, v, m) in enumerate(data):
+        alone, _ = run_alone(d, x, v, m, [dts[k]] * 4)
+        got = b.download(k)
+        assert np.array_equal(got["ids"], alone["ids"]), k
+        for f in ("x", "v", "F", "Jp"):
+            assert rel(got[f], alone[f]) < 1e-5, (scenes[k], f, rel(got[f], alone[f]))
+    b.close()

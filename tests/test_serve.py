@@ -147,3 +147,18 @@ def test_submit_refuses_what_can_never_run():
     with pytest.raises(M.MPMError):
         srv.submit(d, x, v, m, 1)  # CG cap 10000 above the server's 50
     srv.close()
+
+
+def test_2d_requests_match_alone():
+    srv = M.Server(1 << 30, max_scenes=2, max_counts=64, dim=2, keep_results=True)
+    reqs = [(M.scene_generate("disk2d", 64, 0), 2), (M.scene_generate("disk2d", 32, 5, 600), 3)]
+    ids = [srv.submit(d, x, v, m, f) for (d, x, v, m), f in reqs]
+    while any(srv.poll(i)["state"] != M.REQ_DONE for i in ids):
+        srv.tick()
+    for ((d, x, v, m), frames), rid in zip(reqs, ids):
+        alone, steps = run_alone(d, x, v, m, frames)
+        got = srv.download(rid)
+        assert srv.poll(rid)["substeps"] == steps
+        for f in ("x", "v", "F"):
+            assert rel(got[f], alone[f]) < 1e-4, f
+    srv.close()
