@@ -40,6 +40,7 @@ extern "C" {
 #define MPM_E_BUDGET (-6)        /* serving admission: over the HBM budget */
 
 #define MPM_MAX_PLANES 8
+#define MPM_MAX_MATERIALS 8
 
 /* Node flag bits (the reference uses Nova's SPGrid flags,
  * Flag_Setup_Helper.h:32 sets Cell_Type_Interior where m > 0). */
@@ -54,9 +55,11 @@ typedef struct mpm_plane {
     int32_t sticky; /* unused, as in the reference (MPM_Plane_Barrier.h:23) */
 } mpm_plane;
 
-/* Snow material (MPM_Constitutive_Model.h:78-93), per scene instead of per
- * particle.  mu0/lambda0 from Compute_Lame_Parameters
- * (MPM_Constitutive_Model.cpp:22-27). */
+/* A particle material: the constitutive model a reference particle carries
+ * (MPM_Constitutive_Model.h:78-93; mu0/lambda0 from Compute_Lame_Parameters,
+ * MPM_Constitutive_Model.cpp:22-27), or an EOS fluid (MPM_Particle.h:33-37,
+ * 51-55).  A scene holds a small table of them and every particle names
+ * its entry. */
 typedef struct mpm_material {
     float mu0;
     float lambda0;
@@ -65,6 +68,15 @@ typedef struct mpm_material {
     float hardening_factor;  /* xi */
     float hardening_max;     /* FLT_MAX in the reference */
     int32_t plastic;
+    /* EOS fluid particle: no constitutive model; explicit force
+     * (V0/rho) K rho^(gamma-1) grad w (MPM_Example.cpp:774-782), CG force
+     * V0 K gamma rho^(gamma-1) (grad w . v) (MPM_CG_System.h:169-174,
+     * 205-207), density divided by 1 + dt tr(grad v) in G2P
+     * (MPM_Example.cpp:646).  The particle's density rides in its Jp slot. */
+    int32_t eos;
+    float bulk_modulus;      /* K; the reference's default is 1e7 */
+    float gamma;             /* 7 */
+    float density;           /* initial density, 1 */
 } mpm_material;
 
 /* Scene description: what MPM_Example::Parse_Options and the scene hook
@@ -98,6 +110,11 @@ typedef struct mpm_scene_desc {
      * 0 = zero the CG vector at every node inside a plane
      *     ((x_i - s).n < 0). */
     int32_t project_mode;
+    /* Material table: entry 0 is `material`; nmaterials (0 or 1: only
+     * `material`) up to MPM_MAX_MATERIALS, entries 1.. in extra_materials.
+     * Particles uploaded without ids use entry 0. */
+    int32_t nmaterials;
+    mpm_material extra_materials[MPM_MAX_MATERIALS - 1];
 } mpm_scene_desc;
 
 typedef struct mpm_step_stats {
@@ -148,14 +165,25 @@ int mpm_destroy(mpm_handle h);
 int mpm_upload_particles(mpm_handle h, int64_t n, const float* x, const float* v,
                          const float* F, const float* Jp, const float* mass,
                          const float* volume);
+/* The same with each particle's entry in the scene's material table
+ * (MPM_Particle's per-particle constitutive model and eos flag,
+ * MPM_Particle.h:33-37; NULL: all 0).  An EOS particle uploaded with
+ * Jp == NULL starts at its material's density. */
+int mpm_upload_particles_ex(mpm_handle h, int64_t n, const float* x, const float* v, const float* F,
+                            const float* Jp, const float* mass, const float* volume,
+                            const int32_t* material);
 
 /* Copies live particles back in upload order (ids ascending); removed
  * particles are skipped, their ids missing from `ids`.  Any output
  * pointer may be NULL.  Replaces reading MPM_Example::particles
- * (MPM_Example.h:271) and Write_Output_Files (MPM_Example.cpp:1049-1063). */
+ * (MPM_Example.h:271) and Write_Output_Files (MPM_Example.cpp:1049-1063).
+ * For an EOS particle the Jp output holds its density. */
 int mpm_download_particles(mpm_handle h, int64_t cap, float* x, float* v, float* F,
                            float* Jp, float* mass, float* volume, int32_t* ids,
                            int64_t* n_out);
+int mpm_download_particles_ex(mpm_handle h, int64_t cap, float* x, float* v, float* F,
+                              float* Jp, float* mass, float* volume, int32_t* ids,
+                              int32_t* material, int64_t* n_out);
 
 /* MPM_Driver::Initialize (MPM_Driver.cpp:18-32): one grid build, P2G and
  * Process_Waiting_Particles for particles uploaded without volume. */
@@ -283,6 +311,9 @@ int32_t mpm_batch_size(mpm_handle h);
  * before mpm_initialize, which initialises the whole batch. */
 int mpm_batch_upload(mpm_handle h, int32_t scene, int64_t n, const float* x, const float* v,
                      const float* F, const float* Jp, const float* mass, const float* volume);
+int mpm_batch_upload_ex(mpm_handle h, int32_t scene, int64_t n, const float* x, const float* v,
+                        const float* F, const float* Jp, const float* mass, const float* volume,
+                        const int32_t* material);
 /* One substep of every scene s with dt[s]; dt[s] <= 0 leaves scene s
  * unchanged (MPM_Driver::Advance_Step per scene). */
 int mpm_batch_substep(mpm_handle h, const float* dt);
@@ -291,6 +322,9 @@ int mpm_batch_max_velocity(mpm_handle h, float* out);
 int mpm_batch_stats(mpm_handle h, int32_t scene, mpm_step_stats* stats);
 int mpm_batch_download(mpm_handle h, int32_t scene, int64_t cap, float* x, float* v, float* F,
                        float* Jp, float* mass, float* volume, int32_t* ids, int64_t* n_out);
+int mpm_batch_download_ex(mpm_handle h, int32_t scene, int64_t cap, float* x, float* v, float* F,
+                          float* Jp, float* mass, float* volume, int32_t* ids, int32_t* material,
+                          int64_t* n_out);
 /* MPM_Driver::Advance_To_Target_Time (MPM_Driver.cpp:55-80) for every scene:
  * each scene takes the reference's dt rule from its own max speed, and
  * scenes that reach their target idle while the others finish.  time_out
@@ -354,6 +388,10 @@ int mpm_serve_arena(mpm_server_handle s, int64_t* particles, int64_t* slots, int
  * at admission).  MPM_E_CAPACITY if it could never fit the arena. */
 int mpm_serve_submit(mpm_server_handle s, const mpm_scene_desc* desc, int64_t n, const float* x,
                      const float* v, const float* mass, int32_t frames, int64_t* request_id);
+/* The same with each particle's material entry (NULL: all 0). */
+int mpm_serve_submit_ex(mpm_server_handle s, const mpm_scene_desc* desc, int64_t n, const float* x,
+                        const float* v, const float* mass, const int32_t* material, int32_t frames,
+                        int64_t* request_id);
 int mpm_serve_tick(mpm_server_handle s, mpm_tick_stats* out);
 int mpm_serve_poll(mpm_server_handle s, int64_t request_id, mpm_request_status* out);
 /* Particles of a running request (current state) or of a finished one
@@ -370,8 +408,10 @@ int mpm_serve_release(mpm_server_handle s, int64_t request_id);
 
 /* ---- scenes (scene hook Initialize_Particles) ------------------------- */
 /* Fills desc with the preset of a named scene and returns the particle
- * count it generates.  Known names: "disk2d", "two_balls",
- * "avalanche", "ref3d_test18", "ref2d_test18", "blob", "slab". */
+ * count it generates.  Known names: "disk2d", "two_balls", "avalanche",
+ * "blob", "slab", and the reference's Standard_Tests scenes "ref2d_test1",
+ * "ref2d_test15", "ref2d_test16", "ref2d_test17", "ref2d_test18",
+ * "ref2d_test19", "ref3d_test18", "ref3d_test19". */
 int mpm_scene_preset(const char* name, int32_t counts, uint64_t seed, mpm_scene_desc* desc,
                      int64_t* n_particles);
 /* Generates the particles of a named scene (same arguments as the preset);
@@ -388,6 +428,10 @@ int mpm_scene_preset_sized(const char* name, int32_t counts, uint64_t seed, int6
                            mpm_scene_desc* desc, int64_t* n_particles);
 int mpm_scene_generate_sized(const char* name, int32_t counts, uint64_t seed, int64_t target_n,
                              int64_t cap, float* x, float* v, float* mass, int64_t* n_out);
+/* Material entry of every particle of a (sized) scene, in generation order
+ * (the scenes that mix materials: the reference's 2D test 16). */
+int mpm_scene_materials(const char* name, int32_t counts, uint64_t seed, int64_t target_n,
+                        int64_t cap, int32_t* material, int64_t* n_out);
 
 #ifdef __cplusplus
 }

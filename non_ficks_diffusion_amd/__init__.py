@@ -22,6 +22,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libmpm_gpu.so")
 
 MPM_MAX_PLANES = 8
+MPM_MAX_MATERIALS = 8
 MPM_FLAG_INTERIOR = 1
 
 STAGE_BIN, STAGE_P2G, STAGE_WAITING, STAGE_CONSTITUTIVE = 0, 1, 2, 3
@@ -48,7 +49,8 @@ class Material(ctypes.Structure):
     _fields_ = [("mu0", ctypes.c_float), ("lambda0", ctypes.c_float),
                 ("stretching_yield", ctypes.c_float), ("compression_yield", ctypes.c_float),
                 ("hardening_factor", ctypes.c_float), ("hardening_max", ctypes.c_float),
-                ("plastic", ctypes.c_int32)]
+                ("plastic", ctypes.c_int32), ("eos", ctypes.c_int32), ("bulk_modulus", ctypes.c_float),
+                ("gamma", ctypes.c_float), ("density", ctypes.c_float)]
 
 
 class SceneDesc(ctypes.Structure):
@@ -58,7 +60,12 @@ class SceneDesc(ctypes.Structure):
                 ("cg_tolerance", ctypes.c_float), ("cg_max_iterations", ctypes.c_int32),
                 ("cg_min_iterations", ctypes.c_int32), ("nplanes", ctypes.c_int32),
                 ("planes", Plane * MPM_MAX_PLANES), ("material", Material),
-                ("crop_ghost_x", ctypes.c_int32), ("project_mode", ctypes.c_int32)]
+                ("crop_ghost_x", ctypes.c_int32), ("project_mode", ctypes.c_int32),
+                ("nmaterials", ctypes.c_int32), ("extra_materials", Material * (MPM_MAX_MATERIALS - 1))]
+
+    def materials(self):
+        """The scene's material table (entry 0 is `material`)."""
+        return [self.material] + [self.extra_materials[k] for k in range(max(1, self.nmaterials) - 1)]
 
     def copy(self) -> "SceneDesc":
         d = SceneDesc()
@@ -132,6 +139,8 @@ SIGNATURES = {
     "mpm_destroy": (ctypes.c_int, [_H]),
     "mpm_upload_particles": (ctypes.c_int, [_H, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP]),
     "mpm_download_particles": (ctypes.c_int, [_H, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _IP, _LP]),
+    "mpm_upload_particles_ex": (ctypes.c_int, [_H, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _IP]),
+    "mpm_download_particles_ex": (ctypes.c_int, [_H, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _IP, _IP, _LP]),
     "mpm_initialize": (ctypes.c_int, [_H]),
     "mpm_substep": (ctypes.c_int, [_H, ctypes.c_float, ctypes.POINTER(StepStats)]),
     "mpm_advance_to_time": (ctypes.c_int, [_H, ctypes.c_double, ctypes.c_double, _IP,
@@ -158,6 +167,8 @@ SIGNATURES = {
     "mpm_serve_arena": (ctypes.c_int, [_H, _LP, _LP, _LP, _LP]),
     "mpm_serve_submit": (ctypes.c_int, [_H, ctypes.POINTER(SceneDesc), ctypes.c_int64, _FP, _FP, _FP, ctypes.c_int32,
                                         _LP]),
+    "mpm_serve_submit_ex": (ctypes.c_int, [_H, ctypes.POINTER(SceneDesc), ctypes.c_int64, _FP, _FP, _FP, _IP,
+                                           ctypes.c_int32, _LP]),
     "mpm_serve_tick": (ctypes.c_int, [_H, ctypes.POINTER(TickStats)]),
     "mpm_serve_poll": (ctypes.c_int, [_H, ctypes.c_int64, ctypes.POINTER(RequestStatus)]),
     "mpm_serve_download": (ctypes.c_int, [_H, ctypes.c_int64, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _IP,
@@ -168,10 +179,13 @@ SIGNATURES = {
     "mpm_batch_create": (ctypes.c_int, [ctypes.POINTER(SceneDesc), _LP, ctypes.c_int32, ctypes.POINTER(_H)]),
     "mpm_batch_size": (ctypes.c_int32, [_H]),
     "mpm_batch_upload": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP]),
+    "mpm_batch_upload_ex": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _IP]),
     "mpm_batch_substep": (ctypes.c_int, [_H, _FP]),
     "mpm_batch_max_velocity": (ctypes.c_int, [_H, _FP]),
     "mpm_batch_stats": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(StepStats)]),
     "mpm_batch_download": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _IP, _LP]),
+    "mpm_batch_download_ex": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _IP,
+                                             _IP, _LP]),
     "mpm_batch_advance_to_time": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                                  ctypes.POINTER(ctypes.c_double), _IP]),
     "mpm_scene_preset": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint64,
@@ -182,6 +196,8 @@ SIGNATURES = {
                                               ctypes.POINTER(SceneDesc), _LP]),
     "mpm_scene_generate_sized": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64,
                                                 ctypes.c_int64, _FP, _FP, _FP, _LP]),
+    "mpm_scene_materials": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64,
+                                           ctypes.c_int64, _IP, _LP]),
 }
 
 _lib: Optional[ctypes.CDLL] = None
@@ -215,6 +231,21 @@ def _f32(a) -> np.ndarray:
     return np.ascontiguousarray(a, dtype=np.float32)
 
 
+def _i32(a) -> Optional[np.ndarray]:
+    return None if a is None else np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _ip(a: Optional[np.ndarray]):
+    return None if a is None else a.ctypes.data_as(_IP)
+
+
+def _particle_arrays(cap: int, d: int) -> Dict[str, np.ndarray]:
+    return {"x": np.zeros((cap, d), np.float32), "v": np.zeros((cap, d), np.float32),
+            "F": np.zeros((cap, d, d), np.float32), "Jp": np.zeros(cap, np.float32),
+            "mass": np.zeros(cap, np.float32), "volume": np.zeros(cap, np.float32),
+            "ids": np.zeros(cap, np.int32), "material": np.zeros(cap, np.int32)}
+
+
 def device_info() -> str:
     buf = ctypes.create_string_buffer(256)
     _check(lib().mpm_device_info(buf, 256))
@@ -242,6 +273,16 @@ def scene_generate(name: str, counts: int = 0, seed: int = 0, n: int = 0):
     _check(lib().mpm_scene_generate_sized(name.encode(), counts, seed, n, count, _fp(x), _fp(v), _fp(m),
                                           ctypes.byref(out)))
     return d, x, v, m
+
+
+def scene_materials(name: str, counts: int = 0, seed: int = 0, n: int = 0) -> np.ndarray:
+    """Material table entry of every particle of a named scene, in the order
+    scene_generate returns them (all 0 but in scenes that mix materials)."""
+    _, count = scene_preset(name, counts, seed, n)
+    out = np.zeros(count, np.int32)
+    k = ctypes.c_int64()
+    _check(lib().mpm_scene_materials(name.encode(), counts, seed, n, count, _ip(out), ctypes.byref(k)))
+    return out
 
 
 def kernel_class_names():
@@ -313,26 +354,27 @@ class Scene(_LaunchControl):
         self.close()
 
     # -- particles ---------------------------------------------------------
-    def upload(self, x, v, mass, F=None, Jp=None, volume=None) -> None:
+    def upload(self, x, v, mass, F=None, Jp=None, volume=None, material=None) -> None:
+        """material: entry of the scene's material table per particle (None:
+        all 0); an EOS particle's Jp is its density (None: its material's)."""
         x, v, mass = _f32(x), _f32(v), _f32(mass)
         n = mass.shape[0]
         F = None if F is None else _f32(F)
         Jp = None if Jp is None else _f32(Jp)
         volume = None if volume is None else _f32(volume)
-        _check(lib().mpm_upload_particles(self._h, n, _fp(x), _fp(v), _fp(F), _fp(Jp), _fp(mass), _fp(volume)))
+        material = _i32(material)
+        _check(lib().mpm_upload_particles_ex(self._h, n, _fp(x), _fp(v), _fp(F), _fp(Jp), _fp(mass), _fp(volume),
+                                             _ip(material)))
 
     def download(self, cap: Optional[int] = None) -> Dict[str, np.ndarray]:
-        d = self.dim
+        """Live particles by id; Jp holds an EOS particle's density."""
         if cap is None:
             cap = self.capacity
-        out = {"x": np.zeros((cap, d), np.float32), "v": np.zeros((cap, d), np.float32),
-               "F": np.zeros((cap, d, d), np.float32), "Jp": np.zeros(cap, np.float32),
-               "mass": np.zeros(cap, np.float32), "volume": np.zeros(cap, np.float32),
-               "ids": np.zeros(cap, np.int32)}
+        out = _particle_arrays(cap, self.dim)
         n = ctypes.c_int64()
-        _check(lib().mpm_download_particles(self._h, cap, _fp(out["x"]), _fp(out["v"]), _fp(out["F"]),
-                                            _fp(out["Jp"]), _fp(out["mass"]), _fp(out["volume"]),
-                                            out["ids"].ctypes.data_as(_IP), ctypes.byref(n)))
+        _check(lib().mpm_download_particles_ex(self._h, cap, _fp(out["x"]), _fp(out["v"]), _fp(out["F"]),
+                                               _fp(out["Jp"]), _fp(out["mass"]), _fp(out["volume"]),
+                                               _ip(out["ids"]), _ip(out["material"]), ctypes.byref(n)))
         return {k: a[:n.value] for k, a in out.items()}
 
     # -- driver ------------------------------------------------------------
@@ -456,13 +498,14 @@ class Batch(_LaunchControl):
         except Exception:
             pass
 
-    def upload(self, scene: int, x, v, mass, F=None, Jp=None, volume=None) -> None:
+    def upload(self, scene: int, x, v, mass, F=None, Jp=None, volume=None, material=None) -> None:
         x, v, mass = _f32(x), _f32(v), _f32(mass)
         F = None if F is None else _f32(F)
         Jp = None if Jp is None else _f32(Jp)
         volume = None if volume is None else _f32(volume)
-        _check(lib().mpm_batch_upload(self._h, int(scene), mass.shape[0], _fp(x), _fp(v), _fp(F), _fp(Jp), _fp(mass),
-                                      _fp(volume)))
+        material = _i32(material)
+        _check(lib().mpm_batch_upload_ex(self._h, int(scene), mass.shape[0], _fp(x), _fp(v), _fp(F), _fp(Jp),
+                                         _fp(mass), _fp(volume), _ip(material)))
 
     def initialize(self) -> None:
         _check(lib().mpm_initialize(self._h))
@@ -485,14 +528,11 @@ class Batch(_LaunchControl):
         d = self.dim
         if cap is None:
             cap = self.capacities[scene]
-        out = {"x": np.zeros((cap, d), np.float32), "v": np.zeros((cap, d), np.float32),
-               "F": np.zeros((cap, d, d), np.float32), "Jp": np.zeros(cap, np.float32),
-               "mass": np.zeros(cap, np.float32), "volume": np.zeros(cap, np.float32),
-               "ids": np.zeros(cap, np.int32)}
+        out = _particle_arrays(cap, d)
         n = ctypes.c_int64()
-        _check(lib().mpm_batch_download(self._h, int(scene), cap, _fp(out["x"]), _fp(out["v"]), _fp(out["F"]),
-                                        _fp(out["Jp"]), _fp(out["mass"]), _fp(out["volume"]),
-                                        out["ids"].ctypes.data_as(_IP), ctypes.byref(n)))
+        _check(lib().mpm_batch_download_ex(self._h, int(scene), cap, _fp(out["x"]), _fp(out["v"]), _fp(out["F"]),
+                                           _fp(out["Jp"]), _fp(out["mass"]), _fp(out["volume"]),
+                                           _ip(out["ids"]), _ip(out["material"]), ctypes.byref(n)))
         return {k: a[:n.value] for k, a in out.items()}
 
     def advance_to_time(self, time, target):
@@ -552,12 +592,13 @@ class Server:
         _check(lib().mpm_serve_arena(self._h, *[ctypes.byref(x) for x in v]))
         return dict(zip(("particles", "slots", "blocks", "bytes"), (x.value for x in v)))
 
-    def submit(self, desc: SceneDesc, x, v, mass, frames: int) -> int:
+    def submit(self, desc: SceneDesc, x, v, mass, frames: int, material=None) -> int:
         x, v, mass = _f32(x), _f32(v), _f32(mass)
+        material = _i32(material)
         d = desc.copy()
         rid = ctypes.c_int64()
-        _check(lib().mpm_serve_submit(self._h, ctypes.byref(d), mass.shape[0], _fp(x), _fp(v), _fp(mass), int(frames),
-                                      ctypes.byref(rid)))
+        _check(lib().mpm_serve_submit_ex(self._h, ctypes.byref(d), mass.shape[0], _fp(x), _fp(v), _fp(mass),
+                                         _ip(material), int(frames), ctypes.byref(rid)))
         self._n[rid.value] = mass.shape[0]
         return rid.value
 

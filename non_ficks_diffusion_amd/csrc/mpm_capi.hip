@@ -58,7 +58,7 @@ int dalloc(mpm_context* c, T** p, size_t count) {
     return MPM_OK;
 }
 
-int nfields(int D) { return 2 * D + D * D + 3; }
+int nfields(int D) { return 2 * D + D * D + 4; }  // X V F Jp m V0 material
 constexpr int MAX_NODE_GROUPS = 2048;
 constexpr int MAX_GRAPH_CG_ITERATIONS = 256;
 int block_edge(int D) { return D == 3 ? 4 : 8; }
@@ -72,13 +72,22 @@ mpm::SceneParams scene_params(const mpm_scene_desc& d, int slot_base, int nslots
     p.nslots = nslots;
     p.dx = 1.0f / (float)d.counts;
     p.inv_dx = (float)d.counts;
-    p.mat.mu0 = d.material.mu0;
-    p.mat.lambda0 = d.material.lambda0;
-    p.mat.sy = d.material.stretching_yield;
-    p.mat.cy = d.material.compression_yield;
-    p.mat.xi = d.material.hardening_factor;
-    p.mat.hmax = d.material.hardening_max;
-    p.mat.plastic = d.material.plastic;
+    p.nmat = std::max(1, std::min((int)d.nmaterials, mpm::MAX_MATERIALS));
+    for (int k = 0; k < p.nmat; ++k) {
+        const mpm_material& m = k == 0 ? d.material : d.extra_materials[k - 1];
+        mpm::Material& o = p.mats[k];
+        o.mu0 = m.mu0;
+        o.lambda0 = m.lambda0;
+        o.sy = m.stretching_yield;
+        o.cy = m.compression_yield;
+        o.xi = m.hardening_factor;
+        o.hmax = m.hardening_max;
+        o.plastic = m.plastic;
+        o.eos = m.eos;
+        o.K = m.bulk_modulus;
+        o.gamma = m.gamma;
+        o.rho0 = m.density;
+    }
     for (int a = 0; a < 3; ++a) p.gravity[a] = d.gravity[a];
     p.flip = d.flip;
     p.nplanes = d.nplanes;
@@ -531,6 +540,21 @@ int check_desc(const mpm_scene_desc& d, const char* who) {
     if (d.dim != 2 && d.dim != 3) return mpm_set_error(MPM_E_INVALID, std::string(who) + ": dim must be 2 or 3");
     if (d.counts < 4 || d.counts > 4096) return mpm_set_error(MPM_E_INVALID, std::string(who) + ": counts out of range");
     if (d.nplanes < 0 || d.nplanes > MPM_MAX_PLANES) return mpm_set_error(MPM_E_INVALID, std::string(who) + ": too many planes");
+    if (d.nmaterials < 0 || d.nmaterials > MPM_MAX_MATERIALS)
+        return mpm_set_error(MPM_E_INVALID, std::string(who) + ": nmaterials out of range");
+    return MPM_OK;
+}
+static_assert(mpm::MAX_MATERIALS == MPM_MAX_MATERIALS, "material table size");
+
+// Material entry of particle i (material == nullptr: 0), checked against the
+// scene's table; the Jp slot of an EOS particle starts at its density.
+int particle_material(const mpm_scene_desc& d, const int32_t* material, int64_t i, int& id, bool& eos, float& rho0) {
+    id = material ? material[i] : 0;
+    const int nmat = std::max(1, (int)d.nmaterials);
+    if (id < 0 || id >= nmat) return mpm_set_error(MPM_E_INVALID, "particle material outside the scene's table");
+    const mpm_material& m = id == 0 ? d.material : d.extra_materials[id - 1];
+    eos = m.eos != 0;
+    rho0 = m.density;
     return MPM_OK;
 }
 
@@ -638,7 +662,7 @@ int fill_slot_scene(mpm_context* c, int first, int count, int value) {
 
 // Writes scene s's particles into its range of the (unbinned) buffer.
 int upload_scene(mpm_context* c, int s, int64_t n, const float* x, const float* v, const float* F, const float* Jp,
-                 const float* mass, const float* volume) {
+                 const float* mass, const float* volume, const int32_t* material) {
     SceneHost& sh = c->sc[(size_t)s];
     if (n < 0 || n > sh.capacity) return mpm_set_error(MPM_E_CAPACITY, "mpm_upload_particles: n exceeds capacity");
     const int D = c->D, NF = nfields(D);
@@ -652,9 +676,15 @@ int upload_scene(mpm_context* c, int s, int64_t n, const float* x, const float* 
         }
         for (int a = 0; a < D; ++a)
             for (int b = 0; b < D; ++b) f(2 * D + a * D + b, i) = F ? F[(i * D + a) * D + b] : (a == b ? 1.0f : 0.0f);
-        f(2 * D + D * D, i) = Jp ? Jp[i] : 1.0f;
+        int id;
+        bool eos;
+        float rho0;
+        const int rc = particle_material(sh.desc, material, i, id, eos, rho0);
+        if (rc) return rc;
+        f(2 * D + D * D, i) = Jp ? Jp[i] : (eos ? rho0 : 1.0f);
         f(2 * D + D * D + 1, i) = mass[i];
         f(2 * D + D * D + 2, i) = volume ? volume[i] : 0.0f;
+        std::memcpy(&f(2 * D + D * D + 3, i), &id, sizeof(int));
     }
     std::vector<int> ids((size_t)cap), keys((size_t)cap), scene((size_t)cap, s);
     for (int64_t i = 0; i < cap; ++i) {
@@ -688,7 +718,7 @@ int upload_max_speed(mpm_context* c) {
 }
 
 int download_scene(mpm_context* c, int s, int64_t cap, float* x, float* v, float* F, float* Jp, float* mass,
-                   float* volume, int32_t* ids, int64_t* n_out) {
+                   float* volume, int32_t* ids, int64_t* n_out, int32_t* material = nullptr) {
     mpm::Counters hc{};
     int rc = read_counters(c, hc);
     if (rc) return rc;
@@ -738,6 +768,10 @@ int download_scene(mpm_context* c, int s, int64_t cap, float* x, float* v, float
         if (Jp) Jp[k] = g(2 * D + D * D, i);
         if (mass) mass[k] = g(2 * D + D * D + 1, i);
         if (volume) volume[k] = g(2 * D + D * D + 2, i);
+        if (material) {
+            const float mf = g(2 * D + D * D + 3, i);
+            std::memcpy(&material[k], &mf, sizeof(int32_t));
+        }
         if (ids) ids[k] = id[(size_t)i];
     }
     return MPM_OK;
@@ -1120,7 +1154,7 @@ int live_particles(mpm_context* c, int64_t* n) {
 }
 
 int pool_attach(mpm_context* c, const mpm_scene_desc& d, int64_t n, const float* x, const float* v,
-                const float* mass, int* entry) {
+                const float* mass, const int32_t* material, int* entry) {
     if (!c->pool) return mpm_set_error(MPM_E_INVALID, "arena: not an arena");
     int rc = check_desc(d, "arena attach");
     if (rc) return rc;
@@ -1160,9 +1194,15 @@ int pool_attach(mpm_context* c, const mpm_scene_desc& d, int64_t n, const float*
         }
         for (int a = 0; a < D; ++a)
             for (int b = 0; b < D; ++b) f(2 * D + a * D + b, i) = a == b ? 1.0f : 0.0f;
-        f(2 * D + D * D, i) = 1.0f;
+        int id;
+        bool eos;
+        float rho0;
+        rc = particle_material(d, material, i, id, eos, rho0);
+        if (rc) return rc;
+        f(2 * D + D * D, i) = eos ? rho0 : 1.0f;
         f(2 * D + D * D + 1, i) = mass[i];
         f(2 * D + D * D + 2, i) = 0.0f;
+        std::memcpy(&f(2 * D + D * D + 3, i), &id, sizeof(int));
     }
     std::vector<int> ids((size_t)n), keys((size_t)n, 0), scene((size_t)n, e);
     std::iota(ids.begin(), ids.end(), 0);
@@ -1259,6 +1299,11 @@ int mpm_destroy(mpm_handle c) {
 
 int mpm_upload_particles(mpm_handle c, int64_t n, const float* x, const float* v, const float* F, const float* Jp,
                          const float* mass, const float* volume) {
+    return mpm_upload_particles_ex(c, n, x, v, F, Jp, mass, volume, nullptr);
+}
+
+int mpm_upload_particles_ex(mpm_handle c, int64_t n, const float* x, const float* v, const float* F, const float* Jp,
+                            const float* mass, const float* volume, const int32_t* material) {
     if (!c || !x || !v || !mass) return mpm_set_error(MPM_E_INVALID, "mpm_upload_particles: null argument");
     if (c->nscenes != 1) return mpm_set_error(MPM_E_INVALID, "mpm_upload_particles: batch handle, use mpm_batch_upload");
     // a new particle set: the buffer starts over in its unbinned layout
@@ -1271,15 +1316,20 @@ int mpm_upload_particles(mpm_handle c, int64_t n, const float* x, const float* v
     hc.n_prev = (int)c->pcap;
     HIPCHECK(hipMemcpy(c->ctr, &hc, sizeof(hc), hipMemcpyHostToDevice));
     c->initialized = c->binned = c->ever_binned = c->stepped = false;
-    int rc = upload_scene(c, 0, n, x, v, F, Jp, mass, volume);
+    int rc = upload_scene(c, 0, n, x, v, F, Jp, mass, volume, material);
     if (rc) return rc;
     return upload_max_speed(c);
 }
 
 int mpm_download_particles(mpm_handle c, int64_t cap, float* x, float* v, float* F, float* Jp, float* mass,
                            float* volume, int32_t* ids, int64_t* n_out) {
+    return mpm_download_particles_ex(c, cap, x, v, F, Jp, mass, volume, ids, nullptr, n_out);
+}
+
+int mpm_download_particles_ex(mpm_handle c, int64_t cap, float* x, float* v, float* F, float* Jp, float* mass,
+                              float* volume, int32_t* ids, int32_t* material, int64_t* n_out) {
     if (!c) return mpm_set_error(MPM_E_INVALID, "mpm_download_particles: null handle");
-    return download_scene(c, 0, cap, x, v, F, Jp, mass, volume, ids, n_out);
+    return download_scene(c, 0, cap, x, v, F, Jp, mass, volume, ids, n_out, material);
 }
 
 int mpm_initialize(mpm_handle c) {
@@ -1631,10 +1681,15 @@ int32_t mpm_batch_size(mpm_handle c) { return c ? c->nscenes : 0; }
 
 int mpm_batch_upload(mpm_handle c, int32_t scene, int64_t n, const float* x, const float* v, const float* F,
                      const float* Jp, const float* mass, const float* volume) {
+    return mpm_batch_upload_ex(c, scene, n, x, v, F, Jp, mass, volume, nullptr);
+}
+
+int mpm_batch_upload_ex(mpm_handle c, int32_t scene, int64_t n, const float* x, const float* v, const float* F,
+                        const float* Jp, const float* mass, const float* volume, const int32_t* material) {
     if (!c || !x || !v || !mass) return mpm_set_error(MPM_E_INVALID, "mpm_batch_upload: null argument");
     if (scene < 0 || scene >= c->nscenes) return mpm_set_error(MPM_E_INVALID, "mpm_batch_upload: no such scene");
     if (c->ever_binned) return mpm_set_error(MPM_E_NOT_READY, "mpm_batch_upload: the batch is already initialized");
-    int rc = upload_scene(c, scene, n, x, v, F, Jp, mass, volume);
+    int rc = upload_scene(c, scene, n, x, v, F, Jp, mass, volume, material);
     if (rc) return rc;
     return upload_max_speed(c);
 }
@@ -1660,9 +1715,14 @@ int mpm_batch_stats(mpm_handle c, int32_t scene, mpm_step_stats* stats) {
 
 int mpm_batch_download(mpm_handle c, int32_t scene, int64_t cap, float* x, float* v, float* F, float* Jp,
                        float* mass, float* volume, int32_t* ids, int64_t* n_out) {
+    return mpm_batch_download_ex(c, scene, cap, x, v, F, Jp, mass, volume, ids, nullptr, n_out);
+}
+
+int mpm_batch_download_ex(mpm_handle c, int32_t scene, int64_t cap, float* x, float* v, float* F, float* Jp,
+                          float* mass, float* volume, int32_t* ids, int32_t* material, int64_t* n_out) {
     if (!c) return mpm_set_error(MPM_E_INVALID, "mpm_batch_download: null handle");
     if (scene < 0 || scene >= c->nscenes) return mpm_set_error(MPM_E_INVALID, "mpm_batch_download: no such scene");
-    return download_scene(c, scene, cap, x, v, F, Jp, mass, volume, ids, n_out);
+    return download_scene(c, scene, cap, x, v, F, Jp, mass, volume, ids, n_out, material);
 }
 
 int mpm_batch_advance_to_time(mpm_handle c, const double* time, const double* target, double* time_out,

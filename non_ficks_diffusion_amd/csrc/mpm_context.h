@@ -105,7 +105,7 @@ int pool_create(int D, int max_scenes, int64_t pcap, int64_t nslots, int64_t bca
 // a free entry; MPM_E_CAPACITY if the entry, slot range or particle space is
 // not available.
 int pool_attach(mpm_context* c, const mpm_scene_desc& d, int64_t n, const float* x, const float* v,
-                const float* mass, int* entry);
+                const float* mass, const int32_t* material, int* entry);
 // Removes a scene's particles and frees its entry and slot range.
 int pool_detach(mpm_context* c, int entry);
 // Largest free slot range.

@@ -39,7 +39,10 @@ struct Dims<2> {
 struct Material {
     float mu0, lambda0, sy, cy, xi, hmax;
     int plastic;
+    int eos;                 // EOS fluid: K, gamma; density in the particle's Jp slot
+    float K, gamma, rho0;
 };
+constexpr int MAX_MATERIALS = 8;
 
 // ---------------------------------------------------------------------------
 // Quadratic B-spline weights (MPM_Particle.h:79-100).  The arithmetic is

@@ -25,7 +25,8 @@ struct SceneParams {
     int counts, nb;            // cells per axis; block slots per axis (padded)
     int slot_base, nslots;     // the scene's slots: [slot_base, slot_base + nslots)
     float dx, inv_dx;
-    Material mat;
+    Material mats[MAX_MATERIALS];  // the scene's material table (particles hold an entry)
+    int nmat;
     float gravity[3];
     float flip;
     int nplanes;

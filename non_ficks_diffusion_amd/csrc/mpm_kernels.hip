@@ -30,7 +30,8 @@ namespace mpm {
 
 template <int D>
 struct PF {
-    static constexpr int X = 0, V = D, F = 2 * D, J = 2 * D + D * D, M = J + 1, VOL = J + 2, N = J + 3;
+    // MAT: the particle's material entry (int bits); J holds the density of EOS particles
+    static constexpr int X = 0, V = D, F = 2 * D, J = 2 * D + D * D, M = J + 1, VOL = J + 2, MAT = J + 3, N = J + 4;
 };
 
 // Work-groups are dispatched round-robin over the 8 XCDs (work-group w runs
@@ -597,6 +598,14 @@ __device__ inline bool cell_particle(const KP& p, const SceneParams& sc, int i, 
     return same;
 }
 
+// The record of an EOS particle: its CG force V0 K gamma rho^(gamma-1)
+// (grad w . u) grad w (MPM_CG_System.h:169-174, 205-207) is A = c tr(T) I,
+// taking the reference's matrix-plus-scalar as adding to the diagonal.  In
+// the record's layout: U = I, every entry of the diagonal block c, no pair
+// blocks (2D: c at (00,00), (00,11), (11,11)).
+template <int D>
+__device__ inline void store_operator_eos(const KP& p, int i, float c, const float* X);
+
 // ============================================================================
 // Linearised force operator per particle.  MPM_CG_System::Force
 // (MPM_CG_System.h:150-215) maps the node field u to forces through
@@ -802,6 +811,27 @@ __device__ inline void apply_operator(const float* rec, const float* T, float* A
                 if (c != r) A[c] += v * T[r];
             }
     }
+}
+
+template <int D>
+__device__ inline void store_operator_eos(const KP& p, int i, float c, const float* X) {
+    float rec[OpRec<D>::W];
+#pragma unroll
+    for (int k = 0; k < OpRec<D>::W; ++k) rec[k] = 0.0f;
+#pragma unroll
+    for (int a = 0; a < D; ++a) rec[a] = X[a];
+    if constexpr (D == 3) {
+        rec[3] = 1.0f;  // identity quaternion
+#pragma unroll
+        for (int k = 7; k <= 12; ++k) rec[k] = c;  // m00 m01 m02 m11 m12 m22
+    } else {
+        rec[D + op_index<D>(0, 0)] = c;
+        rec[D + op_index<D>(0, 3)] = c;
+        rec[D + op_index<D>(3, 3)] = c;
+    }
+    float4* dst = reinterpret_cast<float4*>(p.pop + (int64_t)i * OpRec<D>::W);
+#pragma unroll
+    for (int k = 0; k < OpRec<D>::W / 4; ++k) dst[k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
 }
 
 // Separable stencil contractions for the CG operator.  With
@@ -1043,13 +1073,40 @@ __global__ __launch_bounds__(BWG) void k_init_volume(KP p) {
 }
 
 // ============================================================================
+// Explicit force of one particle into its wave's tile:
+// f_i += g m w_i - A grad w_i, A = V0 P F^T (MPM_Example.cpp:774-784).
+template <int D>
+__device__ inline void scatter_force(const SceneParams& sc, const int* l, const float (*w)[3], const float (*dw)[3],
+                                     const float* A, float m, float* mine) {
+    constexpr int LTN = Dims<D>::LTN, ST = Dims<D>::ST;
+    float gm[D];
+#pragma unroll
+    for (int a = 0; a < D; ++a) gm[a] = sc.gravity[a] * m;
+#pragma unroll
+    for (int s = 0; s < ST; ++s) {
+        int off[D];
+        const int t = tile_index<D>(l, s, off);
+        float g[D];
+        stencil_grad<D>(w, dw, off, g);
+        const float wt = stencil_weight<D>(w, off);
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            float inner = 0.0f;
+#pragma unroll
+            for (int k = 0; k < D; ++k) inner += A[a * D + k] * g[k];
+            mine[a * LTN + t] += gm[a] * wt - inner;
+        }
+        MPM_STEP_FENCE();
+    }
+}
+
 // Constitutive update (MPM_Example.cpp:563-571) and explicit force P2G
 // (MPM_Example.cpp:757-784): f_i += -V0 P F^T grad w + g m w
 // ============================================================================
 template <int D, bool CONST, bool FORCE>
 __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
     using Dm = Dims<D>;
-    constexpr int LTN = Dm::LTN, DD = D * D, ST = Dm::ST;
+    constexpr int LTN = Dm::LTN, DD = D * D;
     __shared__ float tile[NW * D * LTN];
     const int nblocks = p.ctr->nblocks;
     const int c = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1072,10 +1129,35 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
 #pragma unroll
             for (int k = 0; k < DD; ++k) F[k] = p.pcur[(PF<D>::F + k) * p.pcap + i];
             float Jp = p.pcur[PF<D>::J * p.pcap + i];
+            const Material& mat = sc.mats[__float_as_int(p.pcur[PF<D>::MAT * p.pcap + i])];
+            if (mat.eos) {
+                // EOS fluid (Jp slot = density): no constitutive model
+                const double rho = (double)Jp, V0 = (double)p.pcur[PF<D>::VOL * p.pcap + i];
+                const double rg = pow(rho, (double)mat.gamma - 1.0);
+                if (CONST) {
+                    float Xo[D];
+#pragma unroll
+                    for (int a = 0; a < D; ++a) Xo[a] = p.pcur[(PF<D>::X + a) * p.pcap + i];
+                    store_operator_eos<D>(p, i, (float)(V0 * (double)mat.K * (double)mat.gamma * rg), Xo);
+#pragma unroll
+                    for (int k = 0; k < DD; ++k) p.rcache[k * p.pcap + i] = (k % (D + 1) == 0) ? 1.0f : 0.0f;
+                }
+                if (FORCE) {
+                    float w[D][3], dw[D][3], X[D];
+                    if (!cell_particle<D>(p, sc, i, bb, l, w, dw, X)) continue;
+                    // f += (V0/rho) K rho^(gamma-1) grad w: the stress scatter below with A = -coeff I
+                    const float coeff = (float)(V0 / rho * (double)mat.K * rg);
+                    float A[DD];
+#pragma unroll
+                    for (int k = 0; k < DD; ++k) A[k] = (k % (D + 1) == 0) ? -coeff : 0.0f;
+                    scatter_force<D>(sc, l, w, dw, A, p.pcur[PF<D>::M * p.pcap + i], mine);
+                }
+                continue;
+            }
             if (CONST) {
                 float U[DD];
                 double Sv[D];
-                plasticity<D>(F, Jp, R, sc.mat, U, Sv);
+                plasticity<D>(F, Jp, R, mat, U, Sv);
 #pragma unroll
                 for (int k = 0; k < DD; ++k) {
                     p.pcur[(PF<D>::F + k) * p.pcap + i] = F[k];
@@ -1083,7 +1165,7 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
                 }
                 p.pcur[PF<D>::J * p.pcap + i] = Jp;
                 StressState<D> so;
-                stress_state<D>(F, R, Jp, sc.mat, so);
+                stress_state<D>(F, R, Jp, mat, so);
                 float Xo[D];
 #pragma unroll
                 for (int a = 0; a < D; ++a) Xo[a] = p.pcur[(PF<D>::X + a) * p.pcap + i];
@@ -1096,7 +1178,7 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
                 float w[D][3], dw[D][3], X[D];
                 if (!cell_particle<D>(p, sc, i, bb, l, w, dw, X)) continue;
                 StressState<D> st;
-                stress_state<D>(F, R, Jp, sc.mat, st);
+                stress_state<D>(F, R, Jp, mat, st);
                 float P[DD];
                 first_pk<D>(st, P);
                 const float V0 = p.pcur[PF<D>::VOL * p.pcap + i];
@@ -1111,25 +1193,7 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
                         for (int k = 0; k < D; ++k) s += P[r * D + k] * F[cc * D + k];
                         A[r * D + cc] = s * V0;
                     }
-                float gm[D];
-#pragma unroll
-                for (int a = 0; a < D; ++a) gm[a] = sc.gravity[a] * m;
-#pragma unroll
-                for (int s = 0; s < ST; ++s) {
-                    int off[D];
-                    const int t = tile_index<D>(l, s, off);
-                    float g[D];
-                    stencil_grad<D>(w, dw, off, g);
-                    const float wt = stencil_weight<D>(w, off);
-#pragma unroll
-                    for (int a = 0; a < D; ++a) {
-                        float inner = 0.0f;
-#pragma unroll
-                        for (int k = 0; k < D; ++k) inner += A[a * D + k] * g[k];
-                        mine[a * LTN + t] += gm[a] * wt - inner;
-                    }
-                    MPM_STEP_FENCE();
-                }
+                scatter_force<D>(sc, l, w, dw, A, m, mine);
             }
         }
         if (FORCE) {
@@ -1886,18 +1950,26 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
                     }
                 }
             }
-            float F[DD];
+            if (sc.mats[__float_as_int(p.pcur[PF<D>::MAT * p.pcap + i])].eos) {
+                // EOS: density /= 1 + dt tr(grad v) (MPM_Example.cpp:646); F unchanged
+                float tr = 0.0f;
 #pragma unroll
-            for (int k = 0; k < DD; ++k) F[k] = p.pcur[(PF<D>::F + k) * p.pcap + i];
+                for (int a = 0; a < D; ++a) tr += G[a * D + a];
+                p.pcur[PF<D>::J * p.pcap + i] = p.pcur[PF<D>::J * p.pcap + i] / (1.0f + dt * tr);
+            } else {
+                float F[DD];
 #pragma unroll
-            for (int r = 0; r < D; ++r)
+                for (int k = 0; k < DD; ++k) F[k] = p.pcur[(PF<D>::F + k) * p.pcap + i];
 #pragma unroll
-                for (int cc = 0; cc < D; ++cc) {
-                    float s = 0.0f;
+                for (int r = 0; r < D; ++r)
 #pragma unroll
-                    for (int k = 0; k < D; ++k) s += (dt * G[r * D + k]) * F[k * D + cc];
-                    p.pcur[(PF<D>::F + r * D + cc) * p.pcap + i] = F[r * D + cc] + s;
-                }
+                    for (int cc = 0; cc < D; ++cc) {
+                        float s = 0.0f;
+#pragma unroll
+                        for (int k = 0; k < D; ++k) s += (dt * G[r * D + k]) * F[k * D + cc];
+                        p.pcur[(PF<D>::F + r * D + cc) * p.pcap + i] = F[r * D + cc] + s;
+                    }
+            }
             bool inside = true;
             float v2 = 0.0f;
 #pragma unroll

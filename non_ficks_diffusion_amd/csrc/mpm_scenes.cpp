@@ -4,15 +4,17 @@
 // mpm_2d/Standard_Tests/Standard_Tests.h:47-323), plus the snow scenes the
 // benchmark configurations name.
 //
-// Sampling: stratified jitter.  Every cell of the shape's bounding box is
+// Sampling: stratified jitter (shapes without a fixed count).  Every cell of the shape's bounding box is
 // split into ppc sub-cells (2 per axis), each sub-cell gets one uniform
 // sample from a counter-based generator keyed by (seed, cell, sub-cell),
 // and samples inside the shape are kept.  Particle mass is
-// density * dx^dim / ppc.  The reference's own test 18 scenes draw a fixed
-// number of uniform samples inside a sphere (Random_Numbers::
-// Get_Vector_In_Sphere); those are reproduced by rejection sampling from
-// the same counter-based generator (Nova's generator is not available, so
-// the positions are new, the statistics are the same).
+// density * dx^dim / ppc.  The reference's own Standard_Tests scenes draw a
+// fixed number of uniform samples inside a sphere or box
+// (Random_Numbers::Get_Vector_In_Sphere / Get_Uniform_Vector) with mass
+// mass_density * shape volume / count; those are reproduced by rejection
+// sampling from the same counter-based generator (Nova's generator is not
+// available, so the positions are new, the statistics are the same).  Each
+// shape names its entry in the scene's material table.
 //
 // Sized scenes (target_n > 0): the preset's shapes are scaled toward the
 // volume that holds target_n particles at 8 per cell (clamped so the shapes
@@ -53,6 +55,9 @@ struct Shape {
     double axis[3][3];   // box axes (unit)
     double half[3];      // box half extents
     float v[3];          // initial velocity
+    int count = 0;       // >0: this many uniform samples (reference Standard_Tests style)
+    double mass_density = 0.0;  // with count: particle mass = mass_density * volume / count
+    int material = 0;    // entry of the scene's material table
     bool inside(const double* p, int dim) const {
         if (kind == SPHERE) {
             double s = 0;
@@ -84,9 +89,12 @@ struct Spec {
     mpm_scene_desc desc;
     std::vector<Shape> shapes;
     double density = 400.0;
-    int fixed_count = 0;     // >0: uniform samples in each sphere (reference test 18 style)
-    double fixed_mass_density = 2.0;
     int64_t target_n = 0;    // >0: exactly this many uniform samples over all shapes
+    bool fixed() const {
+        for (const Shape& s : shapes)
+            if (s.count > 0) return true;
+        return false;
+    }
 };
 
 double shape_volume(const Shape& s, int dim) {
@@ -111,6 +119,9 @@ void snow_material(mpm_material& m, double E, double nu, double theta_c, double 
     m.hardening_factor = (float)xi;
     m.hardening_max = FLT_MAX;
     m.plastic = 1;
+    m.bulk_modulus = 1e7f;
+    m.gamma = 7.0f;
+    m.density = 1.0f;
 }
 
 void base_desc(mpm_scene_desc& d, int dim, int counts) {
@@ -133,14 +144,60 @@ void base_desc(mpm_scene_desc& d, int dim, int counts) {
 }
 
 // Upper bound of dt: the reference's 5e-3 (MPM_Driver.cpp:59), tightened to
-// the elastic CFL dx / c_p with c_p = sqrt((lambda0 + 2 mu0) / rho).  The
-// reference's CFL (cfl dx / max|V|) ignores the wave speed and relies on a CG
-// run to 1e-7; with at most 50 unpreconditioned iterations a stiff scene at
-// dt = 5e-3 leaves residuals of order 10^2 on light nodes and blows up.
+// the elastic CFL dx / c_p with c_p = sqrt((lambda0 + 2 mu0) / rho) of the
+// stiffest material (an EOS fluid at its initial density: K gamma instead of
+// lambda0 + 2 mu0).  The reference's CFL (cfl dx / max|V|) ignores the wave
+// speed and relies on a CG run to 1e-7; with at most 50 unpreconditioned
+// iterations a stiff scene at dt = 5e-3 leaves residuals of order 10^2 on
+// light nodes and blows up.
 void elastic_dt_bound(mpm_scene_desc& d, double rho) {
-    const double cp = std::sqrt((d.material.lambda0 + 2.0 * d.material.mu0) / rho);
-    const double dt = (1.0 / d.counts) / cp;
-    if (dt < d.max_dt) d.max_dt = (float)dt;
+    const int nmat = std::max(1, (int)d.nmaterials);
+    for (int k = 0; k < nmat; ++k) {
+        const mpm_material& m = k == 0 ? d.material : d.extra_materials[k - 1];
+        const double modulus = m.eos ? (double)m.bulk_modulus * m.gamma * std::pow((double)m.density, m.gamma - 1.0)
+                                     : m.lambda0 + 2.0 * m.mu0;
+        const double dt = (1.0 / d.counts) / std::sqrt(modulus / rho);
+        if (dt < d.max_dt) d.max_dt = (float)dt;
+    }
+}
+
+// A material of the reference's Standard_Tests:
+// Compute_Lame_Parameters(E, nu) and the plasticity fields
+// (MPM_Constitutive_Model.h:105, MPM_Constitutive_Model.cpp:22-27).
+mpm_material ref_material(double E, double nu, int plastic, double sy, double cy, double xi) {
+    mpm_material m{};
+    m.mu0 = (float)(E / (2.0 * (1.0 + nu)));
+    m.lambda0 = (float)(E * nu / ((1.0 + nu) * (1.0 - 2.0 * nu)));
+    m.stretching_yield = (float)sy;
+    m.compression_yield = (float)cy;
+    m.hardening_factor = (float)xi;
+    m.hardening_max = FLT_MAX;
+    m.plastic = plastic;
+    m.bulk_modulus = 1e7f;  // MPM_Particle::Initialize (MPM_Particle.h:51-56)
+    m.gamma = 7.0f;
+    m.density = 1.0f;
+    return m;
+}
+
+// Scene description of the reference's Standard_Tests scenes: gravity
+// (0,-2,0), CG tolerance 1e-7 and cap 10000 (MPM_Example.cpp:77-82).
+void ref_desc(mpm_scene_desc& d, int dim, int counts) {
+    base_desc(d, dim, counts > 0 ? counts : (dim == 3 ? 32 : 64));
+    d.gravity[1] = -2.0f;
+    d.cg_tolerance = 1e-7f;
+    d.cg_max_iterations = 10000;
+}
+
+Shape box2(double x0, double y0, double x1, double y1) {
+    Shape s{};
+    s.kind = OBOX;
+    for (int a = 0; a < 3; ++a)
+        for (int k = 0; k < 3; ++k) s.axis[k][a] = (a == k) ? 1.0 : 0.0;
+    s.c[0] = 0.5 * (x0 + x1);
+    s.c[1] = 0.5 * (y0 + y1);
+    s.half[0] = 0.5 * (x1 - x0);
+    s.half[1] = 0.5 * (y1 - y0);
+    return s;
 }
 
 void add_plane(mpm_scene_desc& d, double nx, double ny, double nz, double sx, double sy, double sz) {
@@ -206,24 +263,99 @@ bool build_spec(const std::string& name, int counts, uint64_t seed, int64_t targ
         sp.shapes.push_back(s);
     } else if (name == "ref3d_test18" || name == "ref2d_test18") {
         // Reference test 18: two balls r=0.1 at x=0.8 / 0.2 with V=(-2,0.1) / (2,0.1),
-        // E=30, nu=0.4, yields 1.01 / 0.95, xi=10, mass density 2, ground y=0.1,
-        // g=(0,-2,0), CG tolerance 1e-7 (mpm_3d/Standard_Tests/Standard_Tests.h:51-112,
-        // mpm_2d/Standard_Tests/Standard_Tests.h:190-246, MPM_Example.cpp:77-82).
+        // E=30, nu=0.4, yields 1.01 / 0.95, xi=10, mass density 2, ground y=0.1
+        // (mpm_3d/Standard_Tests/Standard_Tests.h:51-112,
+        // mpm_2d/Standard_Tests/Standard_Tests.h:190-246).
         const int dim = name == "ref3d_test18" ? 3 : 2;
-        base_desc(d, dim, counts > 0 ? counts : (dim == 3 ? 32 : 64));
-        d.gravity[1] = -2.0f;
-        d.cg_tolerance = 1e-7f;
-        d.cg_max_iterations = 10000;
-        d.material.mu0 = (float)(30.0 / (2.0 * 1.4));
-        d.material.lambda0 = (float)(30.0 * 0.4 / (1.4 * 0.2));
-        d.material.stretching_yield = 1.01f;
-        d.material.compression_yield = 0.95f;
-        d.material.hardening_factor = 10.0f;
+        ref_desc(d, dim, counts);
+        d.material = ref_material(30.0, 0.4, 1, 1.01, 0.95, 10.0);
         add_plane(d, 0, 1, 0, 0, 0.1, 0);
         sp.shapes.push_back(sphere(0.8, 0.5, 0.5, 0.1, -2.f, 0.1f, 0.f));
         sp.shapes.push_back(sphere(0.2, 0.5, 0.5, 0.1, 2.f, 0.1f, 0.f));
-        sp.fixed_count = dim == 3 ? 20000 : 2000;
-        sp.fixed_mass_density = 2.0;
+        for (Shape& s : sp.shapes) {
+            s.count = dim == 3 ? 20000 : 2000;
+            s.mass_density = 2.0;
+        }
+    } else if (name == "ref2d_test1") {
+        // Reference test 1: one particle in a ball r=0.1 at (0.2,0.5), V=(0.5,0), mass
+        // density 2 over the ball, E=40, nu=0.4, yields 1.01 / 0.95, xi=10, no planes
+        // (mpm_2d/Standard_Tests/Standard_Tests.h:55-81).
+        ref_desc(d, 2, counts);
+        d.material = ref_material(40.0, 0.4, 1, 1.01, 0.95, 10.0);
+        Shape s = sphere(0.2, 0.5, 0, 0.1, 0.5f, 0.f, 0.f);
+        s.count = 1;
+        s.mass_density = 2.0;
+        sp.shapes.push_back(s);
+    } else if (name == "ref2d_test15") {
+        // Reference test 15: 2000 particles in the box [0.4,0.6]^2, V=(-2,0.1), mass
+        // density 2, E=40, nu=0.2, yields 1.01 / 0.95, xi=10, no planes
+        // (mpm_2d/Standard_Tests/Standard_Tests.h:86-109).
+        ref_desc(d, 2, counts);
+        d.material = ref_material(40.0, 0.2, 1, 1.01, 0.95, 10.0);
+        Shape s = box2(0.4, 0.4, 0.6, 0.6);
+        s.v[0] = -2.f;
+        s.v[1] = 0.1f;
+        s.count = 2000;
+        s.mass_density = 2.0;
+        sp.shapes.push_back(s);
+    } else if (name == "ref2d_test16") {
+        // Reference test 16, snowball hitting the ground: a ball r=0.1 at (0.5,0.5) with
+        // V=(0,-1.5), 2000 particles, mass density 2, E=50, nu=0.45, yields 1.005 / 0.975,
+        // xi=10; below it a layer [0,1]x[0.1,0.125] of 1000 particles at rest, mass density
+        // 1, E=5, nu=0.4, yields 1.001 / 0.975, xi=5 (the second material); no planes
+        // (mpm_2d/Standard_Tests/Standard_Tests.h:111-155).
+        ref_desc(d, 2, counts);
+        d.material = ref_material(50.0, 0.45, 1, 1.005, 0.975, 10.0);
+        d.nmaterials = 2;
+        d.extra_materials[0] = ref_material(5.0, 0.4, 1, 1.001, 0.975, 5.0);
+        Shape ball = sphere(0.5, 0.5, 0, 0.1, 0.f, -1.5f, 0.f);
+        ball.count = 2000;
+        ball.mass_density = 2.0;
+        Shape layer = box2(0.0, 0.1, 1.0, 0.125);
+        layer.count = 1000;
+        layer.mass_density = 1.0;
+        layer.material = 1;
+        sp.shapes.push_back(ball);
+        sp.shapes.push_back(layer);
+    } else if (name == "ref2d_test17") {
+        // Reference test 17, snowball hitting the wall: a ball r=0.1 at (0.5,0.5) with
+        // V=(-2,0.1), 2000 particles, mass density 2, E=30, nu=0.4, yields 1.01 / 0.95,
+        // xi=10; a wall x=0.1 and the ground y=0.1
+        // (mpm_2d/Standard_Tests/Standard_Tests.h:157-188).
+        ref_desc(d, 2, counts);
+        d.material = ref_material(30.0, 0.4, 1, 1.01, 0.95, 10.0);
+        add_plane(d, 1, 0, 0, 0.1, 0, 0);
+        add_plane(d, 0, 1, 0, 0, 0.1, 0);
+        Shape s = sphere(0.5, 0.5, 0, 0.1, -2.f, 0.1f, 0.f);
+        s.count = 2000;
+        s.mass_density = 2.0;
+        sp.shapes.push_back(s);
+    } else if (name == "ref2d_test19" || name == "ref3d_test19") {
+        // Reference test 19, fluid: a ball r=0.1 at the centre with V=(0,-1), mass density
+        // 2, every particle an EOS fluid particle with K=1, gamma=7, density 1 (E and nu are
+        // set but unused); 2D: 2000 particles in a box of ground, ceiling and side walls
+        // (mpm_2d/Standard_Tests/Standard_Tests.h:248-283); 3D: 20000 particles, the
+        // ground and four side walls (mpm_3d/Standard_Tests/Standard_Tests.h:113-150).
+        const int dim = name == "ref3d_test19" ? 3 : 2;
+        ref_desc(d, dim, counts);
+        d.material = dim == 3 ? ref_material(30.0, 0.4, 0, 1.0, 1.0, 0.0) : ref_material(40.0, 0.2, 0, 1.0, 1.0, 0.0);
+        d.material.eos = 1;
+        d.material.bulk_modulus = 1.0f;
+        add_plane(d, 0, 1, 0, 0, 0.1, 0);
+        if (dim == 2) {
+            add_plane(d, 0, -1, 0, 0, 0.9, 0);
+            add_plane(d, 1, 0, 0, 0.1, 0, 0);
+            add_plane(d, -1, 0, 0, 0.9, 0, 0);
+        } else {
+            add_plane(d, 1, 0, 0, 0.1, 0, 0);
+            add_plane(d, -1, 0, 0, 0.9, 0, 0);
+            add_plane(d, 0, 0, 1, 0, 0, 0.1);
+            add_plane(d, 0, 0, -1, 0, 0, 0.9);
+        }
+        Shape s = sphere(0.5, 0.5, dim == 3 ? 0.5 : 0.0, 0.1, 0.f, -1.f, 0.f);
+        s.count = dim == 3 ? 20000 : 2000;
+        s.mass_density = 2.0;
+        sp.shapes.push_back(s);
     } else if (name == "blob" || name == "slab") {
         // Seeded random snow blob (sphere) or slab (box) for batched and served scenes.
         base_desc(d, 3, counts > 0 ? counts : 64);
@@ -262,25 +394,32 @@ bool build_spec(const std::string& name, int counts, uint64_t seed, int64_t targ
     } else {
         return false;
     }
-    elastic_dt_bound(d, sp.fixed_count > 0 ? sp.fixed_mass_density : sp.density);
+    double rho = sp.density;
+    if (sp.fixed()) {
+        rho = 1e30;
+        for (const Shape& s : sp.shapes) rho = std::min(rho, s.mass_density);
+    }
+    elastic_dt_bound(d, rho);
     return true;
 }
 
 // Generates particles; if x == nullptr only counts.
-int64_t generate(const Spec& sp, uint64_t seed, int64_t cap, float* x, float* v, float* mass) {
+int64_t generate(const Spec& sp, uint64_t seed, int64_t cap, float* x, float* v, float* mass,
+                 int32_t* material = nullptr) {
     const mpm_scene_desc& d = sp.desc;
     const int dim = d.dim;
     const int n = d.counts;
     const double dx = 1.0 / n;
     int64_t count = 0;
-    auto emit = [&](const double* p, const float* vel, float m) {
+    auto emit = [&](const double* p, const Shape& s, float m) {
         if (x && count < cap) {
             for (int a = 0; a < dim; ++a) {
                 x[count * dim + a] = (float)p[a];
-                v[count * dim + a] = vel[a];
+                v[count * dim + a] = s.v[a];
             }
             mass[count] = m;
         }
+        if (material && count < cap) material[count] = s.material;
         ++count;
     };
     if (sp.target_n > 0) {
@@ -305,25 +444,28 @@ int64_t generate(const Spec& sp, uint64_t seed, int64_t cap, float* x, float* v,
                     p[a] = lo[a] + (hi[a] - lo[a]) * unit_float(key3(seed ^ (si * 0x1000193ull), attempt, (uint64_t)a + 7));
                 ++attempt;
                 if (!s.inside(p, dim)) continue;
-                emit(p, s.v, m);
+                emit(p, s, m);
                 ++i;
             }
         }
         return count;
     }
-    if (sp.fixed_count > 0) {
+    if (sp.fixed()) {
         for (size_t si = 0; si < sp.shapes.size(); ++si) {
             const Shape& s = sp.shapes[si];
-            const double vol = dim == 3 ? 4.0 / 3.0 * M_PI * s.r * s.r * s.r : M_PI * s.r * s.r;
-            const float m = (float)(sp.fixed_mass_density * vol / sp.fixed_count);
+            const float m = (float)(s.mass_density * shape_volume(s, dim) / s.count);
+            double lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
+            s.bounds(dim, lo, hi);
             uint64_t attempt = 0;
-            for (int i = 0; i < sp.fixed_count;) {
+            for (int i = 0; i < s.count;) {
                 double p[3] = {0, 0, 0};
-                for (int a = 0; a < dim; ++a)
-                    p[a] = s.c[a] + s.r * (2.0 * unit_float(key3(seed ^ (si * 0x1000193ull), attempt, a)) - 1.0);
+                for (int a = 0; a < dim; ++a) {
+                    const double u = unit_float(key3(seed ^ (si * 0x1000193ull), attempt, a));
+                    p[a] = s.kind == SPHERE ? s.c[a] + s.r * (2.0 * u - 1.0) : lo[a] + (hi[a] - lo[a]) * u;
+                }
                 ++attempt;
                 if (!s.inside(p, dim)) continue;
-                emit(p, s.v, m);
+                emit(p, s, m);
                 ++i;
             }
         }
@@ -356,7 +498,7 @@ int64_t generate(const Spec& sp, uint64_t seed, int64_t cap, float* x, float* v,
                             p[a] = (ci[a] + (qi[a] + u) / sub) * dx;
                         }
                         if (!s.inside(p, dim)) continue;
-                        emit(p, s.v, m);
+                        emit(p, s, m);
                     }
                 }
     }
@@ -372,7 +514,7 @@ extern "C" int mpm_scene_preset_sized(const char* name, int32_t counts, uint64_t
     Spec sp;
     if (!build_spec(name, counts, seed, target_n, sp))
         return mpm_set_error(MPM_E_INVALID, std::string("unknown scene ") + name);
-    if (target_n > 0 && sp.fixed_count > 0)
+    if (target_n > 0 && sp.fixed())
         return mpm_set_error(MPM_E_INVALID, std::string("scene ") + name + " has a fixed particle count");
     *desc = sp.desc;
     if (n_particles) *n_particles = target_n > 0 ? target_n : generate(sp, seed, 0, nullptr, nullptr, nullptr);
@@ -402,4 +544,20 @@ extern "C" int mpm_scene_preset(const char* name, int32_t counts, uint64_t seed,
 extern "C" int mpm_scene_generate(const char* name, int32_t counts, uint64_t seed, int64_t cap, float* x, float* v,
                                   float* mass, int64_t* n_out) {
     return mpm_scene_generate_sized(name, counts, seed, 0, cap, x, v, mass, n_out);
+}
+
+extern "C" int mpm_scene_materials(const char* name, int32_t counts, uint64_t seed, int64_t target_n, int64_t cap,
+                                   int32_t* material, int64_t* n_out) {
+    if (!name || !material) return mpm_set_error(MPM_E_INVALID, "mpm_scene_materials: null argument");
+    mpm_scene_desc d;
+    int64_t n = 0;
+    int rc = mpm_scene_preset_sized(name, counts, seed, target_n, &d, &n);
+    if (rc) return rc;
+    if (n_out) *n_out = n;
+    if (n > cap) return mpm_set_error(MPM_E_CAPACITY, "mpm_scene_materials: capacity too small");
+    Spec sp;
+    build_spec(name, counts, seed, target_n, sp);
+    std::vector<float> x((size_t)n * d.dim), v((size_t)n * d.dim), m((size_t)n);
+    generate(sp, seed, cap, x.data(), v.data(), m.data(), material);
+    return MPM_OK;
 }

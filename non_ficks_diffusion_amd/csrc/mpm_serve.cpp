@@ -32,6 +32,7 @@ struct Request {
     mpm_scene_desc desc{};
     int64_t n = 0;
     std::vector<float> x, v, m;  // host copy until admitted
+    std::vector<int32_t> mat;    // material entries (empty: all 0)
     int frames_total = 0, frames_done = 0;
     int state = MPM_REQ_QUEUED;
     int entry = -1;
@@ -112,7 +113,8 @@ int admit(mpm_server* s, mpm_tick_stats& st) {
         Request& r = s->req[s->queue.front()];
         if (!fits(s, r)) break;
         int e = -1;
-        int rc = mpm_host::pool_attach(s->c, r.desc, r.n, r.x.data(), r.v.data(), r.m.data(), &e);
+        int rc = mpm_host::pool_attach(s->c, r.desc, r.n, r.x.data(), r.v.data(), r.m.data(),
+                                       r.mat.empty() ? nullptr : r.mat.data(), &e);
         if (rc == MPM_E_CAPACITY) break;  // fragmented slots or dead particles not yet compacted
         if (rc) return rc;
         r.entry = e;
@@ -121,6 +123,7 @@ int admit(mpm_server* s, mpm_tick_stats& st) {
         std::vector<float>().swap(r.x);
         std::vector<float>().swap(r.v);
         std::vector<float>().swap(r.m);
+        std::vector<int32_t>().swap(r.mat);
         s->by_entry[(size_t)e] = s->queue.front();
         s->particles_in_use += r.n;
         s->blocks_in_use += r.blocks;
@@ -199,6 +202,11 @@ int mpm_serve_arena(mpm_server_handle s, int64_t* particles, int64_t* slots, int
 
 int mpm_serve_submit(mpm_server_handle s, const mpm_scene_desc* desc, int64_t n, const float* x, const float* v,
                      const float* mass, int32_t frames, int64_t* id) {
+    return mpm_serve_submit_ex(s, desc, n, x, v, mass, nullptr, frames, id);
+}
+
+int mpm_serve_submit_ex(mpm_server_handle s, const mpm_scene_desc* desc, int64_t n, const float* x, const float* v,
+                        const float* mass, const int32_t* material, int32_t frames, int64_t* id) {
     if (!s || !desc || !x || !v || !mass || !id) return mpm_set_error(MPM_E_INVALID, "mpm_serve_submit: null argument");
     if (desc->dim != s->cfg.dim) return mpm_set_error(MPM_E_INVALID, "mpm_serve_submit: dimension differs from the server's");
     if (desc->counts < 4 || desc->counts > s->cfg.max_counts)
@@ -220,6 +228,13 @@ int mpm_serve_submit(mpm_server_handle s, const mpm_scene_desc* desc, int64_t n,
     r.x.assign(x, x + n * D);
     r.v.assign(v, v + n * D);
     r.m.assign(mass, mass + n);
+    if (material) {
+        const int nmat = std::max(1, (int)desc->nmaterials);
+        for (int64_t i = 0; i < n; ++i)
+            if (material[i] < 0 || material[i] >= nmat)
+                return mpm_set_error(MPM_E_INVALID, "mpm_serve_submit: particle material outside the scene's table");
+        r.mat.assign(material, material + n);
+    }
     r.submitted = s->now();
     const int64_t k = s->next_id++;
     s->req.emplace(k, std::move(r));

@@ -42,7 +42,10 @@ struct Vec {
 };
 
 // MPM_Particle (MPM_Particle.h:14-60) restricted to the state the substep
-// carries: position, velocity, mass, rest volume, elastic F and det(Fp).
+// carries: position, velocity, mass, rest volume, elastic F and det(Fp), and
+// for an EOS fluid particle its density.  The constitutive parameters and the
+// eos flag, bulk modulus and gamma live in the example's material table;
+// `material` names the particle's entry.
 template <class T, int d>
 struct MPM_Particle {
     bool valid = true;
@@ -50,6 +53,8 @@ struct MPM_Particle {
     T mass = 0, volume = 0;
     T Fe[d * d];
     T Jp = 1;
+    int material = 0;
+    T density = 1;
     MPM_Particle() { for (int i = 0; i < d * d; ++i) Fe[i] = (i % (d + 1) == 0) ? T(1) : T(0); }
 };
 
@@ -91,7 +96,8 @@ class MPM_Example {
     T flip = (T)0.9;
     T solver_tolerance = (T)1e-7;
     int solver_iterations = 10000;
-    mpm_material material{};
+    mpm_material material{};                  // table entry 0
+    std::vector<mpm_material> extra_materials;  // entries 1.. (at most MPM_MAX_MATERIALS - 1)
     T min_dt = (T)1e-6, max_dt = (T)5e-3;
     int project_mode = 1, crop_ghost_x = 1;
 
@@ -140,6 +146,9 @@ class MPM_Example {
             desc.planes[i].sticky = barriers[i].sticky ? 1 : 0;
         }
         desc.material = material;
+        desc.nmaterials = 1 + (int)extra_materials.size();
+        for (size_t k = 0; k < extra_materials.size() && k + 1 < MPM_MAX_MATERIALS; ++k)
+            desc.extra_materials[k] = extra_materials[k];
         desc.crop_ghost_x = crop_ghost_x;
         desc.project_mode = project_mode;
         return desc;
@@ -231,6 +240,8 @@ class MPM_Example {
         for (auto& p : particles) {
             g.write((const char*)p.Fe, sizeof(T) * d * d);
             g.write((const char*)&p.Jp, sizeof(T));
+            g.write((const char*)&p.material, sizeof(int));
+            g.write((const char*)&p.density, sizeof(T));
         }
         Write_Hierarchy(frame);
     }
@@ -284,8 +295,8 @@ class MPM_Example {
         std::ofstream ff(dir + "/flags", std::ios::binary);
         ff.write((const char*)flags.data(), (std::streamsize)((size_t)nblocks * 64 * sizeof(uint32_t)));
     }
-    // Read_Output_Files (MPM_Example.cpp:1075-1079); restores F and Jp too when
-    // the state file exists.
+    // Read_Output_Files (MPM_Example.cpp:1075-1079); restores F, Jp, the
+    // material entry and the density too when the state file exists.
     void Read_Output_Files(int frame) {
         const std::string dir = output_directory + "/" + std::to_string(frame);
         std::ifstream f(dir + "/particles", std::ios::binary);
@@ -307,6 +318,8 @@ class MPM_Example {
                 for (auto& p : particles) {
                     g.read((char*)p.Fe, sizeof(T) * d * d);
                     g.read((char*)&p.Jp, sizeof(T));
+                    g.read((char*)&p.material, sizeof(int));
+                    g.read((char*)&p.density, sizeof(T));
                 }
         }
         mpm_scene_desc desc = Scene_Description();
@@ -327,36 +340,47 @@ class MPM_Example {
 
     void Stage(int stage, T dt) { check(mpm_run_stage(handle, stage, (float)dt), "mpm_run_stage"); }
 
+    bool Is_EOS(int entry) const {
+        return entry == 0 ? material.eos != 0 : extra_materials.at((size_t)entry - 1).eos != 0;
+    }
+    // The device keeps an EOS particle's density in its Jp slot.
     void Upload() {
         const size_t n = particles.size();
         std::vector<float> x(n * d), v(n * d), F(n * d * d), Jp(n), m(n), vol(n);
+        std::vector<int32_t> mat(n);
         bool have_volume = n > 0;
         for (size_t i = 0; i < n; ++i) {
             const T_Particle& p = particles[i];
             for (int a = 0; a < d; ++a) { x[i * d + a] = (float)p.X(a); v[i * d + a] = (float)p.V(a); }
             for (int k = 0; k < d * d; ++k) F[i * d * d + k] = (float)p.Fe[k];
-            Jp[i] = (float)p.Jp;
+            mat[i] = p.material;
+            Jp[i] = (float)(Is_EOS(p.material) ? p.density : p.Jp);
             m[i] = (float)p.mass;
             vol[i] = (float)p.volume;
             have_volume = have_volume && p.volume > 0;
         }
-        check(mpm_upload_particles(handle, (int64_t)n, x.data(), v.data(), F.data(), Jp.data(), m.data(),
-                                   have_volume ? vol.data() : nullptr), "mpm_upload_particles");
+        check(mpm_upload_particles_ex(handle, (int64_t)n, x.data(), v.data(), F.data(), Jp.data(), m.data(),
+                                      have_volume ? vol.data() : nullptr, mat.data()),
+              "mpm_upload_particles");
         host_stale = false;
     }
     void Download() {
         if (!host_stale) return;
         const size_t cap = particles.size();
         std::vector<float> x(cap * d), v(cap * d), F(cap * d * d), Jp(cap), m(cap), vol(cap);
+        std::vector<int32_t> mat(cap);
         int64_t n = 0;
-        check(mpm_download_particles(handle, (int64_t)cap, x.data(), v.data(), F.data(), Jp.data(), m.data(),
-                                     vol.data(), nullptr, &n), "mpm_download_particles");
+        check(mpm_download_particles_ex(handle, (int64_t)cap, x.data(), v.data(), F.data(), Jp.data(), m.data(),
+                                        vol.data(), nullptr, mat.data(), &n),
+              "mpm_download_particles");
         particles.assign((size_t)n, T_Particle());
         for (size_t i = 0; i < (size_t)n; ++i) {
             T_Particle& p = particles[i];
             for (int a = 0; a < d; ++a) { p.X(a) = x[i * d + a]; p.V(a) = v[i * d + a]; }
             for (int k = 0; k < d * d; ++k) p.Fe[k] = F[i * d * d + k];
-            p.Jp = Jp[i];
+            p.material = mat[i];
+            if (Is_EOS(p.material)) p.density = Jp[i];
+            else p.Jp = Jp[i];
             p.mass = m[i];
             p.volume = vol[i];
         }

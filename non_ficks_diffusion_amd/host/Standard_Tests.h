@@ -6,8 +6,13 @@
 // mpm_scene_generate), so the executable, the Python tests and the
 // benchmark run the same scenes.
 //
+// test 1  (2D)   one plastic particle moving right
+// test 15 (2D)   a plastic square thrown left
+// test 16 (2D)   a snowball falling on a softer snow layer (two materials)
+// test 17 (2D)   a snowball thrown at a wall above the ground
 // test 18        two plastic balls (E 30, nu 0.4) thrown at each other,
 //                ground y = 0.1 (3D and 2D test 18 of the reference)
+// test 19        an EOS fluid ball falling into a box (3D and 2D)
 // test 100       the benchmark scenes: 3D two snowballs head-on,
 //                2D snow disk dropped on the ground
 // test 101       3D snow slab on a 30-degree incline (avalanche)
@@ -30,6 +35,9 @@ class Standard_Tests : public MPM_Example<T, d> {
 
     static std::string Preset_Name(int test_case) {
         if (test_case == 18) return d == 3 ? "ref3d_test18" : "ref2d_test18";
+        if (test_case == 19) return d == 3 ? "ref3d_test19" : "ref2d_test19";
+        if (d == 2 && (test_case == 1 || test_case == 15 || test_case == 16 || test_case == 17))
+            return "ref2d_test" + std::to_string(test_case);
         if (test_case == 100) return d == 3 ? "two_balls" : "disk2d";
         if (test_case == 101 && d == 3) return "avalanche";
         return "";
@@ -51,13 +59,22 @@ class Standard_Tests : public MPM_Example<T, d> {
         int64_t got = 0;
         check(mpm_scene_generate(name.c_str(), this->counts, seed, n, x.data(), v.data(), m.data(), &got),
               "mpm_scene_generate");
+        std::vector<int32_t> mat((size_t)n);
+        check(mpm_scene_materials(name.c_str(), this->counts, seed, 0, n, mat.data(), &got), "mpm_scene_materials");
+        this->material = desc.material;
+        this->extra_materials.assign(desc.extra_materials,
+                                     desc.extra_materials + std::max(0, (int)desc.nmaterials - 1));
         this->particles.assign((size_t)got, T_Particle());
         for (size_t i = 0; i < (size_t)got; ++i) {
+            T_Particle& p = this->particles[i];
             for (int a = 0; a < d; ++a) {
-                this->particles[i].X(a) = x[i * d + a];
-                this->particles[i].V(a) = v[i * d + a];
+                p.X(a) = x[i * d + a];
+                p.V(a) = v[i * d + a];
             }
-            this->particles[i].mass = m[i];
+            p.mass = m[i];
+            p.material = mat[i];
+            const mpm_material& pm = mat[i] == 0 ? desc.material : desc.extra_materials[mat[i] - 1];
+            p.density = pm.density;
         }
         this->barriers.clear();
         for (int k = 0; k < desc.nplanes; ++k) {
@@ -69,7 +86,6 @@ class Standard_Tests : public MPM_Example<T, d> {
         this->flip = desc.flip;
         this->solver_tolerance = desc.cg_tolerance;
         this->solver_iterations = desc.cg_max_iterations;
-        this->material = desc.material;
         this->min_dt = desc.min_dt;
         this->max_dt = desc.max_dt;
         this->frame_rate = desc.frame_rate;

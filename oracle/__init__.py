@@ -26,6 +26,7 @@ _SIG = {
     "oracle_create": (_H, [ctypes.POINTER(SceneDesc), ctypes.c_int32, ctypes.c_int32, ctypes.c_int32]),
     "oracle_destroy": (None, [_H]),
     "oracle_set_particles": (None, [_H, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP]),
+    "oracle_set_particles_ex": (None, [_H, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, ctypes.POINTER(ctypes.c_int32)]),
     "oracle_num_particles": (ctypes.c_int64, [_H]),
     "oracle_num_simulated": (ctypes.c_int64, [_H]),
     "oracle_get_particles": (None, [_H, _DP, _DP, _DP, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_uint8)]),
@@ -102,9 +103,14 @@ class Oracle:
         except Exception:
             pass
 
-    def set_particles(self, x, v, mass, F=None, Jp=None, volume=None):
+    def set_particles(self, x, v, mass, F=None, Jp=None, volume=None, material=None):
+        """material: entry of the scene's material table per particle (None: all 0);
+        an EOS particle's Jp is its density (None: its material's)."""
         x, v, mass, F, Jp, volume = map(_f32, (x, v, mass, F, Jp, volume))
-        lib().oracle_set_particles(self._h, mass.shape[0], _fp(x), _fp(v), _fp(F), _fp(Jp), _fp(mass), _fp(volume))
+        mat = None if material is None else np.ascontiguousarray(material, dtype=np.int32)
+        mp = None if mat is None else mat.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        lib().oracle_set_particles_ex(self._h, mass.shape[0], _fp(x), _fp(v), _fp(F), _fp(Jp), _fp(mass), _fp(volume), mp)
+        self._keep_mat = mat
 
     def particles(self, only_valid: bool = True) -> Dict[str, np.ndarray]:
         n = lib().oracle_num_particles(self._h)

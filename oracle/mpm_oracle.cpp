@@ -423,6 +423,10 @@ struct Particle {
     T weights[3][d], dweights[3][d];  // weights(k, axis)
     Model<T, d> model;
     Mat<T, d> scp;
+    // EOS fluid particle (MPM_Particle.h:33-37, defaults :51-56)
+    bool eos = false;
+    T density = 1, bulk_modulus = T(1e7), gamma = 7;
+    Mat<T, d> eos_scp;
 
     // Compute_Weights (MPM_Particle.h:90-100)
     void Compute_Weights(T x, const T one_over_dx, const int k) {
@@ -497,7 +501,7 @@ struct Interval {
 struct OracleBase {
     virtual ~OracleBase() {}
     virtual void set_particles(int64_t n, const float* x, const float* v, const float* F, const float* Jp,
-                               const float* mass, const float* volume) = 0;
+                               const float* mass, const float* volume, const int32_t* material) = 0;
     virtual int64_t num_particles() const = 0;
     virtual int64_t num_simulated() const = 0;
     virtual void get_particles(double* x, double* v, double* F, double* Jp, double* mass, double* volume,
@@ -549,24 +553,34 @@ struct Oracle : OracleBase {
     }
 
     // ---- particles --------------------------------------------------------
+    // A Standard_Tests scene sets each particle's constitutive model and EOS
+    // fields (mpm_2d/Standard_Tests/Standard_Tests.h:55-283); here every
+    // particle names an entry of the scene's material table.  An EOS
+    // particle's Jp input is its density (its material's density if Jp is NULL).
     void set_particles(int64_t n, const float* x, const float* v, const float* F, const float* Jp,
-                       const float* mass, const float* volume) override {
+                       const float* mass, const float* volume, const int32_t* material) override {
         particles.assign((size_t)n, Particle<T, d>());
         for (int64_t i = 0; i < n; ++i) {
             Particle<T, d>& p = particles[(size_t)i];
             for (int a = 0; a < d; ++a) { p.X(a) = (T)x[i * d + a]; p.V(a) = (T)v[i * d + a]; }
             p.mass = (T)mass[i];
             p.volume = volume ? (T)volume[i] : T(0);
+            const int id = material ? material[i] : 0;
+            const mpm_material& mat = id == 0 ? desc.material : desc.extra_materials[id - 1];
             Model<T, d>& m = p.model;
-            m.mu0 = (T)desc.material.mu0;
-            m.lambda0 = (T)desc.material.lambda0;
-            m.plastic = desc.material.plastic != 0;
-            m.stretching_yield = (T)desc.material.stretching_yield;
-            m.compression_yield = (T)desc.material.compression_yield;
-            m.hardening_factor = (T)desc.material.hardening_factor;
-            m.hardening_max = (T)desc.material.hardening_max;
+            m.mu0 = (T)mat.mu0;
+            m.lambda0 = (T)mat.lambda0;
+            m.plastic = mat.plastic != 0;
+            m.stretching_yield = (T)mat.stretching_yield;
+            m.compression_yield = (T)mat.compression_yield;
+            m.hardening_factor = (T)mat.hardening_factor;
+            m.hardening_max = (T)mat.hardening_max;
             if (F) for (int a = 0; a < d; ++a) for (int b = 0; b < d; ++b) m.Fe(a, b) = (T)F[(i * d + a) * d + b];
-            const T jp = Jp ? (T)Jp[i] : T(1);
+            p.eos = mat.eos != 0;
+            p.bulk_modulus = (T)mat.bulk_modulus;
+            p.gamma = (T)mat.gamma;
+            p.density = p.eos ? (Jp ? (T)Jp[i] : (T)mat.density) : T(1);
+            const T jp = Jp && !p.eos ? (T)Jp[i] : T(1);
             m.Jp = jp;
             m.Fp = TM::Identity();
             m.Fp(0, 0) = jp;  // only det(Fp) is ever read (MPM_Constitutive_Model.cpp:42)
@@ -588,7 +602,7 @@ struct Oracle : OracleBase {
                 if (v) v[i * d + a] = (double)p.V(a);
             }
             if (F) for (int a = 0; a < d; ++a) for (int b = 0; b < d; ++b) F[(i * d + a) * d + b] = (double)p.model.Fe(a, b);
-            if (Jp) Jp[i] = jp_scalar ? (double)p.model.Jp : (double)Determinant(p.model.Fp);
+            if (Jp) Jp[i] = p.eos ? (double)p.density : jp_scalar ? (double)p.model.Jp : (double)Determinant(p.model.Fp);
             if (mass) mass[i] = (double)p.mass;
             if (volume) volume[i] = (double)p.volume;
             if (valid) valid[i] = p.valid ? 1 : 0;
@@ -783,8 +797,10 @@ struct Oracle : OracleBase {
     // Update_Constitutive_Model_State (MPM_Example.cpp:563-571)
     void Update_Constitutive_Model_State() {
 #pragma omp parallel for num_threads(threads) schedule(static)
-        for (long i = 0; i < (long)simulated_particles.size(); ++i)
-            Precompute(particles[(size_t)simulated_particles[(size_t)i]].model, svd_fp64, jp_scalar);
+        for (long i = 0; i < (long)simulated_particles.size(); ++i) {
+            Particle<T, d>& p = particles[(size_t)simulated_particles[(size_t)i]];
+            if (!p.eos) Precompute(p.model, svd_fp64, jp_scalar);
+        }
     }
 
     // Apply_Explicit_Force (MPM_Example.cpp:757-790) + Explicit_Force_Helper (Explicit_Force_Helper.h:43-56)
@@ -793,8 +809,11 @@ struct Oracle : OracleBase {
         for (int a = 0; a < d; ++a) gravity(a) = (T)desc.gravity[a];
         slab_scatter([&](Particle<T, d>& p, int xlo, int xhi) {
             const T V0 = p.volume;
-            const TM Pm = P(p.model), F = p.model.Fe;
-            const TM V0_P_FT = Pm.Times_Transpose(F) * V0;
+            // EOS: f += V0/rho K rho^(gamma-1) grad w (MPM_Example.cpp:774-782)
+            T eos_coefficient = T(0);
+            TM V0_P_FT;
+            if (p.eos) eos_coefficient = V0 / p.density * p.bulk_modulus * std::pow(p.density, p.gamma - T(1));
+            else V0_P_FT = P(p.model).Times_Transpose(p.model.Fe) * V0;
             for_stencil<d>(xlo, xhi, [&](const int* idx) {
                 int c[3];
                 cell_of(p, idx, c);
@@ -803,8 +822,12 @@ struct Oracle : OracleBase {
                 const T w = p.Weight(idx);
                 TV body_force;
                 for (int a = 0; a < d; ++a) body_force(a) = gravity(a) * p.mass * w;
-                const TV inner_force = V0_P_FT * weight_gradient;
-                for (int a = 0; a < d; ++a) n.ch[CH_F + a] -= inner_force(a);
+                if (p.eos) {
+                    for (int a = 0; a < d; ++a) n.ch[CH_F + a] += eos_coefficient * weight_gradient(a);
+                } else {
+                    const TV inner_force = V0_P_FT * weight_gradient;
+                    for (int a = 0; a < d; ++a) n.ch[CH_F + a] -= inner_force(a);
+                }
                 for (int a = 0; a < d; ++a) n.ch[CH_F + a] += body_force(a);
             });
         });
@@ -854,6 +877,21 @@ struct Oracle : OracleBase {
 #pragma omp parallel for num_threads(threads) schedule(static)
         for (long i = 0; i < (long)simulated_particles.size(); ++i) {
             Particle<T, d>& p = particles[(size_t)simulated_particles[(size_t)i]];
+            if (p.eos) {
+                // eos_scp = V0 K gamma rho^(gamma-1) (sum grad w . x) (MPM_CG_System.h:169-174);
+                // the matrix-plus-scalar is taken as adding to the diagonal
+                T s = T(0);
+                for_stencil<d>(-1, 1, [&](const int* idx) {
+                    int c[3];
+                    cell_of(p, idx, c);
+                    const Node<T>& n = node(c);
+                    const TV g = p.Weight_Gradient(idx);
+                    for (int a = 0; a < d; ++a) s += g(a) * n.ch[x_ch + a];
+                });
+                const T coefficient = p.volume * p.bulk_modulus * p.gamma * std::pow(p.density, p.gamma - T(1));
+                p.eos_scp = TM::Identity() * (s * coefficient);
+                continue;
+            }
             TM tmp;
             for_stencil<d>(-1, 1, [&](const int* idx) {
                 int c[3];
@@ -874,7 +912,8 @@ struct Oracle : OracleBase {
                 int c[3];
                 cell_of(p, idx, c);
                 Node<T>& n = node(c);
-                const TV tmp_vec = p.scp.Transpose_Times(p.Weight_Gradient(idx));
+                // EOS: eos_scp grad w (MPM_CG_System.h:205-207)
+                const TV tmp_vec = p.eos ? p.eos_scp * p.Weight_Gradient(idx) : p.scp.Transpose_Times(p.Weight_Gradient(idx));
                 for (int a = 0; a < d; ++a) n.ch[f_ch + a] += tmp_vec(a);
             });
         });
@@ -1078,7 +1117,9 @@ struct Oracle : OracleBase {
                     grad_Vp += TM::Outer_Product(V_grid, p.Weight_Gradient(idx));
                 }
             });
-            p.model.Fe += grad_Vp * dt * p.model.Fe;
+            // MPM_Example.cpp:645-647
+            if (p.eos) p.density /= T(1) + dt * grad_Vp.Trace();
+            else p.model.Fe += grad_Vp * dt * p.model.Fe;
             for (int a = 0; a < d; ++a) {
                 p.V(a) = V_flip(a) * flip + V_pic(a) * (T(1) - flip);
                 p.X(a) += V_pic(a) * dt;
@@ -1336,7 +1377,12 @@ oracle_handle oracle_create(const mpm_scene_desc* desc, int32_t precision, int32
 void oracle_destroy(oracle_handle h) { delete h; }
 void oracle_set_particles(oracle_handle h, int64_t n, const float* x, const float* v, const float* F,
                           const float* Jp, const float* mass, const float* volume) {
-    h->impl->set_particles(n, x, v, F, Jp, mass, volume);
+    h->impl->set_particles(n, x, v, F, Jp, mass, volume, nullptr);
+}
+
+void oracle_set_particles_ex(oracle_handle h, int64_t n, const float* x, const float* v, const float* F,
+                             const float* Jp, const float* mass, const float* volume, const int32_t* material) {
+    h->impl->set_particles(n, x, v, F, Jp, mass, volume, material);
 }
 int64_t oracle_num_particles(oracle_handle h) { return h->impl->num_particles(); }
 int64_t oracle_num_simulated(oracle_handle h) { return h->impl->num_simulated(); }

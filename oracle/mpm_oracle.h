@@ -68,6 +68,11 @@ void oracle_destroy(oracle_handle h);
  * F, Jp may be NULL (identity, 1). */
 void oracle_set_particles(oracle_handle h, int64_t n, const float* x, const float* v,
                           const float* F, const float* Jp, const float* mass, const float* volume);
+/* The same with each particle's entry in the scene's material table (NULL:
+ * all 0); an EOS particle's Jp is its density (NULL: the material's). */
+void oracle_set_particles_ex(oracle_handle h, int64_t n, const float* x, const float* v,
+                             const float* F, const float* Jp, const float* mass,
+                             const float* volume, const int32_t* material);
 int64_t oracle_num_particles(oracle_handle h);
 int64_t oracle_num_simulated(oracle_handle h);
 /* All particles in creation order; valid[i] = 0 for removed ones. */

@@ -89,3 +89,48 @@ def test_errors_are_status_codes():
     assert rc == -1 and b"dim" in M.lib().mpm_last_error()
     rc = M.lib().mpm_scene_preset(b"no_such_scene", 0, 0, ctypes.byref(M.SceneDesc()), None)
     assert rc == -1
+
+
+def test_reference_test_scenes_counts_mass_and_materials():
+    """The Standard_Tests scenes: particle count, mass = mass density * shape
+    area / count, velocity, planes and material table of each reference test
+    (mpm_2d/Standard_Tests/Standard_Tests.h:55-283,
+    mpm_3d/Standard_Tests/Standard_Tests.h:113-150)."""
+    area = np.pi * 0.1 ** 2
+    cases = {  # name: (count, mass of the first shape's particles, first velocity, planes)
+        "ref2d_test1": (1, 2 * area, (0.5, 0.0), 0),
+        "ref2d_test15": (2000, 2 * 0.04 / 2000, (-2.0, 0.1), 0),
+        "ref2d_test16": (3000, 2 * area / 2000, (0.0, -1.5), 0),
+        "ref2d_test17": (2000, 2 * area / 2000, (-2.0, 0.1), 2),
+        "ref2d_test19": (2000, 2 * area / 2000, (0.0, -1.0), 4),
+        "ref3d_test19": (20000, 2 * (4 / 3 * np.pi * 0.1 ** 3) / 20000, (0.0, -1.0, 0.0), 5),
+    }
+    for name, (count, m0, v0, planes) in cases.items():
+        d, x, v, m = M.scene_generate(name)
+        mat = M.scene_materials(name)
+        assert len(m) == count and len(mat) == count, name
+        assert abs(m[0] - m0) / m0 < 1e-6, name
+        assert np.allclose(v[0], v0), name
+        assert d.nplanes == planes, name
+        assert d.gravity[1] == np.float32(-2.0) and d.cg_max_iterations == 10000, name
+    # test 16: the ball (entry 0) then the layer (entry 1), mass density 1 over [0,1]x[0.1,0.125]
+    d, x, v, m = M.scene_generate("ref2d_test16")
+    mat = M.scene_materials("ref2d_test16")
+    assert d.nmaterials == 2 and list(np.bincount(mat)) == [2000, 1000]
+    layer = mat == 1
+    assert np.allclose(m[layer], 1 * 0.025 / 1000) and np.all(v[layer] == 0)
+    assert x[layer, 1].min() >= 0.1 and x[layer, 1].max() <= 0.125
+    E, nu = 5.0, 0.4
+    lay = d.materials()[1]
+    assert np.isclose(lay.mu0, E / (2 * (1 + nu))) and np.isclose(lay.lambda0, E * nu / ((1 + nu) * (1 - 2 * nu)))
+    assert np.isclose(lay.hardening_factor, 5.0) and np.isclose(lay.stretching_yield, 1.001)
+    # test 19: EOS fluid with K=1, gamma=7, initial density 1
+    for name in ("ref2d_test19", "ref3d_test19"):
+        d, _ = M.scene_preset(name)
+        assert d.material.eos == 1 and d.material.bulk_modulus == 1.0
+        assert d.material.gamma == 7.0 and d.material.density == 1.0
+    # a table entry outside [0, 8] is refused
+    d, _ = M.scene_preset("ref2d_test16")
+    d.nmaterials = 9
+    h = ctypes.c_void_p()
+    assert M.lib().mpm_create(ctypes.byref(d), 10, ctypes.byref(h)) == -1

@@ -27,9 +27,9 @@ def load(scenes):
     return out
 
 
-def run_alone(d, x, v, m, dts):
+def run_alone(d, x, v, m, dts, material=None):
     s = M.Scene(d, len(m))
-    s.upload(x, v, m)
+    s.upload(x, v, m, material=material)
     s.initialize()
     for dt in dts:
         s.substep(dt)
@@ -97,20 +97,24 @@ def test_batch_advance_to_time_matches_alone():
 
 
 def test_batch_2d_matches_scenes_alone():
-    """2D scenes (8^2 blocks) through the same batched kernels."""
-    scenes = [("disk2d", 64, 0, 0), ("disk2d", 48, 3, 800), ("ref2d_test18", 64, 0, 0)]
+    """2D scenes (8^2 blocks) through the same batched kernels, with the
+    two-material and EOS scenes among them (each scene its own table)."""
+    scenes = [("disk2d", 64, 0, 0), ("disk2d", 48, 3, 800), ("ref2d_test18", 64, 0, 0), ("ref2d_test16", 64, 0, 0),
+              ("ref2d_test19", 64, 0, 0)]
     data = [M.scene_generate(n, c, s, k) for n, c, s, k in scenes]
+    mats = [M.scene_materials(n, c, s, k) for n, c, s, k in scenes]
     b = M.Batch([d for d, _, _, _ in data], [len(m) for _, _, _, m in data])
     for k, (d, x, v, m) in enumerate(data):
-        b.upload(k, x, v, m)
+        b.upload(k, x, v, m, material=mats[k])
     b.initialize()
-    dts = [1e-3, 1e-3, 2e-3]
+    dts = [1e-3, 1e-3, 2e-3, 2e-3, 1e-3]
     for _ in range(4):
         b.substep(dts)
     for k, (d, x, v, m) in enumerate(data):
-        alone, _ = run_alone(d, x, v, m, [dts[k]] * 4)
+        alone, _ = run_alone(d, x, v, m, [dts[k]] * 4, material=mats[k])
         got = b.download(k)
         assert np.array_equal(got["ids"], alone["ids"]), k
+        assert np.array_equal(got["material"], mats[k][got["ids"]]), k
         for f in ("x", "v", "F", "Jp"):
             assert rel(got[f], alone[f]) < 1e-5, (scenes[k], f, rel(got[f], alone[f]))
     b.close()

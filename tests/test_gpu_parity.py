@@ -41,12 +41,13 @@ def rel_m(a, b, mass):
 
 def make(name, counts, precision=4, cg_max=None):
     d, x, v, m = M.scene_generate(name, counts, 0)
+    mat = M.scene_materials(name, counts, 0)
     if cg_max is not None:
         d.cg_max_iterations = cg_max
     s = M.Scene(d, len(m))
-    s.upload(x, v, m)
+    s.upload(x, v, m, material=mat)
     o = O.Oracle(d, precision, 4, MATCHED)
-    o.set_particles(x, v, m)
+    o.set_particles(x, v, m, material=mat)
     return d, x, v, m, s, o
 
 
@@ -152,7 +153,14 @@ def run_both_stages(s, o, dt):
     return errs
 
 
-@pytest.mark.parametrize("name,counts", [("two_balls", 32), ("disk2d", 64), ("avalanche", 64)])
+# the reference's other Standard_Tests scenes: one particle (2D test 1), a
+# square (15), two materials (16), a wall (17), EOS fluid (19, 2D and 3D)
+REF_TESTS = [("ref2d_test1", 64), ("ref2d_test15", 64), ("ref2d_test16", 64), ("ref2d_test17", 64),
+             ("ref2d_test19", 64), ("ref3d_test19", 32)]
+
+
+@pytest.mark.parametrize("name,counts", [("two_balls", 32), ("disk2d", 64), ("avalanche", 64), ("ref2d_test16", 64),
+                                         ("ref2d_test19", 64), ("ref3d_test19", 32)])
 def test_stage_by_stage(name, counts):
     d, x, v, m, s, o = make(name, counts, cg_max=20)
     errs = run_both_stages(s, o, 1e-3)
@@ -162,7 +170,8 @@ def test_stage_by_stage(name, counts):
 
 
 @pytest.mark.parametrize("name,counts,dt", [("two_balls", 32, 1e-3), ("disk2d", 64, 1e-3),
-                                            ("avalanche", 64, 1e-3), ("ref3d_test18", 32, 2e-3)])
+                                            ("avalanche", 64, 1e-3), ("ref3d_test18", 32, 2e-3)]
+                         + [(n, c, 2e-3) for n, c in REF_TESTS])
 def test_one_substep(name, counts, dt):
     d, x, v, m, s, o = make(name, counts)
     s.initialize()
@@ -207,7 +216,8 @@ def test_one_substep_full_avalanche_against_fp64_oracle():
         assert e < 1e-5, (k, e)
 
 
-@pytest.mark.parametrize("name,counts,dt,steps", [("disk2d", 64, 2e-3, 100), ("two_balls", 32, 1e-3, 100)])
+@pytest.mark.parametrize("name,counts,dt,steps", [("disk2d", 64, 2e-3, 100), ("two_balls", 32, 1e-3, 100),
+                                                  ("ref2d_test16", 64, 2e-3, 100), ("ref2d_test19", 64, 2e-3, 100)])
 def test_hundred_substeps(name, counts, dt, steps):
     d, x, v, m, s, o = make(name, counts)
     s.initialize()

@@ -18,15 +18,17 @@ needs_driver = pytest.mark.skipif(not os.path.exists(DRIVER), reason="mpm_driver
 
 def read_frame(path, d):
     """Frame files: size_t count, then per particle X[d] V[d] mass volume
-    (floats); `state` holds F[d*d] and Jp per particle."""
+    (floats); `state` holds F[d*d], Jp (float), the material entry (int32)
+    and the density (float) per particle."""
     raw = open(os.path.join(path, "particles"), "rb").read()
     n = int(np.frombuffer(raw[:8], np.uint64)[0])
     rec = np.frombuffer(raw[8:], np.float32).reshape(n, 2 * d + 2)
     st = open(os.path.join(path, "state"), "rb").read()
     assert int(np.frombuffer(st[:8], np.uint64)[0]) == n
-    srec = np.frombuffer(st[8:], np.float32).reshape(n, d * d + 1)
+    srec = np.frombuffer(st[8:], np.float32).reshape(n, d * d + 3)
     return {"x": rec[:, :d], "v": rec[:, d:2 * d], "mass": rec[:, 2 * d], "volume": rec[:, 2 * d + 1],
-            "F": srec[:, :d * d].reshape(n, d, d), "Jp": srec[:, d * d]}
+            "F": srec[:, :d * d].reshape(n, d, d), "Jp": srec[:, d * d],
+            "material": srec[:, d * d + 1].copy().view(np.int32), "density": srec[:, d * d + 2]}
 
 
 def run(args, timeout=300):
@@ -72,6 +74,38 @@ def test_driver_matches_library_frame(tmp_path):
     assert g["x"].shape == f1["x"].shape
     for k in ("x", "v", "F", "Jp", "volume"):
         assert _rel(f1[k], g[k]) < 1e-5, k
+
+
+@pytest.mark.gpu
+@needs_driver
+@pytest.mark.parametrize("dim,test,name,size", [(2, 16, "ref2d_test16", 64), (2, 19, "ref2d_test19", 64),
+                                               (3, 19, "ref3d_test19", 32)])
+def test_driver_material_scenes_match_library_frame(tmp_path, dim, test, name, size):
+    """Reference tests 16 (two materials) and 19 (EOS fluid) through the C++
+    MPM_Driver: the frame files carry each particle's material entry and, for
+    an EOS particle, its density, equal to the library's state after the
+    same frame."""
+    out = tmp_path / "run"
+    r = run(["-d", dim, "-test_number", test, "-size", size, "-last_frame", 1, "-o", out])
+    assert r.returncode == 0, r.stderr
+    f1 = read_frame(out / "1", dim)
+    desc, x, v, m = M.scene_generate(name, size, 0)
+    mat = M.scene_materials(name, size, 0)
+    s = M.Scene(desc, len(m))
+    s.upload(x, v, m, material=mat)
+    s.initialize()
+    s.advance_to_time(0.0, float(np.float32(1.0) / np.float32(desc.frame_rate)))
+    g = s.download()
+    s.close()
+    assert np.array_equal(f1["material"], g["material"])
+    eos = np.array([desc.materials()[k].eos for k in g["material"]], bool)
+    for k in ("x", "v", "F"):
+        assert _rel(f1[k], g[k]) < 1e-5, k
+    if eos.any():
+        assert _rel(f1["density"][eos], g["Jp"][eos]) < 1e-5
+        assert np.all(f1["density"][eos] != 1.0)
+    if (~eos).any():
+        assert _rel(f1["Jp"][~eos], g["Jp"][~eos]) < 1e-5
 
 
 @pytest.mark.gpu

@@ -7,6 +7,8 @@ expressions, library routines (numpy's SVD and linear solver), invariants and
 brute force on tiny inputs.  Each check is chosen so that a plausible slip
 (a dropped term, a wrong sign or index, a transposed operand) fails it.
 """
+import ctypes
+
 import numpy as np
 import pytest
 
@@ -476,3 +478,144 @@ def test_driver_dt_sequence():
     assert abs(t - 1.0 / 24) < 1e-6
     dt0 = max(d.min_dt, min(d.max_dt, d.cfl / d.counts / max(vmax, 1e-2)))
     assert n >= int(np.floor((1.0 / 24) / dt0 * 0.999)) - 1
+
+
+# --------------------------------------------------------------------------- EOS fluid particles
+def eos_scene(dim, n=8, K=3.0, gamma=7.0, seed=3):
+    """Particles alternating between a snow material (entry 0) and an EOS
+    fluid (entry 1) with densities around 1, given volumes, no gravity or
+    planes."""
+    d, x, v, m, F = tiny_scene(dim=dim, n=n, seed=seed, v_scale=0.2)
+    d.gravity[:] = (0.0, 0.0, 0.0)
+    d.nmaterials = 2
+    e = d.extra_materials[0]
+    ctypes.memmove(ctypes.byref(e), ctypes.byref(d.material), ctypes.sizeof(M.Material))
+    e.eos, e.bulk_modulus, e.gamma, e.density = 1, K, gamma, 1.0
+    mat = (np.arange(n) % 2).astype(np.int32)
+    rng = np.random.default_rng(seed + 1)
+    rho = (0.9 + 0.2 * rng.random(n)).astype(np.float32)
+    Jp = np.where(mat == 1, rho, 1.0).astype(np.float32)  # an EOS particle's Jp is its density
+    F[mat == 1] = np.eye(dim)
+    vol = np.full(n, 2e-5, np.float32)
+    return d, x, v, m, F, Jp, vol, mat
+
+
+def node_positions(n, dim):
+    g = np.indices((n + 2,) * dim).reshape(dim, -1).T
+    return ((g - 1 + 0.5) / n).reshape((n + 2,) * dim + (dim,))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_eos_explicit_force_closed_form(dim):
+    """Explicit force of EOS particles f_i = sum_p c_p grad w_ip with
+    c_p = V0/rho K rho^(gamma-1) (MPM_Example.cpp:774-782): since the
+    B-spline reproduces linear fields (sum_i grad w_ip x_i^T = I), the EOS
+    particles' part of sum_i f_i x_i^T is (sum_p c_p) I, and sum_i f_i = 0."""
+    d, x, v, m, F, Jp, vol, mat = eos_scene(dim)
+    o = O.Oracle(d, 8, 1, O.DOT_FP64 | O.SVD_FP64)
+    eos_only = mat == 1
+    o.set_particles(x[eos_only], v[eos_only], m[eos_only], F=F[eos_only], Jp=Jp[eos_only], volume=vol[eos_only],
+                    material=mat[eos_only])
+    for st in (M.STAGE_BIN, M.STAGE_P2G, M.STAGE_CONSTITUTIVE, M.STAGE_EXPLICIT_FORCE):
+        o.run_stage(st, 1e-3)
+    f = o.grid(M.CH_FORCE).reshape(-1, dim)
+    xi = node_positions(d.counts, dim).reshape(-1, dim)
+    rho = Jp[eos_only].astype(np.float64)
+    c = (vol[eos_only] / rho * 3.0 * rho ** 6).sum()
+    assert np.abs(f.sum(0)).max() < 1e-12
+    assert np.allclose(f.T @ xi, c * np.eye(dim), atol=1e-12 * c + 1e-15)
+    # the particles' state is untouched by the constitutive stage (no SVD, no plasticity)
+    p = o.particles()
+    assert np.allclose(p["F"], F[eos_only]) and np.allclose(p["Jp"], rho)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_eos_cg_force_closed_form(dim):
+    """CG force of EOS particles (MPM_CG_System.h:169-174, 205-207) on a
+    linear node field u_i = G x_i: T_p = sum_i grad w_ip u_i^T = G^T, so
+    f_i = sum_p c_p tr(G) grad w_ip with c_p = V0 K gamma rho^(gamma-1)
+    (the matrix-plus-scalar of the reference read as c tr(T) I) and
+    sum_i f_i x_i^T = tr(G) (sum_p c_p) I -- a multiple of I, which the
+    all-entries reading of the same line would not give."""
+    d, x, v, m, F, Jp, vol, mat = eos_scene(dim)
+    eos_only = mat == 1
+    o = O.Oracle(d, 8, 1, O.DOT_FP64 | O.SVD_FP64)
+    o.set_particles(x[eos_only], v[eos_only], m[eos_only], F=F[eos_only], Jp=Jp[eos_only], volume=vol[eos_only],
+                    material=mat[eos_only])
+    dt = 1e-3
+    for st in (M.STAGE_BIN, M.STAGE_P2G, M.STAGE_CONSTITUTIVE, M.STAGE_EXPLICIT_FORCE, M.STAGE_COLLISION):
+        o.run_stage(st, dt)
+    mass = o.grid(M.CH_MASS).reshape(-1)
+    G = RNG.standard_normal((dim, dim))
+    xi = node_positions(d.counts, dim).reshape(-1, dim)
+    u = xi @ G.T
+    q = o.apply_operator(dt, u.ravel()).reshape(-1, dim)
+    live = mass > 0
+    f = np.zeros_like(u)
+    f[live] = (q[live] - u[live]) * mass[live, None] / dt ** 2
+    rho = Jp[eos_only].astype(np.float64)
+    c = (vol[eos_only] * 3.0 * 7.0 * rho ** 6).sum()
+    assert np.abs(f.sum(0)).max() < 1e-9 * c
+    assert np.allclose(f.T @ xi, np.trace(G) * c * np.eye(dim), atol=1e-9 * c)
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_eos_and_snow_g2p_on_a_linear_field(dim):
+    """G2P with grid v = v* = a + B x_i (MPM_Example.cpp:636-660): an EOS
+    particle's density becomes rho / (1 + dt tr B) with F left as it was, a
+    snow particle's F becomes F + dt B F; each particle follows its own
+    material entry."""
+    d, x, v, m, F, Jp, vol, mat = eos_scene(dim, n=10)
+    a = RNG.standard_normal(dim) * 0.2
+    B = 0.5 * RNG.standard_normal((dim, dim))
+    o = O.Oracle(d, 8, 1, O.DOT_FP64 | O.SVD_FP64)
+    o.set_particles(x, v, m, F=F, Jp=Jp, volume=vol, material=mat)
+    o.run_stage(M.STAGE_BIN)
+    o.run_stage(M.STAGE_P2G)
+    field = a + node_positions(d.counts, dim) @ B.T
+    o.set_grid(M.CH_VELOCITY, field)
+    o.set_grid(M.CH_VELOCITY_STAR, field)
+    dt = 1e-3
+    p0 = o.particles()
+    o.run_stage(M.STAGE_G2P, dt)
+    p1 = o.particles()
+    eos = mat == 1
+    assert np.allclose(p1["Jp"][eos], p0["Jp"][eos] / (1 + dt * np.trace(B)), rtol=1e-12)
+    assert np.allclose(p1["F"][eos], p0["F"][eos], atol=0)
+    snow = ~eos
+    assert np.allclose(p1["F"][snow], p0["F"][snow] + dt * np.einsum("ij,pjk->pik", B, p0["F"][snow]), atol=1e-12)
+    assert np.allclose(p1["Jp"][snow], p0["Jp"][snow], atol=0)
+
+
+def test_material_entry_selects_the_particles_parameters():
+    """A particle of table entry k evolves exactly as the same particle in a
+    scene whose only material is entry k (a wrong index or a shared entry
+    fails: the two materials differ in every field)."""
+    d, x, v, m, F = tiny_scene(dim=3, n=12, v_scale=0.3, F_scale=0.05)
+    soft = M.Material()
+    ctypes.memmove(ctypes.byref(soft), ctypes.byref(d.material), ctypes.sizeof(M.Material))
+    soft.mu0 *= 0.3
+    soft.lambda0 *= 0.5
+    soft.stretching_yield = 1.002
+    soft.compression_yield = 0.99
+    soft.hardening_factor = 4.0
+    two = d.copy()
+    two.nmaterials = 2
+    two.extra_materials[0] = soft
+    one = d.copy()
+    one.material = soft
+    mat = np.ones(len(m), np.int32)
+    runs = []
+    for desc, material in ((two, mat), (one, None)):
+        o = O.Oracle(desc, 8, 1, O.DOT_FP64 | O.SVD_FP64)
+        o.set_particles(x, v, m, F=F, material=material)
+        o.initialize()
+        o.advance_step(2e-3)
+        runs.append(o.particles())
+    for k in ("x", "v", "F", "Jp"):
+        assert np.array_equal(runs[0][k], runs[1][k]), k
+    o = O.Oracle(two, 8, 1, O.DOT_FP64 | O.SVD_FP64)
+    o.set_particles(x, v, m, F=F)
+    o.initialize()
+    o.advance_step(2e-3)
+    assert not np.allclose(o.particles()["F"], runs[0]["F"], atol=1e-9)

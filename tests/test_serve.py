@@ -31,9 +31,9 @@ def load():
     return [(M.scene_generate(name, c, seed, n), frames) for name, c, seed, n, frames in REQUESTS]
 
 
-def run_alone(d, x, v, m, frames):
+def run_alone(d, x, v, m, frames, material=None):
     s = M.Scene(d, len(m))
-    s.upload(x, v, m)
+    s.upload(x, v, m, material=material)
     s.initialize()
     t, steps = 0.0, 0
     for f in range(frames):
@@ -150,13 +150,20 @@ def test_submit_refuses_what_can_never_run():
 
 
 def test_2d_requests_match_alone():
-    srv = M.Server(1 << 30, max_scenes=2, max_counts=64, dim=2, keep_results=True)
-    reqs = [(M.scene_generate("disk2d", 64, 0), 2), (M.scene_generate("disk2d", 32, 5, 600), 3)]
-    ids = [srv.submit(d, x, v, m, f) for (d, x, v, m), f in reqs]
+    """2D requests, among them the two-material and EOS scenes (reference 2D
+    tests 16 and 19, their CG cap lowered to the server's 200; their solves
+    converge well before it)."""
+    srv = M.Server(1 << 30, max_scenes=3, max_counts=64, dim=2, keep_results=True, cg_max_iterations=200)
+    reqs = [(M.scene_generate("disk2d", 64, 0), 2), (M.scene_generate("disk2d", 32, 5, 600), 3),
+            (M.scene_generate("ref2d_test16", 64, 0), 1), (M.scene_generate("ref2d_test19", 64, 0), 1)]
+    for (d, _, _, _), _ in reqs[2:]:
+        d.cg_max_iterations = 200
+    mats = [None, None, M.scene_materials("ref2d_test16", 64, 0), M.scene_materials("ref2d_test19", 64, 0)]
+    ids = [srv.submit(d, x, v, m, f, material=mt) for ((d, x, v, m), f), mt in zip(reqs, mats)]
     while any(srv.poll(i)["state"] != M.REQ_DONE for i in ids):
         srv.tick()
-    for ((d, x, v, m), frames), rid in zip(reqs, ids):
-        alone, steps = run_alone(d, x, v, m, frames)
+    for ((d, x, v, m), frames), rid, mt in zip(reqs, ids, mats):
+        alone, steps = run_alone(d, x, v, m, frames, material=mt)
         got = srv.download(rid)
         assert srv.poll(rid)["substeps"] == steps
         for f in ("x", "v", "F"):
