@@ -1101,12 +1101,88 @@ __device__ inline void scatter_force(const SceneParams& sc, const int* l, const 
 }
 
 // Constitutive update (MPM_Example.cpp:563-571) and explicit force P2G
-// (MPM_Example.cpp:757-784): f_i += -V0 P F^T grad w + g m w
+// (MPM_Example.cpp:757-784): f_i += -V0 P F^T grad w + g m w, for particle i
+// of material mat; an EOS particle has no constitutive model and scatters
+// (V0/rho) K rho^(gamma-1) grad w (MPM_Example.cpp:774-782).
 // ============================================================================
+template <int D, bool CONST, bool FORCE>
+__device__ __forceinline__ void particle_force_one(const KP& p, const SceneParams& sc, const Material& mat, int i,
+                                                   const int* bb, const int* l, float* mine) {
+    constexpr int DD = D * D;
+    float Jp = p.pcur[PF<D>::J * p.pcap + i];
+    const float V0 = p.pcur[PF<D>::VOL * p.pcap + i];
+    float A[DD];  // the particle's stress term: f_i -= A grad w
+    if (mat.eos) {
+        // EOS fluid (Jp slot = density): no constitutive model
+        const float rg = powf(Jp, mat.gamma - 1.0f);
+        if (CONST) {
+            float Xo[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) Xo[a] = p.pcur[(PF<D>::X + a) * p.pcap + i];
+            store_operator_eos<D>(p, i, V0 * mat.K * mat.gamma * rg, Xo);
+#pragma unroll
+            for (int k = 0; k < DD; ++k) p.rcache[k * p.pcap + i] = (k % (D + 1) == 0) ? 1.0f : 0.0f;
+        }
+        // f += (V0/rho) K rho^(gamma-1) grad w
+        const float coeff = V0 / Jp * mat.K * rg;
+#pragma unroll
+        for (int k = 0; k < DD; ++k) A[k] = (k % (D + 1) == 0) ? -coeff : 0.0f;
+    } else {
+        float F[DD], R[DD];
+#pragma unroll
+        for (int k = 0; k < DD; ++k) F[k] = p.pcur[(PF<D>::F + k) * p.pcap + i];
+        if (CONST) {
+            float U[DD];
+            double Sv[D];
+            plasticity<D>(F, Jp, R, mat, U, Sv);
+#pragma unroll
+            for (int k = 0; k < DD; ++k) {
+                p.pcur[(PF<D>::F + k) * p.pcap + i] = F[k];
+                p.rcache[k * p.pcap + i] = R[k];
+            }
+            p.pcur[PF<D>::J * p.pcap + i] = Jp;
+            StressState<D> so;
+            stress_state<D>(F, R, Jp, mat, so);
+            float Xo[D];
+#pragma unroll
+            for (int a = 0; a < D; ++a) Xo[a] = p.pcur[(PF<D>::X + a) * p.pcap + i];
+            store_operator<D>(p, i, so, V0, Xo, U, Sv);
+        } else {
+#pragma unroll
+            for (int k = 0; k < DD; ++k) R[k] = p.rcache[k * p.pcap + i];
+        }
+        if (FORCE) {
+            StressState<D> st;
+            stress_state<D>(F, R, Jp, mat, st);
+            float P[DD];
+            first_pk<D>(st, P);
+            // V0 P F^T
+#pragma unroll
+            for (int r = 0; r < D; ++r)
+#pragma unroll
+                for (int cc = 0; cc < D; ++cc) {
+                    float s = 0.0f;
+#pragma unroll
+                    for (int k = 0; k < D; ++k) s += P[r * D + k] * F[cc * D + k];
+                    A[r * D + cc] = s * V0;
+                }
+        }
+    }
+    if (FORCE) {
+        float w[D][3], dw[D][3], X[D];
+        if (!cell_particle<D>(p, sc, i, bb, l, w, dw, X)) return;
+        scatter_force<D>(sc, l, w, dw, A, p.pcur[PF<D>::M * p.pcap + i], mine);
+    }
+}
+
+// The particles of each block, cell by cell (lane = cell, waves interleave
+// a cell's particles), one pass per entry of the scene's material table, each
+// pass taking the particles of that entry (a single pass without a check in
+// a one-material scene): the material's parameters stay scalar operands.
 template <int D, bool CONST, bool FORCE>
 __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
     using Dm = Dims<D>;
-    constexpr int LTN = Dm::LTN, DD = D * D;
+    constexpr int LTN = Dm::LTN;
     __shared__ float tile[NW * D * LTN];
     const int nblocks = p.ctr->nblocks;
     const int c = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1124,76 +1200,12 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
             __syncthreads();
         }
         const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
-        for (int i = i0 + wv; i < i1; i += NW) {
-            float F[DD], R[DD];
-#pragma unroll
-            for (int k = 0; k < DD; ++k) F[k] = p.pcur[(PF<D>::F + k) * p.pcap + i];
-            float Jp = p.pcur[PF<D>::J * p.pcap + i];
-            const Material& mat = sc.mats[__float_as_int(p.pcur[PF<D>::MAT * p.pcap + i])];
-            if (mat.eos) {
-                // EOS fluid (Jp slot = density): no constitutive model
-                const double rho = (double)Jp, V0 = (double)p.pcur[PF<D>::VOL * p.pcap + i];
-                const double rg = pow(rho, (double)mat.gamma - 1.0);
-                if (CONST) {
-                    float Xo[D];
-#pragma unroll
-                    for (int a = 0; a < D; ++a) Xo[a] = p.pcur[(PF<D>::X + a) * p.pcap + i];
-                    store_operator_eos<D>(p, i, (float)(V0 * (double)mat.K * (double)mat.gamma * rg), Xo);
-#pragma unroll
-                    for (int k = 0; k < DD; ++k) p.rcache[k * p.pcap + i] = (k % (D + 1) == 0) ? 1.0f : 0.0f;
-                }
-                if (FORCE) {
-                    float w[D][3], dw[D][3], X[D];
-                    if (!cell_particle<D>(p, sc, i, bb, l, w, dw, X)) continue;
-                    // f += (V0/rho) K rho^(gamma-1) grad w: the stress scatter below with A = -coeff I
-                    const float coeff = (float)(V0 / rho * (double)mat.K * rg);
-                    float A[DD];
-#pragma unroll
-                    for (int k = 0; k < DD; ++k) A[k] = (k % (D + 1) == 0) ? -coeff : 0.0f;
-                    scatter_force<D>(sc, l, w, dw, A, p.pcur[PF<D>::M * p.pcap + i], mine);
-                }
-                continue;
-            }
-            if (CONST) {
-                float U[DD];
-                double Sv[D];
-                plasticity<D>(F, Jp, R, mat, U, Sv);
-#pragma unroll
-                for (int k = 0; k < DD; ++k) {
-                    p.pcur[(PF<D>::F + k) * p.pcap + i] = F[k];
-                    p.rcache[k * p.pcap + i] = R[k];
-                }
-                p.pcur[PF<D>::J * p.pcap + i] = Jp;
-                StressState<D> so;
-                stress_state<D>(F, R, Jp, mat, so);
-                float Xo[D];
-#pragma unroll
-                for (int a = 0; a < D; ++a) Xo[a] = p.pcur[(PF<D>::X + a) * p.pcap + i];
-                store_operator<D>(p, i, so, p.pcur[PF<D>::VOL * p.pcap + i], Xo, U, Sv);
-            } else {
-#pragma unroll
-                for (int k = 0; k < DD; ++k) R[k] = p.rcache[k * p.pcap + i];
-            }
-            if (FORCE) {
-                float w[D][3], dw[D][3], X[D];
-                if (!cell_particle<D>(p, sc, i, bb, l, w, dw, X)) continue;
-                StressState<D> st;
-                stress_state<D>(F, R, Jp, mat, st);
-                float P[DD];
-                first_pk<D>(st, P);
-                const float V0 = p.pcur[PF<D>::VOL * p.pcap + i];
-                const float m = p.pcur[PF<D>::M * p.pcap + i];
-                float A[DD];  // V0 P F^T
-#pragma unroll
-                for (int r = 0; r < D; ++r)
-#pragma unroll
-                    for (int cc = 0; cc < D; ++cc) {
-                        float s = 0.0f;
-#pragma unroll
-                        for (int k = 0; k < D; ++k) s += P[r * D + k] * F[cc * D + k];
-                        A[r * D + cc] = s * V0;
-                    }
-                scatter_force<D>(sc, l, w, dw, A, m, mine);
+        const int nmat = sc.nmat;
+        for (int u = 0; u < nmat; ++u) {
+            const Material& mat = sc.mats[u];
+            for (int i = i0 + wv; i < i1; i += NW) {
+                if (nmat > 1 && __float_as_int(p.pcur[PF<D>::MAT * p.pcap + i]) != u) continue;
+                particle_force_one<D, CONST, FORCE>(p, sc, mat, i, bb, l, mine);
             }
         }
         if (FORCE) {
@@ -1950,7 +1962,8 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
                     }
                 }
             }
-            if (sc.mats[__float_as_int(p.pcur[PF<D>::MAT * p.pcap + i])].eos) {
+            const int mid = sc.nmat > 1 ? __float_as_int(p.pcur[PF<D>::MAT * p.pcap + i]) : 0;
+            if (sc.mats[mid].eos) {
                 // EOS: density /= 1 + dt tr(grad v) (MPM_Example.cpp:646); F unchanged
                 float tr = 0.0f;
 #pragma unroll
