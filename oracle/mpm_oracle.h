@@ -35,6 +35,12 @@
  *   binning          every stencil node has exactly one slab owner for any
  *                    thread count
  *   driver dt        sequence of MPM_Driver.cpp:55-80 by hand
+ *   EOS particles    explicit force: first moment sum_i f_i x_i^T =
+ *                    (sum_p V0/rho K rho^(gamma-1)) I; CG force on a linear
+ *                    field: tr(G) (sum_p V0 K gamma rho^(gamma-1)) I; G2P
+ *                    density rho / (1 + dt tr B) with F unchanged
+ *   material table   a particle of entry k evolves as in a scene whose only
+ *                    material is entry k
  * Parity unpinned (no reference behaviour to check against; the readings
  * taken are the scene options of mpm_gpu.h):
  *   - Project_Helper's predicate (Project_Helper.h:39, Nova operator<):
@@ -42,7 +48,9 @@
  *   - Nova CG internals beyond the solution: the stop-test order, restart
  *     at iteration 0, s.q <= 0 handling and the Multiply count;
  *   - SPGrid handling of stencil writes at ghost index -1 (crop_ghost_x);
- *   - Time_At_Frame's frame rate (24 fps taken).
+ *   - Time_At_Frame's frame rate (24 fps taken);
+ *   - Nova's matrix-plus-scalar in the EOS CG force (MPM_CG_System.h:172),
+ *     taken as adding to the diagonal.
  */
 #ifndef MPM_ORACLE_H
 #define MPM_ORACLE_H
