@@ -542,6 +542,13 @@ int check_desc(const mpm_scene_desc& d, const char* who) {
     if (d.nplanes < 0 || d.nplanes > MPM_MAX_PLANES) return mpm_set_error(MPM_E_INVALID, std::string(who) + ": too many planes");
     if (d.nmaterials < 0 || d.nmaterials > MPM_MAX_MATERIALS)
         return mpm_set_error(MPM_E_INVALID, std::string(who) + ": nmaterials out of range");
+    for (int k = 0; k < std::max(1, (int)d.nmaterials); ++k) {
+        const mpm_material& m = k == 0 ? d.material : d.extra_materials[k - 1];
+        // an EOS particle's pressure needs a positive density, bulk modulus and exponent
+        if (m.eos && !(m.density > 0.0f && m.bulk_modulus >= 0.0f && m.gamma > 0.0f && std::isfinite(m.bulk_modulus) &&
+                       std::isfinite(m.gamma) && std::isfinite(m.density)))
+            return mpm_set_error(MPM_E_INVALID, std::string(who) + ": EOS material needs density > 0, K >= 0, gamma > 0");
+    }
     return MPM_OK;
 }
 static_assert(mpm::MAX_MATERIALS == MPM_MAX_MATERIALS, "material table size");

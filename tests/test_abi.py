@@ -129,8 +129,12 @@ def test_reference_test_scenes_counts_mass_and_materials():
         d, _ = M.scene_preset(name)
         assert d.material.eos == 1 and d.material.bulk_modulus == 1.0
         assert d.material.gamma == 7.0 and d.material.density == 1.0
-    # a table entry outside [0, 8] is refused
+    # a table entry outside [0, 8] is refused, so is an EOS material without a positive density
     d, _ = M.scene_preset("ref2d_test16")
     d.nmaterials = 9
     h = ctypes.c_void_p()
     assert M.lib().mpm_create(ctypes.byref(d), 10, ctypes.byref(h)) == -1
+    d, _ = M.scene_preset("ref2d_test19")
+    d.material.density = 0.0
+    assert M.lib().mpm_create(ctypes.byref(d), 10, ctypes.byref(h)) == -1
+    assert b"EOS" in M.lib().mpm_last_error()
