@@ -950,48 +950,89 @@ __device__ inline void grad_scatter(const float (*w)[3], const float (*dw)[3], i
 }
 
 // ============================================================================
-// P2G (Rasterize, MPM_Example.cpp:520-537): m_i += w m_p, (mv)_i += w m_p V_p
+// P2G (Rasterize, MPM_Example.cpp:520-537): m_i += w m_p, (mv)_i += w m_p V_p.
+// One wave per block (the waves of a work-group take different blocks and
+// share nothing).  Lane c runs every particle of cell c, summing the cell's
+// 3^D-node contribution in registers (the particles of a cell share its
+// stencil), then adds it to the wave's LDS tile in 3^D ordered
+// read-add-write steps (lanes hold distinct cells) and the wave stores the
+// tile.  LDS traffic is one stencil per cell instead of one per particle.
 // ============================================================================
 template <int D>
 __global__ __launch_bounds__(BWG) void k_p2g(KP p) {
     using Dm = Dims<D>;
-    constexpr int LTN = Dm::LTN, ST = Dm::ST, C = D + 1;
-    __shared__ float tile[NW * C * LTN];
-    const int nblocks = p.ctr->nblocks;
-    const int c = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    float* mine = tile + wv * C * LTN;
+    constexpr int LTN = Dm::LTN, C = D + 1, TN = Dm::TN, LX = Dm::LX, LY = Dm::LY;
+    constexpr int NS = D == 3 ? 27 : 9;
+    __shared__ float tile_all[NW][C * LTN];
+    const int c = threadIdx.x & 63;
+    float* mine = tile_all[threadIdx.x >> 6];
     int l[D];
     local_coords<D>(c, l);
-    const XcdRange xr = xcd_blocks(nblocks);
+    int base;  // tile index of this cell's stencil corner
+    {
+        int off0[D];
+        base = tile_index<D>(l, 0, off0);
+    }
+    const XcdRange xr = xcd_wave_blocks(p.ctr->nblocks);
     for (int b = xr.first; b < xr.end; b += xr.step) {
         const SceneParams& sc = p.scn[p.block_scene[b]];
         if (!sc.active || block_empty(p, b)) continue;
-        for (int t = threadIdx.x; t < NW * C * LTN; t += BWG) tile[t] = 0.0f;
+        for (int t = c; t < C * LTN; t += 64) mine[t] = 0.0f;
         int bb[D];
         slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
-        __syncthreads();
+        float acc[NS][C];
+#pragma unroll
+        for (int n = 0; n < NS; ++n)
+#pragma unroll
+            for (int ch = 0; ch < C; ++ch) acc[n][ch] = 0.0f;
         const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
-        for (int i = i0 + wv; i < i1; i += NW) {
+        for (int i = i0; i < i1; ++i) {
             float w[D][3], dw[D][3], X[D];
             if (!cell_particle<D>(p, sc, i, bb, l, w, dw, X)) continue;
-            const float m = p.pcur[PF<D>::M * p.pcap + i];
-            float mv[D];
+            float mv[C];
+            mv[0] = p.pcur[PF<D>::M * p.pcap + i];
 #pragma unroll
-            for (int a = 0; a < D; ++a) mv[a] = m * p.pcur[(PF<D>::V + a) * p.pcap + i];
+            for (int a = 0; a < D; ++a) mv[1 + a] = mv[0] * p.pcur[(PF<D>::V + a) * p.pcap + i];
+            if constexpr (D == 3) {
 #pragma unroll
-            for (int s = 0; s < ST; ++s) {
-                int off[D];
-                const int t = tile_index<D>(l, s, off);
-                const float wt = stencil_weight<D>(w, off);
-                mine[t] += wt * m;
+                for (int ii = 0; ii < 3; ++ii)
 #pragma unroll
-                for (int a = 0; a < D; ++a) mine[(1 + a) * LTN + t] += wt * mv[a];
-                MPM_STEP_FENCE();
+                    for (int jj = 0; jj < 3; ++jj) {
+                        const float wij = w[0][ii] * w[1][jj];
+#pragma unroll
+                        for (int kk = 0; kk < 3; ++kk) {
+                            const float wt = wij * w[2][kk];
+#pragma unroll
+                            for (int ch = 0; ch < C; ++ch) acc[(ii * 3 + jj) * 3 + kk][ch] += wt * mv[ch];
+                        }
+                    }
+            } else {
+#pragma unroll
+                for (int ii = 0; ii < 3; ++ii)
+#pragma unroll
+                    for (int jj = 0; jj < 3; ++jj) {
+                        const float wt = w[0][ii] * w[1][jj];
+#pragma unroll
+                        for (int ch = 0; ch < C; ++ch) acc[ii * 3 + jj][ch] += wt * mv[ch];
+                    }
             }
         }
-        __syncthreads();
-        store_wave_tiles<D, C>(p, b, tile);
-        __syncthreads();
+        MPM_STEP_FENCE();
+#pragma unroll
+        for (int n = 0; n < NS; ++n) {
+            const int t = D == 3 ? base + (n / 9) * LX + ((n / 3) % 3) * LY + n % 3 : base + (n / 3) * LX + n % 3;
+#pragma unroll
+            for (int ch = 0; ch < C; ++ch) mine[ch * LTN + t] += acc[n][ch];
+            MPM_STEP_FENCE();
+        }
+        float* dst = p.tiles + (int64_t)b * C * TN;
+        for (int t = c; t < C * TN; t += 64) {
+            const int cc = t / TN;
+            int nb, local, tc[D];
+            tile_map<D>(t - cc * TN, nb, local, tc);
+            dst[t] = mine[cc * LTN + lds_index<D>(tc)];
+        }
+        MPM_STEP_FENCE();
     }
 }
 
