@@ -185,6 +185,18 @@ int mpm_download_particles_ex(mpm_handle h, int64_t cap, float* x, float* v, flo
                               float* Jp, float* mass, float* volume, int32_t* ids,
                               int32_t* material, int64_t* n_out);
 
+/* MPM_Example::Add_Fluid_Source with Allocate_Particle
+ * (MPM_Example.cpp:925-1027; the call sits, commented out, at the start of
+ * Advance_Step, MPM_Driver.cpp:108): after mpm_initialize, n particles join
+ * scene `scene` (0 for a single scene) between substeps as waiting
+ * particles -- F = I, Jp = 1 (an EOS material: its density), volume from
+ * the grid mass in the next substep's Process_Waiting_Particles -- with the
+ * scene's next ids.  They are binned, and so visible to downloads and
+ * stats, from the next substep on.  material may be NULL (entry 0).
+ * MPM_E_CAPACITY if the handle's particle capacity is exhausted. */
+int mpm_add_particles(mpm_handle h, int32_t scene, int64_t n, const float* x, const float* v,
+                      const float* mass, const int32_t* material);
+
 /* MPM_Driver::Initialize (MPM_Driver.cpp:18-32): one grid build, P2G and
  * Process_Waiting_Particles for particles uploaded without volume. */
 int mpm_initialize(mpm_handle h);
@@ -411,7 +423,8 @@ int mpm_serve_release(mpm_server_handle s, int64_t request_id);
  * count it generates.  Known names: "disk2d", "two_balls", "avalanche",
  * "blob", "slab", and the reference's Standard_Tests scenes "ref2d_test1",
  * "ref2d_test15", "ref2d_test16", "ref2d_test17", "ref2d_test18",
- * "ref2d_test19", "ref3d_test18", "ref3d_test19". */
+ * "ref2d_test19", "ref2d_test20" (no particles: the fluid source is the host's,
+ * through mpm_add_particles), "ref3d_test18", "ref3d_test19". */
 int mpm_scene_preset(const char* name, int32_t counts, uint64_t seed, mpm_scene_desc* desc,
                      int64_t* n_particles);
 /* Generates the particles of a named scene (same arguments as the preset);

@@ -139,6 +139,7 @@ SIGNATURES = {
     "mpm_destroy": (ctypes.c_int, [_H]),
     "mpm_upload_particles": (ctypes.c_int, [_H, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP]),
     "mpm_download_particles": (ctypes.c_int, [_H, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _IP, _LP]),
+    "mpm_add_particles": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int64, _FP, _FP, _FP, _IP]),
     "mpm_upload_particles_ex": (ctypes.c_int, [_H, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _IP]),
     "mpm_download_particles_ex": (ctypes.c_int, [_H, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _IP, _IP, _LP]),
     "mpm_initialize": (ctypes.c_int, [_H]),
@@ -366,6 +367,14 @@ class Scene(_LaunchControl):
         _check(lib().mpm_upload_particles_ex(self._h, n, _fp(x), _fp(v), _fp(F), _fp(Jp), _fp(mass), _fp(volume),
                                              _ip(material)))
 
+    def add_particles(self, x, v, mass, material=None) -> None:
+        """Particles that join the running scene as waiting particles (the
+        fluid source, MPM_Example::Add_Fluid_Source); binned from the next
+        substep on."""
+        x, v, mass = _f32(x), _f32(v), _f32(mass)
+        material = _i32(material)
+        _check(lib().mpm_add_particles(self._h, 0, mass.shape[0], _fp(x), _fp(v), _fp(mass), _ip(material)))
+
     def download(self, cap: Optional[int] = None) -> Dict[str, np.ndarray]:
         """Live particles by id; Jp holds an EOS particle's density."""
         if cap is None:
@@ -509,6 +518,15 @@ class Batch(_LaunchControl):
 
     def initialize(self) -> None:
         _check(lib().mpm_initialize(self._h))
+
+    def add_particles(self, scene: int, x, v, mass, material=None) -> None:
+        """Waiting particles joining scene `scene` (see Scene.add_particles);
+        its download capacity grows by their count."""
+        x, v, mass = _f32(x), _f32(v), _f32(mass)
+        material = _i32(material)
+        _check(lib().mpm_add_particles(self._h, int(scene), mass.shape[0], _fp(x), _fp(v), _fp(mass),
+                                       _ip(material)))
+        self.capacities[scene] += mass.shape[0]
 
     def substep(self, dt) -> None:
         d = np.ascontiguousarray(np.broadcast_to(np.asarray(dt, np.float32), (self.n,)), dtype=np.float32)

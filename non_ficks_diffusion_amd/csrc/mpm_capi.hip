@@ -376,11 +376,15 @@ int substep(mpm_context* c, const float* dts, bool capture = false) {
     stage_bin<D>(c);
     stage_p2g<D>(c);
     mpm::KP p = make_kp(c);
-    if (c->pool) {
-        // scenes attached since the last substep: Process_Waiting_Particles
-        // (every other scene's blocks return at once)
+    if (c->pool || c->emitted) {
+        // scenes attached or particles added since the last substep:
+        // Process_Waiting_Particles (every other scene's blocks return at once)
         timed(c, KC_INIT_VOLUME, [&] { mpm::launch_init_volume<D>(p, c->stream, c->grid_block); });
         mpm::launch_clear_waiting(p, c->stream);
+        if (!capture) {
+            c->emitted = false;
+            for (mpm::SceneParams& t : c->table) t.waiting = 0;
+        }
     }
     timed(c, KC_PFORCE, [&] { mpm::launch_particle_force<D>(p, c->stream, c->grid_block, true, true); });
     timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, true, true, true, false); });
@@ -832,7 +836,8 @@ int graph_substep(mpm_context* c, const float* dts) {
 }
 
 int do_substep(mpm_context* c, const float* dts) {
-    int rc = c->graph ? graph_substep(c, dts) : c->D == 3 ? substep<3>(c, dts) : substep<2>(c, dts);
+    // a substep after mpm_add_particles runs launched: it adds the waiting pass
+    int rc = c->graph && !c->emitted ? graph_substep(c, dts) : c->D == 3 ? substep<3>(c, dts) : substep<2>(c, dts);
     if (rc) return rc;
     for (int s = 0; s < c->nscenes; ++s)
         if (dts[s] > 0.0f) c->sc[(size_t)s].last_dt = dts[s];
@@ -1311,7 +1316,8 @@ int mpm_upload_particles(mpm_handle c, int64_t n, const float* x, const float* v
 
 int mpm_upload_particles_ex(mpm_handle c, int64_t n, const float* x, const float* v, const float* F, const float* Jp,
                             const float* mass, const float* volume, const int32_t* material) {
-    if (!c || !x || !v || !mass) return mpm_set_error(MPM_E_INVALID, "mpm_upload_particles: null argument");
+    // a scene may start empty (the fluid-source test): n = 0 needs no arrays
+    if (!c || (n > 0 && (!x || !v || !mass))) return mpm_set_error(MPM_E_INVALID, "mpm_upload_particles: null argument");
     if (c->nscenes != 1) return mpm_set_error(MPM_E_INVALID, "mpm_upload_particles: batch handle, use mpm_batch_upload");
     // a new particle set: the buffer starts over in its unbinned layout
     HIPCHECK(hipStreamSynchronize(c->stream));
@@ -1326,6 +1332,65 @@ int mpm_upload_particles_ex(mpm_handle c, int64_t n, const float* x, const float
     int rc = upload_scene(c, 0, n, x, v, F, Jp, mass, volume, material);
     if (rc) return rc;
     return upload_max_speed(c);
+}
+
+// MPM_Example::Add_Fluid_Source / Allocate_Particle (MPM_Example.cpp:925-1027):
+// n particles join scene `scene` of a running handle as waiting particles
+// (F = I, Jp = 1 or an EOS material's density, volume from the grid at the
+// next substep), after every live particle, with the next ids of the scene.
+int mpm_add_particles(mpm_handle c, int32_t scene, int64_t n, const float* x, const float* v, const float* mass,
+                      const int32_t* material) {
+    if (!c || !x || !v || !mass) return mpm_set_error(MPM_E_INVALID, "mpm_add_particles: null argument");
+    if (scene < 0 || scene >= c->nscenes) return mpm_set_error(MPM_E_INVALID, "mpm_add_particles: no such scene");
+    if (c->pool) return mpm_set_error(MPM_E_INVALID, "mpm_add_particles: arena scenes are the server's");
+    if (!c->initialized) return mpm_set_error(MPM_E_NOT_READY, "mpm_add_particles: call mpm_initialize first");
+    if (n < 0) return mpm_set_error(MPM_E_INVALID, "mpm_add_particles: negative count");
+    if (n == 0) return MPM_OK;
+    SceneHost& sh = c->sc[(size_t)scene];
+    const mpm_scene_desc& d = sh.desc;
+    int64_t n_live = 0;
+    int rc = mpm_host::live_particles(c, &n_live);
+    if (rc) return rc;
+    if (n_live + n > c->pcap) return mpm_set_error(MPM_E_CAPACITY, "mpm_add_particles: particle capacity exhausted");
+    const int D = c->D, NF = nfields(D);
+    std::vector<float> h((size_t)NF * (size_t)n);
+    auto f = [&](int field, int64_t i) -> float& { return h[(size_t)field * (size_t)n + (size_t)i]; };
+    for (int64_t i = 0; i < n; ++i) {
+        for (int a = 0; a < D; ++a) {
+            if (!std::isfinite(x[i * D + a]) || !std::isfinite(v[i * D + a]))
+                return mpm_set_error(MPM_E_INVALID, "mpm_add_particles: non-finite position or velocity");
+            f(a, i) = x[i * D + a];
+            f(D + a, i) = v[i * D + a];
+        }
+        for (int a = 0; a < D; ++a)
+            for (int b = 0; b < D; ++b) f(2 * D + a * D + b, i) = a == b ? 1.0f : 0.0f;
+        int id;
+        bool eos;
+        float rho0;
+        rc = particle_material(d, material, i, id, eos, rho0);
+        if (rc) return rc;
+        f(2 * D + D * D, i) = eos ? rho0 : 1.0f;
+        f(2 * D + D * D + 1, i) = mass[i];
+        f(2 * D + D * D + 2, i) = 0.0f;
+        std::memcpy(&f(2 * D + D * D + 3, i), &id, sizeof(int));
+    }
+    std::vector<int> ids((size_t)n), keys((size_t)n, 0), sc_of((size_t)n, scene);
+    std::iota(ids.begin(), ids.end(), (int)sh.n_uploaded);
+    const int cur = c->cur;
+    for (int fld = 0; fld < NF; ++fld)
+        HIPCHECK(hipMemcpy(c->pbuf[cur] + (size_t)fld * c->pcap + n_live, &h[(size_t)fld * n], (size_t)n * sizeof(float),
+                           hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->idbuf[cur] + n_live, ids.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->keybuf[cur] + n_live, keys.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    HIPCHECK(hipMemcpy(c->scbuf[cur] + n_live, sc_of.data(), (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    const int nl = (int)(n_live + n);
+    HIPCHECK(hipMemcpy(&c->ctr->n_live, &nl, sizeof(int), hipMemcpyHostToDevice));
+    c->table[(size_t)scene].waiting = 1;
+    HIPCHECK(hipMemcpy(c->scn + scene, &c->table[(size_t)scene], sizeof(mpm::SceneParams), hipMemcpyHostToDevice));
+    sh.n_uploaded += n;
+    c->emitted = true;
+    c->binned = false;
+    return MPM_OK;
 }
 
 int mpm_download_particles(mpm_handle c, int64_t cap, float* x, float* v, float* F, float* Jp, float* mass,
@@ -1693,7 +1758,7 @@ int mpm_batch_upload(mpm_handle c, int32_t scene, int64_t n, const float* x, con
 
 int mpm_batch_upload_ex(mpm_handle c, int32_t scene, int64_t n, const float* x, const float* v, const float* F,
                         const float* Jp, const float* mass, const float* volume, const int32_t* material) {
-    if (!c || !x || !v || !mass) return mpm_set_error(MPM_E_INVALID, "mpm_batch_upload: null argument");
+    if (!c || (n > 0 && (!x || !v || !mass))) return mpm_set_error(MPM_E_INVALID, "mpm_batch_upload: null argument");
     if (scene < 0 || scene >= c->nscenes) return mpm_set_error(MPM_E_INVALID, "mpm_batch_upload: no such scene");
     if (c->ever_binned) return mpm_set_error(MPM_E_NOT_READY, "mpm_batch_upload: the batch is already initialized");
     int rc = upload_scene(c, scene, n, x, v, F, Jp, mass, volume, material);

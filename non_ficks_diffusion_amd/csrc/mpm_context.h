@@ -78,6 +78,7 @@ struct mpm_context {
     bool binned = false;       // particles sit in the block ranges of the last binning
     bool ever_binned = false;  // particle ranges are the binned ones (not the upload ranges)
     bool stepped = false;      // a substep has run (operator timing needs its state)
+    bool emitted = false;      // particles added since the last substep wait for their volumes
     std::vector<void*> allocations;
     // timing
     bool timing = false;

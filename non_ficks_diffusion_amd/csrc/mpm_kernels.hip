@@ -1096,6 +1096,8 @@ __global__ __launch_bounds__(BWG) void k_init_volume(KP p) {
         __syncthreads();
         const int i1 = p.block_pstart[b + 1];
         for (int i = p.block_pstart[b] + (int)threadIdx.x; i < i1; i += BWG) {
+            // the waiting particles are those without a volume yet (new or emitted)
+            if (p.pcur[PF<D>::VOL * p.pcap + i] != 0.0f) continue;
             float w[D][3], dw[D][3], X[D];
             int l[D];
             if (!particle_weights<D>(p, sc, i, bb, w, dw, l, X)) continue;

@@ -330,6 +330,20 @@ bool build_spec(const std::string& name, int counts, uint64_t seed, int64_t targ
         s.count = 2000;
         s.mass_density = 2.0;
         sp.shapes.push_back(s);
+    } else if (name == "ref2d_test20") {
+        // Reference test 20, fluid source: no particles at the start, a box of ground, ceiling
+        // and side walls, and a source region [0.45,0.55]^2 emitting EOS fluid particles (K=1,
+        // gamma=7, density 1; mass 2 * 5e-4 each, V=(0,-1)) through MPM_Example::Add_Fluid_Source
+        // (mpm_2d/Standard_Tests/Standard_Tests.h:285-300, MPM_Example.cpp:977-1001).  The source
+        // is the host's (mpm_add_particles); the preset holds the walls and the fluid material.
+        ref_desc(d, 2, counts);
+        d.material = ref_material(40.0, 0.2, 0, 1.0, 1.0, 0.0);
+        d.material.eos = 1;
+        d.material.bulk_modulus = 1.0f;
+        add_plane(d, 0, 1, 0, 0, 0.1, 0);
+        add_plane(d, 0, -1, 0, 0, 0.9, 0);
+        add_plane(d, 1, 0, 0, 0.1, 0, 0);
+        add_plane(d, -1, 0, 0, 0.9, 0, 0);
     } else if (name == "ref2d_test19" || name == "ref3d_test19") {
         // Reference test 19, fluid: a ball r=0.1 at the centre with V=(0,-1), mass density
         // 2, every particle an EOS fluid particle with K=1, gamma=7, density 1 (E and nu are
@@ -394,7 +408,7 @@ bool build_spec(const std::string& name, int counts, uint64_t seed, int64_t targ
     } else {
         return false;
     }
-    double rho = sp.density;
+    double rho = sp.shapes.empty() ? 2.0 : sp.density;
     if (sp.fixed()) {
         rho = 1e30;
         for (const Shape& s : sp.shapes) rho = std::min(rho, s.mass_density);
@@ -523,12 +537,14 @@ extern "C" int mpm_scene_preset_sized(const char* name, int32_t counts, uint64_t
 
 extern "C" int mpm_scene_generate_sized(const char* name, int32_t counts, uint64_t seed, int64_t target_n,
                                         int64_t cap, float* x, float* v, float* mass, int64_t* n_out) {
-    if (!name || !x || !v || !mass) return mpm_set_error(MPM_E_INVALID, "mpm_scene_generate: null argument");
+    if (!name) return mpm_set_error(MPM_E_INVALID, "mpm_scene_generate: null argument");
     mpm_scene_desc d;
     int64_t n = 0;
     int rc = mpm_scene_preset_sized(name, counts, seed, target_n, &d, &n);
     if (rc) return rc;
     if (n_out) *n_out = n;
+    if (n == 0) return MPM_OK;  // a scene without particles (the fluid-source test)
+    if (!x || !v || !mass) return mpm_set_error(MPM_E_INVALID, "mpm_scene_generate: null argument");
     if (n > cap) return mpm_set_error(MPM_E_CAPACITY, "mpm_scene_generate: capacity too small");
     Spec sp;
     build_spec(name, counts, seed, target_n, sp);
@@ -548,12 +564,14 @@ extern "C" int mpm_scene_generate(const char* name, int32_t counts, uint64_t see
 
 extern "C" int mpm_scene_materials(const char* name, int32_t counts, uint64_t seed, int64_t target_n, int64_t cap,
                                    int32_t* material, int64_t* n_out) {
-    if (!name || !material) return mpm_set_error(MPM_E_INVALID, "mpm_scene_materials: null argument");
+    if (!name) return mpm_set_error(MPM_E_INVALID, "mpm_scene_materials: null argument");
     mpm_scene_desc d;
     int64_t n = 0;
     int rc = mpm_scene_preset_sized(name, counts, seed, target_n, &d, &n);
     if (rc) return rc;
     if (n_out) *n_out = n;
+    if (n == 0) return MPM_OK;
+    if (!material) return mpm_set_error(MPM_E_INVALID, "mpm_scene_materials: null argument");
     if (n > cap) return mpm_set_error(MPM_E_CAPACITY, "mpm_scene_materials: capacity too small");
     Spec sp;
     build_spec(name, counts, seed, target_n, sp);

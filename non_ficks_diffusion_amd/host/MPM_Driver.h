@@ -81,6 +81,7 @@ class MPM_Driver {
     void Advance_Step(const T dt) {
         example.Initialize_SPGrid();
         example.Reset_Grid_Based_Variables();
+        example.Add_Fluid_Source();  // commented out in the reference; runs with -fluid_source
         example.Update_Particle_Weights();
         example.Group_Particles();
         example.Rasterize_Voxels();

@@ -101,6 +101,15 @@ class MPM_Example {
     T min_dt = (T)1e-6, max_dt = (T)5e-3;
     int project_mode = 1, crop_ghost_x = 1;
 
+    // fluid source (MPM_Example.h fluid_source; Add_Fluid_Source,
+    // MPM_Example.cpp:977-1027).  The reference's call is commented out
+    // (MPM_Driver.cpp:108); fluid_source_on turns it on.
+    bool fluid_source_on = false, has_fluid_source = false;
+    TV fluid_source_min, fluid_source_max;
+    int fluid_material = 0;              // entry of the material table (an EOS material)
+    int64_t fluid_capacity = 1 << 20;    // particle room kept for emitted particles
+    uint64_t fluid_seed = 0, fluid_draws = 0;
+
     MPM_Example() {
         gravity(1) = (T)-2;  // MPM_Example.cpp:81
         material.hardening_max = FLT_MAX;
@@ -117,11 +126,44 @@ class MPM_Example {
         Populate_Simulated_Particles();
         mpm_scene_desc desc = Scene_Description();
         const int64_t n = (int64_t)particles.size();
+        const int64_t room = Emitting() ? fluid_capacity : 0;
         if (handle) mpm_destroy(handle);
-        check(mpm_create(&desc, n > 0 ? n : 1, &handle), "mpm_create");
+        check(mpm_create(&desc, std::max<int64_t>(n + room, 1), &handle), "mpm_create");
         Upload();
         waiting = true;
         initialized = false;
+    }
+    bool Emitting() const { return fluid_source_on && has_fluid_source; }
+    // MPM_Example::Add_Fluid_Source (MPM_Example.cpp:977-1027): ten EOS
+    // particles, uniform in the source box, V = (0,-1), mass density 2 over
+    // 5e-4 each in 2D and a tenth of the box in 3D, volume from the grid
+    // (waiting).  Positions come from a counter-based generator (the
+    // reference's Random_Numbers is Nova's).
+    void Add_Fluid_Source() {
+        if (!Emitting()) return;
+        if (!initialized) check(mpm_initialize(handle), "mpm_initialize");
+        initialized = true;
+        waiting = false;
+        constexpr int number_fluid_particles = 10;
+        T measure = 1;
+        for (int a = 0; a < d; ++a) measure *= fluid_source_max(a) - fluid_source_min(a);
+        const T area_per_particle = d == 2 ? (T)5e-4 : measure / number_fluid_particles;
+        const T mass_density = 2;
+        std::vector<float> x(number_fluid_particles * d), v(number_fluid_particles * d, 0.0f),
+            m(number_fluid_particles, (float)(mass_density * area_per_particle));
+        std::vector<int32_t> mat(number_fluid_particles, fluid_material);
+        for (int i = 0; i < number_fluid_particles; ++i) {
+            for (int a = 0; a < d; ++a) {
+                const double u = Unit_Sample(fluid_seed, fluid_draws, (uint64_t)a);
+                x[i * d + a] = (float)(fluid_source_min(a) + (fluid_source_max(a) - fluid_source_min(a)) * u);
+            }
+            v[i * d + 1] = -1.0f;
+            ++fluid_draws;
+        }
+        check(mpm_add_particles(handle, 0, number_fluid_particles, x.data(), v.data(), m.data(), mat.data()),
+              "mpm_add_particles");
+        emitted_since_download += number_fluid_particles;
+        host_stale = true;
     }
     mpm_scene_desc Scene_Description() const {
         mpm_scene_desc desc{};
@@ -337,6 +379,18 @@ class MPM_Example {
   protected:
     mpm_handle handle = nullptr;
     bool waiting = false, initialized = false, host_stale = false;
+    int64_t emitted_since_download = 0;
+
+    // SplitMix64 (Steele, Lea, Flood 2014) over (seed, draw, axis): 24 bits in [0,1)
+    static double Unit_Sample(uint64_t seed, uint64_t draw, uint64_t axis) {
+        auto mix = [](uint64_t z) {
+            z += 0x9E3779B97F4A7C15ull;
+            z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+            z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+            return z ^ (z >> 31);
+        };
+        return (double)(mix(mix(mix(seed ^ 0xF1u) ^ draw) ^ axis) >> 40) / 16777216.0;
+    }
 
     void Stage(int stage, T dt) { check(mpm_run_stage(handle, stage, (float)dt), "mpm_run_stage"); }
 
@@ -366,7 +420,7 @@ class MPM_Example {
     }
     void Download() {
         if (!host_stale) return;
-        const size_t cap = particles.size();
+        const size_t cap = particles.size() + (size_t)emitted_since_download;
         std::vector<float> x(cap * d), v(cap * d), F(cap * d * d), Jp(cap), m(cap), vol(cap);
         std::vector<int32_t> mat(cap);
         int64_t n = 0;
@@ -384,6 +438,7 @@ class MPM_Example {
             p.mass = m[i];
             p.volume = vol[i];
         }
+        emitted_since_download = 0;
         host_stale = false;
     }
     static void Make_Directory(const std::string& path) {

@@ -13,6 +13,8 @@
 // test 18        two plastic balls (E 30, nu 0.4) thrown at each other,
 //                ground y = 0.1 (3D and 2D test 18 of the reference)
 // test 19        an EOS fluid ball falling into a box (3D and 2D)
+// test 20 (2D)   a fluid source: no particles, the box of test 19 and a
+//                source region [0.45,0.55]^2 (emits with -fluid_source)
 // test 100       the benchmark scenes: 3D two snowballs head-on,
 //                2D snow disk dropped on the ground
 // test 101       3D snow slab on a 30-degree incline (avalanche)
@@ -36,6 +38,7 @@ class Standard_Tests : public MPM_Example<T, d> {
     static std::string Preset_Name(int test_case) {
         if (test_case == 18) return d == 3 ? "ref3d_test18" : "ref2d_test18";
         if (test_case == 19) return d == 3 ? "ref3d_test19" : "ref2d_test19";
+        if (test_case == 20 && d == 2) return "ref2d_test20";
         if (d == 2 && (test_case == 1 || test_case == 15 || test_case == 16 || test_case == 17))
             return "ref2d_test" + std::to_string(test_case);
         if (test_case == 100) return d == 3 ? "two_balls" : "disk2d";
@@ -91,6 +94,16 @@ class Standard_Tests : public MPM_Example<T, d> {
         this->frame_rate = desc.frame_rate;
         this->project_mode = desc.project_mode;
         this->crop_ghost_x = desc.crop_ghost_x;
+        this->has_fluid_source = test_case == 20;
+        if (this->has_fluid_source) {
+            // mpm_2d/Standard_Tests/Standard_Tests.h:290-291
+            for (int a = 0; a < d; ++a) {
+                this->fluid_source_min(a) = (T)0.45;
+                this->fluid_source_max(a) = (T)0.55;
+            }
+            this->fluid_material = 0;
+            this->fluid_seed = seed;
+        }
     }
 };
 

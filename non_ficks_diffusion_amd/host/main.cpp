@@ -21,13 +21,14 @@ struct Options {
     int dim = 3, test_number = 18, size = 32, threads = 1, levels = 1, last_frame = 1, restart = -1;
     double cfl = 0.1;
     uint64_t seed = 0;
-    bool staged = false, verbose = false;
+    bool staged = false, verbose = false, fluid_source = false;
     std::string output;
 };
 
 void usage() {
     std::cerr << "usage: mpm_driver [-d 2|3] [-test_number N] [-size n] [-threads t] [-levels l] [-cfl c]\n"
-                 "                  [-last_frame f] [-o dir] [-restart frame] [-seed s] [-staged] [-v]\n";
+                 "                  [-last_frame f] [-o dir] [-restart frame] [-seed s] [-staged] [-v]\n"
+                 "                  [-fluid_source]\n";
 }
 
 bool parse(int argc, char** argv, Options& o) {
@@ -48,6 +49,7 @@ bool parse(int argc, char** argv, Options& o) {
         else if (a == "-restart") o.restart = std::atoi(next("-restart"));
         else if (a == "-seed") o.seed = std::strtoull(next("-seed"), nullptr, 10);
         else if (a == "-staged") o.staged = true;
+        else if (a == "-fluid_source") o.fluid_source = true;
         else if (a == "-v") o.verbose = true;
         else if (a == "-h" || a == "--help") { usage(); std::exit(0); }
         else { std::cerr << "unknown option " << a << "\n"; usage(); return false; }
@@ -69,6 +71,7 @@ int run(const Options& o) {
     example.last_frame = o.last_frame;
     example.seed = o.seed;
     example.staged = o.staged;
+    example.fluid_source_on = o.fluid_source;
     example.cfl = (float)o.cfl;
     example.Parse_Options();
     if (!o.output.empty()) example.output_directory = o.output;

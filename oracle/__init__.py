@@ -27,6 +27,7 @@ _SIG = {
     "oracle_destroy": (None, [_H]),
     "oracle_set_particles": (None, [_H, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP]),
     "oracle_set_particles_ex": (None, [_H, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, ctypes.POINTER(ctypes.c_int32)]),
+    "oracle_add_particles": (None, [_H, ctypes.c_int64, _FP, _FP, _FP, ctypes.POINTER(ctypes.c_int32)]),
     "oracle_num_particles": (ctypes.c_int64, [_H]),
     "oracle_num_simulated": (ctypes.c_int64, [_H]),
     "oracle_get_particles": (None, [_H, _DP, _DP, _DP, _DP, _DP, _DP, ctypes.POINTER(ctypes.c_uint8)]),
@@ -111,6 +112,13 @@ class Oracle:
         mp = None if mat is None else mat.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
         lib().oracle_set_particles_ex(self._h, mass.shape[0], _fp(x), _fp(v), _fp(F), _fp(Jp), _fp(mass), _fp(volume), mp)
         self._keep_mat = mat
+
+    def add_particles(self, x, v, mass, material=None):
+        """Waiting particles that join between substeps (Add_Fluid_Source)."""
+        x, v, mass = map(_f32, (x, v, mass))
+        mat = None if material is None else np.ascontiguousarray(material, dtype=np.int32)
+        mp = None if mat is None else mat.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+        lib().oracle_add_particles(self._h, mass.shape[0], _fp(x), _fp(v), _fp(mass), mp)
 
     def particles(self, only_valid: bool = True) -> Dict[str, np.ndarray]:
         n = lib().oracle_num_particles(self._h)

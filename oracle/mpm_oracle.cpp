@@ -502,6 +502,8 @@ struct OracleBase {
     virtual ~OracleBase() {}
     virtual void set_particles(int64_t n, const float* x, const float* v, const float* F, const float* Jp,
                                const float* mass, const float* volume, const int32_t* material) = 0;
+    virtual void add_particles(int64_t n, const float* x, const float* v, const float* mass,
+                               const int32_t* material) = 0;
     virtual int64_t num_particles() const = 0;
     virtual int64_t num_simulated() const = 0;
     virtual void get_particles(double* x, double* v, double* F, double* Jp, double* mass, double* volume,
@@ -590,6 +592,40 @@ struct Oracle : OracleBase {
         invalid_particles.clear();
         for (int64_t i = 0; i < n; ++i) simulated_particles.push_back((int)i);
         if (!volume) waiting_particles = simulated_particles;
+    }
+    // Add_Fluid_Source with Allocate_Particle (MPM_Example.cpp:925-1027): new
+    // particles join the simulated and waiting lists with volume 0 (their
+    // volume comes from the grid in the next Process_Waiting_Particles).  The
+    // reference recycles the storage of removed particles; here they are
+    // appended, which changes storage order only.
+    void add_particles(int64_t n, const float* x, const float* v, const float* mass,
+                       const int32_t* material) override {
+        for (int64_t i = 0; i < n; ++i) {
+            Particle<T, d> p;
+            for (int a = 0; a < d; ++a) { p.X(a) = (T)x[i * d + a]; p.V(a) = (T)v[i * d + a]; }
+            p.mass = (T)mass[i];
+            p.volume = T(0);
+            const int id = material ? material[i] : 0;
+            const mpm_material& mat = id == 0 ? desc.material : desc.extra_materials[id - 1];
+            Model<T, d>& m = p.model;
+            m.mu0 = (T)mat.mu0;
+            m.lambda0 = (T)mat.lambda0;
+            m.plastic = mat.plastic != 0;
+            m.stretching_yield = (T)mat.stretching_yield;
+            m.compression_yield = (T)mat.compression_yield;
+            m.hardening_factor = (T)mat.hardening_factor;
+            m.hardening_max = (T)mat.hardening_max;
+            p.eos = mat.eos != 0;
+            p.bulk_modulus = (T)mat.bulk_modulus;
+            p.gamma = (T)mat.gamma;
+            p.density = p.eos ? (T)mat.density : T(1);
+            m.Jp = T(1);
+            m.Fp = TM::Identity();
+            const int pid = (int)particles.size();
+            particles.push_back(p);
+            simulated_particles.push_back(pid);
+            waiting_particles.push_back(pid);
+        }
     }
     int64_t num_particles() const override { return (int64_t)particles.size(); }
     int64_t num_simulated() const override { return (int64_t)simulated_particles.size(); }
@@ -1378,6 +1414,11 @@ void oracle_destroy(oracle_handle h) { delete h; }
 void oracle_set_particles(oracle_handle h, int64_t n, const float* x, const float* v, const float* F,
                           const float* Jp, const float* mass, const float* volume) {
     h->impl->set_particles(n, x, v, F, Jp, mass, volume, nullptr);
+}
+
+void oracle_add_particles(oracle_handle h, int64_t n, const float* x, const float* v, const float* mass,
+                          const int32_t* material) {
+    h->impl->add_particles(n, x, v, mass, material);
 }
 
 void oracle_set_particles_ex(oracle_handle h, int64_t n, const float* x, const float* v, const float* F,

@@ -81,6 +81,9 @@ void oracle_set_particles(oracle_handle h, int64_t n, const float* x, const floa
 void oracle_set_particles_ex(oracle_handle h, int64_t n, const float* x, const float* v,
                              const float* F, const float* Jp, const float* mass,
                              const float* volume, const int32_t* material);
+/* New waiting particles between substeps (Add_Fluid_Source, MPM_Example.cpp:977-1027). */
+void oracle_add_particles(oracle_handle h, int64_t n, const float* x, const float* v,
+                          const float* mass, const int32_t* material);
 int64_t oracle_num_particles(oracle_handle h);
 int64_t oracle_num_simulated(oracle_handle h);
 /* All particles in creation order; valid[i] = 0 for removed ones. */
