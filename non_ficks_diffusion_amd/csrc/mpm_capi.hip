@@ -500,6 +500,7 @@ int allocate_context(mpm_context* c) {
     A(dalloc(c, &c->bsq, c->nred));
     A(dalloc(c, &c->bu, (size_t)2 * c->bcap));
     A(dalloc(c, &c->d_dt, (size_t)nscenes));
+    A(dalloc(c, &c->frame, (size_t)nscenes));
     if (rc == MPM_OK) {
         void* hp = nullptr;
         if (hipHostMalloc(&hp, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");
@@ -511,6 +512,12 @@ int allocate_context(mpm_context* c) {
             rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");
         c->h_dt = (float*)hp;
         if (rc == MPM_OK && hipEventCreateWithFlags(&c->dt_ev, hipEventDisableTiming) != hipSuccess)
+            rc = mpm_set_error(MPM_E_HIP, "hipEventCreate failed");
+        hp = nullptr;
+        if (rc == MPM_OK && hipHostMalloc(&hp, sizeof(int), hipHostMallocDefault) != hipSuccess)
+            rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");
+        c->h_fdone = (int*)hp;
+        if (rc == MPM_OK && hipEventCreateWithFlags(&c->fdone_ev, hipEventDisableTiming) != hipSuccess)
             rc = mpm_set_error(MPM_E_HIP, "hipEventCreate failed");
     }
     if (rc == MPM_OK && hipMemset(c->keybuf[0], 0xff, (size_t)c->pcap * sizeof(int)) != hipSuccess)
@@ -789,11 +796,96 @@ int download_scene(mpm_context* c, int s, int64_t cap, float* x, float* v, float
 }
 
 void invalidate_graphs(mpm_context* c) {
-    for (int k = 0; k < 2; ++k)
+    for (int k = 0; k < 2; ++k) {
         if (c->gexec[k]) {
             (void)hipGraphExecDestroy(c->gexec[k]);
             c->gexec[k] = nullptr;
         }
+        if (c->fexec[k]) {
+            (void)hipGraphExecDestroy(c->fexec[k]);
+            c->fexec[k] = nullptr;
+        }
+    }
+}
+
+// Device frame loop (MPM_Driver::Advance_To_Target_Time for every scene, the
+// dt rule on the device): a graph of FRAME_K substeps, each one k_frame_dt
+// (the next dt of every scene from its last G2P) and the captured substep,
+// replayed until every scene reached its frame end.  The host reads one
+// count per replay instead of the max speeds after every substep.  FRAME_K is
+// even, so a replay leaves the particle buffers at the parity it found.
+constexpr int FRAME_K = 4;
+
+int capture_frame_graph(mpm_context* c, int parity) {
+    const bool timing = c->timing;
+    c->timing = false;
+    HIPCHECK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = MPM_OK;
+    for (int k = 0; k < FRAME_K && rc == MPM_OK; ++k) {
+        mpm::launch_frame_dt(make_kp(c), c->stream, c->frame, c->d_dt);
+        rc = c->D == 3 ? substep<3>(c, nullptr, true) : substep<2>(c, nullptr, true);
+    }
+    hipGraph_t g = nullptr;
+    const hipError_t e = hipStreamEndCapture(c->stream, &g);
+    c->timing = timing;
+    c->cur = parity;  // capture records launches only
+    if (rc != MPM_OK || e != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return rc != MPM_OK ? rc : mpm_set_error(MPM_E_HIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+    }
+    const hipError_t ei = hipGraphInstantiate(&c->fexec[parity], g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+        c->fexec[parity] = nullptr;
+        return mpm_set_error(MPM_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+    }
+    return MPM_OK;
+}
+
+// Advances every scene s from time[s] to target[s] (float, as the host loop)
+// with the device dt rule; writes the times reached and the substeps taken.
+int device_frames(mpm_context* c, const float* time, const float* target, float* time_out, int* steps_out) {
+    const int n = c->nscenes;
+    std::vector<mpm::FrameState> f((size_t)n);
+    int ndone = 0;
+    for (int s = 0; s < n; ++s) {
+        const mpm_scene_desc& d = c->sc[(size_t)s].desc;
+        mpm::FrameState& st = f[(size_t)s];
+        st.time = time[s];
+        st.target = target[s];
+        st.cfl = d.cfl;
+        st.dx = 1.0f / (float)d.counts;
+        st.min_dt = d.min_dt;
+        st.max_dt = d.max_dt;
+        st.last_dt = c->sc[(size_t)s].last_dt;
+        st.done = !(target[s] > time[s]) ? 1 : 0;
+        st.steps = 0;
+        ndone += st.done;
+    }
+    HIPCHECK(hipMemcpyAsync(c->frame, f.data(), sizeof(mpm::FrameState) * (size_t)n, hipMemcpyHostToDevice, c->stream));
+    while (ndone < n) {
+        const int parity = c->cur;
+        if (!c->fexec[parity]) {
+            const int rc = capture_frame_graph(c, parity);
+            if (rc) return rc;
+        }
+        HIPCHECK(hipGraphLaunch(c->fexec[parity], c->stream));
+        c->binned = false;
+        c->ever_binned = true;
+        c->stepped = true;
+        HIPCHECK(hipMemcpyAsync(c->h_fdone, &c->ctr->frame_ndone, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        HIPCHECK(hipEventRecord(c->fdone_ev, c->stream));
+        HIPCHECK(hipEventSynchronize(c->fdone_ev));
+        ndone = *c->h_fdone;
+    }
+    HIPCHECK(hipMemcpy(f.data(), c->frame, sizeof(mpm::FrameState) * (size_t)n, hipMemcpyDeviceToHost));
+    for (int s = 0; s < n; ++s) {
+        const mpm::FrameState& st = f[(size_t)s];
+        if (time_out) time_out[s] = st.time;
+        if (steps_out) steps_out[s] = st.steps;
+        if (st.steps > 0) c->sc[(size_t)s].last_dt = st.last_dt;
+    }
+    return check_launch("device frame loop");
 }
 
 // One substep as a graph replay: the dts go to d_dt through pinned staging,
@@ -1304,6 +1396,8 @@ int mpm_destroy(mpm_handle c) {
     invalidate_graphs(c);
     if (c->dt_ev) (void)hipEventDestroy(c->dt_ev);
     if (c->h_dt) (void)hipHostFree(c->h_dt);
+    if (c->fdone_ev) (void)hipEventDestroy(c->fdone_ev);
+    if (c->h_fdone) (void)hipHostFree(c->h_fdone);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
     return MPM_OK;
@@ -1444,6 +1538,11 @@ int mpm_advance_to_time(mpm_handle c, double time_in, double target_in, int32_t*
     const float target_time = (float)target_in;
     bool done = false;
     int n = 0;
+    if (c->graph && !c->emitted) {  // the device frame loop
+        int rc = device_frames(c, &time, &target_time, &time, &n);
+        if (rc) return rc;
+        done = true;
+    }
     while (!done) {
         float max_v = 0.0f;
         int rc = mpm_max_particle_velocity(c, &max_v);
@@ -1809,6 +1908,11 @@ int mpm_batch_advance_to_time(mpm_handle c, const double* time, const double* ta
         t[s] = (float)time[s];
         tgt[s] = (float)target[s];
         done[s] = !(tgt[s] > t[s]);
+    }
+    if (c->graph && !c->emitted) {  // the device frame loop
+        int rc = device_frames(c, t.data(), tgt.data(), t.data(), steps.data());
+        if (rc) return rc;
+        std::fill(done.begin(), done.end(), 1);
     }
     for (;;) {
         bool any = false;

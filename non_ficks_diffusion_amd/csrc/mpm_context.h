@@ -69,6 +69,12 @@ struct mpm_context {
     float* h_dt = nullptr;     // pinned staging of d_dt
     hipEvent_t dt_ev = nullptr;
     bool dt_pending = false;
+    // device frame loop: per-scene FrameState, graphs of FRAME_K substeps per
+    // buffer parity, pinned read-back of the frame-done count
+    mpm::FrameState* frame = nullptr;
+    hipGraphExec_t fexec[2] = {nullptr, nullptr};
+    int* h_fdone = nullptr;
+    hipEvent_t fdone_ev = nullptr;
     // serving arena: scenes attach to free entries, slot ranges and particle space
     bool pool = false;
     int pool_cg_max = 0;                            // CG cap of every arena scene

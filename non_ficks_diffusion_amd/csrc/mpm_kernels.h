@@ -58,7 +58,19 @@ struct Counters {
     int overflow;       // block capacity exceeded
     int cg_ndone;       // scenes whose CG solve is finished
     int slot_hw;        // slots [0, slot_hw) hold every scene's range (the binning scans them)
-    int pad[2];
+    int frame_ndone;    // device frame loop: scenes that reached their frame end
+    int pad;
+};
+
+// Device frame loop of one scene (MPM_Driver::Advance_To_Target_Time,
+// MPM_Driver.cpp:55-80): simulated time toward the frame end, the dt rule's
+// parameters (host-rounded exactly as the host rule rounds them), and the
+// substeps taken.
+struct FrameState {
+    float time, target;
+    float cfl, dx, min_dt, max_dt;
+    float last_dt;
+    int done, steps, pad;
 };
 
 // CG scalars of one scene.  rho of iteration k lives in rho_hist[k & 1].
@@ -133,7 +145,8 @@ struct StepArgs {
 
 // launchers (host)
 void launch_set_step(const KP& p, hipStream_t s, const StepArgs& a);
-void launch_set_step_dev(const KP& p, hipStream_t s, const float* dt);  // nscenes dts in device memory
+void launch_set_step_dev(const KP& p, hipStream_t s, const float* dt);
+void launch_frame_dt(const KP& p, hipStream_t s, FrameState* f, float* dt);  // nscenes dts in device memory
 void launch_clear_waiting(const KP& p, hipStream_t s);
 void launch_remove_scene(const KP& p, hipStream_t s, int grid, int scene);
 template <int D> void launch_bin(const KP& p, hipStream_t s, int grid);

@@ -2185,6 +2185,61 @@ __global__ __launch_bounds__(WG) void k_set_step_dev(KP p, const float* dt) {
     sc.active = d > 0.0f ? 1 : 0;
 }
 
+// The dt of the next substep of every scene in the device frame loop: the
+// reference's rule (MPM_Driver.cpp:55-80) on the scene's max |V| from its last
+// G2P (MPM_Example::Max_Particle_Velocity), in the same float operations as
+// the host loop, so both take the same dts: dt = clamp(cfl dx / max(vmax,
+// 1e-2), min_dt, max_dt); the last substep of the frame takes what remains
+// when that is under 1.001 dt, and the one before half of it when it is
+// under 2 dt.  A scene at its frame end gets dt 0 (it idles).  One work-group.
+__global__ __launch_bounds__(WG) void k_frame_dt(KP p, FrameState* f, float* dt_out) {
+#pragma clang fp contract(off)
+    int done = 0;
+    for (int s = threadIdx.x; s < p.nscenes; s += WG) {
+        FrameState st = f[s];
+        float d = 0.0f;
+        if (!st.done) {
+            const float v2 = __uint_as_float(p.sst[s].max_v2);
+            const float vmax = (float)sqrt((double)v2);  // the correctly rounded float sqrt
+            const float vfloor = vmax < 1e-2f ? 1e-2f : vmax;
+            const float cfl_dt = st.cfl * st.dx / vfloor;
+            const float capped = cfl_dt < st.max_dt ? cfl_dt : st.max_dt;
+            d = st.min_dt < capped ? capped : st.min_dt;
+            bool last = false;
+            const float rem = st.target - st.time;
+            if (rem < d * 1.001f) {
+                d = rem;
+                last = true;
+            } else if (rem < 2.0f * d) {
+                d = rem * 0.5f;
+            }
+            if (!(d > 0.0f)) {
+                st.done = 1;
+                d = 0.0f;
+            } else {
+                st.time += d;
+                st.steps += 1;
+                st.last_dt = d;
+                if (last) st.done = 1;
+            }
+            f[s] = st;
+        }
+        dt_out[s] = d;
+        done += st.done ? 1 : 0;
+    }
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) done += __shfl_xor(done, off);
+    __shared__ int wsum[WG / 64];
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = done;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int total = 0;
+#pragma unroll
+        for (int w = 0; w < WG / 64; ++w) total += wsum[w];
+        p.ctr->frame_ndone = total;
+    }
+}
+
 // ============================================================================
 // launchers
 // ============================================================================
@@ -2193,6 +2248,9 @@ void launch_set_step(const KP& p, hipStream_t s, const StepArgs& a) {
 }
 void launch_set_step_dev(const KP& p, hipStream_t s, const float* dt) {
     hipLaunchKernelGGL(k_set_step_dev, dim3((p.nscenes + WG - 1) / WG), dim3(WG), 0, s, p, dt);
+}
+void launch_frame_dt(const KP& p, hipStream_t s, FrameState* f, float* dt) {
+    hipLaunchKernelGGL(k_frame_dt, dim3(1), dim3(WG), 0, s, p, f, dt);
 }
 void launch_clear_waiting(const KP& p, hipStream_t s) {
     hipLaunchKernelGGL(k_clear_waiting, dim3((p.nscenes + WG - 1) / WG), dim3(WG), 0, s, p);

@@ -152,3 +152,71 @@ def test_single_scene_operator_modes_agree():
     assert_same(runs[1].download(), runs[0].download(), "modes")
     for s in runs:
         s.close()
+
+
+def host_rule_steps(d, time, target, vmax=0.0):
+    """MPM_Driver::Advance_To_Target_Time's dt sequence in float32 arithmetic
+    (MPM_Driver.cpp:55-80) for a constant max speed."""
+    f = np.float32
+    t, tgt, n = f(time), f(target), 0
+    dx = f(1.0) / f(d.counts)
+    while True:
+        v = f(vmax)
+        dt = f(d.cfl) * dx / (f(1e-2) if v < f(1e-2) else v)
+        dt = dt if dt < f(d.max_dt) else f(d.max_dt)
+        dt = dt if f(d.min_dt) < dt else f(d.min_dt)
+        last = False
+        if tgt - t < dt * f(1.001):
+            dt, last = tgt - t, True
+        elif tgt - t < f(2.0) * dt:
+            dt = (tgt - t) * f(0.5)
+        if not dt > 0:
+            break
+        t = f(t + dt)
+        n += 1
+        if last:
+            break
+    return float(t), n
+
+
+def test_device_frame_loop_takes_the_host_rules_dts():
+    """Captured substeps advance to a frame end with the dt rule on the
+    device.  A scene at rest without gravity keeps max |V| = 0, so the dt
+    sequence is fixed: the substep count and the end time equal the rule's
+    in float32 arithmetic, including the end-of-frame halving."""
+    d, x, v, m = M.scene_generate("two_balls", 24, 0)
+    d.gravity[:] = (0.0, 0.0, 0.0)
+    v = np.zeros_like(v)
+    s = M.Scene(d, len(m))
+    s.upload(x, v, m)
+    s.initialize()
+    s.use_graph(True)
+    t = 0.0
+    for frame in (1, 2, 3):
+        target = float(np.float32(frame) / np.float32(24.0))
+        t_exp, n_exp = host_rule_steps(d, t, target)
+        t, n, _ = s.advance_to_time(t, target)
+        assert n == n_exp and t == t_exp, (frame, n, n_exp, t, t_exp)
+    assert s.max_particle_velocity() == 0.0
+    s.close()
+
+
+def test_device_frame_loop_matches_host_loop():
+    """A batch advanced a frame by the device loop (captured substeps) and by
+    the host loop (launched substeps): equal substep counts per scene, end
+    times within float rounding, states to the bar of a substep chain."""
+    data = load()
+    host, dev = make_batch(data), make_batch(data)
+    dev.use_graph(True)
+    target = float(np.float32(1.0) / np.float32(24.0))
+    th, nh = host.advance_to_time(0.0, target)
+    td, nd = dev.advance_to_time(0.0, target)
+    assert np.array_equal(nh, nd), (nh, nd)
+    assert np.allclose(th, td, rtol=1e-6, atol=0)
+    for k in range(len(data)):
+        a, b = host.download(k), dev.download(k)
+        assert np.array_equal(a["ids"], b["ids"])
+        for f in ("x", "v", "F"):
+            assert rel(b[f], a[f]) < 1e-4, (k, f, rel(b[f], a[f]))
+    host.close()
+    dev.close()
