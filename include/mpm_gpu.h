@@ -371,7 +371,7 @@ typedef struct mpm_serve_config {
                                     count has doubled since the last tune (from 1M particles on) */
 } mpm_serve_config;
 
-enum { MPM_REQ_QUEUED = 0, MPM_REQ_RUNNING = 1, MPM_REQ_DONE = 2 };
+enum { MPM_REQ_QUEUED = 0, MPM_REQ_RUNNING = 1, MPM_REQ_DONE = 2, MPM_REQ_CANCELLED = 3 };
 
 typedef struct mpm_request_status {
     int32_t state;               /* MPM_REQ_* */
@@ -411,6 +411,12 @@ int mpm_serve_poll(mpm_server_handle s, int64_t request_id, mpm_request_status* 
 int mpm_serve_download(mpm_server_handle s, int64_t request_id, int64_t cap, float* x, float* v,
                        float* F, float* Jp, float* mass, float* volume, int32_t* ids,
                        int64_t* n_out);
+/* Withdraws a queued request, or a running one (its particles leave the
+ * arena at once, freeing room for admissions); the request is then
+ * MPM_REQ_CANCELLED and can be released.  MPM_E_INVALID if it already
+ * finished.  (No reference counterpart: the reference runs one scene per
+ * process.) */
+int mpm_serve_cancel(mpm_server_handle s, int64_t request_id);
 /* Completion times (server clock, seconds) of a request's frames so far;
  * *n = frames completed, at most cap times written. */
 int mpm_serve_frame_times(mpm_server_handle s, int64_t request_id, int32_t cap, double* times,

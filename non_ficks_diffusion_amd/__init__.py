@@ -123,7 +123,7 @@ class TickStats(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-REQ_QUEUED, REQ_RUNNING, REQ_DONE = 0, 1, 2
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_CANCELLED = 0, 1, 2, 3
 
 _FP = ctypes.POINTER(ctypes.c_float)
 _IP = ctypes.POINTER(ctypes.c_int32)
@@ -175,6 +175,7 @@ SIGNATURES = {
     "mpm_serve_download": (ctypes.c_int, [_H, ctypes.c_int64, ctypes.c_int64, _FP, _FP, _FP, _FP, _FP, _FP, _IP,
                                           _LP]),
     "mpm_serve_release": (ctypes.c_int, [_H, ctypes.c_int64]),
+    "mpm_serve_cancel": (ctypes.c_int, [_H, ctypes.c_int64]),
     "mpm_serve_frame_times": (ctypes.c_int, [_H, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double),
                                              _IP]),
     "mpm_batch_create": (ctypes.c_int, [ctypes.POINTER(SceneDesc), _LP, ctypes.c_int32, ctypes.POINTER(_H)]),
@@ -654,4 +655,8 @@ class Server:
 
     def release(self, rid: int) -> None:
         _check(lib().mpm_serve_release(self._h, int(rid)))
+
+    def cancel(self, rid: int) -> None:
+        """Withdraws a queued or running request (REQ_CANCELLED)."""
+        _check(lib().mpm_serve_cancel(self._h, int(rid)))
         self._n.pop(rid, None)

@@ -441,7 +441,7 @@ int fill_stats(mpm_context* c, int s, mpm_step_stats* out) {
     std::memcpy(&v2, &st.max_v2, sizeof(v2));
     out->max_speed = std::sqrt((double)v2);
     if (h.overflow) return mpm_set_error(MPM_E_CAPACITY, "active blocks exceed the block capacity");
-    if (st.nonfinite) return mpm_set_error(MPM_E_NONFINITE, "non-finite particle position");
+    if (st.nonfinite) return mpm_set_error(MPM_E_NONFINITE, "non-finite particle state (the particles were removed)");
     return MPM_OK;
 }
 
@@ -564,6 +564,24 @@ int check_desc(const mpm_scene_desc& d, const char* who) {
 }
 static_assert(mpm::MAX_MATERIALS == MPM_MAX_MATERIALS, "material table size");
 
+// Particle inputs: finite positions, velocities and (when given) F, Jp and
+// volumes, and a finite positive mass.
+int check_particles(const char* who, int D, int64_t n, const float* x, const float* v, const float* F, const float* Jp,
+                    const float* mass, const float* volume) {
+    for (int64_t i = 0; i < n; ++i) {
+        bool ok = std::isfinite(mass[i]) && mass[i] > 0.0f;
+        for (int a = 0; a < D; ++a) ok = ok && std::isfinite(x[i * D + a]) && std::isfinite(v[i * D + a]);
+        if (F)
+            for (int k = 0; k < D * D; ++k) ok = ok && std::isfinite(F[i * D * D + k]);
+        if (Jp) ok = ok && std::isfinite(Jp[i]);
+        if (volume) ok = ok && std::isfinite(volume[i]) && volume[i] >= 0.0f;
+        if (!ok)
+            return mpm_set_error(MPM_E_INVALID, std::string(who) + ": particle " + std::to_string(i) +
+                                                    " has a non-finite value or a mass that is not positive");
+    }
+    return MPM_OK;
+}
+
 // Material entry of particle i (material == nullptr: 0), checked against the
 // scene's table; the Jp slot of an EOS particle starts at its density.
 int particle_material(const mpm_scene_desc& d, const int32_t* material, int64_t i, int& id, bool& eos, float& rho0) {
@@ -683,6 +701,10 @@ int upload_scene(mpm_context* c, int s, int64_t n, const float* x, const float* 
                  const float* mass, const float* volume, const int32_t* material) {
     SceneHost& sh = c->sc[(size_t)s];
     if (n < 0 || n > sh.capacity) return mpm_set_error(MPM_E_CAPACITY, "mpm_upload_particles: n exceeds capacity");
+    {
+        const int rc = check_particles("mpm_upload_particles", c->D, n, x, v, F, Jp, mass, volume);
+        if (rc) return rc;
+    }
     const int D = c->D, NF = nfields(D);
     const int64_t cap = sh.capacity;
     std::vector<float> h((size_t)NF * (size_t)cap, 0.0f);
@@ -1266,6 +1288,8 @@ int pool_attach(mpm_context* c, const mpm_scene_desc& d, int64_t n, const float*
     if (d.cg_max_iterations < 1 || d.cg_max_iterations > c->pool_cg_max)
         return mpm_set_error(MPM_E_INVALID, "arena attach: CG cap above the arena's");
     if (n < 1 || !x || !v || !mass) return mpm_set_error(MPM_E_INVALID, "arena attach: no particles");
+    rc = check_particles("arena attach", c->D, n, x, v, nullptr, nullptr, mass, nullptr);
+    if (rc) return rc;
     int e = -1;
     for (int k = 0; k < c->nscenes; ++k)
         if (!c->entry_used[(size_t)k]) { e = k; break; }
@@ -1354,6 +1378,10 @@ int pool_detach(mpm_context* c, int e) {
     return update_slot_hw(c);
 }
 
+int check_particles(const char* who, int D, int64_t n, const float* x, const float* v, const float* F,
+                    const float* Jp, const float* mass, const float* volume) {
+    return ::check_particles(who, D, n, x, v, F, Jp, mass, volume);
+}
 int substep(mpm_context* c, const float* dts) { return do_substep(c, dts); }
 int max_velocities(mpm_context* c, float* out) { return ::max_velocities(c, out); }
 int download(mpm_context* c, int scene, int64_t cap, float* x, float* v, float* F, float* Jp, float* mass,
@@ -1442,8 +1470,10 @@ int mpm_add_particles(mpm_handle c, int32_t scene, int64_t n, const float* x, co
     if (n == 0) return MPM_OK;
     SceneHost& sh = c->sc[(size_t)scene];
     const mpm_scene_desc& d = sh.desc;
+    int rc = check_particles("mpm_add_particles", c->D, n, x, v, nullptr, nullptr, mass, nullptr);
+    if (rc) return rc;
     int64_t n_live = 0;
-    int rc = mpm_host::live_particles(c, &n_live);
+    rc = mpm_host::live_particles(c, &n_live);
     if (rc) return rc;
     if (n_live + n > c->pcap) return mpm_set_error(MPM_E_CAPACITY, "mpm_add_particles: particle capacity exhausted");
     const int D = c->D, NF = nfields(D);
@@ -1451,8 +1481,6 @@ int mpm_add_particles(mpm_handle c, int32_t scene, int64_t n, const float* x, co
     auto f = [&](int field, int64_t i) -> float& { return h[(size_t)field * (size_t)n + (size_t)i]; };
     for (int64_t i = 0; i < n; ++i) {
         for (int a = 0; a < D; ++a) {
-            if (!std::isfinite(x[i * D + a]) || !std::isfinite(v[i * D + a]))
-                return mpm_set_error(MPM_E_INVALID, "mpm_add_particles: non-finite position or velocity");
             f(a, i) = x[i * D + a];
             f(D + a, i) = v[i * D + a];
         }

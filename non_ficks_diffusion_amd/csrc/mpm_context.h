@@ -113,6 +113,10 @@ int pool_create(int D, int max_scenes, int64_t pcap, int64_t nslots, int64_t bca
 // not available.
 int pool_attach(mpm_context* c, const mpm_scene_desc& d, int64_t n, const float* x, const float* v,
                 const float* mass, const int32_t* material, int* entry);
+// Finite particle inputs with a positive mass (F, Jp, volume may be NULL);
+// MPM_E_INVALID otherwise.
+int check_particles(const char* who, int D, int64_t n, const float* x, const float* v, const float* F,
+                    const float* Jp, const float* mass, const float* volume);
 // Removes a scene's particles and frees its entry and slot range.
 int pool_detach(mpm_context* c, int entry);
 // Largest free slot range.

@@ -2026,7 +2026,7 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
                         p.pcur[(PF<D>::F + r * D + cc) * p.pcap + i] = F[r * D + cc] + s;
                     }
             }
-            bool inside = true;
+            bool inside = true, finite = true;
             float v2 = 0.0f;
 #pragma unroll
             for (int a = 0; a < D; ++a) {
@@ -2035,8 +2035,12 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
                 p.pcur[(PF<D>::V + a) * p.pcap + i] = V;
                 p.pcur[(PF<D>::X + a) * p.pcap + i] = Xn;
                 inside = inside && Xn >= lo && Xn <= hi;
+                finite = finite && isfinite(V) && isfinite(Xn);
                 v2 += V * V;
             }
+            // a NaN/Inf state leaves the domain test false (removal, as in the
+            // reference) and flags the scene for the caller
+            if (!finite) atomicOr(&p.sst[p.block_scene[b]].nonfinite, 1);
             if (!inside) {
                 p.keycur[i] = -1;
                 ++removed;

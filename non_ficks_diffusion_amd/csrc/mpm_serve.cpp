@@ -214,6 +214,10 @@ int mpm_serve_submit_ex(mpm_server_handle s, const mpm_scene_desc* desc, int64_t
     if (desc->cg_max_iterations < 1 || desc->cg_max_iterations > s->cfg.cg_max_iterations)
         return mpm_set_error(MPM_E_INVALID, "mpm_serve_submit: CG cap above the server's");
     if (n < 1 || frames < 1) return mpm_set_error(MPM_E_INVALID, "mpm_serve_submit: no particles or frames");
+    {
+        const int rc = mpm_host::check_particles("mpm_serve_submit", desc->dim, n, x, v, nullptr, nullptr, mass, nullptr);
+        if (rc) return rc;
+    }
     if (!(desc->frame_rate > 0.0f)) return mpm_set_error(MPM_E_INVALID, "mpm_serve_submit: frame rate must be positive");
     Request r;
     r.desc = *desc;
@@ -407,8 +411,37 @@ int mpm_serve_release(mpm_server_handle s, int64_t id) {
     if (!s) return mpm_set_error(MPM_E_INVALID, "mpm_serve_release: null handle");
     auto it = s->req.find(id);
     if (it == s->req.end()) return mpm_set_error(MPM_E_INVALID, "mpm_serve_release: unknown request");
-    if (it->second.state != MPM_REQ_DONE) return mpm_set_error(MPM_E_NOT_READY, "mpm_serve_release: request not finished");
+    if (it->second.state != MPM_REQ_DONE && it->second.state != MPM_REQ_CANCELLED)
+        return mpm_set_error(MPM_E_NOT_READY, "mpm_serve_release: request not finished");
     s->req.erase(it);
+    return MPM_OK;
+}
+
+int mpm_serve_cancel(mpm_server_handle s, int64_t id) {
+    if (!s) return mpm_set_error(MPM_E_INVALID, "mpm_serve_cancel: null handle");
+    auto it = s->req.find(id);
+    if (it == s->req.end()) return mpm_set_error(MPM_E_INVALID, "mpm_serve_cancel: unknown request");
+    Request& r = it->second;
+    if (r.state == MPM_REQ_DONE || r.state == MPM_REQ_CANCELLED)
+        return mpm_set_error(MPM_E_INVALID, "mpm_serve_cancel: request already finished");
+    if (r.state == MPM_REQ_QUEUED) {
+        s->queue.erase(std::find(s->queue.begin(), s->queue.end(), id));
+        std::vector<float>().swap(r.x);
+        std::vector<float>().swap(r.v);
+        std::vector<float>().swap(r.m);
+        std::vector<int32_t>().swap(r.mat);
+    } else {
+        // leaves the arena at once: its particles, entry and slot range are freed
+        const int rc = mpm_host::pool_detach(s->c, r.entry);
+        if (rc) return rc;
+        s->by_entry[(size_t)r.entry] = -1;
+        s->particles_in_use -= r.n;
+        s->blocks_in_use -= r.blocks;
+        s->bytes_in_use -= r.bytes;
+        r.entry = -1;
+    }
+    r.state = MPM_REQ_CANCELLED;
+    r.finished = s->now();
     return MPM_OK;
 }
 

@@ -1,0 +1,181 @@
+"""Failure handling through the C ABI, every condition forced from the host
+(no kernel is made to fault): non-finite or non-positive inputs are refused
+with a status, a substep whose dt drives the state non-finite reports
+MPM_E_NONFINITE and drops those particles (the reference drops them
+silently, MPM_Example.cpp:650-660), particles outside the domain are
+dropped at binning, a CG capped below convergence reports it in its stats
+(the reference ignores the solver's result, MPM_Example.cpp:711), inverted
+and singular F go through the SVD, and the request server cancels queued
+and running requests.
+"""
+import numpy as np
+import pytest
+
+import non_ficks_diffusion_amd as M
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+E_INVALID, E_NONFINITE = -1, -5
+
+
+def rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-30))
+
+
+def scene(name="two_balls", counts=24):
+    d, x, v, m = M.scene_generate(name, counts, 0)
+    return d, x.copy(), v.copy(), m.copy()
+
+
+def test_non_finite_and_non_positive_inputs_are_refused():
+    d, x, v, m = scene()
+    s = M.Scene(d, len(m) + 10)
+    for field, value in (("x", np.nan), ("v", np.inf), ("m", 0.0), ("m", -1.0), ("F", np.nan), ("Jp", np.inf)):
+        xx, vv, mm = x.copy(), v.copy(), m.copy()
+        F = np.tile(np.eye(3, dtype=np.float32), (len(m), 1, 1))
+        Jp = np.ones(len(m), np.float32)
+        {"x": xx[3], "v": vv[5], "m": mm[7:8], "F": F[2, 1], "Jp": Jp[4:5]}[field][...] = value
+        with pytest.raises(M.MPMError) as e:
+            s.upload(xx, vv, mm, F=F, Jp=Jp)
+        assert e.value.code == E_INVALID, field
+    s.upload(x, v, m)
+    s.initialize()
+    bad = x[:2].copy()
+    bad[1, 0] = np.nan
+    with pytest.raises(M.MPMError):
+        s.add_particles(bad, v[:2], m[:2])
+    s.substep(1e-3)  # the refused calls left the scene as it was
+    assert s.stats()["live_particles"] == len(m)
+    s.close()
+    srv = M.Server(1 << 30, max_scenes=2, max_counts=64)
+    vv = v.copy()
+    vv[0, 1] = np.nan
+    with pytest.raises(M.MPMError):
+        srv.submit(d, x, vv, m, 1)
+    srv.close()
+
+
+def test_non_finite_state_is_reported_and_dropped():
+    """dt = 1e30 drives v* and the CG to Inf/NaN: the substep's stats report
+    MPM_E_NONFINITE, the non-finite particles are gone, and the handle keeps
+    running on what is left."""
+    d, x, v, m = scene()
+    s = M.Scene(d, len(m))
+    s.upload(x, v, m)
+    s.initialize()
+    with pytest.raises(M.MPMError) as e:
+        s.substep(1e30, stats=True)
+    assert e.value.code == E_NONFINITE
+    g = s.download()
+    assert np.all(np.isfinite(g["x"])) and len(g["ids"]) < len(m)
+    s.close()
+
+
+def test_particles_outside_the_domain_are_dropped():
+    d, x, v, m = scene()
+    x[0] = (1.2, 0.5, 0.5)
+    x[1] = (-0.1, 0.5, 0.5)
+    s = M.Scene(d, len(m))
+    s.upload(x, v, m)
+    s.initialize()
+    s.substep(1e-3)
+    g = s.download()
+    assert 0 not in g["ids"] and 1 not in g["ids"] and len(g["ids"]) == len(m) - 2
+    assert s.stats()["live_particles"] == len(m) - 2
+    s.close()
+
+
+def test_cg_capped_below_convergence_is_reported():
+    d, x, v, m = scene()
+    d.cg_max_iterations = 1
+    d.cg_tolerance = 1e-12
+    s = M.Scene(d, len(m))
+    s.upload(x, v, m)
+    s.initialize()
+    st = s.substep(2e-3, stats=True)
+    assert st["cg_converged"] == 0 and st["cg_iterations"] == 1
+    d.cg_max_iterations = 50
+    d.cg_tolerance = 1e-6
+    s2 = M.Scene(d, len(m))
+    s2.upload(x, v, m)
+    s2.initialize()
+    assert s2.substep(2e-3, stats=True)["cg_converged"] == 1
+    s.close()
+    s2.close()
+
+
+def test_inverted_and_singular_deformation_gradients():
+    """An inverted F (a reflection, det < 0) follows the FP64 oracle to the
+    one-substep bar; a singular F (rank 1) gives a finite state.  At J = -1
+    the lambda (J - 1) J term of P is large, so the material is softened and
+    the CG given room to converge: the comparison is of converged solves."""
+    d, x, v, m = scene()
+    n = len(m)
+    F = np.tile(np.eye(3, dtype=np.float32), (n, 1, 1))
+    inv = np.arange(n) % 7 == 0
+    F[inv] = np.diag([1.02, 0.97, -1.01]).astype(np.float32)  # det < 0
+    d.material.plastic = 0
+    d.material.mu0 *= 1e-3
+    d.material.lambda0 *= 1e-3
+    d.cg_tolerance = 1e-7
+    d.cg_max_iterations = 200
+    s = M.Scene(d, n)
+    s.upload(x, v, m, F=F)
+    s.initialize()
+    st = s.substep(1e-3, stats=True)
+    assert st["cg_converged"] == 1
+    g = s.download()
+    o = O.Oracle(d, 8, 4, O.DOT_FP64 | O.SVD_FP64)
+    o.set_particles(x, v, m, F=F)
+    o.initialize()
+    o.advance_step(1e-3)
+    r = o.particles()
+    for k in ("x", "v", "F"):
+        assert rel(g[k], r[k]) < 1e-5, (k, rel(g[k], r[k]))
+    assert np.all(np.linalg.det(g["F"][inv]) < 0)
+    s.close()
+    Fs = np.tile(np.eye(3, dtype=np.float32), (n, 1, 1))
+    Fs[::5] = np.outer([1.0, 0.5, 0.2], [0.3, 0.9, 0.1]).astype(np.float32)  # rank 1
+    s = M.Scene(d, n)
+    s.upload(x, v, m, F=Fs)
+    s.initialize()
+    s.substep(1e-3, stats=True)
+    g = s.download()
+    for k in ("x", "v", "F", "Jp"):
+        assert np.all(np.isfinite(g[k])), k
+    s.close()
+
+
+def test_server_cancels_queued_and_running_requests():
+    srv = M.Server(1 << 30, max_scenes=1, max_counts=64, keep_results=True)
+    reqs = [M.scene_generate("two_balls", 32, 0, 3000), M.scene_generate("blob", 32, 1003, 2000),
+            M.scene_generate("slab", 32, 1004, 2500)]
+    ids = [srv.submit(d, x, v, m, 2) for d, x, v, m in reqs]
+    srv.tick()
+    assert srv.poll(ids[0])["state"] == M.REQ_RUNNING and srv.poll(ids[1])["state"] == M.REQ_QUEUED
+    srv.cancel(ids[1])  # queued
+    assert srv.poll(ids[1])["state"] == M.REQ_CANCELLED
+    srv.cancel(ids[0])  # running: leaves the arena, the entry goes to the next request
+    assert srv.poll(ids[0])["state"] == M.REQ_CANCELLED
+    t = srv.tick()
+    assert t["admitted"] == 1 and srv.poll(ids[2])["state"] == M.REQ_RUNNING
+    while srv.poll(ids[2])["state"] != M.REQ_DONE:
+        srv.tick()
+    # the survivor's result is that of the scene alone
+    d, x, v, m = reqs[2]
+    s = M.Scene(d, len(m))
+    s.upload(x, v, m)
+    s.initialize()
+    t = 0.0
+    for f in range(2):
+        t, _, _ = s.advance_to_time(t, float(np.float32(f + 1) / np.float32(24.0)))
+    assert rel(srv.download(ids[2])["x"], s.download()["x"]) < 1e-4
+    s.close()
+    with pytest.raises(M.MPMError):
+        srv.cancel(ids[2])  # already finished
+    for rid in ids:
+        srv.release(rid)
+    srv.close()
