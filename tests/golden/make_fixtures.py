@@ -22,20 +22,24 @@ CASES = [
     ("disk2d_64_5steps", "disk2d", 64, 5e-4, 5),
     ("two_balls_16_3steps", "two_balls", 16, 1e-3, 3),
     ("avalanche_32_3steps", "avalanche", 32, 1e-3, 3),
+    # the reference's 2D test 16 (two materials) and test 19 (EOS fluid)
+    ("ref2d_test16_64_5steps", "ref2d_test16", 64, 2e-3, 5),
+    ("ref2d_test19_64_5steps", "ref2d_test19", 64, 2e-3, 5),
 ]
 
 
 def make(name, scene, counts, dt, steps):
     d, x, v, m = M.scene_generate(scene, counts, 0)
+    mat = M.scene_materials(scene, counts, 0)
     o = O.Oracle(d, 8, 1, O.DOT_FP64 | O.SVD_FP64)
-    o.set_particles(x, v, m)
+    o.set_particles(x, v, m, material=mat)
     o.initialize()
     for _ in range(steps):
         o.advance_step(dt)
     p = o.particles()
     st = o.cg_stats()
     return dict(scene=np.array(scene), counts=np.int32(counts), dt=np.float64(dt), steps=np.int32(steps),
-                x0=x, v0=v, m0=m, ids=p["ids"], x=p["x"], v=p["v"], F=p["F"], Jp=p["Jp"], volume=p["volume"],
+                x0=x, v0=v, m0=m, material0=mat, ids=p["ids"], x=p["x"], v=p["v"], F=p["F"], Jp=p["Jp"], volume=p["volume"],
                 cg_iterations=np.int32(st["cg_iterations"]))
 
 

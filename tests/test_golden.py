@@ -28,8 +28,9 @@ def test_oracle_reproduces_fixture(path):
     f = load(path)
     d, x, v, m = M.scene_generate(str(f["scene"]), int(f["counts"]), 0)
     assert np.array_equal(x, f["x0"]) and np.array_equal(m, f["m0"])
+    mat = f.get("material0")
     o = O.Oracle(d, 8, 1, O.DOT_FP64 | O.SVD_FP64)
-    o.set_particles(x, v, m)
+    o.set_particles(x, v, m, material=mat)
     o.initialize()
     for _ in range(int(f["steps"])):
         o.advance_step(float(f["dt"]))
@@ -45,7 +46,7 @@ def test_gpu_matches_fixture(path):
     f = load(path)
     d, x, v, m = M.scene_generate(str(f["scene"]), int(f["counts"]), 0)
     s = M.Scene(d, len(m))
-    s.upload(x, v, m)
+    s.upload(x, v, m, material=f.get("material0"))
     s.initialize()
     for _ in range(int(f["steps"])):
         s.substep(float(f["dt"]))
