@@ -10,7 +10,8 @@ import non_ficks_diffusion_amd as M
 
 pytestmark = pytest.mark.gpu
 
-SCENES = [("two_balls", 24, 0), ("blob", 32, 1), ("slab", 28, 2), ("avalanche", 32, 0), ("ref3d_test18", 20, 0)]
+SCENES = [("two_balls", 24, 0), ("blob", 32, 1), ("slab", 28, 2), ("avalanche", 32, 0), ("ref3d_test18", 20, 0),
+          ("ref3d_test19", 32, 0)]  # the last: EOS fluid (every particle entry 0, an EOS material)
 
 
 def rel(a, b):
@@ -42,7 +43,8 @@ def run_alone(d, x, v, m, dts, material=None):
 def test_batch_matches_scenes_alone():
     data = load(SCENES)
     # per-scene dt sequences of different lengths: a scene with dt 0 idles
-    dts = [[1e-3, 1e-3, 5e-4], [2e-3, 1e-3, 1e-3], [1e-3, 0.0, 1e-3], [5e-4, 5e-4, 5e-4], [2e-3, 2e-3, 0.0]]
+    dts = [[1e-3, 1e-3, 5e-4], [2e-3, 1e-3, 1e-3], [1e-3, 0.0, 1e-3], [5e-4, 5e-4, 5e-4], [2e-3, 2e-3, 0.0],
+           [2e-3, 1e-3, 2e-3]]
     b = M.Batch([d for d, _, _, _ in data], [len(m) for _, _, _, m in data])
     for k, (d, x, v, m) in enumerate(data):
         b.upload(k, x, v, m)
