@@ -1965,6 +1965,17 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
         if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
         slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
+        // the first particle's position and velocity load while the tile is gathered
+        const int i0 = p.block_pstart[b] + (int)threadIdx.x, i1 = p.block_pstart[b + 1];
+        float nX[D], nV[D];
+        auto load = [&](int i) {
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                nX[a] = p.pcur[(PF<D>::X + a) * p.pcap + i];
+                nV[a] = p.pcur[(PF<D>::V + a) * p.pcap + i];
+            }
+        };
+        if (i0 < i1) load(i0);
         __syncthreads();
         const float* src[2 * D];
 #pragma unroll
@@ -1974,16 +1985,29 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
         }
         gather_tile<D, 2 * D>(p, nbr, vt, src);
         __syncthreads();
-        const int i1 = p.block_pstart[b + 1];
-        for (int i = p.block_pstart[b] + (int)threadIdx.x; i < i1; i += BWG) {
-            float w[D][3], dw[D][3], X[D];
+        for (int i = i0; i < i1; i += BWG) {
+            float w[D][3], dw[D][3], X[D], V0[D];
             int l[D];
-            if (!particle_weights<D>(p, sc, i, bb, w, dw, l, X)) continue;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                X[a] = nX[a];
+                V0[a] = nV[a];
+            }
+            if (i + BWG < i1) load(i + BWG);
+            bool in_block = true;
+#pragma unroll
+            for (int a = 0; a < D; ++a) {
+                int cl;
+                bspline_weights(X[a], sc.dx, sc.inv_dx, cl, w[a], dw[a]);
+                l[a] = cl + Dm::B - Dm::B * bb[a];
+                in_block = in_block && l[a] >= 0 && l[a] < Dm::B;
+            }
+            if (!in_block) continue;
             float vpic[D], vflip[D], G[DD];
 #pragma unroll
             for (int a = 0; a < D; ++a) {
                 vpic[a] = 0.0f;
-                vflip[a] = p.pcur[(PF<D>::V + a) * p.pcap + i];
+                vflip[a] = V0[a];
             }
 #pragma unroll
             for (int k = 0; k < DD; ++k) G[k] = 0.0f;
