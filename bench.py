@@ -30,6 +30,11 @@ OPERATOR_REPS = 20
 # HBM bytes per launch of the CG operator at this workload, from rocprofv3
 # FETCH_SIZE and WRITE_SIZE runs scaled by a copy kernel of known size
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "cg_force_avalanche_256_traffic.txt")
+# VALU wave-instructions per launch of the CG operator at this workload, from
+# a rocprofv3 SQ_INSTS_VALU run, and MI355X's VALU issue peak without packed
+# FP32 (256 CUs x 4 SIMDs x one wave64 instruction per 4 cycles x 2.4 GHz)
+INSTRUCTIONS_FILE = os.path.join(ROOT, "profiles", "cg_force_avalanche_256_instructions.txt")
+VALU_PEAK_G = 614.4
 WORKLOAD = "avalanche"
 COUNTS = 256
 
@@ -66,6 +71,27 @@ def cpu_baseline(desc, x, v, m, dt, substeps=2):
     return {"value": live * substeps / el, "unit": "particle-substeps/s", "cores": cores, "kind": "port",
             "sample": f"{WORKLOAD} {COUNTS}^3, {len(m)} particles, 1 warm-up + {substeps} fixed-dt substeps "
                       f"(dt {dt:.2e}, CG cap {desc.cg_max_iterations}), {el:.1f} s"}
+
+
+def measured_valu_insts():
+    """cg_force_valu_insts_per_launch from the committed counter summary, or None."""
+    try:
+        with open(INSTRUCTIONS_FILE) as f:
+            for line in f:
+                if line.startswith("cg_force_valu_insts_per_launch:"):
+                    return int(line.split(":", 1)[1].split()[0])
+    except OSError:
+        pass
+    return None
+
+
+def valu_roofline(avg_cg_ms):
+    insts = measured_valu_insts()
+    if insts is None or not avg_cg_ms > 0:
+        return None
+    achieved = insts / (avg_cg_ms * 1e-3) / 1e9
+    return {"kernel": "cg_force", "unit": "G wave-instructions/s", "insts_per_launch": insts,
+            "achieved": round(achieved, 1), "peak": VALU_PEAK_G, "frac": round(achieved / VALU_PEAK_G, 4)}
 
 
 def measured_traffic():
@@ -467,6 +493,9 @@ def main():
                      "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "avg_launch_ms": round(avg_cg_ms, 5), "launches_timed": OPERATOR_REPS,
                      "algorithmic_bytes_per_launch": int(cb["cg_force"])},
+        # the operator's own bound: VALU issue (counter-measured instructions per launch
+        # over the live launch time, against the issue peak)
+        "valu_roofline": valu_roofline(avg_cg_ms),
         "substep_roofline": {"algorithmic_bytes": int(cb["substep"]), "kernel_ms": round(sub_ms, 4),
                              "achieved_GBs": round(cb["substep"] / (sub_ms * 1e-3) / 1e9, 1)},
         "kernel_GBs": per_kernel_gbs,
