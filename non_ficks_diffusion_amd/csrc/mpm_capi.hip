@@ -564,8 +564,9 @@ int check_desc(const mpm_scene_desc& d, const char* who) {
 }
 static_assert(mpm::MAX_MATERIALS == MPM_MAX_MATERIALS, "material table size");
 
-// Particle inputs: finite positions, velocities and (when given) F, Jp and
-// volumes, and a finite positive mass.
+// Particle inputs: finite positions, velocities and (when given) F and
+// volumes, a positive Jp (det Fp, or an EOS particle's density) and a finite
+// positive mass.
 int check_particles(const char* who, int D, int64_t n, const float* x, const float* v, const float* F, const float* Jp,
                     const float* mass, const float* volume) {
     for (int64_t i = 0; i < n; ++i) {
@@ -573,11 +574,11 @@ int check_particles(const char* who, int D, int64_t n, const float* x, const flo
         for (int a = 0; a < D; ++a) ok = ok && std::isfinite(x[i * D + a]) && std::isfinite(v[i * D + a]);
         if (F)
             for (int k = 0; k < D * D; ++k) ok = ok && std::isfinite(F[i * D * D + k]);
-        if (Jp) ok = ok && std::isfinite(Jp[i]);
+        if (Jp) ok = ok && std::isfinite(Jp[i]) && Jp[i] > 0.0f;
         if (volume) ok = ok && std::isfinite(volume[i]) && volume[i] >= 0.0f;
         if (!ok)
             return mpm_set_error(MPM_E_INVALID, std::string(who) + ": particle " + std::to_string(i) +
-                                                    " has a non-finite value or a mass that is not positive");
+                                                    " has a non-finite value, or a mass or Jp that is not positive");
     }
     return MPM_OK;
 }

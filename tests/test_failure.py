@@ -33,7 +33,8 @@ def scene(name="two_balls", counts=24):
 def test_non_finite_and_non_positive_inputs_are_refused():
     d, x, v, m = scene()
     s = M.Scene(d, len(m) + 10)
-    for field, value in (("x", np.nan), ("v", np.inf), ("m", 0.0), ("m", -1.0), ("F", np.nan), ("Jp", np.inf)):
+    for field, value in (("x", np.nan), ("v", np.inf), ("m", 0.0), ("m", -1.0), ("F", np.nan), ("Jp", np.inf),
+                         ("Jp", 0.0)):
         xx, vv, mm = x.copy(), v.copy(), m.copy()
         F = np.tile(np.eye(3, dtype=np.float32), (len(m), 1, 1))
         Jp = np.ones(len(m), np.float32)
