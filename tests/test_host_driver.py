@@ -110,17 +110,19 @@ def test_driver_material_scenes_match_library_frame(tmp_path, dim, test, name, s
 
 @pytest.mark.gpu
 @needs_driver
-def test_driver_staged_equals_fused(tmp_path):
+@pytest.mark.parametrize("test", [18, 16, 19])
+def test_driver_staged_equals_fused(tmp_path, test):
     """-staged runs each MPM_Driver::Advance_Step stage as its own
-    mpm_run_stage call; the result equals the fused substep to FP32 rounding."""
+    mpm_run_stage call; the result equals the fused substep to FP32 rounding
+    (test 18 plastic balls, test 16 two materials, test 19 EOS fluid)."""
     a, b = tmp_path / "fused", tmp_path / "staged"
-    r = run(["-d", 2, "-test_number", 18, "-size", 64, "-last_frame", 1, "-o", a])
+    r = run(["-d", 2, "-test_number", test, "-size", 64, "-last_frame", 1, "-o", a])
     assert r.returncode == 0, r.stderr
-    r = run(["-d", 2, "-test_number", 18, "-size", 64, "-last_frame", 1, "-o", b, "-staged"])
+    r = run(["-d", 2, "-test_number", test, "-size", 64, "-last_frame", 1, "-o", b, "-staged"])
     assert r.returncode == 0, r.stderr
     fa, fb = read_frame(a / "1", 2), read_frame(b / "1", 2)
     assert fa["x"].shape == fb["x"].shape
-    for k in ("x", "v", "F"):
+    for k in ("x", "v", "F", "density"):
         assert _rel(fa[k], fb[k]) < 1e-4, k
 
 
