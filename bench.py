@@ -73,24 +73,25 @@ def cpu_baseline(desc, x, v, m, dt, substeps=2):
                       f"(dt {dt:.2e}, CG cap {desc.cg_max_iterations}), {el:.1f} s"}
 
 
-def measured_valu_insts():
-    """cg_force_valu_insts_per_launch from the committed counter summary, or None."""
+def measured_valu_insts(kernel="cg_force"):
+    """<kernel>_valu_insts_per_launch from the committed counter summary, or None."""
     try:
         with open(INSTRUCTIONS_FILE) as f:
             for line in f:
-                if line.startswith("cg_force_valu_insts_per_launch:"):
+                if line.startswith(kernel + "_valu_insts_per_launch:"):
                     return int(line.split(":", 1)[1].split()[0])
     except OSError:
         pass
     return None
 
 
-def valu_roofline(avg_cg_ms):
-    insts = measured_valu_insts()
-    if insts is None or not avg_cg_ms > 0:
+def valu_roofline(kernel, ms):
+    """VALU issue rate of one launch of `kernel` taking `ms`, against the issue peak."""
+    insts = measured_valu_insts(kernel)
+    if insts is None or not ms > 0:
         return None
-    achieved = insts / (avg_cg_ms * 1e-3) / 1e9
-    return {"kernel": "cg_force", "unit": "G wave-instructions/s", "insts_per_launch": insts,
+    achieved = insts / (ms * 1e-3) / 1e9
+    return {"kernel": kernel, "unit": "G wave-instructions/s", "insts_per_launch": insts,
             "achieved": round(achieved, 1), "peak": VALU_PEAK_G, "frac": round(achieved / VALU_PEAK_G, 4)}
 
 
@@ -241,20 +242,72 @@ def latency_summary(first, inter):
             "frames": len(first) + len(inter)}
 
 
+def fifo_replay(arrivals, service, frame_ends):
+    """One server, first in first out: request k starts at max(its arrival,
+    the end of request k-1) and its frames end frame_ends[k] seconds after
+    it starts (its service does not depend on when it starts).  Returns
+    (first-frame latencies ms, between-frame gaps ms, makespan s from the
+    first arrival)."""
+    first, inter = [], []
+    end = -np.inf
+    for a, s, fe in zip(arrivals, service, frame_ends):
+        start = max(a, end)
+        ft = [start + e for e in fe]
+        first.append((ft[0] - a) * 1e3)
+        inter.extend((np.diff(ft) * 1e3).tolist())
+        end = start + s
+    return first, inter, end - arrivals[0]
+
+
 def bench_serve(args):
     """Serving form: a Poisson stream of snow-impact / avalanche requests,
-    served by the request server (continuous batching on one device arena)
-    and by the plain loop (one scene at a time, in arrival order), each
-    against the same arrival clock."""
+    served by the plain loop (one scene at a time, in arrival order) and by
+    the request server (continuous batching on one device arena), both
+    against the same arrival clock.  The arrival rate defaults to 1.5 times
+    the plain loop's capacity on the stream (requests / total service time)."""
     import non_ficks_diffusion_amd as M
     from non_ficks_diffusion_amd import workloads as W
 
-    reqs = W.serve_requests(args.requests, args.rate, frames_min=args.frames_min, frames_max=args.frames_max)
-    data = [r.spec.generate() for r in reqs]
-    check = sorted({int(np.argmin([r.spec.n for r in reqs])), int(np.argmax([r.spec.n for r in reqs]))})
+    unit = W.serve_requests(args.requests, 1.0, frames_min=args.frames_min, frames_max=args.frames_max)
+    data = [r.spec.generate() for r in unit]
+    check = sorted({int(np.argmin([r.spec.n for r in unit])), int(np.argmax([r.spec.n for r in unit]))})
+    if args.no_plain and args.rate is None:
+        sys.exit("bench.py: --workload serve --no-plain needs --rate (the default rate is set from the plain loop)")
 
-    def work_of(k, substeps):
-        return reqs[k].spec.n * substeps
+    plain = None
+    if not args.no_plain:
+        # plain form: one scene at a time, back to back.  A request's service
+        # (handle, upload, frames) does not depend on when it starts, so its
+        # waits against the arrival clock follow from first-in-first-out order.
+        service, frame_ends, steps_p, kept = [], [], [], {}
+        work_p = 0
+        for k, r in enumerate(unit):
+            t0 = time.perf_counter()
+            d, x, v, m = data[k]
+            s = M.Scene(d, len(m))
+            s.upload(x, v, m)
+            s.initialize()
+            t, n_steps, ends = 0.0, 0, []
+            for f in range(r.frames):
+                target = float(np.float32(f + 1) / np.float32(d.frame_rate))
+                t, n, _ = s.advance_to_time(t, target)
+                n_steps += n
+                s.synchronize()
+                ends.append(time.perf_counter() - t0)
+            work_p += r.spec.n * n_steps
+            steps_p.append(n_steps)
+            if k in check:
+                kept[k] = s.download()
+            s.close()
+            service.append(time.perf_counter() - t0)
+            frame_ends.append(ends)
+        capacity = len(unit) / sum(service)
+        plain = (service, frame_ends, steps_p, kept, work_p, capacity)
+        print(f"plain loop: {len(unit)} requests served in {sum(service):.1f} s ({capacity:.3f} requests/s)",
+              file=sys.stderr, flush=True)
+    rate = args.rate if args.rate is not None else 1.5 * plain[5]
+    reqs = W.serve_requests(args.requests, rate, frames_min=args.frames_min, frames_max=args.frames_max)
+    assert [r.spec for r in reqs] == [r.spec for r in unit] and [r.frames for r in reqs] == [r.frames for r in unit]
 
     # batched form
     srv = M.Server(int(args.budget_gb * 1e9), max_scenes=args.max_scenes, max_counts=256, graph=args.graph,
@@ -295,6 +348,7 @@ def bench_serve(args):
     srv.close()
     lat_b = latency_summary(first, inter)
     ns = [r.spec.n for r in reqs]
+    rate_note = (f"{rate:.3f} Hz (1.5x the plain loop's capacity)" if args.rate is None else f"{rate} Hz")
     out = {
         "metric": "particle-substeps/s and p50/p99 frame latency, mixed request stream",
         "value": work / span_b,
@@ -308,9 +362,10 @@ def bench_serve(args):
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"{len(reqs)} Poisson requests at {args.rate} Hz, snow-impact (two_balls) or "
+        "config": {"workload": f"{len(reqs)} Poisson requests at {rate_note}, snow-impact (two_balls) or "
                                f"avalanche, N log-uniform 10k-2M, grids 64^3/128^3/256^3, "
                                f"{args.frames_min}-{args.frames_max} frames at 24 fps",
+                   "arrival_rate_hz": rate,
                    "particles_min": int(min(ns)), "particles_max": int(max(ns)),
                    "frames_total": int(sum(r.frames for r in reqs)), "hbm_budget_gb": args.budget_gb,
                    "arena": arena, "max_scenes": args.max_scenes, "graph": bool(args.graph),
@@ -320,49 +375,23 @@ def bench_serve(args):
         "latency": lat_b,
         "makespan_s": span_b,
     }
-    if args.no_plain:
+    if plain is None:
         print(json.dumps(out), flush=True)
         return
 
-    # plain form: one scene at a time, in arrival order
-    first, inter = [], []
-    work_p = 0
-    plain = {}
-    steps_p = []
-    t0 = time.perf_counter()
-    for k, r in enumerate(reqs):
-        now = time.perf_counter() - t0
-        if r.arrival > now:
-            time.sleep(r.arrival - now)
-        d, x, v, m = data[k]
-        s = M.Scene(d, len(m))
-        s.upload(x, v, m)
-        s.initialize()
-        t, n_steps = 0.0, 0
-        prev = r.arrival
-        for f in range(r.frames):
-            target = float(np.float32(f + 1) / np.float32(d.frame_rate))
-            t, n, _ = s.advance_to_time(t, target)
-            n_steps += n
-            s.synchronize()
-            t_end = time.perf_counter() - t0
-            (first if f == 0 else inter).append((t_end - prev) * 1e3)
-            prev = t_end
-        work_p += work_of(k, n_steps)
-        steps_p.append(n_steps)
-        if k in check:
-            plain[k] = s.download()
-        s.close()
-    span_p = time.perf_counter() - t0 - reqs[0].arrival
+    service, frame_ends, steps_p, kept, work_p, capacity = plain
+    first, inter, span_p = fifo_replay([r.arrival for r in reqs], service, frame_ends)
     lat_p = latency_summary(first, inter)
     err = {"x": 0.0, "v": 0.0, "F": 0.0}
     for k in check:
         for f in err:
-            err[f] = max(err[f], rel_l2(served[k][f], plain[k][f]))
+            err[f] = max(err[f], rel_l2(served[k][f], kept[k][f]))
     value, value_p = work / span_b, work_p / span_p
     out.update({
         "plain_form": {"value": value_p, "latency": lat_p, "makespan_s": span_p,
-                       "form": "one scene at a time in arrival order, a handle per request"},
+                       "capacity_hz": capacity, "service_s": sum(service),
+                       "form": "one scene at a time in arrival order, a handle per request; served back to back, "
+                               "waits against the arrival clock by first-in-first-out order"},
         "speedup_vs_plain": value / value_p,
         "substeps_equal": steps_b == steps_p,
         "served_vs_plain_rel_l2": {k: float(f"{v:.3e}") for k, v in err.items()},
@@ -384,7 +413,8 @@ def main():
                          "batched against one scene per handle; serve: a Poisson request stream served with "
                          "continuous batching against the one-scene-at-a-time loop")
     ap.add_argument("--requests", type=int, default=24, help="serve: number of requests")
-    ap.add_argument("--rate", type=float, default=4.0, help="serve: Poisson arrival rate (requests/s)")
+    ap.add_argument("--rate", type=float, default=None,
+                    help="serve: Poisson arrival rate (requests/s); default 1.5x the plain loop's capacity")
     ap.add_argument("--frames-min", type=int, default=60, help="serve: frames per request, lower bound")
     ap.add_argument("--frames-max", type=int, default=240, help="serve: frames per request, upper bound")
     ap.add_argument("--budget-gb", type=float, default=200.0, help="serve: HBM budget of the arena")
@@ -495,7 +525,9 @@ def main():
                      "algorithmic_bytes_per_launch": int(cb["cg_force"])},
         # the operator's own bound: VALU issue (counter-measured instructions per launch
         # over the live launch time, against the issue peak)
-        "valu_roofline": valu_roofline(avg_cg_ms),
+        "valu_roofline": valu_roofline("cg_force", avg_cg_ms),
+        # the FP64-SVD constitutive update and explicit force: VALU too
+        "valu_roofline_constitutive": valu_roofline("constitutive_force", breakdown.get("constitutive_force", 0.0)),
         "substep_roofline": {"algorithmic_bytes": int(cb["substep"]), "kernel_ms": round(sub_ms, 4),
                              "achieved_GBs": round(cb["substep"] / (sub_ms * 1e-3) / 1e9, 1)},
         "kernel_GBs": per_kernel_gbs,
