@@ -499,6 +499,9 @@ def main():
             per_kernel_gbs[k] = round(cb[k] / (msv / n * 1e-3) / 1e9, 1)
     per_kernel_gbs["cg_force"] = round(achieved, 1)
     per_kernel_frac = {k: round(v / HBM_PEAK_GBS, 4) for k, v in per_kernel_gbs.items()}
+    # measured roof beside the 8 TB/s peak: a streaming 1 GiB copy
+    copy_gbs = M.measure_copy_bandwidth(1 << 30, 10)
+    per_kernel_copy_frac = {k: round(v / copy_gbs, 4) for k, v in per_kernel_gbs.items()}
 
     out = {
         "metric": "particle-substeps/s, 3D 1M snow",
@@ -532,6 +535,9 @@ def main():
                              "achieved_GBs": round(cb["substep"] / (sub_ms * 1e-3) / 1e9, 1)},
         "kernel_GBs": per_kernel_gbs,
         "kernel_frac_of_hbm_peak": per_kernel_frac,
+        "copy_roof": {"GBs": round(copy_gbs, 1), "bytes": 1 << 30, "launches": 10,
+                      "form": "streaming device-to-device copy, read plus written bytes"},
+        "kernel_frac_of_copy_roof": per_kernel_copy_frac,
         "kernel_ms_per_substep": breakdown,
     }
     s.close()

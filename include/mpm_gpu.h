@@ -284,6 +284,14 @@ int32_t mpm_kernel_class_count(void);
  * state of that substep unchanged except the operator's tile buffer. */
 int mpm_time_cg_operator(mpm_handle h, int32_t reps, float* avg_ms);
 
+/* Benchmark support (no counterpart in the reference): the measured copy
+ * bandwidth roof of the current device.  A streaming copy kernel moves
+ * `bytes` (a multiple of 16) from one device buffer to another `reps` times
+ * back to back, after one untimed launch, between two HIP events on a stream
+ * of its own; *gbs = 2 * bytes * reps / elapsed (read plus write).  The call
+ * allocates and frees the two buffers. */
+int mpm_measure_copy_bandwidth(int64_t bytes, int32_t reps, double* gbs);
+
 /* ---- launch shapes, autotuner, captured substeps ------------------------ */
 /* (No counterpart in the reference.)  Current launch shapes of a handle. */
 int mpm_get_launch_config(mpm_handle h, mpm_launch_config* cfg);

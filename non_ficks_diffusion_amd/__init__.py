@@ -159,6 +159,7 @@ SIGNATURES = {
     "mpm_kernel_class_name": (ctypes.c_char_p, [ctypes.c_int32]),
     "mpm_kernel_class_count": (ctypes.c_int32, []),
     "mpm_time_cg_operator": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
+    "mpm_measure_copy_bandwidth": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
     "mpm_get_launch_config": (ctypes.c_int, [_H, ctypes.POINTER(LaunchConfig)]),
     "mpm_set_launch_config": (ctypes.c_int, [_H, ctypes.POINTER(LaunchConfig)]),
     "mpm_autotune": (ctypes.c_int, [_H, _FP, ctypes.c_int32, ctypes.POINTER(LaunchConfig)]),
@@ -290,6 +291,14 @@ def scene_materials(name: str, counts: int = 0, seed: int = 0, n: int = 0) -> np
 def kernel_class_names():
     L = lib()
     return [L.mpm_kernel_class_name(k).decode() for k in range(L.mpm_kernel_class_count())]
+
+
+def measure_copy_bandwidth(nbytes: int = 1 << 30, reps: int = 10) -> float:
+    """GB/s of a streaming device-to-device copy of nbytes (read plus write),
+    the measured bandwidth roof of the current device."""
+    gbs = ctypes.c_double()
+    _check(lib().mpm_measure_copy_bandwidth(int(nbytes), int(reps), ctypes.byref(gbs)))
+    return gbs.value
 
 
 class _LaunchControl:

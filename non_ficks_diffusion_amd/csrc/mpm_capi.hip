@@ -1813,6 +1813,45 @@ int mpm_time_cg_operator(mpm_handle c, int32_t reps, float* avg_ms) {
     return MPM_OK;
 }
 
+int mpm_measure_copy_bandwidth(int64_t bytes, int32_t reps, double* gbs) {
+    if (!gbs || reps <= 0 || bytes < 16 || bytes % 16 != 0)
+        return mpm_set_error(MPM_E_INVALID, "mpm_measure_copy_bandwidth: bad argument");
+    hipStream_t s = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    void *src = nullptr, *dst = nullptr;
+    float ms = 0.0f;
+    const int64_t n4 = bytes / 16;
+    if ((n4 + mpm::WG - 1) / mpm::WG > INT32_MAX)
+        return mpm_set_error(MPM_E_INVALID, "mpm_measure_copy_bandwidth: bytes too large");
+    hipError_t e = hipStreamCreate(&s);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipMalloc(&src, (size_t)bytes);
+    if (e == hipSuccess) e = hipMalloc(&dst, (size_t)bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(src, 0, (size_t)bytes, s);
+    if (e == hipSuccess) {
+        mpm::launch_copy((const float4*)src, (float4*)dst, n4, s);  // untimed first launch
+        e = hipEventRecord(e0, s);
+    }
+    if (e == hipSuccess) {
+        for (int r = 0; r < reps; ++r) mpm::launch_copy((const float4*)src, (float4*)dst, n4, s);
+        e = hipGetLastError();
+    }
+    if (e == hipSuccess) e = hipEventRecord(e1, s);
+    if (e == hipSuccess) e = hipEventSynchronize(e1);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (s) (void)hipStreamSynchronize(s);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (s) (void)hipStreamDestroy(s);
+    if (e != hipSuccess) return mpm_set_error(MPM_E_HIP, std::string("mpm_measure_copy_bandwidth: ") + hipGetErrorString(e));
+    if (!(ms > 0.0f)) return mpm_set_error(MPM_E_HIP, "mpm_measure_copy_bandwidth: no elapsed time");
+    *gbs = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    return MPM_OK;
+}
+
 int mpm_get_launch_config(mpm_handle c, mpm_launch_config* cfg) {
     if (!c || !cfg) return mpm_set_error(MPM_E_INVALID, "mpm_get_launch_config: null argument");
     *cfg = current_config(c);

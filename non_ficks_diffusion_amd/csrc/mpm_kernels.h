@@ -151,6 +151,7 @@ void launch_clear_waiting(const KP& p, hipStream_t s);
 void launch_remove_scene(const KP& p, hipStream_t s, int grid, int scene);
 template <int D> void launch_bin(const KP& p, hipStream_t s, int grid);
 void launch_scan(const KP& p, hipStream_t s);
+void launch_copy(const float4* src, float4* dst, int64_t n4, hipStream_t s);
 template <int D> void launch_block_setup(const KP& p, hipStream_t s, int grid);
 void launch_scene_ranges(const KP& p, hipStream_t s);
 template <int D> void launch_reorder(const KP& p, hipStream_t s, int grid);

@@ -89,6 +89,11 @@ def test_errors_are_status_codes():
     assert rc == -1 and b"dim" in M.lib().mpm_last_error()
     rc = M.lib().mpm_scene_preset(b"no_such_scene", 0, 0, ctypes.byref(M.SceneDesc()), None)
     assert rc == -1
+    # the copy-bandwidth probe checks its arguments before touching a device
+    g = ctypes.c_double()
+    for nbytes, reps in ((0, 1), (1000, 1), (1 << 20, 0)):
+        assert M.lib().mpm_measure_copy_bandwidth(nbytes, reps, ctypes.byref(g)) == -1
+    assert M.lib().mpm_measure_copy_bandwidth(1 << 20, 1, None) == -1
 
 
 def test_reference_test_scenes_counts_mass_and_materials():

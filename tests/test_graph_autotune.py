@@ -220,3 +220,10 @@ def test_device_frame_loop_matches_host_loop():
             assert rel(b[f], a[f]) < 1e-4, (k, f, rel(b[f], a[f]))
     host.close()
     dev.close()
+
+
+def test_copy_bandwidth_roof_is_plausible():
+    # a streaming copy of 256 MiB: well above PCIe rates, below the 8 TB/s peak
+    gbs = M.measure_copy_bandwidth(1 << 28, 5)
+    print("copy GB/s", gbs)
+    assert 500.0 < gbs < 8000.0
