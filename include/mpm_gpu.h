@@ -198,12 +198,16 @@ int mpm_add_particles(mpm_handle h, int32_t scene, int64_t n, const float* x, co
                       const float* mass, const int32_t* material);
 
 /* MPM_Driver::Initialize (MPM_Driver.cpp:18-32): one grid build, P2G and
- * Process_Waiting_Particles for particles uploaded without volume. */
+ * Process_Waiting_Particles for particles uploaded without volume.
+ * MPM_E_CAPACITY if the particles need more active blocks than the handle
+ * stores (nothing is built; upload a set that fits). */
 int mpm_initialize(mpm_handle h);
 
 /* MPM_Driver::Advance_Step (MPM_Driver.cpp:103-116): one substep with the
  * given dt.  Asynchronous unless stats != NULL (then it waits and fills
- * stats). */
+ * stats).  A substep whose particles need more active blocks than the
+ * handle stores changes nothing and returns MPM_E_CAPACITY (seen when the
+ * CG's done poll or the stats read the device counters). */
 int mpm_substep(mpm_handle h, float dt, mpm_step_stats* stats);
 
 /* MPM_Driver::Advance_To_Target_Time (MPM_Driver.cpp:55-80): substeps from

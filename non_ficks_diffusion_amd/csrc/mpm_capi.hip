@@ -5,6 +5,7 @@
 
 #include <algorithm>
 #include <cfloat>
+#include <cstddef>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -306,7 +307,9 @@ int stage_cg(mpm_context* c, bool do_init, bool fixed = false) {
         const int n = launched == 0 ? first : std::min(chunk, max_it - launched);
         for (int j = 0; j < n; ++j) iteration(launched + j);
         launched += n;
-        HIPCHECK(hipMemcpyAsync(&c->h_done[slot], &c->ctr->cg_ndone, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        // the overflow flag and the done count are adjacent: one copy reads both
+        static_assert(offsetof(mpm::Counters, cg_ndone) == offsetof(mpm::Counters, overflow) + sizeof(int), "layout");
+        HIPCHECK(hipMemcpyAsync(&c->h_done[2 * slot], &c->ctr->overflow, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipEventRecord(c->done_ev[slot], c->stream));
         pending[slot] = true;
         boundary[slot] = launched;
@@ -314,7 +317,8 @@ int stage_cg(mpm_context* c, bool do_init, bool fixed = false) {
         if (pending[slot]) {
             HIPCHECK(hipEventSynchronize(c->done_ev[slot]));
             pending[slot] = false;
-            if (c->h_done[slot] >= c->nscenes) {
+            c->overflow_seen = c->overflow_seen || c->h_done[2 * slot] != 0;
+            if (c->h_done[2 * slot + 1] >= c->nscenes) {
                 // done by the first boundary: try one fewer next time
                 c->cg_predict = boundary[slot] == first ? first - 1 : boundary[slot];
                 finished = true;
@@ -373,6 +377,7 @@ template <int D>
 int substep(mpm_context* c, const float* dts, bool capture = false) {
     if (capture) mpm::launch_set_step_dev(make_kp(c), c->stream, c->d_dt);
     else set_step(c, dts, 0.0f, false, false);
+    c->overflow_seen = false;
     stage_bin<D>(c);
     stage_p2g<D>(c);
     mpm::KP p = make_kp(c);
@@ -391,6 +396,9 @@ int substep(mpm_context* c, const float* dts, bool capture = false) {
     int rc = stage_cg<D>(c, false, capture);
     if (rc != MPM_OK) return rc;
     stage_g2p<D>(c);
+    // a binning over the block capacity made the substep a no-op (seen when
+    // the CG's done poll read the counters; stats always read them)
+    if (c->overflow_seen) return mpm_set_error(MPM_E_CAPACITY, "active blocks exceed the block capacity");
     return MPM_OK;
 }
 
@@ -503,7 +511,7 @@ int allocate_context(mpm_context* c) {
     A(dalloc(c, &c->frame, (size_t)nscenes));
     if (rc == MPM_OK) {
         void* hp = nullptr;
-        if (hipHostMalloc(&hp, 2 * sizeof(int), hipHostMallocDefault) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");
+        if (hipHostMalloc(&hp, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");
         c->h_done = (int*)hp;
         for (int k = 0; k < 2 && rc == MPM_OK; ++k)
             if (hipEventCreateWithFlags(&c->done_ev[k], hipEventDisableTiming) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipEventCreate failed");
@@ -1533,6 +1541,12 @@ int mpm_initialize(mpm_handle c) {
         if (!s.uploaded) return mpm_set_error(MPM_E_NOT_READY, "mpm_initialize: no particles uploaded");
     int rc = c->D == 3 ? initialize<3>(c) : initialize<2>(c);
     if (rc) return rc;
+    mpm::Counters h{};
+    rc = read_counters(c, h);
+    if (rc) return rc;
+    // the particles need more active blocks than the handle stores: nothing
+    // was built, and the handle waits for an upload that fits
+    if (h.overflow) return mpm_set_error(MPM_E_CAPACITY, "mpm_initialize: active blocks exceed the block capacity");
     c->initialized = true;
     rc = check_launch("mpm_initialize");
     if (rc) return rc;

@@ -57,7 +57,8 @@ struct mpm_context {
     double *bsum = nullptr, *bmax = nullptr, *bsq = nullptr;
     unsigned* bu = nullptr;
     int cg_predict = 0;    // iterations launched before the done count was seen, last solve
-    int* h_done = nullptr;  // pinned
+    int* h_done = nullptr;  // pinned: per poll slot, the overflow flag and the CG done count
+    bool overflow_seen = false;  // the last substep's poll saw a block-capacity overflow
     hipEvent_t done_ev[2] = {nullptr, nullptr};
     int grid_particle = 0, grid_block = 0, grid_node = 0;
     int op_mode = 0, grid_op = 0;  // CG operator: 0 four waves per block, 1 one wave per block

@@ -415,6 +415,7 @@ __global__ __launch_bounds__(WG) void k_scan_top(KP p) {
             p.ctr->overflow = 1;
             p.ctr->nblocks = 0;
         } else {
+            p.ctr->overflow = 0;  // a binning that fits clears an earlier overflow
             p.ctr->n_live = tc;
             p.ctr->nblocks = ta;
             p.block_pstart[ta] = tc;
@@ -439,6 +440,9 @@ __global__ __launch_bounds__(WG) void k_scan_apply(KP p) {
     int ta, tc;
     int ra = wg_exclusive_scan(sa, ta) + p.scan_part[blockIdx.x];
     int rc = wg_exclusive_scan(sc, tc) + p.scan_part[p.nparts + blockIdx.x];
+    // after an overflow no block is set up, so the binning's cell counters of
+    // the touched slots are cleared here instead of by the block set-up
+    const bool overflow = p.ctr->overflow != 0;
 #pragma unroll
     for (int k = 0; k < SCAN_PER_THREAD; ++k) {
         const int s = base + k;
@@ -449,6 +453,8 @@ __global__ __launch_bounds__(WG) void k_scan_apply(KP p) {
                 p.block_slot[ra] = s;
                 p.block_pstart[ra] = rc;
             }
+            if (overflow)
+                for (int cell = 0; cell < 64; ++cell) p.cell_count[s * 64 + cell] = 0;
         } else {
             p.slot_active[s] = -1;
         }

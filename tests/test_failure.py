@@ -5,8 +5,10 @@ MPM_E_NONFINITE and drops those particles (the reference drops them
 silently, MPM_Example.cpp:650-660), particles outside the domain are
 dropped at binning, a CG capped below convergence reports it in its stats
 (the reference ignores the solver's result, MPM_Example.cpp:711), inverted
-and singular F go through the SVD, and the request server cancels queued
-and running requests.
+and singular F go through the SVD, the request server cancels queued
+and running requests, and particles that need more active blocks than a
+handle stores are refused at initialisation and make a substep a no-op
+that reports MPM_E_CAPACITY.
 """
 import numpy as np
 import pytest
@@ -16,7 +18,7 @@ import oracle as O
 
 pytestmark = pytest.mark.gpu
 
-E_INVALID, E_NONFINITE = -1, -5
+E_INVALID, E_CAPACITY, E_NONFINITE = -1, -3, -5
 
 
 def rel(a, b):
@@ -180,3 +182,55 @@ def test_server_cancels_queued_and_running_requests():
     for rid in ids:
         srv.release(rid)
     srv.close()
+
+
+def block_lattice(z_layers):
+    """One particle inside each 4^3 block of a 128^3 grid, blocks 1..30 in x
+    and y and 1..z_layers in z: every particle's stencil stays in its block,
+    so the lattice needs exactly 900 * z_layers active blocks."""
+    g = np.stack(np.meshgrid(np.arange(1, 31), np.arange(1, 31), np.arange(1, z_layers + 1), indexing="ij"),
+                 -1).reshape(-1, 3)
+    x = ((4 * g + 2.3) / 128).astype(np.float32)
+    return x, np.zeros_like(x), np.full(len(x), 1e-3, np.float32)
+
+
+def test_block_capacity_overflow_is_refused_and_changes_nothing():
+    d, _ = M.scene_preset("two_balls", 128, 0)
+    x, v, m = block_lattice(6)  # 5400 blocks; a 5400-particle handle stores 4096
+    s = M.Scene(d, len(m))
+    s.upload(x, v, m)
+    with pytest.raises(M.MPMError) as e:
+        s.initialize()
+    assert e.value.code == E_CAPACITY
+    with pytest.raises(M.MPMError) as e:
+        s.substep(1e-4)  # not initialised
+    # the handle takes a set that fits, and gives what a fresh handle gives
+    x1, v1, m1 = block_lattice(1)
+    s.upload(x1, v1, m1)
+    s.initialize()
+    s.substep(1e-4)
+    got = s.download()
+    ref_s = M.Scene(d, len(m1))
+    ref_s.upload(x1, v1, m1)
+    ref_s.initialize()
+    ref_s.substep(1e-4)
+    ref = ref_s.download()
+    ref_s.close()
+    assert np.array_equal(got["ids"], ref["ids"])
+    for f in ("x", "v", "F"):
+        assert rel(got[f], ref[f]) < 1e-6, f
+    # particles that join later push the blocks over capacity: the substep is
+    # a no-op that reports it, and the particles already there keep their state
+    x2, v2, m2 = block_lattice(6)
+    extra = x2[:, 2] > 8 / 128  # blocks z = 2..6: 4500 more
+    before = s.download()
+    s.add_particles(x2[extra], v2[extra], m2[extra])
+    with pytest.raises(M.MPMError) as e:
+        s.substep(1e-4)
+    assert e.value.code == E_CAPACITY
+    after = s.download()
+    keep = np.isin(after["ids"], before["ids"])
+    assert keep.sum() == len(before["ids"])
+    for f in ("x", "v", "F", "Jp", "volume"):
+        assert np.array_equal(after[f][keep], before[f]), f
+    s.close()
