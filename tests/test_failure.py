@@ -234,3 +234,41 @@ def test_block_capacity_overflow_is_refused_and_changes_nothing():
     for f in ("x", "v", "F", "Jp", "volume"):
         assert np.array_equal(after[f][keep], before[f]), f
     s.close()
+
+
+def test_call_order_and_argument_errors():
+    # every condition is checked on the host and answered with a status; the
+    # handle stays usable afterwards
+    E_NOT_READY = -4
+    d, x, v, m = scene()
+    n = len(m)
+    s = M.Scene(d, n)
+
+    def code(f, *a):
+        with pytest.raises(M.MPMError) as e:
+            f(*a)
+        return e.value.code
+
+    assert code(s.initialize) == E_NOT_READY  # nothing uploaded
+    big = np.concatenate([x, x[:1]]), np.concatenate([v, v[:1]]), np.concatenate([m, m[:1]])
+    assert code(s.upload, *big) == E_CAPACITY  # one more than the capacity
+    s.upload(x, v, m)
+    assert code(s.substep, 1e-4) == E_NOT_READY  # before initialize
+    assert code(s.time_cg_operator, 5) == E_NOT_READY  # before a substep
+    s.initialize()
+    for dt in (0.0, -1e-4, float("nan"), float("inf")):
+        assert code(s.substep, dt) == E_INVALID
+    s.substep(1e-4)
+    assert code(s.download, n - 1) == E_CAPACITY  # cap below the live count
+    got = s.download()
+    assert len(got["ids"]) == n
+    s.close()
+
+    b = M.Batch([d, d], [n, n])
+    b.upload(0, x, v, m)
+    b.upload(1, x, v, m)
+    b.initialize()
+    assert code(b.upload, 0, x, v, m) == E_NOT_READY  # the batch is built
+    b.substep(1e-4)
+    assert code(b.stats, 2) == E_INVALID  # no such scene
+    b.close()
