@@ -55,8 +55,10 @@ def cfl_dt(desc, vmax):
     return float(max(desc.min_dt, min(desc.max_dt, desc.cfl * dx / max(vmax, 1e-2))))
 
 
-def cpu_baseline(desc, x, v, m, dt, substeps=2):
-    """Times the oracle (reference CPU path restated) on the host cores."""
+def cpu_baseline(desc, x, v, m, dt, min_seconds=10.0, max_substeps=64):
+    """Times the oracle (reference CPU path restated) on the host cores: one
+    warm-up substep, then fixed-dt substeps until min_seconds have passed
+    (a bounded sample of the same workload)."""
     import oracle as O
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     o = O.Oracle(desc, 4, cores, O.SVD_FP64)  # float state, float CG dot products (as the reference)
@@ -64,9 +66,11 @@ def cpu_baseline(desc, x, v, m, dt, substeps=2):
     o.initialize()
     o.advance_step(dt)  # warm-up substep
     t0 = time.perf_counter()
-    for _ in range(substeps):
+    substeps, el = 0, 0.0
+    while substeps < max_substeps and (substeps == 0 or el < min_seconds):
         o.advance_step(dt)
-    el = time.perf_counter() - t0
+        substeps += 1
+        el = time.perf_counter() - t0
     live = o.particles()["x"].shape[0]
     return {"value": live * substeps / el, "unit": "particle-substeps/s", "cores": cores, "kind": "port",
             "sample": f"{WORKLOAD} {COUNTS}^3, {len(m)} particles, 1 warm-up + {substeps} fixed-dt substeps "
@@ -407,7 +411,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--counts", type=int, default=COUNTS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--cpu-substeps", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=10.0,
+                    help="CPU baseline: time fixed-dt oracle substeps for at least this long")
     ap.add_argument("--workload", choices=["avalanche", "batch", "serve"], default="avalanche",
                     help="avalanche: the 3D 1M-particle scene (default); batch: 256 scenes of uneven size, "
                          "batched against one scene per handle; serve: a Poisson request stream served with "
@@ -542,7 +547,7 @@ def main():
     }
     s.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(desc, x, v, m, dt, args.cpu_substeps)
+        out["cpu_baseline"] = cpu_baseline(desc, x, v, m, dt, args.cpu_seconds)
         out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
     print(json.dumps(out), flush=True)
 
