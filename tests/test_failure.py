@@ -272,3 +272,42 @@ def test_call_order_and_argument_errors():
     b.substep(1e-4)
     assert code(b.stats, 2) == E_INVALID  # no such scene
     b.close()
+
+
+def test_scene_emptied_by_advection_keeps_running_and_refills():
+    # every particle leaves the domain in the first substep (fast and near the
+    # x = 1 edge); later substeps run on an empty scene, and particles added
+    # afterwards are simulated as in a scene that started with them
+    d, _ = M.scene_preset("two_balls", 32, 0)
+    rng = np.random.default_rng(3)
+    n = 200
+    x = np.column_stack([rng.uniform(0.93, 0.95, n), rng.uniform(0.4, 0.6, n), rng.uniform(0.4, 0.6, n)])
+    x = x.astype(np.float32)
+    v = np.tile(np.float32([400.0, 0.0, 0.0]), (n, 1))
+    m = np.full(n, 1e-3, np.float32)
+    s = M.Scene(d, 2 * n)
+    s.upload(x, v, m)
+    s.initialize()
+    s.substep(1e-3)  # 0.4 in x: everyone leaves
+    assert len(s.download()["ids"]) == 0
+    for _ in range(3):
+        st = s.substep(1e-3, stats=True)
+        assert st["live_particles"] == 0 and st["active_blocks"] == 0
+    assert s.max_particle_velocity() == 0.0
+    x2 = x - np.float32([0.5, 0.0, 0.0])
+    v2 = np.zeros_like(v)
+    s.add_particles(x2, v2, m)
+    for _ in range(3):
+        s.substep(1e-4)
+    got = s.download()
+    ref_s = M.Scene(d, n)
+    ref_s.upload(x2, v2, m)
+    ref_s.initialize()
+    for _ in range(3):
+        ref_s.substep(1e-4)
+    ref = ref_s.download()
+    ref_s.close()
+    assert len(got["ids"]) == n and np.array_equal(got["ids"] - n, ref["ids"])
+    for f in ("x", "v", "F", "volume"):
+        assert rel(got[f], ref[f]) < 1e-5, (f, rel(got[f], ref[f]))
+    s.close()
