@@ -288,6 +288,16 @@ int32_t mpm_kernel_class_count(void);
  * state of that substep unchanged except the operator's tile buffer. */
 int mpm_time_cg_operator(mpm_handle h, int32_t reps, float* avg_ms);
 
+/* Benchmark support (no counterpart in the reference): launches one
+ * stage kernel of the last substep `reps` times back to back on its state,
+ * bracketed by two HIP events on the handle's stream, and returns the
+ * average milliseconds per launch.  stage: MPM_STAGE_P2G (the scatter
+ * kernel), MPM_STAGE_CONSTITUTIVE (the substep's fused constitutive update
+ * and explicit force) or MPM_STAGE_G2P.  Needs a completed substep; the
+ * particle state is saved before the launches and restored after them, the
+ * grid scratch they write is not (the next substep rebuilds it). */
+int mpm_time_stage(mpm_handle h, int32_t stage, int32_t reps, float* avg_ms);
+
 /* Benchmark support (no counterpart in the reference): the measured copy
  * bandwidth roof of the current device.  A streaming copy kernel moves
  * `bytes` (a multiple of 16) from one device buffer to another `reps` times

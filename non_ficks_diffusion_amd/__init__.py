@@ -160,6 +160,7 @@ SIGNATURES = {
     "mpm_kernel_class_count": (ctypes.c_int32, []),
     "mpm_time_cg_operator": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
     "mpm_measure_copy_bandwidth": (ctypes.c_int, [ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(ctypes.c_double)]),
+    "mpm_time_stage": (ctypes.c_int, [_H, ctypes.c_int32, ctypes.c_int32, ctypes.POINTER(ctypes.c_float)]),
     "mpm_get_launch_config": (ctypes.c_int, [_H, ctypes.POINTER(LaunchConfig)]),
     "mpm_set_launch_config": (ctypes.c_int, [_H, ctypes.POINTER(LaunchConfig)]),
     "mpm_autotune": (ctypes.c_int, [_H, _FP, ctypes.c_int32, ctypes.POINTER(LaunchConfig)]),
@@ -473,6 +474,14 @@ class Scene(_LaunchControl):
     # -- timing ------------------------------------------------------------
     def enable_timing(self, on: bool = True) -> None:
         _check(lib().mpm_enable_timing(self._h, 1 if on else 0))
+
+    def time_stage(self, stage: int, reps: int = 20) -> float:
+        """Average ms of one launch of a stage kernel (STAGE_P2G, STAGE_CONSTITUTIVE
+        or STAGE_G2P) on the last substep's state, back to back between HIP
+        events; the particle state is restored afterwards."""
+        ms = ctypes.c_float()
+        _check(lib().mpm_time_stage(self._h, int(stage), int(reps), ctypes.byref(ms)))
+        return ms.value
 
     def time_cg_operator(self, reps: int = 20) -> float:
         """Average ms of one CG operator launch, timed back to back with HIP events."""

@@ -1827,6 +1827,55 @@ int mpm_time_cg_operator(mpm_handle c, int32_t reps, float* avg_ms) {
     return MPM_OK;
 }
 
+int mpm_time_stage(mpm_handle c, int32_t stage, int32_t reps, float* avg_ms) {
+    if (!c || !avg_ms || reps <= 0 ||
+        (stage != MPM_STAGE_P2G && stage != MPM_STAGE_CONSTITUTIVE && stage != MPM_STAGE_G2P))
+        return mpm_set_error(MPM_E_INVALID, "mpm_time_stage: bad argument");
+    if (!c->stepped) return mpm_set_error(MPM_E_NOT_READY, "mpm_time_stage: run a substep first");
+    // the particle fields and keys go to the other buffer (unused between
+    // substeps) and come back after the timed launches
+    const size_t n = (size_t)c->nscenes;
+    const size_t pbytes = (size_t)nfields(c->D) * (size_t)c->pcap * sizeof(float);
+    const int cur = c->cur;
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    std::vector<float> dts(n);
+    for (size_t s = 0; s < n; ++s) dts[s] = c->sc[s].last_dt;
+    set_step(c, dts.data(), 0.0f, false, false);
+    HIPCHECK(hipMemcpyAsync(c->pbuf[cur ^ 1], c->pbuf[cur], pbytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->keybuf[cur ^ 1], c->keybuf[cur], (size_t)c->pcap * sizeof(int), hipMemcpyDeviceToDevice,
+                            c->stream));
+    hipEvent_t e0, e1;
+    HIPCHECK(hipEventCreate(&e0));
+    HIPCHECK(hipEventCreate(&e1));
+    mpm::KP p = make_kp(c);
+    HIPCHECK(hipEventRecord(e0, c->stream));
+    for (int r = 0; r < reps; ++r) {
+        if (stage == MPM_STAGE_P2G) {
+            if (c->D == 3) mpm::launch_p2g<3>(p, c->stream, c->grid_block);
+            else mpm::launch_p2g<2>(p, c->stream, c->grid_block);
+        } else if (stage == MPM_STAGE_CONSTITUTIVE) {  // the substep's fused constitutive update and force
+            if (c->D == 3) mpm::launch_particle_force<3>(p, c->stream, c->grid_block, true, true);
+            else mpm::launch_particle_force<2>(p, c->stream, c->grid_block, true, true);
+        } else {
+            if (c->D == 3) mpm::launch_g2p<3>(p, c->stream, c->grid_block);
+            else mpm::launch_g2p<2>(p, c->stream, c->grid_block);
+        }
+    }
+    HIPCHECK(hipEventRecord(e1, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->pbuf[cur], c->pbuf[cur ^ 1], pbytes, hipMemcpyDeviceToDevice, c->stream));
+    HIPCHECK(hipMemcpyAsync(c->keybuf[cur], c->keybuf[cur ^ 1], (size_t)c->pcap * sizeof(int), hipMemcpyDeviceToDevice,
+                            c->stream));
+    HIPCHECK(hipStreamSynchronize(c->stream));
+    float ms = 0.0f;
+    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
+    HIPCHECK(hipEventDestroy(e0));
+    HIPCHECK(hipEventDestroy(e1));
+    int rc = check_launch("mpm_time_stage");
+    if (rc) return rc;
+    *avg_ms = ms / (float)reps;
+    return MPM_OK;
+}
+
 int mpm_measure_copy_bandwidth(int64_t bytes, int32_t reps, double* gbs) {
     if (!gbs || reps <= 0 || bytes < 16 || bytes % 16 != 0)
         return mpm_set_error(MPM_E_INVALID, "mpm_measure_copy_bandwidth: bad argument");

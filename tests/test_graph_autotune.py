@@ -227,3 +227,24 @@ def test_copy_bandwidth_roof_is_plausible():
     gbs = M.measure_copy_bandwidth(1 << 28, 5)
     print("copy GB/s", gbs)
     assert 500.0 < gbs < 8000.0
+
+
+def test_stage_timing_leaves_the_state_as_it_was():
+    d, x, v, m = M.scene_generate("two_balls", 24, 0)
+    runs = []
+    for timed in (False, True):
+        s = M.Scene(d, len(m))
+        s.upload(x, v, m)
+        s.initialize()
+        s.substep(1e-3)
+        before = s.download()
+        if timed:
+            for stage in (M.STAGE_P2G, M.STAGE_G2P, M.STAGE_CONSTITUTIVE):
+                assert s.time_stage(stage, 5) > 0.0
+            after = s.download()
+            for f in ("ids", "x", "v", "F", "Jp", "volume"):
+                assert np.array_equal(after[f], before[f]), f
+        s.substep(1e-3)
+        runs.append(s.download())
+        s.close()
+    assert_same(runs[1], runs[0], "substep after stage timing")
