@@ -1844,12 +1844,13 @@ int mpm_time_stage(mpm_handle c, int32_t stage, int32_t reps, float* avg_ms) {
     HIPCHECK(hipMemcpyAsync(c->pbuf[cur ^ 1], c->pbuf[cur], pbytes, hipMemcpyDeviceToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->keybuf[cur ^ 1], c->keybuf[cur], (size_t)c->pcap * sizeof(int), hipMemcpyDeviceToDevice,
                             c->stream));
-    hipEvent_t e0, e1;
-    HIPCHECK(hipEventCreate(&e0));
-    HIPCHECK(hipEventCreate(&e1));
+    // from here on the saved state is restored on every path
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e == hipSuccess) e = hipEventRecord(e0, c->stream);
     mpm::KP p = make_kp(c);
-    HIPCHECK(hipEventRecord(e0, c->stream));
-    for (int r = 0; r < reps; ++r) {
+    for (int r = 0; r < reps && e == hipSuccess; ++r) {
         if (stage == MPM_STAGE_P2G) {
             if (c->D == 3) mpm::launch_p2g<3>(p, c->stream, c->grid_block);
             else mpm::launch_p2g<2>(p, c->stream, c->grid_block);
@@ -1861,17 +1862,17 @@ int mpm_time_stage(mpm_handle c, int32_t stage, int32_t reps, float* avg_ms) {
             else mpm::launch_g2p<2>(p, c->stream, c->grid_block);
         }
     }
-    HIPCHECK(hipEventRecord(e1, c->stream));
+    if (e == hipSuccess) e = hipGetLastError();
+    if (e == hipSuccess) e = hipEventRecord(e1, c->stream);
     HIPCHECK(hipMemcpyAsync(c->pbuf[cur], c->pbuf[cur ^ 1], pbytes, hipMemcpyDeviceToDevice, c->stream));
     HIPCHECK(hipMemcpyAsync(c->keybuf[cur], c->keybuf[cur ^ 1], (size_t)c->pcap * sizeof(int), hipMemcpyDeviceToDevice,
                             c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
     float ms = 0.0f;
-    HIPCHECK(hipEventElapsedTime(&ms, e0, e1));
-    HIPCHECK(hipEventDestroy(e0));
-    HIPCHECK(hipEventDestroy(e1));
-    int rc = check_launch("mpm_time_stage");
-    if (rc) return rc;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return mpm_set_error(MPM_E_HIP, std::string("mpm_time_stage: ") + hipGetErrorString(e));
     *avg_ms = ms / (float)reps;
     return MPM_OK;
 }
