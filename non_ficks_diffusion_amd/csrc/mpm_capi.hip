@@ -117,6 +117,8 @@ mpm::KP make_kp(const mpm_context* c) {
     p.cg = c->cg;
     p.slot_scene = c->slot_scene;
     p.nscenes = c->nscenes;
+    p.solo = c->solo >= 0 ? c->solo : 0;
+    p.cg_target = (c->nscenes == 1 || c->solo >= 0) ? 1 : c->nscenes;
     p.pcur = c->pbuf[c->cur];
     p.palt = c->pbuf[c->cur ^ 1];
     p.idcur = c->idbuf[c->cur];
@@ -252,6 +254,10 @@ void stage_waiting(mpm_context* c) {
     timed(c, KC_INIT_VOLUME, [&] { mpm::launch_init_volume<D>(p, c->stream, c->grid_block); });
 }
 
+// particles from which a batch substep with one active scene is launched
+// (single-scene CG kernels, polled) rather than replayed from the batch graph
+constexpr int64_t SOLO_LAUNCHED_MIN = 65536;
+
 int max_cg_iterations(const mpm_context* c) {
     if (c->pool) return c->pool_cg_max;
     int m = 0;
@@ -267,7 +273,7 @@ int max_cg_iterations(const mpm_context* c) {
 template <int D>
 int stage_cg(mpm_context* c, bool do_init, bool fixed = false) {
     mpm::KP p = make_kp(c);
-    const bool one = c->nscenes == 1;
+    const bool one = c->nscenes == 1 || c->solo >= 0;
     const int gn = c->grid_node;
     if (do_init)
         timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, gn, false, false, true, false); });
@@ -318,7 +324,7 @@ int stage_cg(mpm_context* c, bool do_init, bool fixed = false) {
             HIPCHECK(hipEventSynchronize(c->done_ev[slot]));
             pending[slot] = false;
             c->overflow_seen = c->overflow_seen || c->h_done[2 * slot] != 0;
-            if (c->h_done[2 * slot + 1] >= c->nscenes) {
+            if (c->h_done[2 * slot + 1] >= (one ? 1 : c->nscenes)) {
                 // done by the first boundary: try one fewer next time
                 c->cg_predict = boundary[slot] == first ? first - 1 : boundary[slot];
                 finished = true;
@@ -343,6 +349,7 @@ void stage_g2p(mpm_context* c) {
 
 template <int D>
 int run_stage(mpm_context* c, int stage, float dt) {
+    c->solo = -1;
     set_step(c, nullptr, dt, true, false);
     mpm::KP p = make_kp(c);
     switch (stage) {
@@ -375,6 +382,16 @@ int run_stage(mpm_context* c, int stage, float dt) {
 // capture: the launch sequence of a graph (dts from d_dt, fixed CG count).
 template <int D>
 int substep(mpm_context* c, const float* dts, bool capture = false) {
+    // a batch or arena substep with one active scene runs the single-scene
+    // CG kernels on it (captured substeps keep the batch kernels: their
+    // active set changes between replays)
+    c->solo = -1;
+    if (!capture && c->nscenes > 1) {
+        int active = 0, last = -1;
+        for (int s = 0; s < c->nscenes; ++s)
+            if (dts[s] > 0.0f) { ++active; last = s; }
+        if (active == 1) c->solo = last;
+    }
     if (capture) mpm::launch_set_step_dev(make_kp(c), c->stream, c->d_dt);
     else set_step(c, dts, 0.0f, false, false);
     c->overflow_seen = false;
@@ -959,8 +976,16 @@ int graph_substep(mpm_context* c, const float* dts) {
 }
 
 int do_substep(mpm_context* c, const float* dts) {
-    // a substep after mpm_add_particles runs launched: it adds the waiting pass
-    int rc = c->graph && !c->emitted ? graph_substep(c, dts) : c->D == 3 ? substep<3>(c, dts) : substep<2>(c, dts);
+    // a substep after mpm_add_particles runs launched: it adds the waiting
+    // pass; so does a batch substep with one large active scene, which then
+    // takes the single-scene CG kernels with the CG's done poll
+    int active = 0, last = -1;
+    for (int s = 0; s < c->nscenes; ++s)
+        if (dts[s] > 0.0f) { ++active; last = s; }
+    const bool solo = c->nscenes > 1 && active == 1 && c->sc[(size_t)last].n_uploaded >= SOLO_LAUNCHED_MIN;
+    int rc = c->graph && !c->emitted && !solo ? graph_substep(c, dts)
+             : c->D == 3                      ? substep<3>(c, dts)
+                                              : substep<2>(c, dts);
     if (rc) return rc;
     for (int s = 0; s < c->nscenes; ++s)
         if (dts[s] > 0.0f) c->sc[(size_t)s].last_dt = dts[s];

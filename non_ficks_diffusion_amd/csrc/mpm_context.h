@@ -39,6 +39,7 @@ struct mpm_context {
     int* keybuf[2] = {nullptr, nullptr};
     int* scbuf[2] = {nullptr, nullptr};
     int cur = 0;
+    int solo = -1;  // batch or arena substep with one active scene: it takes the single-scene CG
     int* prank = nullptr;
     float* rcache = nullptr;
     float* pop = nullptr;

@@ -86,6 +86,8 @@ struct KP {
     CGState* cg;
     const int* slot_scene;  // per global slot: the scene that owns it, -1 if none
     int nscenes;
+    int solo;        // the scene of the single-scene CG kernels (0 for a one-scene context)
+    int cg_target;   // finished solves that end the CG: 1 on the single-scene path, else nscenes
     // particles: field-major arrays of capacity pcap
     float* pcur;
     float* palt;

@@ -1274,12 +1274,14 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
 template <int D, bool FORCE, bool COLL, bool CGINIT, bool WRITE_F>
 __global__ __launch_bounds__(WG) void k_grid_update(KP p) {
     if (CGINIT) {
-        // a new solve for every scene (inactive scenes finish at the first test)
+        // a new solve for every scene (inactive scenes finish at the first
+        // test; on the single-scene path the others are finished here)
         for (int si = blockIdx.x * WG + threadIdx.x; si < p.nscenes; si += gridDim.x * WG) {
             CGState& cg = p.cg[si];
+            const int other = p.cg_target == 1 && si != p.solo ? 1 : 0;
             cg.iter = 0;
-            cg.done = 0;
-            cg.converged = 0;
+            cg.done = other;
+            cg.converged = other;
             cg.multiplies = 0;
             cg.rho_hist[0] = cg.rho_hist[1] = 0.0;
             cg.alpha = cg.beta = cg.conv = cg.rr = cg.sq = 0.0;
@@ -1371,7 +1373,11 @@ __device__ inline float cg_direction(float r, float s, float beta, int k) {
     return k == 0 ? r : fmaf(beta, s, r);
 }
 
-__device__ inline bool cg_all_done(const KP& p) { return p.ctr->cg_ndone >= p.nscenes; }
+__device__ inline bool cg_all_done(const KP& p) { return p.ctr->cg_ndone >= p.cg_target; }
+
+// Single-scene CG kernels in a batch or arena (one active scene): blocks of
+// the other scenes are skipped.
+__device__ inline bool other_scene(const KP& p, int b) { return p.nscenes > 1 && p.block_scene[b] != p.solo; }
 
 // MPM_CG_System::Force (MPM_CG_System.h:150-215) with the linearised operator
 // record: T = sum_i grad w_ip (x) u_i;  A = L T;  q_i += A^T grad w_ip.
@@ -1398,7 +1404,8 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p, int k) {
     const XcdRange xr = xcd_blocks(nblocks);
     for (int b = xr.first; b < xr.end; b += xr.step) {
         // one scene: no per-block lookups (its solve is live: cg_all_done above)
-        const int si = ONE ? 0 : p.block_scene[b];
+        if (ONE && other_scene(p, b)) continue;
+        const int si = ONE ? p.solo : p.block_scene[b];
         const SceneParams& sc = p.scn[si];
         if (!sc.active || (!ONE && p.cg[si].done) || block_empty(p, b)) continue;
         const float beta = (RESTART || ONE) ? 0.0f : (float)p.cg[si].beta;
@@ -1559,7 +1566,8 @@ __global__ __launch_bounds__(WG) void k_cg_force_cell(KP p, int k) {
     }
     const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
     for (int b = r.first; b < r.end; b += r.step) {
-        const int si = ONE ? 0 : p.block_scene[b];
+        if (ONE && other_scene(p, b)) continue;
+        const int si = ONE ? p.solo : p.block_scene[b];
         const SceneParams& sc = p.scn[si];
         if (!sc.active || (!ONE && p.cg[si].done) || block_empty(p, b)) continue;
         const float beta = (RESTART || ONE) ? 0.0f : (float)p.cg[si].beta;
@@ -1740,7 +1748,8 @@ __global__ __launch_bounds__(WG) void k_cg_restart_node(KP p) {
     const int lane = threadIdx.x & 63;
     double wsum = 0.0, wmx = 0.0;
     for (int b = r.first; b < r.end; b += r.step) {
-        const SceneParams& sc = p.scn[ONE ? 0 : p.block_scene[b]];
+        if (ONE && other_scene(p, b)) continue;
+        const SceneParams& sc = p.scn[ONE ? p.solo : p.block_scene[b]];
         if (!sc.active) continue;
         const int n = b * 64 + lane;
         const float sds = sc.dt * sc.dt;
@@ -1796,10 +1805,11 @@ __global__ __launch_bounds__(WG) void k_cg_update_s(KP p, int k, int nparts) {
     double rr, mx;
     reduce_range(p.bsum, p.bmax, 0, nparts, rr, mx);
     float beta;
-    if (cg_test(p, 0, k, rr, mx, blockIdx.x == 0 && threadIdx.x == 0, beta)) return;
+    if (cg_test(p, p.solo, k, rr, mx, blockIdx.x == 0 && threadIdx.x == 0, beta)) return;
     const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
     const int lane = threadIdx.x & 63;
     for (int b = r.first; b < r.end; b += r.step) {
+        if (other_scene(p, b)) continue;
         const int n = b * 64 + lane;
         if (!(p.g_flags[n] & MPM_FLAG_INTERIOR_DEV)) continue;
 #pragma unroll
@@ -1818,7 +1828,8 @@ __global__ __launch_bounds__(WG) void k_cg_node1(KP p, int k) {
     const int lane = threadIdx.x & 63;
     double wsum = 0.0;
     for (int b = r.first; b < r.end; b += r.step) {
-        const int si = ONE ? 0 : p.block_scene[b];
+        if (ONE && other_scene(p, b)) continue;
+        const int si = ONE ? p.solo : p.block_scene[b];
         const SceneParams& sc = p.scn[si];
         if (!sc.active || (!ONE && p.cg[si].done)) continue;
         const float beta = ONE ? 0.0f : (float)p.cg[si].beta;
@@ -1867,7 +1878,7 @@ __global__ __launch_bounds__(WG) void k_cg_node2(KP p, int k, int nparts) {
     if (cg_all_done(p)) return;
     float alpha1 = 0.0f;
     if (ONE) {
-        CGState& cg = p.cg[0];
+        CGState& cg = p.cg[p.solo];
         if (cg.done) return;
         double sq, unused;
         reduce_range(p.bsq, nullptr, 0, nparts, sq, unused);
@@ -1883,7 +1894,8 @@ __global__ __launch_bounds__(WG) void k_cg_node2(KP p, int k, int nparts) {
     const int lane = threadIdx.x & 63;
     double wsum = 0.0, wmx = 0.0;
     for (int b = r.first; b < r.end; b += r.step) {
-        const int si = ONE ? 0 : p.block_scene[b];
+        if (ONE && other_scene(p, b)) continue;
+        const int si = ONE ? p.solo : p.block_scene[b];
         const SceneParams& sc = p.scn[si];
         if (!sc.active || (!ONE && p.cg[si].done)) continue;
         const float alpha = ONE ? alpha1 : (float)p.cg[si].alpha;

@@ -120,3 +120,30 @@ def test_batch_2d_matches_scenes_alone():
         for f in ("x", "v", "F", "Jp"):
             assert rel(got[f], alone[f]) < 1e-5, (scenes[k], f, rel(got[f], alone[f]))
     b.close()
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_one_active_scene_takes_the_single_scene_solver(graph):
+    # substeps where exactly one scene advances run the single-scene CG
+    # kernels on it (launched, with the done poll, when it is large, even in
+    # graph mode); every scene still matches itself alone
+    specs = [("two_balls", 24, 0, 0), ("blob", 64, 1001, 70000), ("slab", 40, 1002, 8000)]
+    data = [M.scene_generate(n, c, s, k) for n, c, s, k in specs]
+    dts = [[1e-3, 0.0, 1e-3, 0.0], [0.0, 1e-3, 1e-3, 0.0], [0.0, 0.0, 1e-3, 1e-3]]
+    b = M.Batch([d for d, _, _, _ in data], [len(m) for _, _, _, m in data])
+    for k, (d, x, v, m) in enumerate(data):
+        b.upload(k, x, v, m)
+    b.initialize()
+    if graph:
+        b.use_graph(True)
+    for step in range(4):
+        b.substep([dts[k][step] for k in range(3)])
+    for k, (d, x, v, m) in enumerate(data):
+        alone, st_alone = run_alone(d, x, v, m, [t for t in dts[k] if t > 0])
+        got = b.download(k)
+        assert np.array_equal(got["ids"], alone["ids"]), k
+        for f in ("x", "v", "F", "Jp", "volume"):
+            assert rel(got[f], alone[f]) < 1e-5, (specs[k], f, rel(got[f], alone[f]))
+        st = b.stats(k)
+        assert st["live_particles"] == st_alone["live_particles"]
+    b.close()
