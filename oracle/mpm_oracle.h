@@ -27,8 +27,16 @@
  *   P, dP            P = dPsi/dF and dP = directional derivative of P by
  *                    finite differences; dP symmetric
  *   force map        symmetric, block structure in the SVD frame
+ *   explicit force   snow: sum_i f_i = g sum_p m_p and the first moment
+ *                    sum_i f_i x_i^T = g (sum_p m_p x_p)^T - sum_p V0 P F^T
+ *                    with P from the textbook closed form
  *   CG operator      M-symmetric; solve matches a dense numpy solve; rigid
- *                    translation maps to itself
+ *                    translation maps to itself; K u equals the central
+ *                    difference of a numpy restatement of the elastic force
+ *                    with F_p -> (I + e grad u_p) F_p (fixes V0, dt^2 and
+ *                    the transposes)
+ *   waiting volume   isolated particle V0 = dx^d / sum_i w_ip^2; a cluster
+ *                    against a brute-force textbook-weight sum
  *   collision        closed form of Grid_Based_Collision_Helper.h:72-80
  *   P2G / G2P        mass and momentum conservation; G2P reproduces a linear
  *                    velocity field; free fall without elasticity

@@ -619,3 +619,154 @@ def test_material_entry_selects_the_particles_parameters():
     o.initialize()
     o.advance_step(2e-3)
     assert not np.allclose(o.particles()["F"], runs[0]["F"], atol=1e-9)
+
+
+# --------------------------------------------------------------------------- snow force, operator, waiting
+def textbook_stencil(X, n):
+    """The 3^d stencil of a particle at X on an n-cell grid from the textbook
+    B-spline: dense node indices over [-1, n] (index = cell + 1), weights and
+    weight gradients (Tools/Influence_Iterator.h:85-99 restated)."""
+    dim = len(X)
+    dx = 1.0 / n
+    cell = np.floor(X * n).astype(int)
+    out = []
+    for off in np.ndindex(*(3,) * dim):
+        node = cell - 1 + np.array(off)
+        r = (X - (node + 0.5) * dx) / dx
+        w1 = np.array([bspline(r[a]) for a in range(dim)])
+        d1 = np.array([bspline_d(r[a]) / dx for a in range(dim)])
+        g = np.array([np.prod([d1[a] if a == b else w1[a] for a in range(dim)]) for b in range(dim)])
+        out.append((tuple(node + 1), np.prod(w1), g))
+    return out
+
+
+def fixed_corotated_P(F, mu, lam):
+    """P = 2 mu (F - R) + lambda (J - 1) cof(F) (MPM_Constitutive_Model.cpp:63-67),
+    R the polar rotation from numpy's SVD, cof(F) = J F^-T."""
+    U, _, Vt = np.linalg.svd(F)
+    R = U @ Vt
+    J = np.linalg.det(F)
+    return 2 * mu * (F - R) + lam * (J - 1) * J * np.linalg.inv(F).T
+
+
+def hardened(mat, Jp):
+    """Lame parameters after hardening (MPM_Constitutive_Model.cpp:43-45)."""
+    e = np.exp(min(mat.hardening_factor * (1.0 - Jp), mat.hardening_max))
+    return mat.mu0 * e, mat.lambda0 * e
+
+
+def snow_force_scene(n=10, seed=5):
+    d, x, v, m, F = tiny_scene(dim=3, n=n, seed=seed, v_scale=0.3, F_scale=0.04)
+    d.gravity[:] = (0.3, -9.8, 0.7)
+    rng = np.random.default_rng(seed + 7)
+    vol = (2e-5 * (0.8 + 0.4 * rng.random(n))).astype(np.float32)
+    m = (1e-4 * (0.8 + 0.4 * rng.random(n))).astype(np.float32)
+    return d, x, v, m, F, vol
+
+
+def test_snow_explicit_force_moments():
+    """Explicit force of snow particles (MPM_Example.cpp:757-790),
+    f_i = sum_p (g m_p w_ip - V0_p P_p F_p^T grad w_ip).  Partition of unity and
+    sum_i grad w_ip = 0 give sum_i f_i = g sum_p m_p; linear reproduction
+    (sum_i w_ip x_i = x_p, sum_i grad w_ip x_i^T = I) gives
+    sum_i f_i x_i^T = g (sum_p m_p x_p)^T - sum_p V0_p P_p F_p^T, with P from the
+    textbook fixed-corotated closed form on the post-plasticity F.  A flipped
+    sign, a dropped V0 or P F in place of P F^T fails the second identity."""
+    d, x, v, m, F, vol = snow_force_scene()
+    o = O.Oracle(d, 8, 1, O.DOT_FP64 | O.SVD_FP64)
+    o.set_particles(x, v, m, F=F, volume=vol)
+    for st in (M.STAGE_BIN, M.STAGE_P2G, M.STAGE_CONSTITUTIVE, M.STAGE_EXPLICIT_FORCE):
+        o.run_stage(st, 1e-3)
+    p = o.particles()
+    f = o.grid(M.CH_FORCE).reshape(-1, 3)
+    xi = node_positions(d.counts, 3).reshape(-1, 3)
+    g = np.array(d.gravity[:3], dtype=np.float64)
+    mp = p["mass"]
+    assert np.allclose(f.sum(0), g * mp.sum(), rtol=1e-10, atol=1e-14)
+    stress = np.zeros((3, 3))
+    for k in range(len(mp)):
+        mu, lam = hardened(d.material, p["Jp"][k])
+        stress += p["volume"][k] * fixed_corotated_P(p["F"][k], mu, lam) @ p["F"][k].T
+    expect = np.outer(g, (mp[:, None] * p["x"]).sum(0)) - stress
+    assert np.abs(stress).max() > 1e-3 * np.abs(np.outer(g, (mp[:, None] * p["x"]).sum(0))).max()
+    assert np.allclose(f.T @ xi, expect, rtol=1e-9, atol=1e-9 * np.abs(expect).max())
+
+
+def test_cg_operator_is_the_derivative_of_the_elastic_force():
+    """MPM_CG_System::Force (MPM_CG_System.h:150-215) applied to a node field u
+    is K u = -d/de f_el(e) at e = 0, where f_el is the elastic part of the
+    explicit force, f_el,i = -sum_p V0_p P((I + e grad u_p) F_p) F_p^T grad w_ip
+    with grad u_p = sum_i u_i grad w_ip^T and the trailing F_p^T held at the
+    substep's F (Stomakhin et al. 2013, eq. 6).  The force is read back from
+    Multiply as (q - u) m / dt^2 (MPM_CG_System.h:73-81, Multiply_Helper.h:40-53).
+    The derivative is a central difference of a numpy restatement (textbook
+    weights, closed-form P), so a wrong V0 or dt^2 scale, or a transposed dF
+    or F, fails it."""
+    d, x, v, m, F, vol = snow_force_scene(n=8, seed=11)
+    d.gravity[:] = (0.0, 0.0, 0.0)
+    o = O.Oracle(d, 8, 1, O.DOT_FP64 | O.SVD_FP64)
+    o.set_particles(x, v, m, F=F, volume=vol)
+    dt = 2e-3
+    for st in (M.STAGE_BIN, M.STAGE_P2G, M.STAGE_CONSTITUTIVE, M.STAGE_EXPLICIT_FORCE, M.STAGE_COLLISION):
+        o.run_stage(st, dt)
+    p = o.particles()
+    mass = o.grid(M.CH_MASS)
+    live = mass > 0
+    rng = np.random.default_rng(17)
+    u = np.zeros(mass.shape + (3,))
+    u[live] = rng.standard_normal((live.sum(), 3))
+    q = o.apply_operator(dt, u.ravel()).reshape(u.shape)
+    f_op = np.zeros_like(u)
+    f_op[live] = (q[live] - u[live]) * mass[live][:, None] / dt ** 2
+    stencils = [textbook_stencil(p["x"][k], d.counts) for k in range(len(p["mass"]))]
+
+    def f_el(eps):
+        out = np.zeros_like(u)
+        for k, st in enumerate(stencils):
+            mu, lam = hardened(d.material, p["Jp"][k])
+            Fk = p["F"][k]
+            gu = sum(np.outer(u[node], g) for node, _, g in st)
+            A = p["volume"][k] * fixed_corotated_P((np.eye(3) + eps * gu) @ Fk, mu, lam) @ Fk.T
+            for node, _, g in st:
+                out[node] -= A @ g
+        return out
+
+    h = 1e-6
+    Ku = -(f_el(h) - f_el(-h)) / (2 * h)
+    scale = np.abs(Ku[live]).max()
+    assert scale > 0
+    assert np.abs(f_op[live] - Ku[live]).max() < 1e-6 * scale
+    # nodes without mass receive no force in the operator's view either
+    assert np.abs(Ku[~live]).max() < 1e-9 * scale
+
+
+def test_waiting_particle_volume():
+    """Process_Waiting_Particles (MPM_Example.cpp:827-840):
+    V0_p = m_p / (sum_i w_ip m_i / dx^d) with m_i = sum_q w_iq m_q.  An isolated
+    particle gets dx^d / sum_i w_ip^2 whatever its mass; a cluster is checked
+    against the same sum by brute force over textbook weights (w in place of
+    w^2, or a missing dx^d, fails)."""
+    d, x, v, m, F = tiny_scene(dim=3, n=1, seed=3)
+    d.counts = 32
+    X = np.array([[0.4137, 0.5561, 0.6029]], np.float32)
+    o = O.Oracle(d, 8, 1, O.DOT_FP64 | O.SVD_FP64)
+    o.set_particles(X, np.zeros((1, 3), np.float32), np.array([3.7e-4], np.float32))
+    o.initialize()
+    st = textbook_stencil(X[0].astype(np.float64), 32)
+    assert abs(o.particles()["volume"][0] - (1.0 / 32) ** 3 / sum(w * w for _, w, _ in st)) < 1e-12 * (1.0 / 32) ** 3
+    # a cluster of overlapping particles
+    d, x, v, m, F = tiny_scene(dim=3, n=9, seed=4)
+    d.counts = 32
+    m = (1e-4 * (0.5 + np.arange(9) / 9.0)).astype(np.float32)
+    o = O.Oracle(d, 8, 1, O.DOT_FP64 | O.SVD_FP64)
+    o.set_particles(x, v, m)
+    o.initialize()
+    stencils = [textbook_stencil(x[k].astype(np.float64), 32) for k in range(9)]
+    node_mass = {}
+    for k, stc in enumerate(stencils):
+        for node, w, _ in stc:
+            node_mass[node] = node_mass.get(node, 0.0) + w * float(m[k])
+    vol = o.particles()["volume"]
+    for k, stc in enumerate(stencils):
+        rho = sum(w * node_mass[node] for node, w, _ in stc) * 32 ** 3
+        assert abs(vol[k] - float(m[k]) / rho) < 1e-10 * vol[k]
