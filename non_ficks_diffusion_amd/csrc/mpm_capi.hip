@@ -257,6 +257,8 @@ void stage_waiting(mpm_context* c) {
 // particles from which a batch substep with one active scene is launched
 // (single-scene CG kernels, polled) rather than replayed from the batch graph
 constexpr int64_t SOLO_LAUNCHED_MIN = 65536;
+// iterations per launch batch of the polled CG after the first
+constexpr int CG_CHUNK = 4;
 
 int max_cg_iterations(const mpm_context* c) {
     if (c->pool) return c->pool_cg_max;
@@ -303,14 +305,14 @@ int stage_cg(mpm_context* c, bool do_init, bool fixed = false) {
     // too many is harmless.  The first batch is the previous solve's iteration
     // count; after it the host polls the done count one chunk behind the GPU
     // (double-buffered events), so the queue never runs dry while it waits.
-    const int chunk = 4;
-    const int first = std::min(max_it, std::max(c->cg_predict, chunk));
+    const int first = std::min(max_it, std::max(c->cg_predict, CG_CHUNK));
+    const int target = one ? 1 : c->nscenes;
     int slot = 0, launched = 0;
     int boundary[2] = {0, 0};
     bool pending[2] = {false, false};
     bool finished = false;
     while (launched < max_it) {
-        const int n = launched == 0 ? first : std::min(chunk, max_it - launched);
+        const int n = launched == 0 ? first : std::min(CG_CHUNK, max_it - launched);
         for (int j = 0; j < n; ++j) iteration(launched + j);
         launched += n;
         // the overflow flag and the done count are adjacent: one copy reads both
@@ -324,7 +326,7 @@ int stage_cg(mpm_context* c, bool do_init, bool fixed = false) {
             HIPCHECK(hipEventSynchronize(c->done_ev[slot]));
             pending[slot] = false;
             c->overflow_seen = c->overflow_seen || c->h_done[2 * slot] != 0;
-            if (c->h_done[2 * slot + 1] >= (one ? 1 : c->nscenes)) {
+            if (c->h_done[2 * slot + 1] >= target) {
                 // done by the first boundary: try one fewer next time
                 c->cg_predict = boundary[slot] == first ? first - 1 : boundary[slot];
                 finished = true;
@@ -333,10 +335,38 @@ int stage_cg(mpm_context* c, bool do_init, bool fixed = false) {
         }
     }
     if (!finished) {
-        c->cg_predict = max_it;
         // one scene: the convergence test after the last iteration (batches ran it in reduce_rr)
         if (one) timed(c, KC_CG_NODE, [&] { mpm::launch_cg_update_s<D>(p, c->stream, gn, max_it); });
+        // the polls still in flight are read once the rest of the substep is queued
+        c->poll_open = true;
+        c->poll_older = slot;  // the slot written first of the two
+        c->poll_target = target;
+        c->poll_max_it = max_it;
+        for (int k = 0; k < 2; ++k) {
+            c->poll_pending[k] = pending[k];
+            c->poll_boundary[k] = boundary[k];
+        }
     }
+    return MPM_OK;
+}
+
+// Reads the done polls of a solve that ran to its last chunk: the overflow
+// flag they carry makes the substep report MPM_E_CAPACITY, and the next
+// solve's first batch moves one chunk below where this one was seen done (or
+// below the cap when it never was), so the prediction comes back down.
+int finish_cg_poll(mpm_context* c) {
+    if (!c->poll_open) return MPM_OK;
+    c->poll_open = false;
+    int done_at = -1;
+    for (int j = 0; j < 2; ++j) {
+        const int slot = (c->poll_older + j) & 1;
+        if (!c->poll_pending[slot]) continue;
+        c->poll_pending[slot] = false;
+        HIPCHECK(hipEventSynchronize(c->done_ev[slot]));
+        c->overflow_seen = c->overflow_seen || c->h_done[2 * slot] != 0;
+        if (done_at < 0 && c->h_done[2 * slot + 1] >= c->poll_target) done_at = c->poll_boundary[slot];
+    }
+    c->cg_predict = std::max(CG_CHUNK, (done_at > 0 ? done_at : c->poll_max_it) - CG_CHUNK);
     return MPM_OK;
 }
 
@@ -369,7 +399,10 @@ int run_stage(mpm_context* c, int stage, float dt) {
         case MPM_STAGE_COLLISION:
             timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, false, true, false, false); });
             break;
-        case MPM_STAGE_CG: return stage_cg<D>(c, true);
+        case MPM_STAGE_CG: {
+            const int rc = stage_cg<D>(c, true);
+            return rc ? rc : finish_cg_poll(c);
+        }
         case MPM_STAGE_G2P: stage_g2p<D>(c); break;
         default: return mpm_set_error(MPM_E_INVALID, "unknown stage");
     }
@@ -386,6 +419,10 @@ int substep(mpm_context* c, const float* dts, bool capture = false) {
     // CG kernels on it (captured substeps keep the batch kernels: their
     // active set changes between replays)
     c->solo = -1;
+    struct SoloReset {  // only this substep's launches see the solo scene
+        mpm_context* c;
+        ~SoloReset() { c->solo = -1; }
+    } solo_reset{c};
     if (!capture && c->nscenes > 1) {
         int active = 0, last = -1;
         for (int s = 0; s < c->nscenes; ++s)
@@ -398,6 +435,8 @@ int substep(mpm_context* c, const float* dts, bool capture = false) {
     stage_bin<D>(c);
     stage_p2g<D>(c);
     mpm::KP p = make_kp(c);
+    const bool was_emitted = c->emitted;
+    std::vector<int> was_waiting;
     if (c->pool || c->emitted) {
         // scenes attached or particles added since the last substep:
         // Process_Waiting_Particles (every other scene's blocks return at once)
@@ -405,7 +444,10 @@ int substep(mpm_context* c, const float* dts, bool capture = false) {
         mpm::launch_clear_waiting(p, c->stream);
         if (!capture) {
             c->emitted = false;
-            for (mpm::SceneParams& t : c->table) t.waiting = 0;
+            for (mpm::SceneParams& t : c->table) {
+                was_waiting.push_back(t.waiting);
+                t.waiting = 0;
+            }
         }
     }
     timed(c, KC_PFORCE, [&] { mpm::launch_particle_force<D>(p, c->stream, c->grid_block, true, true); });
@@ -413,9 +455,18 @@ int substep(mpm_context* c, const float* dts, bool capture = false) {
     int rc = stage_cg<D>(c, false, capture);
     if (rc != MPM_OK) return rc;
     stage_g2p<D>(c);
+    if (!capture) {
+        rc = finish_cg_poll(c);
+        if (rc != MPM_OK) return rc;
+    }
     // a binning over the block capacity made the substep a no-op (seen when
-    // the CG's done poll read the counters; stats always read them)
-    if (c->overflow_seen) return mpm_set_error(MPM_E_CAPACITY, "active blocks exceed the block capacity");
+    // the CG's done poll read the counters; stats always read them); waiting
+    // particles keep waiting (k_clear_waiting left the device flags set)
+    if (c->overflow_seen) {
+        c->emitted = was_emitted;
+        for (size_t k = 0; k < was_waiting.size(); ++k) c->table[k].waiting = was_waiting[k];
+        return mpm_set_error(MPM_E_CAPACITY, "active blocks exceed the block capacity");
+    }
     return MPM_OK;
 }
 
@@ -539,7 +590,8 @@ int allocate_context(mpm_context* c) {
         if (rc == MPM_OK && hipEventCreateWithFlags(&c->dt_ev, hipEventDisableTiming) != hipSuccess)
             rc = mpm_set_error(MPM_E_HIP, "hipEventCreate failed");
         hp = nullptr;
-        if (rc == MPM_OK && hipHostMalloc(&hp, sizeof(int), hipHostMallocDefault) != hipSuccess)
+        // the overflow flag through the frame-done count (Counters::overflow .. frame_ndone)
+        if (rc == MPM_OK && hipHostMalloc(&hp, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess)
             rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");
         c->h_fdone = (int*)hp;
         if (rc == MPM_OK && hipEventCreateWithFlags(&c->fdone_ev, hipEventDisableTiming) != hipSuccess)
@@ -921,10 +973,15 @@ int device_frames(mpm_context* c, const float* time, const float* target, float*
         c->binned = false;
         c->ever_binned = true;
         c->stepped = true;
-        HIPCHECK(hipMemcpyAsync(c->h_fdone, &c->ctr->frame_ndone, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        // one copy reads the overflow flag and the frame-done count
+        static_assert(offsetof(mpm::Counters, frame_ndone) == offsetof(mpm::Counters, overflow) + 3 * sizeof(int),
+                      "layout");
+        HIPCHECK(hipMemcpyAsync(c->h_fdone, &c->ctr->overflow, 4 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
         HIPCHECK(hipEventRecord(c->fdone_ev, c->stream));
         HIPCHECK(hipEventSynchronize(c->fdone_ev));
-        ndone = *c->h_fdone;
+        // a binning over the block capacity made the replayed substeps no-ops
+        if (c->h_fdone[0] != 0) return mpm_set_error(MPM_E_CAPACITY, "active blocks exceed the block capacity");
+        ndone = c->h_fdone[3];
     }
     HIPCHECK(hipMemcpy(f.data(), c->frame, sizeof(mpm::FrameState) * (size_t)n, hipMemcpyDeviceToHost));
     for (int s = 0; s < n; ++s) {
@@ -995,6 +1052,9 @@ int do_substep(mpm_context* c, const float* dts) {
     return collect_timing(c);
 }
 
+// Max |V| per scene after the last G2P.  Also reports a block-capacity
+// overflow of the last binning (the substeps it made no-ops may have been
+// queued without a poll: replayed graphs).
 int max_velocities(mpm_context* c, float* out) {
     HIPCHECK(hipStreamSynchronize(c->stream));
     std::vector<mpm::SceneState> st((size_t)c->nscenes);
@@ -1004,6 +1064,9 @@ int max_velocities(mpm_context* c, float* out) {
         std::memcpy(&v2, &st[(size_t)s].max_v2, sizeof(v2));
         out[s] = std::sqrt(v2);
     }
+    int overflow = 0;
+    HIPCHECK(hipMemcpy(&overflow, &c->ctr->overflow, sizeof(int), hipMemcpyDeviceToHost));
+    if (overflow) return mpm_set_error(MPM_E_CAPACITY, "active blocks exceed the block capacity");
     return MPM_OK;
 }
 
@@ -1863,6 +1926,10 @@ int mpm_time_stage(mpm_handle c, int32_t stage, int32_t reps, float* avg_ms) {
     const size_t pbytes = (size_t)nfields(c->D) * (size_t)c->pcap * sizeof(float);
     const int cur = c->cur;
     HIPCHECK(hipStreamSynchronize(c->stream));
+    // G2P's reduction rewrites each scene's max speed, removal count and
+    // non-finite flag: they come back with the particles
+    std::vector<mpm::SceneState> sst_saved(n);
+    HIPCHECK(hipMemcpy(sst_saved.data(), c->sst, sizeof(mpm::SceneState) * n, hipMemcpyDeviceToHost));
     std::vector<float> dts(n);
     for (size_t s = 0; s < n; ++s) dts[s] = c->sc[s].last_dt;
     set_step(c, dts.data(), 0.0f, false, false);
@@ -1893,6 +1960,7 @@ int mpm_time_stage(mpm_handle c, int32_t stage, int32_t reps, float* avg_ms) {
     HIPCHECK(hipMemcpyAsync(c->keybuf[cur], c->keybuf[cur ^ 1], (size_t)c->pcap * sizeof(int), hipMemcpyDeviceToDevice,
                             c->stream));
     HIPCHECK(hipStreamSynchronize(c->stream));
+    HIPCHECK(hipMemcpy(c->sst, sst_saved.data(), sizeof(mpm::SceneState) * n, hipMemcpyHostToDevice));
     float ms = 0.0f;
     if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
     if (e0) (void)hipEventDestroy(e0);

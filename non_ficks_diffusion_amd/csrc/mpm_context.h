@@ -61,6 +61,12 @@ struct mpm_context {
     int* h_done = nullptr;  // pinned: per poll slot, the overflow flag and the CG done count
     bool overflow_seen = false;  // the last substep's poll saw a block-capacity overflow
     hipEvent_t done_ev[2] = {nullptr, nullptr};
+    // done polls that a solve running to its last chunk left unread; they are
+    // read once the rest of the substep is queued (finish_cg_poll)
+    bool poll_open = false;
+    bool poll_pending[2] = {false, false};
+    int poll_boundary[2] = {0, 0};
+    int poll_older = 0, poll_target = 1, poll_max_it = 0;
     int grid_particle = 0, grid_block = 0, grid_node = 0;
     int op_mode = 0, grid_op = 0;  // CG operator: 0 four waves per block, 1 one wave per block
     size_t nred = 0;           // entries of the reduction partial arrays

@@ -2208,9 +2208,11 @@ __global__ __launch_bounds__(WG) void k_set_step(KP p, StepArgs a) {
 }
 
 // Process_Waiting_Particles ran for every waiting scene: clear the flags.
+// After a binning over the block capacity no block was set up and the
+// waiting particles have no volume yet: they keep waiting.
 __global__ __launch_bounds__(WG) void k_clear_waiting(KP p) {
     const int i = blockIdx.x * WG + threadIdx.x;
-    if (i < p.nscenes) p.scn[i].waiting = 0;
+    if (i < p.nscenes && !p.ctr->overflow) p.scn[i].waiting = 0;
 }
 
 // Removes every particle of scene `scene` (key -1: the next binning drops it).

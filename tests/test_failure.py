@@ -236,6 +236,50 @@ def test_block_capacity_overflow_is_refused_and_changes_nothing():
     s.close()
 
 
+@pytest.mark.parametrize("graph", [False, True])
+def test_overflow_after_capped_solves_is_reported(graph):
+    """A CG solve that runs to its cap (here: one iteration) leaves its last
+    done poll unread until the rest of the substep is queued; the overflow
+    flag that poll carries must still make the substep report
+    MPM_E_CAPACITY.  A lattice
+    of 3,600 particles, one per 4^3 block (3,600 active blocks of the 4,096
+    the handle stores), moves one cell diagonally in a substep, after which
+    every particle's stencil spans 8 blocks (4,805 in all).  Replayed
+    substeps read no poll: the next call that reads the device state reports
+    it (the max speed, the device frame loop)."""
+    d, _ = M.scene_preset("two_balls", 128, 0)
+    d.cg_max_iterations = 1  # every solve is one chunk: its poll is left for after G2P
+    d.cg_tolerance = 1e-12
+    d.nplanes = 0
+    d.gravity[:] = (0.0, 0.0, 0.0)
+    x, _, m = block_lattice(4)
+    dt = 1e-4
+    cell_per_dt = np.float32(1.0 / 128 / dt)
+    rng = np.random.default_rng(8)
+    v = (cell_per_dt + 0.2 * rng.standard_normal(x.shape)).astype(np.float32)
+    s = M.Scene(d, len(m))
+    s.upload(x, v, m)
+    s.initialize()
+    if graph:
+        s.use_graph(True)
+    s.substep(dt)  # the particles cross into the next cell
+    assert s.stats()["cg_iterations"] <= 1
+    if graph:
+        s.substep(dt)  # a replay: no poll
+    else:
+        with pytest.raises(M.MPMError) as e:
+            s.substep(dt)
+        assert e.value.code == E_CAPACITY
+    with pytest.raises(M.MPMError) as e:
+        s.max_particle_velocity()
+    assert e.value.code == E_CAPACITY
+    if graph:
+        with pytest.raises(M.MPMError) as e:
+            s.advance_to_time(0.0, 1e-3)
+        assert e.value.code == E_CAPACITY
+    s.close()
+
+
 def test_call_order_and_argument_errors():
     # every condition is checked on the host and answered with a status; the
     # handle stays usable afterwards

@@ -244,7 +244,19 @@ def test_stage_timing_leaves_the_state_as_it_was():
             after = s.download()
             for f in ("ids", "x", "v", "F", "Jp", "volume"):
                 assert np.array_equal(after[f], before[f]), f
+        # what the driver reads next: the max speed, the stats and a frame
+        # advanced with the dt rule from that max speed
+        vmax = s.max_particle_velocity()
+        st = s.stats()
         s.substep(1e-3)
-        runs.append(s.download())
+        after_step = s.download()
+        t, n, _ = s.advance_to_time(0.0, 1.0 / 240.0)
+        runs.append((vmax, st, after_step, n, t, s.download()))
         s.close()
-    assert_same(runs[1], runs[0], "substep after stage timing")
+    (v0, st0, a0, n0, t0, f0), (v1, st1, a1, n1, t1, f1) = runs
+    assert v1 == v0, (v1, v0)
+    for k in ("live_particles", "max_speed", "active_blocks"):
+        assert st1[k] == st0[k], (k, st1[k], st0[k])
+    assert_same(a1, a0, "substep after stage timing")
+    assert n1 == n0 and abs(t1 - t0) <= 1e-6 * t0, (n1, n0, t1, t0)
+    assert_same(f1, f0, "frame after stage timing")
