@@ -306,6 +306,38 @@ int mpm_time_stage(mpm_handle h, int32_t stage, int32_t reps, float* avg_ms);
  * allocates and frees the two buffers. */
 int mpm_measure_copy_bandwidth(int64_t bytes, int32_t reps, double* gbs);
 
+/* ---- one substep without allocation ------------------------------------ */
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): MPM_Driver::Advance_Step (MPM_Driver.cpp:103-116) of one
+ * scene on particle arrays in device memory that the caller owns, with no
+ * handle and no allocation.  Arrays are particle-major as on the host (x[i*dim+a],
+ * F row-major per particle); x, v, F, Jp and volume are read and then
+ * overwritten with the state after the substep at each particle's index; F
+ * and Jp may be NULL (identity and 1, not written).  A particle of volume 0
+ * gets its volume from the grid mass first (Process_Waiting_Particles,
+ * MPM_Example.cpp:827-840), so a first call on fresh particles is
+ * MPM_Driver::Initialize plus one substep.  valid[i] = 1 for particles inside
+ * the domain after the substep, 0 for removed ones (MPM_Example.cpp:650-660).
+ * Every particle takes material entry 0; the CG runs a fixed count of
+ * cg_max_iterations iterations (those after convergence return at once).
+ * stream: a hipStream_t, or NULL for the null stream.  With stats == NULL the
+ * call is asynchronous on the stream and calls no synchronizing HIP routine;
+ * with stats it waits and returns MPM_E_CAPACITY when the particles needed
+ * more than MPM_NOALLOC_MAX_BLOCKS active blocks (the substep was then a
+ * no-op) and MPM_E_NONFINITE as mpm_substep does.
+ * Every working buffer lives in one static device array per GPU (40 MiB,
+ * held until the process ends) that every call on that GPU shares and that
+ * keeps the last call's contents: calls that overlap on one GPU race.  Sizes
+ * beyond the limits are refused with MPM_E_CAPACITY before any GPU work;
+ * CG caps above 256 with MPM_E_INVALID. */
+#define MPM_NOALLOC_MAX_PARTICLES 65536
+#define MPM_NOALLOC_MAX_COUNTS 64
+#define MPM_NOALLOC_MAX_BLOCKS 2048
+int mpm_substep_noalloc(const mpm_scene_desc* desc, int64_t n, float* x, float* v, float* F, float* Jp,
+                        const float* mass, float* volume, int32_t* valid, float dt, void* stream,
+                        mpm_step_stats* stats);
+
 /* ---- launch shapes, autotuner, captured substeps ------------------------ */
 /* (No counterpart in the reference.)  Current launch shapes of a handle. */
 int mpm_get_launch_config(mpm_handle h, mpm_launch_config* cfg);

@@ -192,6 +192,8 @@ SIGNATURES = {
                                              _IP, _LP]),
     "mpm_batch_advance_to_time": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
                                                  ctypes.POINTER(ctypes.c_double), _IP]),
+    "mpm_substep_noalloc": (ctypes.c_int, [ctypes.POINTER(SceneDesc), ctypes.c_int64] + [ctypes.c_void_p] * 7
+                            + [ctypes.c_float, ctypes.c_void_p, ctypes.POINTER(StepStats)]),
     "mpm_scene_preset": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint64,
                                         ctypes.POINTER(SceneDesc), _LP]),
     "mpm_scene_generate": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int32, ctypes.c_uint64, ctypes.c_int64,
@@ -287,6 +289,24 @@ def scene_materials(name: str, counts: int = 0, seed: int = 0, n: int = 0) -> np
     k = ctypes.c_int64()
     _check(lib().mpm_scene_materials(name.encode(), counts, seed, n, count, _ip(out), ctypes.byref(k)))
     return out
+
+
+NOALLOC_MAX_PARTICLES, NOALLOC_MAX_COUNTS, NOALLOC_MAX_BLOCKS = 65536, 64, 2048
+
+
+def substep_noalloc(desc: SceneDesc, n: int, x: int, v: int, F: Optional[int], Jp: Optional[int], mass: int,
+                    volume: int, valid: int, dt: float, stream: Optional[int] = None, stats: bool = True):
+    """One substep of one scene on particle arrays in device memory that the
+    caller owns (device addresses), allocating nothing (mpm_substep_noalloc):
+    x, v, F, Jp and volume are overwritten with the state after the substep,
+    valid[i] = 1 for particles still inside the domain.  stream: a hipStream_t
+    address or None.  Returns the step stats (stats=True waits for the
+    stream) or None."""
+    d = desc.copy()
+    st = StepStats() if stats else None
+    _check(lib().mpm_substep_noalloc(ctypes.byref(d), int(n), x, v, F, Jp, mass, volume, valid, float(dt), stream,
+                                     ctypes.byref(st) if st is not None else None))
+    return st.as_dict() if st is not None else None
 
 
 def kernel_class_names():

@@ -521,62 +521,79 @@ int fill_stats(mpm_context* c, int s, mpm_step_stats* out) {
     return MPM_OK;
 }
 
-// Device buffers, stream, events and launch shapes of a context whose D,
-// nscenes, pcap, nslots and bcap are set.
-int allocate_context(mpm_context* c) {
-    const int D = c->D, nscenes = c->nscenes;
+// Sizes that follow from a context's D, pcap, nslots and bcap: grid nodes,
+// scan parts, reduction partials and the default launch shapes.
+void size_context(mpm_context* c) {
     c->gcap = (int64_t)c->bcap * 64;
     c->nparts = (c->nslots + mpm::SCAN_TILE - 1) / mpm::SCAN_TILE;
-    int rc = MPM_OK;
-    auto A = [&](int r) { if (rc == MPM_OK) rc = r; };
-    if (hipStreamCreate(&c->stream) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipStreamCreate failed");
-    for (int k = 0; k < 2; ++k) {
-        A(dalloc(c, &c->pbuf[k], (size_t)nfields(D) * c->pcap));
-        A(dalloc(c, &c->idbuf[k], (size_t)c->pcap));
-        A(dalloc(c, &c->keybuf[k], (size_t)c->pcap));
-        A(dalloc(c, &c->scbuf[k], (size_t)c->pcap));
-    }
-    A(dalloc(c, &c->prank, (size_t)c->pcap));
-    A(dalloc(c, &c->rcache, (size_t)D * D * c->pcap));
-    A(dalloc(c, &c->pop, (size_t)mpm::op_record_floats(D) * c->pcap));
-    A(dalloc(c, &c->slot_count, (size_t)c->nslots));
-    A(dalloc(c, &c->slot_touched, (size_t)c->nslots));
-    A(dalloc(c, &c->slot_active, (size_t)c->nslots));
-    A(dalloc(c, &c->slot_scene, (size_t)c->nslots));
-    A(dalloc(c, &c->block_slot, (size_t)c->bcap));
-    A(dalloc(c, &c->block_scene, (size_t)c->bcap));
-    A(dalloc(c, &c->block_pstart, (size_t)c->bcap + 1));
-    A(dalloc(c, &c->block_nbr, (size_t)c->bcap * (D == 3 ? 27 : 9)));
-    A(dalloc(c, &c->block_tnbr, (size_t)c->bcap * (D == 3 ? 27 : 9)));
-    A(dalloc(c, &c->cell_count, (size_t)c->nslots * 64));
-    A(dalloc(c, &c->cell_start, (size_t)c->bcap * 65 + 1));
-    A(dalloc(c, &c->tiles, (size_t)c->bcap * (D + 1) * (D == 3 ? 216 : 100)));
-    A(dalloc(c, &c->scan_part, (size_t)2 * c->nparts));
-    A(dalloc(c, &c->g_m, (size_t)c->gcap));
-    A(dalloc(c, &c->g_v, (size_t)D * c->gcap));
-    A(dalloc(c, &c->g_vs, (size_t)D * c->gcap));
-    A(dalloc(c, &c->g_q, (size_t)D * c->gcap));
-    A(dalloc(c, &c->g_s, (size_t)D * c->gcap));
-    A(dalloc(c, &c->g_r, (size_t)D * c->gcap));
-    A(dalloc(c, &c->g_flags, (size_t)c->gcap));
-    A(dalloc(c, &c->ctr, 1));
-    A(dalloc(c, &c->scn, (size_t)nscenes));
-    A(dalloc(c, &c->sst, (size_t)nscenes));
-    A(dalloc(c, &c->cg, (size_t)nscenes));
+    // reduction partials: per block (batches) or per node work-group (one scene)
+    c->nred = (size_t)std::max(c->bcap, MAX_NODE_GROUPS);
     c->grid_particle = (int)std::min<int64_t>(((c->pcap + mpm::WG - 1) / mpm::WG + 7) / 8 * 8, 4096);
     c->grid_block = (int)std::min<int64_t>(((int64_t)c->bcap + 7) / 8 * 8, 8192);
     c->grid_op = c->grid_block;
     c->op_mode = 0;
     // node kernels: one wave per block, up to 1024 work-groups (a multiple of 8 for the XCD ranges)
     c->grid_node = (int)std::min<int64_t>(((int64_t)c->bcap / (mpm::WG / 64) + 7) / 8 * 8 + 8, 1024);
-    // reduction partials: per block (batches) or per node work-group (one scene)
-    c->nred = (size_t)std::max(c->bcap, MAX_NODE_GROUPS);
-    A(dalloc(c, &c->bsum, c->nred));
-    A(dalloc(c, &c->bmax, c->nred));
-    A(dalloc(c, &c->bsq, c->nred));
-    A(dalloc(c, &c->bu, (size_t)2 * c->bcap));
-    A(dalloc(c, &c->d_dt, (size_t)nscenes));
-    A(dalloc(c, &c->frame, (size_t)nscenes));
+}
+
+// Every device buffer of a sized context, in one fixed order: a handle
+// allocates each of them (allocate_context), the allocation-free substep
+// carves them from its static scratch (mpm_substep_noalloc).
+template <class Take>
+void layout_buffers(mpm_context* c, Take&& take) {
+    const int D = c->D;
+    const size_t nscenes = (size_t)c->nscenes, pcap = (size_t)c->pcap, nslots = (size_t)c->nslots,
+                 bcap = (size_t)c->bcap, gcap = (size_t)c->gcap;
+    for (int k = 0; k < 2; ++k) {
+        take(&c->pbuf[k], (size_t)nfields(D) * pcap);
+        take(&c->idbuf[k], pcap);
+        take(&c->keybuf[k], pcap);
+        take(&c->scbuf[k], pcap);
+    }
+    take(&c->prank, pcap);
+    take(&c->rcache, (size_t)D * D * pcap);
+    take(&c->pop, (size_t)mpm::op_record_floats(D) * pcap);
+    take(&c->slot_count, nslots);
+    take(&c->slot_touched, nslots);
+    take(&c->slot_active, nslots);
+    take(&c->slot_scene, nslots);
+    take(&c->block_slot, bcap);
+    take(&c->block_scene, bcap);
+    take(&c->block_pstart, bcap + 1);
+    take(&c->block_nbr, bcap * (D == 3 ? 27 : 9));
+    take(&c->block_tnbr, bcap * (D == 3 ? 27 : 9));
+    take(&c->cell_count, nslots * 64);
+    take(&c->cell_start, bcap * 65 + 1);
+    take(&c->tiles, bcap * (D + 1) * (D == 3 ? 216 : 100));
+    take(&c->scan_part, (size_t)2 * c->nparts);
+    take(&c->g_m, gcap);
+    take(&c->g_v, (size_t)D * gcap);
+    take(&c->g_vs, (size_t)D * gcap);
+    take(&c->g_q, (size_t)D * gcap);
+    take(&c->g_s, (size_t)D * gcap);
+    take(&c->g_r, (size_t)D * gcap);
+    take(&c->g_flags, gcap);
+    take(&c->ctr, 1);
+    take(&c->scn, nscenes);
+    take(&c->sst, nscenes);
+    take(&c->cg, nscenes);
+    take(&c->bsum, c->nred);
+    take(&c->bmax, c->nred);
+    take(&c->bsq, c->nred);
+    take(&c->bu, (size_t)2 * bcap);
+    take(&c->d_dt, nscenes);
+    take(&c->frame, nscenes);
+}
+
+// Device buffers, stream, events and launch shapes of a context whose D,
+// nscenes, pcap, nslots and bcap are set.
+int allocate_context(mpm_context* c) {
+    const int nscenes = c->nscenes;
+    size_context(c);
+    int rc = MPM_OK;
+    auto A = [&](int r) { if (rc == MPM_OK) rc = r; };
+    if (hipStreamCreate(&c->stream) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipStreamCreate failed");
+    layout_buffers(c, [&](auto** ptr, size_t count) { A(dalloc(c, ptr, count)); });
     if (rc == MPM_OK) {
         void* hp = nullptr;
         if (hipHostMalloc(&hp, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");
@@ -2171,3 +2188,124 @@ int mpm_batch_advance_to_time(mpm_handle c, const double* time, const double* ta
 }
 
 }  // extern "C"
+
+// ---- allocation-free substep ------------------------------------------------
+namespace {
+
+// Scratch of mpm_substep_noalloc: every device buffer of a one-scene context
+// (sorted particle copies, block tables, grid channels, tiles, CG partials,
+// counters), carved afresh by each call from this array.  Each device that
+// runs the entry point holds its own copy until the process ends.  In host
+// code the array's name is a host-side stand-in and never a kernel argument:
+// its device address comes from hipGetSymbolAddress, once per device.
+constexpr size_t NOALLOC_SCRATCH_BYTES = (size_t)40 << 20;
+static __device__ __align__(256) unsigned char scratch[NOALLOC_SCRATCH_BYTES];
+static_assert(MPM_NOALLOC_MAX_BLOCKS <= 2048 && MPM_NOALLOC_MAX_COUNTS <= 64 && MPM_NOALLOC_MAX_PARTICLES <= 65536,
+              "the scratch size is computed for these limits");
+
+struct NoallocDevice {
+    unsigned char* base = nullptr;  // device address of `scratch` on this device
+    mpm_context* c = nullptr;       // the host view every call on this device reuses
+};
+constexpr int NOALLOC_MAX_DEVICES = 64;
+std::mutex g_noalloc_mutex;  // held for the host part of a call
+NoallocDevice g_noalloc[NOALLOC_MAX_DEVICES];
+
+// Set-up on the first call on a device (with that device current): the
+// scratch's device address and the reusable host view.
+int noalloc_setup(NoallocDevice& nd) {
+    if (nd.base) return MPM_OK;
+    void* ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || ptr == nullptr)
+        return mpm_set_error(MPM_E_HIP, std::string("mpm_substep_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+    mpm_context* c = new mpm_context;
+    c->nscenes = 1;
+    c->sc.resize(1);
+    c->table.resize(1);
+    nd.c = c;
+    nd.base = static_cast<unsigned char*>(ptr);
+    return MPM_OK;
+}
+
+template <int D>
+int noalloc_substep(NoallocDevice& nd, const mpm_scene_desc& d, int64_t n, float* x, float* v, float* F, float* Jp,
+                    const float* mass, float* volume, int32_t* valid, float dt, hipStream_t stream,
+                    mpm_step_stats* stats) {
+    mpm_context* c = nd.c;
+    c->D = D;
+    c->nscenes = 1;
+    c->pcap = n;
+    c->nslots = slots_of_desc(d);
+    c->bcap = std::min(c->nslots, MPM_NOALLOC_MAX_BLOCKS);
+    size_context(c);
+    c->stream = stream;
+    c->sc[0] = SceneHost{};
+    c->sc[0].desc = d;
+    c->sc[0].capacity = n;
+    c->sc[0].n_uploaded = n;
+    c->sc[0].uploaded = true;
+    c->cur = 0;
+    c->solo = -1;
+    c->pool = false;
+    c->graph = false;
+    c->timing = false;
+    c->emitted = true;  // every call runs Process_Waiting_Particles for particles of volume 0
+    c->ever_binned = true;
+    size_t off = 0;
+    bool fits = true;
+    layout_buffers(c, [&](auto** ptr, size_t count) {
+        using T = std::remove_reference_t<decltype(**ptr)>;
+        const size_t bytes = std::max<size_t>(count, 1) * sizeof(T);
+        off = (off + 255) & ~(size_t)255;
+        if (off + bytes > NOALLOC_SCRATCH_BYTES) {
+            fits = false;
+            *ptr = nullptr;
+            return;
+        }
+        *ptr = reinterpret_cast<T*>(nd.base + off);
+        off += bytes;
+    });
+    if (!fits) return mpm_set_error(MPM_E_CAPACITY, "mpm_substep_noalloc: the scene does not fit the scratch");
+    mpm::SceneParams sp = scene_params(d, 0, c->nslots);
+    sp.waiting = 1;
+    mpm::NoallocIn in{x, v, F, Jp, mass, volume, valid, c->d_dt, c->slot_scene, (int)n, dt};
+    mpm::launch_noalloc_load<D>(make_kp(c), stream, c->grid_particle, sp, in);
+    int rc = substep<D>(c, nullptr, true);  // dt from d_dt, CG of a fixed count (no host polls)
+    if (rc) return rc;
+    const mpm::NoallocOut o{x, v, F, Jp, volume, valid};
+    mpm::launch_noalloc_store<D>(make_kp(c), stream, c->grid_particle, o);
+    rc = check_launch("mpm_substep_noalloc");
+    if (rc) return rc;
+    if (stats) return fill_stats(c, 0, stats);
+    return MPM_OK;
+}
+
+}  // namespace
+
+extern "C" int mpm_substep_noalloc(const mpm_scene_desc* desc, int64_t n, float* x, float* v, float* F, float* Jp,
+                                   const float* mass, float* volume, int32_t* valid, float dt, void* stream,
+                                   mpm_step_stats* stats) {
+    if (!desc || !x || !v || !mass || !volume || !valid)
+        return mpm_set_error(MPM_E_INVALID, "mpm_substep_noalloc: null argument");
+    int rc = check_desc(*desc, "mpm_substep_noalloc");
+    if (rc) return rc;
+    if (n < 1 || n > MPM_NOALLOC_MAX_PARTICLES)
+        return mpm_set_error(MPM_E_CAPACITY, "mpm_substep_noalloc: n must be in [1, " +
+                                                 std::to_string(MPM_NOALLOC_MAX_PARTICLES) + "]");
+    if (desc->counts > MPM_NOALLOC_MAX_COUNTS)
+        return mpm_set_error(MPM_E_CAPACITY, "mpm_substep_noalloc: counts above " + std::to_string(MPM_NOALLOC_MAX_COUNTS));
+    if (desc->cg_max_iterations < 1 || desc->cg_max_iterations > MAX_GRAPH_CG_ITERATIONS)
+        return mpm_set_error(MPM_E_INVALID, "mpm_substep_noalloc: the CG cap must be in [1, 256]");
+    if (!(dt > 0.0f) || !std::isfinite(dt)) return mpm_set_error(MPM_E_INVALID, "mpm_substep_noalloc: dt must be positive");
+    int dev = 0;
+    HIPCHECK(hipGetDevice(&dev));
+    if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return mpm_set_error(MPM_E_INVALID, "mpm_substep_noalloc: device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_noalloc_mutex);
+    NoallocDevice& nd = g_noalloc[dev];
+    rc = noalloc_setup(nd);
+    if (rc) return rc;
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    return desc->dim == 3 ? noalloc_substep<3>(nd, *desc, n, x, v, F, Jp, mass, volume, valid, dt, st, stats)
+                          : noalloc_substep<2>(nd, *desc, n, x, v, F, Jp, mass, volume, valid, dt, st, stats);
+}

@@ -145,6 +145,30 @@ struct StepArgs {
     float dt[STEP_ARGS_MAX];
 };
 
+// The caller's particle-major arrays of an allocation-free substep (device
+// memory): inputs, and the outputs written back at each particle's index.
+struct NoallocIn {
+    const float* x;
+    const float* v;
+    const float* F;       // may be null: identity
+    const float* Jp;      // may be null: 1 (an EOS material: its density)
+    const float* mass;
+    const float* volume;  // 0: the particle waits for Process_Waiting_Particles
+    int* valid;
+    float* d_dt;          // the scene's dt in device memory (read by k_set_step_dev)
+    int* slot_scene;      // the context's slot owners (all slots: scene 0)
+    int n;
+    float dt;
+};
+struct NoallocOut {
+    float* x;
+    float* v;
+    float* F;   // may be null
+    float* Jp;  // may be null
+    float* volume;
+    int* valid;  // 1: binned and inside the domain after G2P; 0: removed
+};
+
 // launchers (host)
 void launch_set_step(const KP& p, hipStream_t s, const StepArgs& a);
 void launch_set_step_dev(const KP& p, hipStream_t s, const float* dt);
@@ -174,5 +198,7 @@ void launch_cg_reduce_sq(const KP& p, hipStream_t s, int k);
 void launch_cg_reduce_rr(const KP& p, hipStream_t s, int k);
 template <int D> void launch_g2p(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_max_speed(const KP& p, hipStream_t s, int grid);
+template <int D> void launch_noalloc_load(const KP& p, hipStream_t s, int grid, const SceneParams& sp, const NoallocIn& a);
+template <int D> void launch_noalloc_store(const KP& p, hipStream_t s, int grid, const NoallocOut& o);
 
 }  // namespace mpm

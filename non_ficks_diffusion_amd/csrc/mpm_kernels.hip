@@ -2289,8 +2289,85 @@ __global__ __launch_bounds__(WG) void k_frame_dt(KP p, FrameState* f, float* dt_
 }
 
 // ============================================================================
+// Allocation-free substep: the scene table, the counters and the binning's
+// slot and cell counters of the static scratch start over at every call (the
+// scratch keeps the last call's contents), and the caller's particle-major
+// arrays become binning's field-major input; afterwards every binned
+// particle goes back to the caller's arrays at its original index.
+// ============================================================================
+template <int D>
+__global__ __launch_bounds__(WG) void k_noalloc_load(KP p, SceneParams sp, NoallocIn a) {
+    const int gid = blockIdx.x * WG + threadIdx.x, stride = gridDim.x * WG;
+    if (gid == 0) {
+        p.scn[0] = sp;
+        Counters c{};
+        c.n_live = a.n;
+        c.n_prev = a.n;
+        c.slot_hw = p.nslots;
+        *p.ctr = c;
+        p.sst[0] = SceneState{};
+        a.d_dt[0] = a.dt;
+    }
+    for (int s = gid; s < p.nslots; s += stride) {
+        p.slot_count[s] = 0;
+        p.slot_touched[s] = 0;
+        a.slot_scene[s] = 0;
+    }
+    for (int64_t k = gid; k < (int64_t)p.nslots * 64; k += stride) p.cell_count[k] = 0;
+    const Material& mat = sp.mats[0];
+    for (int i = gid; i < a.n; i += stride) {
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            p.pcur[(PF<D>::X + c) * p.pcap + i] = a.x[(int64_t)i * D + c];
+            p.pcur[(PF<D>::V + c) * p.pcap + i] = a.v[(int64_t)i * D + c];
+        }
+#pragma unroll
+        for (int k = 0; k < D * D; ++k)
+            p.pcur[(PF<D>::F + k) * p.pcap + i] = a.F ? a.F[(int64_t)i * D * D + k] : (k % (D + 1) == 0 ? 1.0f : 0.0f);
+        p.pcur[PF<D>::J * p.pcap + i] = a.Jp ? a.Jp[i] : (mat.eos ? mat.rho0 : 1.0f);
+        p.pcur[PF<D>::M * p.pcap + i] = a.mass[i];
+        p.pcur[PF<D>::VOL * p.pcap + i] = a.volume[i];
+        p.pcur[PF<D>::MAT * p.pcap + i] = __int_as_float(0);
+        p.idcur[i] = i;
+        p.keycur[i] = 0;
+        p.scur[i] = 0;
+        a.valid[i] = 0;
+    }
+}
+
+template <int D>
+__global__ __launch_bounds__(WG) void k_noalloc_store(KP p, NoallocOut o) {
+    const int n = p.ctr->n_live;  // binned particles (dropped ones keep valid = 0)
+    for (int j = blockIdx.x * WG + threadIdx.x; j < n; j += gridDim.x * WG) {
+        const int64_t i = p.idcur[j];
+        o.valid[i] = p.keycur[j] >= 0 ? 1 : 0;
+#pragma unroll
+        for (int c = 0; c < D; ++c) {
+            o.x[i * D + c] = p.pcur[(PF<D>::X + c) * p.pcap + j];
+            o.v[i * D + c] = p.pcur[(PF<D>::V + c) * p.pcap + j];
+        }
+        if (o.F) {
+#pragma unroll
+            for (int k = 0; k < D * D; ++k) o.F[i * D * D + k] = p.pcur[(PF<D>::F + k) * p.pcap + j];
+        }
+        if (o.Jp) o.Jp[i] = p.pcur[PF<D>::J * p.pcap + j];
+        o.volume[i] = p.pcur[PF<D>::VOL * p.pcap + j];
+    }
+}
+
+// ============================================================================
 // launchers
 // ============================================================================
+template <int D> void launch_noalloc_load(const KP& p, hipStream_t s, int grid, const SceneParams& sp, const NoallocIn& a) {
+    hipLaunchKernelGGL(k_noalloc_load<D>, dim3(grid), dim3(WG), 0, s, p, sp, a);
+}
+template <int D> void launch_noalloc_store(const KP& p, hipStream_t s, int grid, const NoallocOut& o) {
+    hipLaunchKernelGGL(k_noalloc_store<D>, dim3(grid), dim3(WG), 0, s, p, o);
+}
+template void launch_noalloc_load<2>(const KP&, hipStream_t, int, const SceneParams&, const NoallocIn&);
+template void launch_noalloc_load<3>(const KP&, hipStream_t, int, const SceneParams&, const NoallocIn&);
+template void launch_noalloc_store<2>(const KP&, hipStream_t, int, const NoallocOut&);
+template void launch_noalloc_store<3>(const KP&, hipStream_t, int, const NoallocOut&);
 void launch_set_step(const KP& p, hipStream_t s, const StepArgs& a) {
     hipLaunchKernelGGL(k_set_step, dim3((a.n + WG - 1) / WG), dim3(WG), 0, s, p, a);
 }
