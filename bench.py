@@ -10,9 +10,13 @@ first CFL substep, kept for every timed substep.  Particle state, grid and
 CG vectors stay resident in HBM; the timed region contains only kernel work.
 
 Prints one JSON line: particle-substeps/s, the roofline of the dominant
-kernel (CG operator) measured with HIP events on the scene's stream, and the
-CPU baseline (the oracle, the C++/OpenMP restatement of the reference CPU
-path, timed on the host cores on a bounded sample of the same workload).
+kernel (CG operator) measured with HIP events on the scene's stream, a
+second window after the avalanche has developed, the CPU baseline (the
+oracle, the C++/OpenMP restatement of the reference CPU path, timed on the
+host cores on a bounded sample of the same workload) and the serving leg:
+aggregate particle-substeps/s and p50/p99 frame latency of a bounded mixed
+request stream, served with continuous batching and by the plain
+one-scene-at-a-time loop.
 """
 import argparse
 import json
@@ -37,6 +41,11 @@ INSTRUCTIONS_FILE = os.path.join(ROOT, "profiles", "cg_force_avalanche_256_instr
 VALU_PEAK_G = 614.4
 WORKLOAD = "avalanche"
 COUNTS = 256
+# substeps between the first timed window and the second (the developed avalanche)
+DEVELOP = 1000
+# the default run's bounded mixed request stream
+SERVE_REQUESTS = 8
+SERVE_FRAMES_MIN, SERVE_FRAMES_MAX = 1, 2
 
 
 def canonical_bytes(N, Nn, I, M):
@@ -55,10 +64,22 @@ def cfl_dt(desc, vmax):
     return float(max(desc.min_dt, min(desc.max_dt, desc.cfl * dx / max(vmax, 1e-2))))
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(desc, x, v, m, dt, min_seconds=10.0, max_substeps=64):
     """Times the oracle (reference CPU path restated) on the host cores: one
     warm-up substep, then fixed-dt substeps until min_seconds have passed
-    (a bounded sample of the same workload)."""
+    (a bounded sample of the same workload).  Each substep's CG iteration
+    count is logged, the work-parity record beside the GPU's."""
     import oracle as O
     cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
     o = O.Oracle(desc, 4, cores, O.SVD_FP64)  # float state, float CG dot products (as the reference)
@@ -66,15 +87,19 @@ def cpu_baseline(desc, x, v, m, dt, min_seconds=10.0, max_substeps=64):
     o.initialize()
     o.advance_step(dt)  # warm-up substep
     t0 = time.perf_counter()
-    substeps, el = 0, 0.0
+    substeps, el, its, timed = 0, 0.0, [], 0.0
     while substeps < max_substeps and (substeps == 0 or el < min_seconds):
+        t1 = time.perf_counter()
         o.advance_step(dt)
+        timed += time.perf_counter() - t1
+        its.append(o.cg_stats()["cg_iterations"])
         substeps += 1
         el = time.perf_counter() - t0
     live = o.particles()["x"].shape[0]
-    return {"value": live * substeps / el, "unit": "particle-substeps/s", "cores": cores, "kind": "port",
+    return {"value": live * substeps / timed, "unit": "particle-substeps/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(), "cg_iterations": its,
             "sample": f"{WORKLOAD} {COUNTS}^3, {len(m)} particles, 1 warm-up + {substeps} fixed-dt substeps "
-                      f"(dt {dt:.2e}, CG cap {desc.cg_max_iterations}), {el:.1f} s"}
+                      f"(dt {dt:.2e}, CG cap {desc.cg_max_iterations}), {timed:.1f} s"}
 
 
 def measured_valu_insts(kernel="cg_force"):
@@ -269,6 +294,10 @@ def bench_serve(args):
     the request server (continuous batching on one device arena), both
     against the same arrival clock.  The arrival rate defaults to 1.5 times
     the plain loop's capacity on the stream (requests / total service time)."""
+    print(json.dumps(serve_stream(args)), flush=True)
+
+
+def serve_stream(args):
     import non_ficks_diffusion_amd as M
     from non_ficks_diffusion_amd import workloads as W
 
@@ -380,8 +409,7 @@ def bench_serve(args):
         "makespan_s": span_b,
     }
     if plain is None:
-        print(json.dumps(out), flush=True)
-        return
+        return out
 
     service, frame_ends, steps_p, kept, work_p, capacity = plain
     first, inter, span_p = fifo_replay([r.arrival for r in reqs], service, frame_ends)
@@ -401,7 +429,7 @@ def bench_serve(args):
         "served_vs_plain_rel_l2": {k: float(f"{v:.3e}") for k, v in err.items()},
         "checked_requests": check,
     })
-    print(json.dumps(out), flush=True)
+    return out
 
 
 def main():
@@ -411,6 +439,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--counts", type=int, default=COUNTS)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the mixed-stream serving leg of the default run")
+    ap.add_argument("--develop", type=int, default=DEVELOP,
+                    help="substeps between the first timed window and the second (0: no second window)")
     ap.add_argument("--cpu-seconds", type=float, default=10.0,
                     help="CPU baseline: time fixed-dt oracle substeps for at least this long")
     ap.add_argument("--workload", choices=["avalanche", "batch", "serve"], default="avalanche",
@@ -459,25 +490,25 @@ def main():
     s.initialize()
     dt = cfl_dt(desc, s.max_particle_velocity())
 
+    def timed_window(steps):
+        """K substeps between two stream synchronisations: particle-substeps/s
+        and ms per substep (live particles averaged over the window's ends)."""
+        s.synchronize()
+        live0 = s.stats()["live_particles"]
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            s.substep(dt)
+        s.synchronize()
+        el = time.perf_counter() - t0
+        live1 = s.stats()["live_particles"]
+        return steps * 0.5 * (live0 + live1) / el, el * 1e3 / steps
+
     for _ in range(args.warmup):
         s.substep(dt)
-    s.synchronize()
-    live0 = s.stats()["live_particles"]
-    # timed region: K substeps, synchronised on both sides
-    t0 = time.perf_counter()
-    live_sum = 0
-    for _ in range(args.steps):
-        s.substep(dt)
-    s.synchronize()
-    el = time.perf_counter() - t0
-    st = s.stats()
-    # live particles before the timed region plus after; removal is rare and
-    # monotone, so the average bounds the per-substep count
-    live_sum = args.steps * 0.5 * (live0 + st["live_particles"])
-    value = live_sum / el
-    ms_per_step = el * 1e3 / args.steps
+    value, ms_per_step = timed_window(args.steps)
 
-    # per-kernel HIP-event timing pass (outside the timed region)
+    # per-kernel HIP-event timing pass (outside the timed region): the stats
+    # of these substeps give the window's N, nodes and CG counts
     s.enable_timing(True)
     tsteps = 3
     stats = []
@@ -487,8 +518,8 @@ def main():
     s.enable_timing(False)
     N = stats[-1]["live_particles"]
     Nn = stats[-1]["active_blocks"] * 64
-    I = stats[-1]["cg_iterations"]
-    Mul = stats[-1]["cg_multiplies"]
+    I = float(np.mean([st["cg_iterations"] for st in stats]))
+    Mul = I + 1.0
     cb = canonical_bytes(N, Nn, I, Mul)
     # dominant kernel: the CG operator, launched back to back on the state of
     # the last substep between two HIP events on the scene's stream
@@ -496,7 +527,6 @@ def main():
     achieved = cb["cg_force"] / (avg_cg_ms * 1e-3) / 1e9
     traffic = measured_traffic()
     breakdown = {k: round(msv / tsteps, 4) for k, (msv, n) in kt.items() if n}
-    sub_ms = sum(msv for msv, _ in kt.values()) / tsteps
     # P2G, G2P and the constitutive+force kernel timed the same way: launched
     # back to back on the last substep's state (restored afterwards)
     live_ms = {"p2g": s.time_stage(M.STAGE_P2G, OPERATOR_REPS),
@@ -508,6 +538,24 @@ def main():
     # measured roof beside the 8 TB/s peak: a streaming 1 GiB copy
     copy_gbs = M.measure_copy_bandwidth(1 << 30, 10)
     per_kernel_copy_frac = {k: round(v / copy_gbs, 4) for k, v in per_kernel_gbs.items()}
+
+    # the developed avalanche: --develop more substeps at the same dt
+    # (untimed), then the same K substeps timed again; the CG counts are those
+    # of three substeps after the window
+    developed = None
+    if args.develop > 0:
+        for _ in range(args.develop):
+            s.substep(dt)
+        value_dev, ms_dev = timed_window(args.steps)
+        st_dev = [s.substep(dt, stats=True) for _ in range(tsteps)]
+        I_dev = float(np.mean([st["cg_iterations"] for st in st_dev]))
+        cb_dev = canonical_bytes(st_dev[-1]["live_particles"], st_dev[-1]["active_blocks"] * 64, I_dev, I_dev + 1.0)
+        done = args.warmup + args.steps + tsteps + args.develop
+        developed = {"after_substeps": done, "sim_time_s": round(done * dt, 4),
+                     "value": value_dev, "ms_per_step": ms_dev, "steps": args.steps,
+                     "particles": int(st_dev[-1]["live_particles"]), "active_blocks": int(st_dev[-1]["active_blocks"]),
+                     "cg_iterations": [int(st["cg_iterations"]) for st in st_dev],
+                     "substep_achieved_GBs": round(cb_dev["substep"] / (ms_dev * 1e-3) / 1e9, 1)}
 
     out = {
         "metric": "particle-substeps/s, 3D 1M snow",
@@ -525,7 +573,11 @@ def main():
         "config": {"workload": f"{WORKLOAD} (3D sparse avalanche, 30-degree incline)", "grid": f"{args.counts}^3",
                    "particles": int(N), "active_blocks": int(stats[-1]["active_blocks"]),
                    "active_block_fraction": round(stats[-1]["active_blocks"] / ((args.counts // 4) ** 3), 4),
-                   "dt": dt, "cg_iterations": int(I), "cg_multiplies": int(Mul),
+                   "dt": dt,
+                   "dt_rule": "fixed-dt mode: the first substep's dt, clamp(cfl dx / max(vmax, 1e-2), min_dt, max_dt) "
+                              "with max_dt the elastic bound dx / c_p (c_p = sqrt((lambda0 + 2 mu0) / rho)), not "
+                              "the 5e-3 cap: from rest it is the elastic bound",
+                   "cg_iterations": round(I, 2), "cg_multiplies": round(Mul, 2),
                    "cg_tolerance": float(desc.cg_tolerance), "cg_max_iterations": int(desc.cg_max_iterations),
                    "precision": "FP32 state, FP64 CG reductions and SVD", "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "kernel": "cg_force", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
@@ -537,21 +589,59 @@ def main():
         "valu_roofline": valu_roofline("cg_force", avg_cg_ms),
         # the FP64-SVD constitutive update and explicit force: VALU too
         "valu_roofline_constitutive": valu_roofline("constitutive_force", live_ms["constitutive_force"]),
-        "substep_roofline": {"algorithmic_bytes": int(cb["substep"]), "kernel_ms": round(sub_ms, 4),
-                             "achieved_GBs": round(cb["substep"] / (sub_ms * 1e-3) / 1e9, 1)},
+        # the whole substep: canonical bytes over the timed window's ms per substep
+        "substep_roofline": {"algorithmic_bytes": int(cb["substep"]), "ms": round(ms_per_step, 4),
+                             "achieved_GBs": round(cb["substep"] / (ms_per_step * 1e-3) / 1e9, 1),
+                             "frac": round(cb["substep"] / (ms_per_step * 1e-3) / 1e9 / HBM_PEAK_GBS, 4)},
         "kernel_GBs": per_kernel_gbs,
         "kernel_frac_of_hbm_peak": per_kernel_frac,
         "copy_roof": {"GBs": round(copy_gbs, 1), "bytes": 1 << 30, "launches": 10,
                       "form": "streaming device-to-device copy, read plus written bytes"},
         "kernel_frac_of_copy_roof": per_kernel_copy_frac,
-        "kernel_ms_per_substep": breakdown,
+        # each kernel class between its own HIP events over three substeps: the
+        # events serialise the launches, so the sum exceeds ms_per_step
+        "event_timed_ms_per_substep": breakdown,
         "kernel_live_ms": {k: round(v, 5) for k, v in {**live_ms, "cg_force": avg_cg_ms}.items()},
+        "developed": developed,
     }
     s.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(desc, x, v, m, dt, args.cpu_seconds)
-        out["gpu_over_cpu"] = round(value / out["cpu_baseline"]["value"], 1)
+        cpu = cpu_baseline(desc, x, v, m, dt, args.cpu_seconds)
+        out["cpu_baseline"] = cpu
+        out["gpu_over_cpu"] = round(value / cpu["value"], 1)
+        gpu_its = [int(st["cg_iterations"]) for st in stats]
+        # per CG operator application (iterations + 1 per substep): the same work unit on both sides
+        per_app = (value * (np.mean(gpu_its) + 1.0)) / (cpu["value"] * (np.mean(cpu["cg_iterations"]) + 1.0))
+        out["work_parity"] = {"gpu_cg_iterations": gpu_its, "cpu_cg_iterations": cpu["cg_iterations"],
+                              "gpu_over_cpu_per_operator_application": round(float(per_app), 1),
+                              "note": "same scene, dt and CG cap; the CPU sample runs the first substeps from "
+                                      "the start, the GPU figures are from its timed window"}
+    if not args.no_serve:
+        out["serving"] = serving_leg(args)
     print(json.dumps(out), flush=True)
+
+
+def serving_leg(args):
+    """The mixed request stream on a bounded sample: SERVE_REQUESTS Poisson
+    requests of SERVE_FRAMES_MIN-SERVE_FRAMES_MAX frames, served by the request
+    server (graph replays, autotuned launch shapes) and by the plain
+    one-scene-at-a-time loop against the same arrival clock."""
+    sa = argparse.Namespace(requests=SERVE_REQUESTS, rate=None, frames_min=SERVE_FRAMES_MIN,
+                            frames_max=SERVE_FRAMES_MAX, budget_gb=200.0, max_scenes=32, graph=True,
+                            autotune=True, tick_substeps=32, no_plain=False)
+    r = serve_stream(sa)
+    lat, lat_p = r["latency"], r["plain_form"]["latency"]
+    return {"workload": r["config"]["workload"], "requests": SERVE_REQUESTS,
+            "frames_total": r["config"]["frames_total"], "value": r["value"], "unit": "particle-substeps/s",
+            "frame_latency_p50_ms": lat["frame_p50_ms"], "frame_latency_p99_ms": lat["frame_p99_ms"],
+            "first_frame_p50_ms": lat["first_frame_p50_ms"], "first_frame_p99_ms": lat["first_frame_p99_ms"],
+            "plain_value": r["plain_form"]["value"],
+            "plain_frame_latency_p50_ms": lat_p["frame_p50_ms"], "plain_frame_latency_p99_ms": lat_p["frame_p99_ms"],
+            "plain_first_frame_p50_ms": lat_p["first_frame_p50_ms"],
+            "plain_first_frame_p99_ms": lat_p["first_frame_p99_ms"],
+            "speedup_vs_plain": r["speedup_vs_plain"], "substeps_equal": r["substeps_equal"],
+            "served_vs_plain_rel_l2": r["served_vs_plain_rel_l2"], "makespan_s": r["makespan_s"],
+            "plain_makespan_s": r["plain_form"]["makespan_s"]}
 
 
 if __name__ == "__main__":

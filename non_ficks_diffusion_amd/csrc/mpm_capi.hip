@@ -259,6 +259,13 @@ void stage_waiting(mpm_context* c) {
 constexpr int64_t SOLO_LAUNCHED_MIN = 65536;
 // iterations per launch batch of the polled CG after the first
 constexpr int CG_CHUNK = 4;
+// A done poll copies Counters::overflow .. cg_iters (five ints) into a slot
+// of POLL_INTS pinned ints: [0] overflow, [1] finished solves, [4] the
+// largest iteration count among them.
+constexpr int POLL_INTS = 8;
+static_assert(offsetof(mpm::Counters, cg_ndone) == offsetof(mpm::Counters, overflow) + sizeof(int) &&
+                  offsetof(mpm::Counters, cg_iters) == offsetof(mpm::Counters, overflow) + 4 * sizeof(int),
+              "Counters layout of the done poll");
 
 int max_cg_iterations(const mpm_context* c) {
     if (c->pool) return c->pool_cg_max;
@@ -315,9 +322,9 @@ int stage_cg(mpm_context* c, bool do_init, bool fixed = false) {
         const int n = launched == 0 ? first : std::min(CG_CHUNK, max_it - launched);
         for (int j = 0; j < n; ++j) iteration(launched + j);
         launched += n;
-        // the overflow flag and the done count are adjacent: one copy reads both
-        static_assert(offsetof(mpm::Counters, cg_ndone) == offsetof(mpm::Counters, overflow) + sizeof(int), "layout");
-        HIPCHECK(hipMemcpyAsync(&c->h_done[2 * slot], &c->ctr->overflow, 2 * sizeof(int), hipMemcpyDeviceToHost, c->stream));
+        // one copy reads the overflow flag, the done count and the iterations
+        HIPCHECK(hipMemcpyAsync(&c->h_done[POLL_INTS * slot], &c->ctr->overflow, 5 * sizeof(int), hipMemcpyDeviceToHost,
+                                c->stream));
         HIPCHECK(hipEventRecord(c->done_ev[slot], c->stream));
         pending[slot] = true;
         boundary[slot] = launched;
@@ -325,10 +332,12 @@ int stage_cg(mpm_context* c, bool do_init, bool fixed = false) {
         if (pending[slot]) {
             HIPCHECK(hipEventSynchronize(c->done_ev[slot]));
             pending[slot] = false;
-            c->overflow_seen = c->overflow_seen || c->h_done[2 * slot] != 0;
-            if (c->h_done[2 * slot + 1] >= target) {
-                // done by the first boundary: try one fewer next time
-                c->cg_predict = boundary[slot] == first ? first - 1 : boundary[slot];
+            const int* h = &c->h_done[POLL_INTS * slot];
+            c->overflow_seen = c->overflow_seen || h[0] != 0;
+            if (h[1] >= target) {
+                // the next solve's first batch: this one's iterations (iteration
+                // k's convergence test ends a solve of k updates)
+                c->cg_predict = h[4] + 1;
                 finished = true;
                 break;
             }
@@ -351,22 +360,25 @@ int stage_cg(mpm_context* c, bool do_init, bool fixed = false) {
 }
 
 // Reads the done polls of a solve that ran to its last chunk: the overflow
-// flag they carry makes the substep report MPM_E_CAPACITY, and the next
-// solve's first batch moves one chunk below where this one was seen done (or
-// below the cap when it never was), so the prediction comes back down.
+// flag they carry makes the substep report MPM_E_CAPACITY, and the
+// iteration count they carry sets the next solve's first batch, so the
+// prediction comes back down after a solve that needed many.
 int finish_cg_poll(mpm_context* c) {
     if (!c->poll_open) return MPM_OK;
     c->poll_open = false;
-    int done_at = -1;
+    int iters = -1;
     for (int j = 0; j < 2; ++j) {
         const int slot = (c->poll_older + j) & 1;
         if (!c->poll_pending[slot]) continue;
         c->poll_pending[slot] = false;
         HIPCHECK(hipEventSynchronize(c->done_ev[slot]));
-        c->overflow_seen = c->overflow_seen || c->h_done[2 * slot] != 0;
-        if (done_at < 0 && c->h_done[2 * slot + 1] >= c->poll_target) done_at = c->poll_boundary[slot];
+        const int* h = &c->h_done[POLL_INTS * slot];
+        c->overflow_seen = c->overflow_seen || h[0] != 0;
+        if (iters < 0 && h[1] >= c->poll_target) iters = h[4];
     }
-    c->cg_predict = std::max(CG_CHUNK, (done_at > 0 ? done_at : c->poll_max_it) - CG_CHUNK);
+    // seen done: its iterations; never seen done (the cap's test runs after
+    // the last poll): the cap
+    c->cg_predict = iters >= 0 ? iters + 1 : c->poll_max_it;
     return MPM_OK;
 }
 
@@ -596,7 +608,8 @@ int allocate_context(mpm_context* c) {
     layout_buffers(c, [&](auto** ptr, size_t count) { A(dalloc(c, ptr, count)); });
     if (rc == MPM_OK) {
         void* hp = nullptr;
-        if (hipHostMalloc(&hp, 4 * sizeof(int), hipHostMallocDefault) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");
+        if (hipHostMalloc(&hp, 2 * POLL_INTS * sizeof(int), hipHostMallocDefault) != hipSuccess)
+            rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");
         c->h_done = (int*)hp;
         for (int k = 0; k < 2 && rc == MPM_OK; ++k)
             if (hipEventCreateWithFlags(&c->done_ev[k], hipEventDisableTiming) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipEventCreate failed");

@@ -57,8 +57,8 @@ struct mpm_context {
     int* slot_scene = nullptr;   // per slot: owning scene, -1 if free
     double *bsum = nullptr, *bmax = nullptr, *bsq = nullptr;
     unsigned* bu = nullptr;
-    int cg_predict = 0;    // iterations launched before the done count was seen, last solve
-    int* h_done = nullptr;  // pinned: per poll slot, the overflow flag and the CG done count
+    int cg_predict = 0;    // the next solve's first launch batch: the last solve's iterations plus one
+    int* h_done = nullptr;  // pinned: per poll slot (8 ints), Counters::overflow .. cg_iters
     bool overflow_seen = false;  // the last substep's poll saw a block-capacity overflow
     hipEvent_t done_ev[2] = {nullptr, nullptr};
     // done polls that a solve running to its last chunk left unread; they are

@@ -14,7 +14,7 @@ namespace mpm {
 
 constexpr int WG = 256;          // work-group size of every kernel
 // floats per particle of the CG force operator record (see OpRec)
-constexpr int op_record_floats(int D) { return D == 3 ? 24 : 12; }
+constexpr int op_record_floats(int D) { return D == 3 ? 16 : 12; }
 constexpr int SCAN_PER_THREAD = 16;
 constexpr int SCAN_TILE = WG * SCAN_PER_THREAD;
 constexpr int MAX_PLANES = 8;
@@ -59,7 +59,7 @@ struct Counters {
     int cg_ndone;       // scenes whose CG solve is finished
     int slot_hw;        // slots [0, slot_hw) hold every scene's range (the binning scans them)
     int frame_ndone;    // device frame loop: scenes that reached their frame end
-    int pad;
+    int cg_iters;       // largest iteration count of the solves finished in this substep
 };
 
 // Device frame loop of one scene (MPM_Driver::Advance_To_Target_Time,

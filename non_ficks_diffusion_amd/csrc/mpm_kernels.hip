@@ -625,9 +625,11 @@ __device__ inline void store_operator_eos(const KP& p, int i, float c, const flo
 // re-deriving the stress state, and reads one contiguous record.
 // ============================================================================
 template <int D> struct OpRec;
-// 3D: X[3], q[4] (unit quaternion of U), the 3x3 block on the diagonal
-// entries (upper triangle, 6) and the three 2x2 pair blocks (3 each), pad 2.
-template <> struct OpRec<3> { static constexpr int W = 24; };  // 6 x float4
+// 3D: X[3], q[4] (unit quaternion of U), the singular values S[3] and the
+// four scalars A = 2 mu V0, B = lambda V0 (J - 1), P = B J, Q = lambda V0 J^2
+// from which the 3x3 block on the diagonal entries and the three 2x2 pair
+// blocks follow, pad 2 (64 B: four float4).
+template <> struct OpRec<3> { static constexpr int W = 16; };  // 4 x float4
 // 2D: X[2], the full symmetric 4x4 operator (upper triangle, 10).
 template <> struct OpRec<2> { static constexpr int L = 10, W = 12; };  // 3 x float4
 static_assert(OpRec<3>::W == op_record_floats(3) && OpRec<2>::W == op_record_floats(2), "operator record size");
@@ -723,13 +725,15 @@ __device__ inline void rotate_out(const float* U, const float* Ah, float* out) {
 // with T^ = U^T T U, dF^ = T^T S, the rotation term W^_ij = (dF^_ij -
 // dF^_ji) / (s_i + s_j) and the derivative of cof(S), A^ = U^T A U = V0 S dP^T
 // is a 3x3 block on the diagonal entries,
-//   V0 (2 mu s_a^2 [c == a] + lambda J^2 + lambda J (J - 1) [c != a]),
+//   V0 (2 mu s_a^2 [c == a] + lambda J^2 + lambda J (J - 1) [c != a])
+//   = diag(A s_a^2 - P) + (P + Q) 1 1^T,
 // plus per off-diagonal pair (lo, hi), with s = s_lo + s_hi and k the third
 // index, the symmetric block
-//   [[2 mu V0 s_lo^2 (s - 1) / s,  V0 s_lo s_hi (2 mu / s - lambda (J - 1) s_k)],
-//    [            ...,             2 mu V0 s_hi^2 (s - 1) / s]].
-// The entries are formed in FP64 from the SVD's singular values and stored
-// in FP32.
+//   [[A s_lo^2 (s - 1) / s,  s_lo s_hi (A / s - B s_k)],
+//    [         ...,          A s_hi^2 (s - 1) / s   ]],
+// where A = 2 mu V0, B = lambda V0 (J - 1), P = B J, Q = lambda V0 J^2.  The
+// record holds S and these four scalars (formed in FP64, stored in FP32);
+// apply_operator forms the block entries from them.
 template <int D>
 __device__ inline void store_operator(const KP& p, int i, const StressState<D>& st, float V0, const float* X,
                                       const float* U, const double* S) {
@@ -743,22 +747,14 @@ __device__ inline void store_operator(const KP& p, int i, const StressState<D>& 
         quat_of(U, rec + 3);
         const double v0 = (double)V0, mu = (double)st.mu, lam = (double)st.lambda;
         const double J = S[0] * S[1] * S[2];
-        const double same = lam * J * J, cross = lam * J * (J - 1.0) + same;
-        rec[7] = (float)(v0 * (2.0 * mu * S[0] * S[0] + same));
-        rec[8] = (float)(v0 * cross);
-        rec[9] = (float)(v0 * cross);
-        rec[10] = (float)(v0 * (2.0 * mu * S[1] * S[1] + same));
-        rec[11] = (float)(v0 * cross);
-        rec[12] = (float)(v0 * (2.0 * mu * S[2] * S[2] + same));
-        const int lo[3] = {0, 0, 1}, hi[3] = {1, 2, 2}, third[3] = {2, 1, 0};
-#pragma unroll
-        for (int pr = 0; pr < 3; ++pr) {
-            const double sl = S[lo[pr]], sh = S[hi[pr]], sm = sl + sh;
-            const double g = 2.0 * mu * v0 * (sm - 1.0) / sm;
-            rec[13 + 3 * pr] = (float)(g * sl * sl);
-            rec[14 + 3 * pr] = (float)(v0 * sl * sh * (2.0 * mu / sm - lam * (J - 1.0) * S[third[pr]]));
-            rec[15 + 3 * pr] = (float)(g * sh * sh);
-        }
+        const double B = lam * v0 * (J - 1.0);
+        rec[7] = (float)S[0];
+        rec[8] = (float)S[1];
+        rec[9] = (float)S[2];
+        rec[10] = (float)(2.0 * mu * v0);
+        rec[11] = (float)B;
+        rec[12] = (float)(B * J);
+        rec[13] = (float)(lam * v0 * J * J);
     } else {
         float L[DD][DD];
 #pragma unroll
@@ -780,29 +776,40 @@ __device__ inline void store_operator(const KP& p, int i, const StressState<D>& 
     for (int k = 0; k < OpRec<D>::W / 4; ++k) dst[k] = make_float4(rec[4 * k], rec[4 * k + 1], rec[4 * k + 2], rec[4 * k + 3]);
 }
 
+// The 2x2 block of pair (lo, hi) with third index k applied to
+// (Th_lo,hi, Th_hi,lo).
+__device__ inline void apply_pair(float A, float B, float slo, float shi, float sk, float tlh, float thl, float& alh,
+                                  float& ahl) {
+    const float sm = slo + shi;
+    const float inv = __builtin_amdgcn_rcpf(sm);
+    const float g = A * (sm - 1.0f) * inv;
+    const float a = g * slo * slo, c = g * shi * shi;
+    const float b = slo * shi * (A * inv - B * sk);
+    alh = a * tlh + b * thl;
+    ahl = b * tlh + c * thl;
+}
+
 // A = L T from a particle's operator record
 template <int D>
 __device__ inline void apply_operator(const float* rec, const float* T, float* A) {
     constexpr int DD = D * D;
     if constexpr (D == 3) {
-        const float4 r1 = *reinterpret_cast<const float4*>(rec + 4);   // q1 q2 q3 m00
         const float4 r0 = *reinterpret_cast<const float4*>(rec);       // X0 X1 X2 q0
-        const float4 r2 = *reinterpret_cast<const float4*>(rec + 8);   // m01 m02 m11 m12
-        const float4 r3 = *reinterpret_cast<const float4*>(rec + 12);  // m22 p0 p0 p0
-        const float4 r4 = *reinterpret_cast<const float4*>(rec + 16);  // p1 p1 p1 p2
-        const float4 r5 = *reinterpret_cast<const float4*>(rec + 20);  // p2 p2 - -
+        const float4 r1 = *reinterpret_cast<const float4*>(rec + 4);   // q1 q2 q3 s0
+        const float4 r2 = *reinterpret_cast<const float4*>(rec + 8);   // s1 s2 A B
+        const float4 r3 = *reinterpret_cast<const float4*>(rec + 12);  // P Q - -
         const float q[4] = {r0.w, r1.x, r1.y, r1.z};
+        const float s0 = r1.w, s1 = r2.x, s2 = r2.y, a2 = r2.z, bl = r2.w, pj = r3.x, kappa = r3.x + r3.y;
         float U[9], Th[9], Ah[9];
         rot_of(q, U);
         rotate_in(U, T, Th);
-        const float m00 = r1.w, m01 = r2.x, m02 = r2.y, m11 = r2.z, m12 = r2.w, m22 = r3.x;
-        Ah[0] = m00 * Th[0] + m01 * Th[4] + m02 * Th[8];
-        Ah[4] = m01 * Th[0] + m11 * Th[4] + m12 * Th[8];
-        Ah[8] = m02 * Th[0] + m12 * Th[4] + m22 * Th[8];
-        // pairs (0,1), (0,2), (1,2): (Ah_lo,hi, Ah_hi,lo) = [[a b][b c]] (Th_lo,hi, Th_hi,lo)
-        Ah[1] = r3.y * Th[1] + r3.z * Th[3];  Ah[3] = r3.z * Th[1] + r3.w * Th[3];
-        Ah[2] = r4.x * Th[2] + r4.y * Th[6];  Ah[6] = r4.y * Th[2] + r4.z * Th[6];
-        Ah[5] = r4.w * Th[5] + r5.x * Th[7];  Ah[7] = r5.x * Th[5] + r5.y * Th[7];
+        const float tr = Th[0] + Th[4] + Th[8];
+        Ah[0] = (a2 * s0 * s0 - pj) * Th[0] + kappa * tr;
+        Ah[4] = (a2 * s1 * s1 - pj) * Th[4] + kappa * tr;
+        Ah[8] = (a2 * s2 * s2 - pj) * Th[8] + kappa * tr;
+        apply_pair(a2, bl, s0, s1, s2, Th[1], Th[3], Ah[1], Ah[3]);
+        apply_pair(a2, bl, s0, s2, s1, Th[2], Th[6], Ah[2], Ah[6]);
+        apply_pair(a2, bl, s1, s2, s0, Th[5], Th[7], Ah[5], Ah[7]);
         rotate_out(U, Ah, A);
     } else {
         const float* tri = rec + D;
@@ -827,9 +834,11 @@ __device__ inline void store_operator_eos(const KP& p, int i, float c, const flo
 #pragma unroll
     for (int a = 0; a < D; ++a) rec[a] = X[a];
     if constexpr (D == 3) {
-        rec[3] = 1.0f;  // identity quaternion
-#pragma unroll
-        for (int k = 7; k <= 12; ++k) rec[k] = c;  // m00 m01 m02 m11 m12 m22
+        // U = I, S = (1, 1, 1), A = B = P = 0, Q = c: every entry of the
+        // diagonal block c, empty pair blocks
+        rec[3] = 1.0f;
+        rec[7] = rec[8] = rec[9] = 1.0f;
+        rec[13] = c;
     } else {
         rec[D + op_index<D>(0, 0)] = c;
         rec[D + op_index<D>(0, 3)] = c;
@@ -1149,18 +1158,17 @@ __device__ inline void scatter_force(const SceneParams& sc, const int* l, const 
     }
 }
 
-// Constitutive update (MPM_Example.cpp:563-571) and explicit force P2G
-// (MPM_Example.cpp:757-784): f_i += -V0 P F^T grad w + g m w, for particle i
-// of material mat; an EOS particle has no constitutive model and scatters
-// (V0/rho) K rho^(gamma-1) grad w (MPM_Example.cpp:774-782).
+// Constitutive update (MPM_Example.cpp:563-571) of particle i of material
+// mat (CONST: plasticity, F, Jp and the CG operator record) and its stress
+// term A of the explicit force (FORCE: f_i -= A grad w_i with A = V0 P F^T,
+// MPM_Example.cpp:757-784); an EOS particle has no constitutive model and its
+// term is A = -(V0/rho) K rho^(gamma-1) I (MPM_Example.cpp:774-782).
 // ============================================================================
 template <int D, bool CONST, bool FORCE>
-__device__ __forceinline__ void particle_force_one(const KP& p, const SceneParams& sc, const Material& mat, int i,
-                                                   const int* bb, const int* l, float* mine) {
+__device__ __forceinline__ void particle_stress(const KP& p, const Material& mat, int i, float* A) {
     constexpr int DD = D * D;
     float Jp = p.pcur[PF<D>::J * p.pcap + i];
     const float V0 = p.pcur[PF<D>::VOL * p.pcap + i];
-    float A[DD];  // the particle's stress term: f_i -= A grad w
     if (mat.eos) {
         // EOS fluid (Jp slot = density): no constitutive model
         const float rg = powf(Jp, mat.gamma - 1.0f);
@@ -1169,8 +1177,10 @@ __device__ __forceinline__ void particle_force_one(const KP& p, const SceneParam
 #pragma unroll
             for (int a = 0; a < D; ++a) Xo[a] = p.pcur[(PF<D>::X + a) * p.pcap + i];
             store_operator_eos<D>(p, i, V0 * mat.K * mat.gamma * rg, Xo);
+            if (!FORCE) {
 #pragma unroll
-            for (int k = 0; k < DD; ++k) p.rcache[k * p.pcap + i] = (k % (D + 1) == 0) ? 1.0f : 0.0f;
+                for (int k = 0; k < DD; ++k) p.rcache[k * p.pcap + i] = (k % (D + 1) == 0) ? 1.0f : 0.0f;
+            }
         }
         // f += (V0/rho) K rho^(gamma-1) grad w
         const float coeff = V0 / Jp * mat.K * rg;
@@ -1187,7 +1197,8 @@ __device__ __forceinline__ void particle_force_one(const KP& p, const SceneParam
 #pragma unroll
             for (int k = 0; k < DD; ++k) {
                 p.pcur[(PF<D>::F + k) * p.pcap + i] = F[k];
-                p.rcache[k * p.pcap + i] = R[k];
+                // R is read back only by the staged explicit force
+                if (!FORCE) p.rcache[k * p.pcap + i] = R[k];
             }
             p.pcur[PF<D>::J * p.pcap + i] = Jp;
             StressState<D> so;
@@ -1217,22 +1228,24 @@ __device__ __forceinline__ void particle_force_one(const KP& p, const SceneParam
                 }
         }
     }
-    if (FORCE) {
-        float w[D][3], dw[D][3], X[D];
-        if (!cell_particle<D>(p, sc, i, bb, l, w, dw, X)) return;
-        scatter_force<D>(sc, l, w, dw, A, p.pcur[PF<D>::M * p.pcap + i], mine);
-    }
 }
 
-// The particles of each block, cell by cell (lane = cell, waves interleave
-// a cell's particles), one pass per entry of the scene's material table, each
-// pass taking the particles of that entry (a single pass without a check in
-// a one-material scene): the material's parameters stay scalar operands.
+// One work-group per block, its particles in chunks of PF_CHUNK (a full
+// block of 8 per cell is one chunk).  Each chunk is first updated a particle
+// per thread in storage order (coalesced loads and stores; a wave past the
+// block's last particle skips its part), one pass per entry of the scene's
+// material table, each pass taking the particles of that entry (a single
+// pass without a check in a one-material scene): the material's parameters
+// stay scalar operands.  With FORCE the chunk's stress terms go to LDS and
+// are then scattered cell by cell (lane = cell, waves interleave a cell's
+// particles) into the wave tiles.
+constexpr int PF_CHUNK = 2 * BWG;
 template <int D, bool CONST, bool FORCE>
 __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
     using Dm = Dims<D>;
-    constexpr int LTN = Dm::LTN;
+    constexpr int LTN = Dm::LTN, DD = D * D;
     __shared__ float tile[NW * D * LTN];
+    __shared__ float stress[DD][PF_CHUNK];
     const int nblocks = p.ctr->nblocks;
     const int c = threadIdx.x & 63, wv = threadIdx.x >> 6;
     float* mine = tile + wv * D * LTN;
@@ -1244,21 +1257,41 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
         if (!sc.active || block_empty(p, b)) continue;
         int bb[D];
         slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
-        if (FORCE) {
+        if (FORCE)
             for (int t = threadIdx.x; t < NW * D * LTN; t += BWG) tile[t] = 0.0f;
-            __syncthreads();
-        }
-        const int i0 = p.cell_start[b * 65 + c], i1 = p.cell_start[b * 65 + c + 1];
+        const int p0 = p.block_pstart[b], p1 = p.block_pstart[b + 1];
+        const int c0 = p.cell_start[b * 65 + c], c1 = p.cell_start[b * 65 + c + 1];
         const int nmat = sc.nmat;
-        for (int u = 0; u < nmat; ++u) {
-            const Material& mat = sc.mats[u];
-            for (int i = i0 + wv; i < i1; i += NW) {
-                if (nmat > 1 && __float_as_int(p.pcur[PF<D>::MAT * p.pcap + i]) != u) continue;
-                particle_force_one<D, CONST, FORCE>(p, sc, mat, i, bb, l, mine);
+        for (int chunk = p0; chunk < p1; chunk += PF_CHUNK) {
+            for (int half = 0; half < PF_CHUNK; half += BWG) {
+                const int t = half + (int)threadIdx.x, i = chunk + t;
+                if (chunk + half + (int)(threadIdx.x & ~63u) >= p1) continue;  // the wave's part is past the block
+                for (int u = 0; u < nmat; ++u) {
+                    if (i < p1 && (nmat == 1 || __float_as_int(p.pcur[PF<D>::MAT * p.pcap + i]) == u)) {
+                        float A[DD];
+                        particle_stress<D, CONST, FORCE>(p, sc.mats[u], i, A);
+                        if (FORCE) {
+#pragma unroll
+                            for (int k = 0; k < DD; ++k) stress[k][t] = A[k];
+                        }
+                    }
+                }
+            }
+            if (FORCE) {
+                __syncthreads();  // the chunk's stress terms (and, first, the zeroed tiles)
+                const int lo = max(c0, chunk), hi = min(c1, chunk + PF_CHUNK);
+                for (int j = lo + wv; j < hi; j += NW) {
+                    float w[D][3], dw[D][3], X[D];
+                    if (!cell_particle<D>(p, sc, j, bb, l, w, dw, X)) continue;
+                    float A[DD];
+#pragma unroll
+                    for (int k = 0; k < DD; ++k) A[k] = stress[k][j - chunk];
+                    scatter_force<D>(sc, l, w, dw, A, p.pcur[PF<D>::M * p.pcap + j], mine);
+                }
+                __syncthreads();  // before the next chunk overwrites the stress terms
             }
         }
         if (FORCE) {
-            __syncthreads();
             store_wave_tiles<D, D>(p, b, tile);
             __syncthreads();
         }
@@ -1286,7 +1319,10 @@ __global__ __launch_bounds__(WG) void k_grid_update(KP p) {
             cg.rho_hist[0] = cg.rho_hist[1] = 0.0;
             cg.alpha = cg.beta = cg.conv = cg.rr = cg.sq = 0.0;
         }
-        if (blockIdx.x == 0 && threadIdx.x == 0) p.ctr->cg_ndone = 0;
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            p.ctr->cg_ndone = 0;
+            p.ctr->cg_iters = 0;
+        }
     }
     const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
     const int lane = threadIdx.x & 63;
@@ -1730,6 +1766,7 @@ __device__ inline bool cg_test(const KP& p, int si, int k, double rr, double mx,
         if (done) {
             cg.converged = (pass || empty) ? 1 : 0;
             cg.done = 1;
+            atomicMax(&p.ctr->cg_iters, k);
             atomicAdd(&p.ctr->cg_ndone, 1);
         } else {
             cg.beta = beta;
