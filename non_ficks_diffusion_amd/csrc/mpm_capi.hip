@@ -140,6 +140,7 @@ mpm::KP make_kp(const mpm_context* c) {
     p.block_pstart = c->block_pstart;
     p.block_nbr = c->block_nbr;
     p.block_tnbr = c->block_tnbr;
+    p.block_org = c->block_org;
     p.cell_count = c->cell_count;
     p.cell_start = c->cell_start;
     p.bcap = c->bcap;
@@ -574,6 +575,7 @@ void layout_buffers(mpm_context* c, Take&& take) {
     take(&c->block_pstart, bcap + 1);
     take(&c->block_nbr, bcap * (D == 3 ? 27 : 9));
     take(&c->block_tnbr, bcap * (D == 3 ? 27 : 9));
+    take(&c->block_org, bcap * 3);
     take(&c->cell_count, nslots * 64);
     take(&c->cell_start, bcap * 65 + 1);
     take(&c->tiles, bcap * (D + 1) * (D == 3 ? 216 : 100));
@@ -1354,7 +1356,7 @@ UnitBytes unit_bytes(int D) {
     u.slot = 4 * 4 + 64 * 4;
     // block tables, neighbours, cell ranges, tiles, 64 nodes of grid channels, partials
     const int64_t nbr = D == 3 ? 27 : 9, tile = D == 3 ? 216 : 100;
-    u.block = 3 * 4 + 2 * nbr * 4 + 65 * 4 + (D + 1) * tile * 4 + 64 * (1 + 5 * D + 1) * 4 + 3 * 8 + 2 * 4;
+    u.block = 3 * 4 + 2 * nbr * 4 + 3 * 4 + 65 * 4 + (D + 1) * tile * 4 + 64 * (1 + 5 * D + 1) * 4 + 3 * 8 + 2 * 4;
     return u;
 }
 

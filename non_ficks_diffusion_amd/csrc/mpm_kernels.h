@@ -112,6 +112,7 @@ struct KP {
     int* block_pstart;
     int* block_nbr;    // per active block, the 3^D neighbours' active indices (-1: none)
     int* block_tnbr;   // the same, -1 also for neighbours without particles (no tile)
+    int* block_org;    // per active block, its D coordinates in its scene's slot grid (3 ints)
     int* cell_count;   // per slot, 64 cell counters (zero between substeps)
     int* cell_start;   // per active block, 65 particle offsets (cell ranges)
     int bcap;

@@ -75,6 +75,14 @@ __device__ inline void slot_coords(int slot, int nb, int* b) {
     }
 }
 
+// Coordinates of active block b in its scene's slot grid (written by
+// k_block_setup; a load instead of the slot index's divisions).
+template <int D>
+__device__ inline void block_coords(const KP& p, int b, int* bb) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) bb[a] = p.block_org[b * 3 + a];
+}
+
 template <int D>
 __device__ inline void local_coords(int l, int* lc) {
     if constexpr (D == 3) { lc[0] = l >> 4; lc[1] = (l >> 2) & 3; lc[2] = l & 3; }
@@ -187,7 +195,7 @@ __device__ inline void gather_node_tiles(const KP& p, const SceneParams& sc, int
     for (int c = 0; c < C; ++c) out[c] = 0.0f;
     if (sc.crop_ghost_x) {
         int bb[D];
-        slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
+        block_coords<D>(p, b, bb);
         const int ix = B * bb[0] + lc[0] - B;
         if (ix < 0 || ix > sc.counts - 1) return;
     }
@@ -236,7 +244,7 @@ template <int D>
 __device__ inline void node_center(const KP& p, const SceneParams& sc, int n, float* x) {
     constexpr int B = Dims<D>::B;
     int bb[D], lc[D];
-    slot_coords<D>(p.block_slot[n >> 6] - sc.slot_base, sc.nb, bb);
+    block_coords<D>(p, n >> 6, bb);
     local_coords<D>(n & 63, lc);
 #pragma unroll
     for (int a = 0; a < D; ++a) {
@@ -482,6 +490,7 @@ __global__ __launch_bounds__(WG) void k_block_setup(KP p) {
         if (lane < Dm::NBR) {
             int bb[D];
             slot_coords<D>(slot - sc.slot_base, sc.nb, bb);
+            if (lane < D) p.block_org[b * 3 + lane] = bb[lane];
             int o[3];
             if constexpr (D == 3) { o[0] = lane / 9; o[1] = (lane / 3) % 3; o[2] = lane % 3; }
             else { o[0] = lane / 3; o[1] = lane % 3; }
@@ -572,6 +581,56 @@ constexpr int BWG = 64 * NW;
 // so no LDS access may move across it (a wave's LDS operations execute in
 // issue order).
 #define MPM_STEP_FENCE() __asm__ __volatile__("" ::: "memory")
+
+// The tile nodes a thread gathers and stores in a D-channel block kernel
+// depend on its thread index only, so the kernel forms them once:
+//   g: gather of tile node threadIdx.x: neighbour | local node << 5 | LDS
+//      index << 11 (~0u for threads past the tile);
+//   s: store of tile values t = threadIdx.x + k BWG (k < 3): the
+//      channel-major LDS index of each, 10 bits apiece.
+template <int D>
+struct TileMaps {
+    unsigned g, s;
+    __device__ void init() {
+        constexpr int TN = Dims<D>::TN, LTN = Dims<D>::LTN;
+        static_assert(D * TN <= 3 * BWG && D * LTN <= 1024, "three 10-bit store slots");
+        g = ~0u;
+        if ((int)threadIdx.x < TN) {
+            int nb, local, tc[D];
+            tile_map<D>(threadIdx.x, nb, local, tc);
+            g = (unsigned)nb | ((unsigned)local << 5) | ((unsigned)lds_index<D>(tc) << 11);
+        }
+        s = 0u;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int t = threadIdx.x + k * BWG;
+            if (t < D * TN) {
+                const int c = t / TN;
+                int nb, local, tc[D];
+                tile_map<D>(t - c * TN, nb, local, tc);
+                s |= (unsigned)(c * LTN + lds_index<D>(tc)) << (10 * k);
+            }
+        }
+    }
+};
+
+// store_wave_tiles<D, D> with the thread's maps
+template <int D>
+__device__ inline void store_wave_tiles_mapped(const KP& p, int b, const float* lds, unsigned smap) {
+    constexpr int TN = Dims<D>::TN, LTN = Dims<D>::LTN;
+    float* dst = p.tiles + (int64_t)b * D * TN;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int t = threadIdx.x + k * BWG;
+        if (t < D * TN) {
+            const int li = (int)((smap >> (10 * k)) & 1023u);
+            float v = 0.0f;
+#pragma unroll
+            for (int w = 0; w < NW; ++w) v += lds[w * D * LTN + li];
+            dst[t] = v;
+        }
+    }
+}
 
 // Sum of the NW wave tiles (C channels each, LDS layout) of block b into
 // its tile slot (compact layout, node (x,y,z) at (x*TE + y)*TE + z).
@@ -994,7 +1053,7 @@ __global__ __launch_bounds__(BWG) void k_p2g(KP p) {
         if (!sc.active || block_empty(p, b)) continue;
         for (int t = c; t < C * LTN; t += 64) mine[t] = 0.0f;
         int bb[D];
-        slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
+        block_coords<D>(p, b, bb);
         float acc[NS][C];
 #pragma unroll
         for (int n = 0; n < NS; ++n)
@@ -1104,7 +1163,7 @@ __global__ __launch_bounds__(BWG) void k_init_volume(KP p) {
         const float inv_vol = 1.0f / vol;
         if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
-        slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
+        block_coords<D>(p, b, bb);
         __syncthreads();
         const float* src[1] = {p.g_m};
         gather_tile<D, 1>(p, nbr, tile, src);
@@ -1244,7 +1303,7 @@ template <int D, bool CONST, bool FORCE>
 __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
     using Dm = Dims<D>;
     constexpr int LTN = Dm::LTN, DD = D * D;
-    __shared__ float tile[NW * D * LTN];
+    __shared__ __align__(16) float tile[NW * D * LTN];
     __shared__ float stress[DD][PF_CHUNK];
     const int nblocks = p.ctr->nblocks;
     const int c = threadIdx.x & 63, wv = threadIdx.x >> 6;
@@ -1256,7 +1315,7 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
         const SceneParams& sc = p.scn[p.block_scene[b]];
         if (!sc.active || block_empty(p, b)) continue;
         int bb[D];
-        slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
+        slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);  // computed: the loaded form costs occupancy here
         if (FORCE)
             for (int t = threadIdx.x; t < NW * D * LTN; t += BWG) tile[t] = 0.0f;
         const int p0 = p.block_pstart[b], p1 = p.block_pstart[b + 1];
@@ -1424,12 +1483,14 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p, int k) {
     using Dm = Dims<D>;
     constexpr int LTN = Dm::LTN, NBR = Dm::NBR, DD = D * D;
     __shared__ float ut[D][LTN];
-    __shared__ float qt[NW * D * LTN];
+    __shared__ __align__(16) float qt[NW * D * LTN];
     __shared__ int nbr[NBR];
     if (cg_all_done(p)) return;
     const int nblocks = p.ctr->nblocks;
     const int c = threadIdx.x & 63, wv = threadIdx.x >> 6;
     float* mine = qt + wv * D * LTN;
+    TileMaps<D> tm;
+    tm.init();
     int l[D];
     local_coords<D>(c, l);
     int base;  // tile index of this cell's stencil corner
@@ -1447,15 +1508,15 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p, int k) {
         const float beta = (RESTART || ONE) ? 0.0f : (float)p.cg[si].beta;
         if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
-        slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
-        for (int t = threadIdx.x; t < NW * D * LTN; t += BWG) qt[t] = 0.0f;
+        block_coords<D>(p, b, bb);
+        static_assert((NW * D * LTN) % 4 == 0, "float4 zeroing");
+        for (int t = threadIdx.x; t < NW * D * LTN / 4; t += BWG)
+            reinterpret_cast<float4*>(qt)[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         __syncthreads();
-        for (int t = threadIdx.x; t < Dm::TN; t += BWG) {
-            int nb, local, tc[D];
-            tile_map<D>(t, nb, local, tc);
-            const int blk = nbr[nb];
-            const int li = lds_index<D>(tc);
-            const int64_t n = (int64_t)blk * 64 + local;
+        if (tm.g != ~0u) {
+            const int blk = nbr[tm.g & 31u];
+            const int li = (int)(tm.g >> 11);
+            const int64_t n = (int64_t)blk * 64 + (int)((tm.g >> 5) & 63u);
 #pragma unroll
             for (int a = 0; a < D; ++a) {
                 float u = 0.0f;
@@ -1490,7 +1551,7 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p, int k) {
             grad_scatter<D>(w, dw, base, A, mine);
         }
         __syncthreads();
-        store_wave_tiles<D, D>(p, b, qt);
+        store_wave_tiles_mapped<D>(p, b, qt, tm.s);
         __syncthreads();
     }
 }
@@ -1609,7 +1670,7 @@ __global__ __launch_bounds__(WG) void k_cg_force_cell(KP p, int k) {
         const float beta = (RESTART || ONE) ? 0.0f : (float)p.cg[si].beta;
         if (c < NBR) nbr[c] = p.block_nbr[b * NBR + c];
         int bb[D];
-        slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
+        block_coords<D>(p, b, bb);
         for (int t = c; t < D * LTN; t += 64) qt[t] = 0.0f;
         MPM_STEP_FENCE();
         for (int t = c; t < TN; t += 64) {
@@ -2019,7 +2080,7 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
         int removed = 0;
         if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
         int bb[D];
-        slot_coords<D>(p.block_slot[b] - sc.slot_base, sc.nb, bb);
+        block_coords<D>(p, b, bb);
         // the first particle's position and velocity load while the tile is gathered
         const int i0 = p.block_pstart[b] + (int)threadIdx.x, i1 = p.block_pstart[b + 1];
         float nX[D], nV[D];
