@@ -3,8 +3,8 @@
  *
  * TEST INFRASTRUCTURE ONLY.  This is a C++/OpenMP restatement of the
  * reference's algorithm (SoldierDown/Non_Ficks_Diffusion, which cannot be
- * built here: its Nova dependency is not vendored).  It is the parity
- * checker for the GPU path and the CPU baseline that bench.py times; the
+ * built here: its Nova dependency is not vendored).  The GPU path's parity
+ * tests compare against it and bench.py times it as the CPU baseline; the
  * product never links, loads or calls it.
  *
  * Every stage follows the reference stage for stage: AoS particles,

@@ -1,4 +1,4 @@
-"""GPU parity of the driver rule, the Project reading and the workload sizes
+"""GPU parity of MPM_Driver's dt rule, the Project reading and the workload sizes
 against the CPU oracle, through the C ABI.
 
 * The dt rule and max speed (MPM_Driver.cpp:55-80, MPM_Example.cpp:375-387)

@@ -1,6 +1,6 @@
 """GPU parity: the HIP path through the C ABI against the CPU oracle.
 
-Bars (north-star tolerances): block binning and node flags bit-exact;
+Bars: block binning and node flags bit-exact;
 FP32 state within 1e-5 relative L2 after one substep and 1e-3 after 100;
 P2G conserves grid mass and momentum to 1e-6 relative.  The oracle runs in
 the same precision mode as the GPU path (FP32 state, FP64 CG reductions and

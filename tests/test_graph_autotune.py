@@ -244,7 +244,7 @@ def test_stage_timing_leaves_the_state_as_it_was():
             after = s.download()
             for f in ("ids", "x", "v", "F", "Jp", "volume"):
                 assert np.array_equal(after[f], before[f]), f
-        # what the driver reads next: the max speed, the stats and a frame
+        # what MPM_Driver reads next: the max speed, the stats and a frame
         # advanced with the dt rule from that max speed
         vmax = s.max_particle_velocity()
         st = s.stats()
