@@ -141,6 +141,7 @@ mpm::KP make_kp(const mpm_context* c) {
     p.block_nbr = c->block_nbr;
     p.block_tnbr = c->block_tnbr;
     p.block_org = c->block_org;
+    p.block_order = c->block_order;
     p.cell_count = c->cell_count;
     p.cell_start = c->cell_start;
     p.bcap = c->bcap;
@@ -235,6 +236,7 @@ void stage_bin(mpm_context* c) {
     timed(c, KC_SETUP, [&] {
         mpm::launch_block_setup<D>(p, c->stream, c->grid_block);
         mpm::launch_scene_ranges(p, c->stream);
+        mpm::launch_block_order(p, c->stream);
     });
     timed(c, KC_REORDER, [&] { mpm::launch_reorder<D>(p, c->stream, c->grid_particle); });
     c->cur ^= 1;
@@ -576,6 +578,7 @@ void layout_buffers(mpm_context* c, Take&& take) {
     take(&c->block_nbr, bcap * (D == 3 ? 27 : 9));
     take(&c->block_tnbr, bcap * (D == 3 ? 27 : 9));
     take(&c->block_org, bcap * 3);
+    take(&c->block_order, bcap);
     take(&c->cell_count, nslots * 64);
     take(&c->cell_start, bcap * 65 + 1);
     take(&c->tiles, bcap * (D + 1) * (D == 3 ? 216 : 100));
@@ -1356,7 +1359,7 @@ UnitBytes unit_bytes(int D) {
     u.slot = 4 * 4 + 64 * 4;
     // block tables, neighbours, cell ranges, tiles, 64 nodes of grid channels, partials
     const int64_t nbr = D == 3 ? 27 : 9, tile = D == 3 ? 216 : 100;
-    u.block = 3 * 4 + 2 * nbr * 4 + 3 * 4 + 65 * 4 + (D + 1) * tile * 4 + 64 * (1 + 5 * D + 1) * 4 + 3 * 8 + 2 * 4;
+    u.block = 3 * 4 + 2 * nbr * 4 + 4 * 4 + 65 * 4 + (D + 1) * tile * 4 + 64 * (1 + 5 * D + 1) * 4 + 3 * 8 + 2 * 4;
     return u;
 }
 

@@ -113,6 +113,7 @@ struct KP {
     int* block_nbr;    // per active block, the 3^D neighbours' active indices (-1: none)
     int* block_tnbr;   // the same, -1 also for neighbours without particles (no tile)
     int* block_org;    // per active block, its D coordinates in its scene's slot grid (3 ints)
+    int* block_order;  // each XCD's eighth of the active blocks, by descending particle count
     int* cell_count;   // per slot, 64 cell counters (zero between substeps)
     int* cell_start;   // per active block, 65 particle offsets (cell ranges)
     int bcap;
@@ -181,6 +182,8 @@ void launch_scan(const KP& p, hipStream_t s);
 void launch_copy(const float4* src, float4* dst, int64_t n4, hipStream_t s);
 template <int D> void launch_block_setup(const KP& p, hipStream_t s, int grid);
 void launch_scene_ranges(const KP& p, hipStream_t s);
+// block_order: every XCD range of the active blocks sorted by descending particle count
+void launch_block_order(const KP& p, hipStream_t s);
 template <int D> void launch_reorder(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_p2g(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_normalize(const KP& p, hipStream_t s, int grid);

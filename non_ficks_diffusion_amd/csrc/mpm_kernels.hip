@@ -545,6 +545,33 @@ __global__ __launch_bounds__(WG) void k_scene_ranges(KP p) {
     st.pend = nb > 0 ? p.block_pstart[b1] : 0;
 }
 
+// One work-group per XCD range of the active blocks (the ranges of
+// xcd_blocks): the range's blocks by descending particle count, bucketed by
+// eight particles (a counting sort; the order within a bucket is that of
+// the LDS atomics).  Block loops that take the hardware's dispatch order
+// then start the heavy blocks first and leave light ones for the tail.  Only
+// which work-group computes a block changes, never a block's result.
+__global__ __launch_bounds__(WG) void k_block_order(KP p) {
+    constexpr int NBK = WG;  // buckets, one per thread for the scan
+    __shared__ int hist[NBK];
+    const int n = p.ctr->nblocks, x = (int)blockIdx.x;
+    const int lo = (int)(((int64_t)n * x) >> 3), hi = (int)(((int64_t)n * (x + 1)) >> 3);
+    auto key = [&](int b) {  // descending count: the heaviest bucket first
+        const int cnt = p.block_pstart[b + 1] - p.block_pstart[b];
+        return NBK - 1 - min(NBK - 1, cnt >> 3);
+    };
+    hist[threadIdx.x] = 0;
+    __syncthreads();
+    for (int b = lo + (int)threadIdx.x; b < hi; b += WG) atomicAdd(&hist[key(b)], 1);
+    __syncthreads();
+    int total;
+    const int off = wg_exclusive_scan(hist[threadIdx.x], total);
+    __syncthreads();
+    hist[threadIdx.x] = off;
+    __syncthreads();
+    for (int b = lo + (int)threadIdx.x; b < hi; b += WG) p.block_order[lo + atomicAdd(&hist[key(b)], 1)] = b;
+}
+
 // scatter into (block, cell) order
 template <int D>
 __global__ __launch_bounds__(WG) void k_reorder(KP p) {
@@ -1048,7 +1075,8 @@ __global__ __launch_bounds__(BWG) void k_p2g(KP p) {
         base = tile_index<D>(l, 0, off0);
     }
     const XcdRange xr = xcd_wave_blocks(p.ctr->nblocks);
-    for (int b = xr.first; b < xr.end; b += xr.step) {
+    for (int kb = xr.first; kb < xr.end; kb += xr.step) {
+        const int b = p.block_order[kb];  // heavy blocks first within the XCD's range
         const SceneParams& sc = p.scn[p.block_scene[b]];
         if (!sc.active || block_empty(p, b)) continue;
         for (int t = c; t < C * LTN; t += 64) mine[t] = 0.0f;
@@ -1155,7 +1183,8 @@ __global__ __launch_bounds__(BWG) void k_init_volume(KP p) {
     __shared__ int nbr[NBR];
     const int nblocks = p.ctr->nblocks;
     const XcdRange xr = xcd_blocks(nblocks);
-    for (int b = xr.first; b < xr.end; b += xr.step) {
+    for (int kb = xr.first; kb < xr.end; kb += xr.step) {
+        const int b = p.block_order[kb];  // heavy blocks first within the XCD's range
         const SceneParams& sc = p.scn[p.block_scene[b]];
         if (!sc.waiting || block_empty(p, b)) continue;
         float vol = 1.0f;
@@ -1311,7 +1340,8 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
     int l[D];
     local_coords<D>(c, l);
     const XcdRange xr = xcd_blocks(nblocks);
-    for (int b = xr.first; b < xr.end; b += xr.step) {
+    for (int kb = xr.first; kb < xr.end; kb += xr.step) {
+        const int b = p.block_order[kb];  // heavy blocks first within the XCD's range
         const SceneParams& sc = p.scn[p.block_scene[b]];
         if (!sc.active || block_empty(p, b)) continue;
         int bb[D];
@@ -1499,7 +1529,8 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p, int k) {
         base = tile_index<D>(l, 0, off0);
     }
     const XcdRange xr = xcd_blocks(nblocks);
-    for (int b = xr.first; b < xr.end; b += xr.step) {
+    for (int kb = xr.first; kb < xr.end; kb += xr.step) {
+        const int b = p.block_order[kb];  // heavy blocks first within the XCD's range
         // one scene: no per-block lookups (its solve is live: cg_all_done above)
         if (ONE && other_scene(p, b)) continue;
         const int si = ONE ? p.solo : p.block_scene[b];
@@ -2064,7 +2095,8 @@ __global__ __launch_bounds__(BWG) void k_g2p(KP p) {
     __shared__ int wrem[NW];
     const int nblocks = p.ctr->nblocks;
     const XcdRange xr = xcd_blocks(nblocks);
-    for (int b = xr.first; b < xr.end; b += xr.step) {
+    for (int kb = xr.first; kb < xr.end; kb += xr.step) {
+        const int b = p.block_order[kb];  // heavy blocks first within the XCD's range
         const SceneParams& sc = p.scn[p.block_scene[b]];
         if (!sc.active) continue;
         if (block_empty(p, b)) {  // no particles: empty partials for the scene's reduction
@@ -2502,6 +2534,7 @@ template <int D> void launch_block_setup(const KP& p, hipStream_t s, int grid) {
 void launch_scene_ranges(const KP& p, hipStream_t s) {
     hipLaunchKernelGGL(k_scene_ranges, dim3((p.nscenes + WG - 1) / WG), dim3(WG), 0, s, p);
 }
+void launch_block_order(const KP& p, hipStream_t s) { hipLaunchKernelGGL(k_block_order, dim3(8), dim3(WG), 0, s, p); }
 template <int D> void launch_reorder(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_reorder<D>, dim3(grid), dim3(WG), 0, s, p); }
 template <int D> void launch_p2g(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_p2g<D>, dim3(grid), dim3(BWG), 0, s, p); }
 template <int D> void launch_normalize(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_normalize<D>, dim3(grid), dim3(WG), 0, s, p); }
