@@ -142,6 +142,7 @@ mpm::KP make_kp(const mpm_context* c) {
     p.block_tnbr = c->block_tnbr;
     p.block_org = c->block_org;
     p.block_order = c->block_order;
+    p.block_full = c->block_full;
     p.cell_count = c->cell_count;
     p.cell_start = c->cell_start;
     p.bcap = c->bcap;
@@ -578,7 +579,8 @@ void layout_buffers(mpm_context* c, Take&& take) {
     take(&c->block_nbr, bcap * (D == 3 ? 27 : 9));
     take(&c->block_tnbr, bcap * (D == 3 ? 27 : 9));
     take(&c->block_org, bcap * 3);
-    take(&c->block_order, bcap);
+    take(&c->block_order, 2 * bcap);
+    take(&c->block_full, bcap);
     take(&c->cell_count, nslots * 64);
     take(&c->cell_start, bcap * 65 + 1);
     take(&c->tiles, bcap * (D + 1) * (D == 3 ? 216 : 100));
@@ -1359,7 +1361,7 @@ UnitBytes unit_bytes(int D) {
     u.slot = 4 * 4 + 64 * 4;
     // block tables, neighbours, cell ranges, tiles, 64 nodes of grid channels, partials
     const int64_t nbr = D == 3 ? 27 : 9, tile = D == 3 ? 216 : 100;
-    u.block = 3 * 4 + 2 * nbr * 4 + 4 * 4 + 65 * 4 + (D + 1) * tile * 4 + 64 * (1 + 5 * D + 1) * 4 + 3 * 8 + 2 * 4;
+    u.block = 3 * 4 + 2 * nbr * 4 + 6 * 4 + 65 * 4 + (D + 1) * tile * 4 + 64 * (1 + 5 * D + 1) * 4 + 3 * 8 + 2 * 4;
     return u;
 }
 

@@ -47,7 +47,9 @@ struct mpm_context {
     int *block_slot = nullptr, *block_scene = nullptr, *block_pstart = nullptr, *block_nbr = nullptr;
     int* block_tnbr = nullptr;
     int* block_org = nullptr;
-    int* block_order = nullptr;  // per XCD range, the active blocks by descending particle count
+    int* block_order = nullptr;  // per XCD range: the active blocks by descending particle count, then the
+                                 // same by descending fullest cell (2 bcap ints)
+    int* block_full = nullptr;   // per active block, its fullest cell's particle count
     int *scan_part = nullptr, *cell_count = nullptr, *cell_start = nullptr;
     float *g_m = nullptr, *g_v = nullptr, *g_vs = nullptr, *g_q = nullptr, *g_s = nullptr, *g_r = nullptr;
     unsigned* g_flags = nullptr;

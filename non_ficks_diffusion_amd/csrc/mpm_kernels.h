@@ -113,7 +113,9 @@ struct KP {
     int* block_nbr;    // per active block, the 3^D neighbours' active indices (-1: none)
     int* block_tnbr;   // the same, -1 also for neighbours without particles (no tile)
     int* block_org;    // per active block, its D coordinates in its scene's slot grid (3 ints)
-    int* block_order;  // each XCD's eighth of the active blocks, by descending particle count
+    int* block_order;  // each XCD's eighth of the active blocks by descending particle count ([0, bcap))
+                       // and by descending fullest cell ([bcap, 2 bcap), for the CG operator)
+    int* block_full;   // per active block, the particle count of its fullest cell
     int* cell_count;   // per slot, 64 cell counters (zero between substeps)
     int* cell_start;   // per active block, 65 particle offsets (cell ranges)
     int bcap;

@@ -517,6 +517,10 @@ __global__ __launch_bounds__(WG) void k_block_setup(KP p) {
         const int start = p.block_pstart[b] + x - cnt;
         p.cell_start[b * 65 + lane] = start;
         if (lane == 63) p.cell_start[b * 65 + 64] = start + cnt;
+        int full = cnt;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) full = max(full, __shfl_xor(full, off));
+        if (lane == 0) p.block_full[b] = full;
         p.cell_count[slot * 64 + lane] = 0;
     }
 }
@@ -546,19 +550,25 @@ __global__ __launch_bounds__(WG) void k_scene_ranges(KP p) {
 }
 
 // One work-group per XCD range of the active blocks (the ranges of
-// xcd_blocks): the range's blocks by descending particle count, bucketed by
-// eight particles (a counting sort; the order within a bucket is that of
-// the LDS atomics).  Block loops that take the hardware's dispatch order
-// then start the heavy blocks first and leave light ones for the tail.  Only
-// which work-group computes a block changes, never a block's result.
+// xcd_blocks) and per key: the range's blocks by descending particle count
+// (bucketed by eight; work-groups 0-7) and by descending count of their
+// fullest cell (work-groups 8-15), each a counting sort whose order within
+// a bucket is that of the LDS atomics.  The CG operator's time per block
+// follows its fullest cell (its lane-per-cell passes are as many as that
+// cell's particles); P2G, the constitutive/force kernel and G2P measured
+// fastest in particle-count order.  Block loops that follow the hardware's
+// dispatch order then start the long blocks first and leave short ones for
+// the tail.  Only which work-group computes a block changes, never a
+// block's result.
 __global__ __launch_bounds__(WG) void k_block_order(KP p) {
     constexpr int NBK = WG;  // buckets, one per thread for the scan
     __shared__ int hist[NBK];
-    const int n = p.ctr->nblocks, x = (int)blockIdx.x;
+    const int n = p.ctr->nblocks, x = (int)(blockIdx.x & 7), by_cell = (int)(blockIdx.x >> 3);
     const int lo = (int)(((int64_t)n * x) >> 3), hi = (int)(((int64_t)n * (x + 1)) >> 3);
-    auto key = [&](int b) {  // descending count: the heaviest bucket first
-        const int cnt = p.block_pstart[b + 1] - p.block_pstart[b];
-        return NBK - 1 - min(NBK - 1, cnt >> 3);
+    int* order = p.block_order + (by_cell ? p.bcap : 0);
+    auto key = [&](int b) {  // the longest first
+        const int w = by_cell ? p.block_full[b] : (p.block_pstart[b + 1] - p.block_pstart[b]) >> 3;
+        return NBK - 1 - min(NBK - 1, w);
     };
     hist[threadIdx.x] = 0;
     __syncthreads();
@@ -569,7 +579,7 @@ __global__ __launch_bounds__(WG) void k_block_order(KP p) {
     __syncthreads();
     hist[threadIdx.x] = off;
     __syncthreads();
-    for (int b = lo + (int)threadIdx.x; b < hi; b += WG) p.block_order[lo + atomicAdd(&hist[key(b)], 1)] = b;
+    for (int b = lo + (int)threadIdx.x; b < hi; b += WG) order[lo + atomicAdd(&hist[key(b)], 1)] = b;
 }
 
 // scatter into (block, cell) order
@@ -1530,7 +1540,7 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p, int k) {
     }
     const XcdRange xr = xcd_blocks(nblocks);
     for (int kb = xr.first; kb < xr.end; kb += xr.step) {
-        const int b = p.block_order[kb];  // heavy blocks first within the XCD's range
+        const int b = p.block_order[p.bcap + kb];  // the fullest cells first within the XCD's range
         // one scene: no per-block lookups (its solve is live: cg_all_done above)
         if (ONE && other_scene(p, b)) continue;
         const int si = ONE ? p.solo : p.block_scene[b];
@@ -2534,7 +2544,7 @@ template <int D> void launch_block_setup(const KP& p, hipStream_t s, int grid) {
 void launch_scene_ranges(const KP& p, hipStream_t s) {
     hipLaunchKernelGGL(k_scene_ranges, dim3((p.nscenes + WG - 1) / WG), dim3(WG), 0, s, p);
 }
-void launch_block_order(const KP& p, hipStream_t s) { hipLaunchKernelGGL(k_block_order, dim3(8), dim3(WG), 0, s, p); }
+void launch_block_order(const KP& p, hipStream_t s) { hipLaunchKernelGGL(k_block_order, dim3(16), dim3(WG), 0, s, p); }
 template <int D> void launch_reorder(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_reorder<D>, dim3(grid), dim3(WG), 0, s, p); }
 template <int D> void launch_p2g(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_p2g<D>, dim3(grid), dim3(BWG), 0, s, p); }
 template <int D> void launch_normalize(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_normalize<D>, dim3(grid), dim3(WG), 0, s, p); }
