@@ -1940,14 +1940,31 @@ __global__ __launch_bounds__(WG) void k_cg_restart_node(KP p) {
 // s = r (k = 0) or r + beta s on the interior nodes.
 template <int D>
 __global__ __launch_bounds__(WG) void k_cg_update_s(KP p, int k, int nparts) {
+    const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
+    const int lane = threadIdx.x & 63;
+    // the wave's first block: r and s load while the partials are reduced
+    float r0[D], s0[D];
+    bool first = false;
+    if (r.first < r.end) {
+        const int n = r.first * 64 + lane;
+        first = !other_scene(p, r.first) && (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            r0[a] = p.g_r[a * p.gcap + n];
+            s0[a] = p.g_s[a * p.gcap + n];
+        }
+    }
     if (cg_all_done(p)) return;
     double rr, mx;
     reduce_range(p.bsum, p.bmax, 0, nparts, rr, mx);
     float beta;
     if (cg_test(p, p.solo, k, rr, mx, blockIdx.x == 0 && threadIdx.x == 0, beta)) return;
-    const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
-    const int lane = threadIdx.x & 63;
-    for (int b = r.first; b < r.end; b += r.step) {
+    if (first) {
+        const int n = r.first * 64 + lane;
+#pragma unroll
+        for (int a = 0; a < D; ++a) p.g_s[a * p.gcap + n] = cg_direction(r0[a], s0[a], beta, k);
+    }
+    for (int b = r.first + r.step; b < r.end; b += r.step) {
         if (other_scene(p, b)) continue;
         const int n = b * 64 + lane;
         if (!(p.g_flags[n] & MPM_FLAG_INTERIOR_DEV)) continue;
@@ -2014,6 +2031,24 @@ __global__ __launch_bounds__(WG) void k_cg_node1(KP p, int k) {
 // (work-group 0 records alpha); batches read alpha from k_cg_reduce_sq.
 template <int D, bool ONE>
 __global__ __launch_bounds__(WG) void k_cg_node2(KP p, int k, int nparts) {
+    const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
+    const int lane = threadIdx.x & 63;
+    // one scene: the wave's first block's node values load while the s.q
+    // partials are reduced
+    float pre_s[D], pre_q[D], pre_x[D], pre_r[D], pre_m = 0.0f;
+    bool pre_in = false;
+    if (ONE && r.first < r.end) {
+        const int n = r.first * 64 + lane;
+        pre_in = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
+        pre_m = p.g_m[n];
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            pre_s[a] = p.g_s[a * p.gcap + n];
+            pre_q[a] = p.g_q[a * p.gcap + n];
+            pre_x[a] = p.g_vs[a * p.gcap + n];
+            pre_r[a] = p.g_r[a * p.gcap + n];
+        }
+    }
     if (cg_all_done(p)) return;
     float alpha1 = 0.0f;
     if (ONE) {
@@ -2029,8 +2064,6 @@ __global__ __launch_bounds__(WG) void k_cg_node2(KP p, int k, int nparts) {
         }
         alpha1 = (float)alpha_d;
     }
-    const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
-    const int lane = threadIdx.x & 63;
     double wsum = 0.0, wmx = 0.0;
     for (int b = r.first; b < r.end; b += r.step) {
         if (ONE && other_scene(p, b)) continue;
@@ -2039,15 +2072,18 @@ __global__ __launch_bounds__(WG) void k_cg_node2(KP p, int k, int nparts) {
         if (!sc.active || (!ONE && p.cg[si].done)) continue;
         const float alpha = ONE ? alpha1 : (float)p.cg[si].alpha;
         const int n = b * 64 + lane;
-        const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
+        const bool pre = ONE && b == r.first;
+        const bool interior = pre ? pre_in : (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
         double sum = 0.0, mx = 0.0;
         if (interior) {
-            const double m = (double)p.g_m[n];
+            const double m = (double)(pre ? pre_m : p.g_m[n]);
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                const float sv = p.g_s[a * p.gcap + n], q = p.g_q[a * p.gcap + n];
-                p.g_vs[a * p.gcap + n] = alpha * sv + p.g_vs[a * p.gcap + n];
-                const float rv = -alpha * q + p.g_r[a * p.gcap + n];
+                const float sv = pre ? pre_s[a] : p.g_s[a * p.gcap + n];
+                const float q = pre ? pre_q[a] : p.g_q[a * p.gcap + n];
+                const float xv = pre ? pre_x[a] : p.g_vs[a * p.gcap + n];
+                p.g_vs[a * p.gcap + n] = alpha * sv + xv;
+                const float rv = -alpha * q + (pre ? pre_r[a] : p.g_r[a * p.gcap + n]);
                 p.g_r[a * p.gcap + n] = rv;
                 sum += (double)rv * ((double)rv * m);
                 mx += (double)rv * (double)rv;
