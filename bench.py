@@ -467,7 +467,7 @@ def main():
                          "for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build runs each scene on a single MI355X "
+        print(f"bench.py measures one GPU: each scene runs on a single MI355X "
               f"(--gpus {args.gpus} is not supported)", file=sys.stderr)
         sys.exit(2)
 
