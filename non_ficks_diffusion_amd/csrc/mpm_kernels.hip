@@ -298,19 +298,25 @@ __global__ __launch_bounds__(WG) void k_bin(KP p) {
             key = slot * 64 + cell;
         }
         // wave-aggregated atomics: one per distinct (block, cell) for the rank in
-        // the cell, one per distinct block for the block total
-        int rank = 0;
+        // the cell, one per distinct block for the block total.  The groups are
+        // found first (ballots only), then every group's first lane adds its
+        // group's size at once, so a wave waits for one round of atomics.
+        int leader = 0, in_group = 0, group_size = 0;
         unsigned long long todo = __ballot(valid);
         while (todo) {
-            const int leader = __ffsll((long long)todo) - 1;
-            const int lkey = __shfl(key, leader);
+            const int l = __ffsll((long long)todo) - 1;
+            const int lkey = __shfl(key, l);
             const unsigned long long grp = __ballot(valid && key == lkey);
-            int r0 = 0;
-            if (lane == leader) r0 = atomicAdd(&p.cell_count[lkey], __popcll(grp));
-            r0 = __shfl(r0, leader);
-            if (valid && key == lkey) rank = r0 + __popcll(grp & ((1ull << lane) - 1ull));
+            if (valid && key == lkey) {
+                leader = l;
+                in_group = __popcll(grp & ((1ull << lane) - 1ull));
+            }
+            if (lane == l) group_size = __popcll(grp);
             todo &= ~grp;
         }
+        int r0 = 0;
+        if (valid && lane == leader) r0 = atomicAdd(&p.cell_count[key], group_size);
+        const int rank = __shfl(r0, leader) + in_group;
         todo = __ballot(valid);
         while (todo) {
             const int leader = __ffsll((long long)todo) - 1;
