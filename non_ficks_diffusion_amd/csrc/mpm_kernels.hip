@@ -1092,7 +1092,10 @@ __global__ __launch_bounds__(BWG) void k_p2g(KP p) {
     }
     const XcdRange xr = xcd_wave_blocks(p.ctr->nblocks);
     for (int kb = xr.first; kb < xr.end; kb += xr.step) {
-        const int b = p.block_order[kb];  // heavy blocks first within the XCD's range
+        // one scene: heavy blocks first within the XCD's range; batches keep
+        // slot order, in which neighbouring waves share a scene (P2G's work per
+        // block is small enough there that the scene's parameters matter)
+        const int b = p.nscenes > 1 ? kb : p.block_order[kb];
         const SceneParams& sc = p.scn[p.block_scene[b]];
         if (!sc.active || block_empty(p, b)) continue;
         for (int t = c; t < C * LTN; t += 64) mine[t] = 0.0f;
