@@ -1093,8 +1093,8 @@ __global__ __launch_bounds__(BWG) void k_p2g(KP p) {
     const XcdRange xr = xcd_wave_blocks(p.ctr->nblocks);
     for (int kb = xr.first; kb < xr.end; kb += xr.step) {
         // one scene: heavy blocks first within the XCD's range; batches keep
-        // slot order, in which neighbouring waves share a scene (P2G's work per
-        // block is small enough there that the scene's parameters matter)
+        // slot order, measured faster there (small blocks of many scenes, and
+        // neighbouring waves share a scene in slot order)
         const int b = p.nscenes > 1 ? kb : p.block_order[kb];
         const SceneParams& sc = p.scn[p.block_scene[b]];
         if (!sc.active || block_empty(p, b)) continue;
