@@ -8,7 +8,8 @@ for the same particles (the FP32 bar of one substep, 1e-5 relative L2: only
 the order of particles inside a cell, which binning's atomics decide,
 differs; both CG solves converge).  Two calls in a row, the second with
 fewer particles and a smaller grid, so the second runs on the scratch the
-first left behind.  Sizes beyond the limits are refused before any GPU work.
+first left behind; then the 2D path.  Sizes beyond the limits are refused
+before any GPU work.
 """
 import ctypes
 
@@ -124,6 +125,16 @@ def test_noalloc_substep_matches_the_handle_path_twice():
         got, st_got = noalloc_path(d, x, v, m, dt, hip, use_stream)
         check(got, st_got, ref, st_ref, f"{name} {counts}^3 ({len(m)} particles)")
     assert sizes[1] < sizes[0]
+
+
+def test_noalloc_substep_2d_matches_the_handle_path():
+    """The 2D path of the entry point (8^2 blocks, 2x2 F) on the 2D disk."""
+    hip = Hip()
+    d, x, v, m = M.scene_generate("disk2d", 64, 0)
+    assert d.dim == 2
+    ref, st_ref = handle_path(d, x, v, m, 2e-3)
+    got, st_got = noalloc_path(d, x, v, m, 2e-3, hip, True)
+    check(got, st_got, ref, st_ref, f"disk2d 64^2 ({len(m)} particles)")
 
 
 def test_noalloc_refuses_sizes_beyond_its_limits():
