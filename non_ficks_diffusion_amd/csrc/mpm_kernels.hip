@@ -191,47 +191,66 @@ __device__ inline void gather_node_tiles(const KP& p, const SceneParams& sc, int
     const int b = n >> 6;
     int lc[D];
     local_coords<D>(n & 63, lc);
+    // per axis: option 0 is the node's own block; a node on a block face
+    // also lies in the face neighbour's tile (option 1)
+    int od1[D], td0[D], td1[D], face = 0;
 #pragma unroll
-    for (int c = 0; c < C; ++c) out[c] = 0.0f;
-    if (sc.crop_ghost_x) {
+    for (int a = 0; a < D; ++a) {
+        td0[a] = lc[a] + 1;
+        face |= (lc[a] == 0 || lc[a] == B - 1) ? 1 << (D - 1 - a) : 0;
+        od1[a] = lc[a] == 0 ? 0 : 2;
+        td1[a] = lc[a] == 0 ? TE - 1 : 0;
+    }
+    // every candidate tile's neighbour entry, then every tile value, is
+    // loaded before the first sum (no load waits on another candidate's);
+    // candidate m takes option 1 on the axes of its set bits, and the sum
+    // runs over the candidates in lexicographic option order
+    constexpr int NC = 1 << D;
+    const int* nbr = p.block_tnbr + (int64_t)b * NBR;
+    int blk[NC];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+        int o = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) o = o * 3 + (((m >> (D - 1 - a)) & 1) ? od1[a] : 1);
+        const int e = nbr[o];  // o is an entry of the table whether or not it is used
+        blk[m] = (m & ~face) == 0 ? e : -1;
+    }
+    // a candidate on the node's faces loads even where its block wrote no
+    // tile (from block 0's slot, in bounds, and not summed), so that no sum
+    // shares its load's condition and none is scheduled ahead of the loads
+    float v[NC][C];
+#pragma unroll
+    for (int m = 0; m < NC; ++m) {
+        int t = 0;
+#pragma unroll
+        for (int a = 0; a < D; ++a) t = t * TE + (((m >> (D - 1 - a)) & 1) ? td1[a] : td0[a]);
+#pragma unroll
+        for (int c = 0; c < C; ++c) v[m][c] = 0.0f;
+        if ((m & ~face) == 0) {
+            const float* src = p.tiles + (int64_t)max(blk[m], 0) * C * TN + t;
+#pragma unroll
+            for (int c = 0; c < C; ++c) v[m][c] = src[c * TN];
+        }
+    }
+    // the crop test's loads go out with the tile loads (a cropped node sums
+    // nothing)
+    bool keep = true;
+    {
+        const int crop = sc.crop_ghost_x, counts = sc.counts;
         int bb[D];
         block_coords<D>(p, b, bb);
         const int ix = B * bb[0] + lc[0] - B;
-        if (ix < 0 || ix > sc.counts - 1) return;
+        keep = (crop == 0) | ((ix >= 0) & (ix <= counts - 1));
     }
-    int od[D][2], td[D][2], nopt[D];
 #pragma unroll
-    for (int a = 0; a < D; ++a) {
-        od[a][0] = 1;
-        td[a][0] = lc[a] + 1;
-        nopt[a] = 1;
-        if (lc[a] == 0) { od[a][1] = 0; td[a][1] = TE - 1; nopt[a] = 2; }
-        else if (lc[a] == B - 1) { od[a][1] = 2; td[a][1] = 0; nopt[a] = 2; }
-        else { od[a][1] = 1; td[a][1] = lc[a] + 1; }
-    }
-    const int* nbr = p.block_tnbr + (int64_t)b * NBR;
-    if constexpr (D == 3) {
-        for (int i = 0; i < nopt[0]; ++i)
-            for (int j = 0; j < nopt[1]; ++j)
-                for (int k = 0; k < nopt[2]; ++k) {
-                    const int blk = nbr[(od[0][i] * 3 + od[1][j]) * 3 + od[2][k]];
-                    if (blk < 0) continue;
-                    const int t = (td[0][i] * TE + td[1][j]) * TE + td[2][k];
-                    const float* src = p.tiles + (int64_t)blk * C * TN + t;
+    for (int c = 0; c < C; ++c) out[c] = 0.0f;
 #pragma unroll
-                    for (int c = 0; c < C; ++c) out[c] += src[c * TN];
-                }
-    } else {
-        for (int i = 0; i < nopt[0]; ++i)
-            for (int j = 0; j < nopt[1]; ++j) {
-                const int blk = nbr[od[0][i] * 3 + od[1][j]];
-                if (blk < 0) continue;
-                const int t = td[0][i] * TE + td[1][j];
-                const float* src = p.tiles + (int64_t)blk * C * TN + t;
+    for (int m = 0; m < NC; ++m)
+        if (keep && blk[m] >= 0) {
 #pragma unroll
-                for (int c = 0; c < C; ++c) out[c] += src[c * TN];
-            }
-    }
+            for (int c = 0; c < C; ++c) out[c] += v[m][c];
+        }
 }
 
 __device__ inline bool inside_plane(const SceneParams& sc, int id, const float* x, int D) {
@@ -2001,15 +2020,20 @@ __global__ __launch_bounds__(WG) void k_cg_node1(KP p, int k) {
         const float sds = sc.dt * sc.dt;
         const int n = b * 64 + lane;
         const bool interior = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
-        float q[D], sv[D];
+        float q[D], sv[D], mn = 1.0f;
+        // one scene: s and m load alongside the flags and the tile gather
+        // (every node of an active block has them; only interior ones use them)
+        if (ONE) {
+            mn = p.g_m[n];
+#pragma unroll
+            for (int a = 0; a < D; ++a) sv[a] = p.g_s[a * p.gcap + n];
+        }
         gather_node_tiles<D, D>(p, sc, n, q);
         if (interior) {
-            const float cm = sds / p.g_m[n];
+            const float cm = sds / (ONE ? mn : p.g_m[n]);
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                if (ONE) {
-                    sv[a] = p.g_s[a * p.gcap + n];
-                } else {
+                if (!ONE) {
                     sv[a] = cg_direction(p.g_r[a * p.gcap + n], p.g_s[a * p.gcap + n], beta, k);
                     p.g_s[a * p.gcap + n] = sv[a];
                 }
@@ -2024,7 +2048,7 @@ __global__ __launch_bounds__(WG) void k_cg_node1(KP p, int k) {
         for (int a = 0; a < D; ++a) p.g_q[a * p.gcap + n] = q[a];
         double sum = 0.0, mx = 0.0;
         if (interior) {
-            const double m = (double)p.g_m[n];
+            const double m = (double)(ONE ? mn : p.g_m[n]);
 #pragma unroll
             for (int a = 0; a < D; ++a) sum += (double)sv[a] * ((double)q[a] * m);
         }
