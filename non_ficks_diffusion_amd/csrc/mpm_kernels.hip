@@ -1850,14 +1850,21 @@ __device__ inline void reduce_range(const double* sums, const double* maxs, int 
 #pragma unroll
     for (int u = 0; u < U; ++u) { sa[u] = 0.0; ma[u] = 0.0; }
     for (int base = (int)threadIdx.x; base < count; base += U * WG) {
+        // the chunk's loads all go out before its first add (an index past
+        // the range reads the range's last entry and is not added)
+        double vs[U], vm[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int j = base + u * WG;
-            if (j < count) {
-                sa[u] += sums[first + j];
-                if (maxs) ma[u] = fmax(ma[u], maxs[first + j]);
-            }
+            const int j = min(base + u * WG, count - 1);
+            vs[u] = sums[first + j];
+            vm[u] = maxs ? maxs[first + j] : 0.0;
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (base + u * WG < count) {
+                sa[u] += vs[u];
+                if (maxs) ma[u] = fmax(ma[u], vm[u]);
+            }
     }
     double s = 0.0, m = 0.0;
 #pragma unroll
@@ -1880,21 +1887,39 @@ __device__ inline void reduce_range(const double* sums, const double* maxs, int 
 // L-inf over nodes of |r|) on r.r and max |r|^2, and beta = rho_k / rho_{k-1}.
 // A scene is done when the test passes (after cg_min iterations) or at its
 // cap cg_max.  `record`: this thread writes the scene's state.
-__device__ inline bool cg_test(const KP& p, int si, int k, double rr, double mx, bool record, float& beta_f) {
-    CGState& cg = p.cg[si];
+// The scene's inputs of the test, read in one go (before the partial
+// reduction, so that their loads overlap it).
+struct CGTestIn {
+    double rho_prev, tol;
+    int cg_min, cg_max;
+    bool empty;
+};
+__device__ inline CGTestIn cg_test_in(const KP& p, int si, int k) {
     const SceneParams& sc = p.scn[si];
+    CGTestIn in;
+    in.rho_prev = p.cg[si].rho_hist[(k + 1) & 1];
+    in.tol = sc.cg_tol;
+    in.cg_min = sc.cg_min;
+    in.cg_max = sc.cg_max;
+    const int active = sc.active, b0 = p.sst[si].block_begin, b1 = p.sst[si].block_end;
+    in.empty = (active == 0) | (b1 <= b0);
+    return in;
+}
+
+__device__ inline bool cg_test(const KP& p, int si, int k, double rr, double mx, bool record, float& beta_f,
+                               const CGTestIn& in) {
+    CGState& cg = p.cg[si];
     const double conv = sqrt(mx);
-    const bool pass = conv <= sc.cg_tol && (k >= sc.cg_min || conv < (double)FLT_EPSILON);
-    const bool empty = !sc.active || p.sst[si].block_end <= p.sst[si].block_begin;
-    const bool done = pass || k >= sc.cg_max || empty;
-    const double beta = k == 0 ? 0.0 : rr / cg.rho_hist[(k + 1) & 1];
+    const bool pass = conv <= in.tol && (k >= in.cg_min || conv < (double)FLT_EPSILON);
+    const bool done = pass || k >= in.cg_max || in.empty;
+    const double beta = k == 0 ? 0.0 : rr / in.rho_prev;
     if (record) {
         if (k == 0) cg.multiplies = 1;  // the restart Multiply
         cg.iter = k;
         cg.conv = conv;
         cg.rr = rr;
         if (done) {
-            cg.converged = (pass || empty) ? 1 : 0;
+            cg.converged = (pass || in.empty) ? 1 : 0;
             cg.done = 1;
             atomicMax(&p.ctr->cg_iters, k);
             atomicAdd(&p.ctr->cg_ndone, 1);
@@ -1971,26 +1996,27 @@ __global__ __launch_bounds__(WG) void k_cg_update_s(KP p, int k, int nparts) {
     const XcdRange r = xcd_wave_blocks(p.ctr->nblocks);
     const int lane = threadIdx.x & 63;
     // the wave's first block: r and s load while the partials are reduced
+    // (a wave without blocks loads block 0's nodes and uses nothing)
     float r0[D], s0[D];
-    bool first = false;
-    if (r.first < r.end) {
-        const int n = r.first * 64 + lane;
-        first = !other_scene(p, r.first) && (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
+    const bool has = r.first < r.end;
+    const bool mine = has && !other_scene(p, r.first);
+    const int n0 = (has ? r.first : 0) * 64 + lane;
+    const unsigned flags0 = p.g_flags[n0];
 #pragma unroll
-        for (int a = 0; a < D; ++a) {
-            r0[a] = p.g_r[a * p.gcap + n];
-            s0[a] = p.g_s[a * p.gcap + n];
-        }
+    for (int a = 0; a < D; ++a) {
+        r0[a] = p.g_r[a * p.gcap + n0];
+        s0[a] = p.g_s[a * p.gcap + n0];
     }
     if (cg_all_done(p)) return;
+    // the test's inputs load alongside the partials
+    const CGTestIn tin = cg_test_in(p, p.solo, k);
     double rr, mx;
     reduce_range(p.bsum, p.bmax, 0, nparts, rr, mx);
     float beta;
-    if (cg_test(p, p.solo, k, rr, mx, blockIdx.x == 0 && threadIdx.x == 0, beta)) return;
-    if (first) {
-        const int n = r.first * 64 + lane;
+    if (cg_test(p, p.solo, k, rr, mx, blockIdx.x == 0 && threadIdx.x == 0, beta, tin)) return;
+    if (mine && (flags0 & MPM_FLAG_INTERIOR_DEV) != 0) {
 #pragma unroll
-        for (int a = 0; a < D; ++a) p.g_s[a * p.gcap + n] = cg_direction(r0[a], s0[a], beta, k);
+        for (int a = 0; a < D; ++a) p.g_s[a * p.gcap + n0] = cg_direction(r0[a], s0[a], beta, k);
     }
     for (int b = r.first + r.step; b < r.end; b += r.step) {
         if (other_scene(p, b)) continue;
@@ -2069,10 +2095,10 @@ __global__ __launch_bounds__(WG) void k_cg_node2(KP p, int k, int nparts) {
     // one scene: the wave's first block's node values load while the s.q
     // partials are reduced
     float pre_s[D], pre_q[D], pre_x[D], pre_r[D], pre_m = 0.0f;
-    bool pre_in = false;
+    unsigned pre_flags = 0u;
     if (ONE && r.first < r.end) {
         const int n = r.first * 64 + lane;
-        pre_in = (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
+        pre_flags = p.g_flags[n];
         pre_m = p.g_m[n];
 #pragma unroll
         for (int a = 0; a < D; ++a) {
@@ -2085,11 +2111,15 @@ __global__ __launch_bounds__(WG) void k_cg_node2(KP p, int k, int nparts) {
     if (cg_all_done(p)) return;
     float alpha1 = 0.0f;
     if (ONE) {
+        // the scene's done flag and rho load alongside the partials (a
+        // finished solve writes nothing)
         CGState& cg = p.cg[p.solo];
-        if (cg.done) return;
+        const bool stop = cg.done != 0;
+        const double rho = cg.rho_hist[k & 1];
         double sq, unused;
         reduce_range(p.bsq, nullptr, 0, nparts, sq, unused);
-        const double alpha_d = sq != 0.0 ? cg.rho_hist[k & 1] / sq : (double)FLT_MAX;
+        if (stop) return;
+        const double alpha_d = sq != 0.0 ? rho / sq : (double)FLT_MAX;
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             cg.sq = sq;
             cg.alpha = alpha_d;
@@ -2106,7 +2136,7 @@ __global__ __launch_bounds__(WG) void k_cg_node2(KP p, int k, int nparts) {
         const float alpha = ONE ? alpha1 : (float)p.cg[si].alpha;
         const int n = b * 64 + lane;
         const bool pre = ONE && b == r.first;
-        const bool interior = pre ? pre_in : (p.g_flags[n] & MPM_FLAG_INTERIOR_DEV) != 0;
+        const bool interior = ((pre ? pre_flags : p.g_flags[n]) & MPM_FLAG_INTERIOR_DEV) != 0;
         double sum = 0.0, mx = 0.0;
         if (interior) {
             const double m = (double)(pre ? pre_m : p.g_m[n]);
@@ -2155,10 +2185,11 @@ __global__ __launch_bounds__(WG) void k_cg_reduce_rr(KP p, int k) {
     const int si = blockIdx.x;
     if (p.cg[si].done) return;
     const int b0 = p.sst[si].block_begin, b1 = p.sst[si].block_end;
+    const CGTestIn tin = cg_test_in(p, si, k);
     double rr, mx;
     reduce_range(p.bsum, p.bmax, b0, b1 - b0, rr, mx);
     float beta;
-    (void)cg_test(p, si, k, rr, mx, threadIdx.x == 0, beta);
+    (void)cg_test(p, si, k, rr, mx, threadIdx.x == 0, beta, tin);
 }
 
 // ============================================================================
