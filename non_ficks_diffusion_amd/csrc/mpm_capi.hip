@@ -128,6 +128,7 @@ mpm::KP make_kp(const mpm_context* c) {
     p.scur = c->scbuf[c->cur];
     p.salt = c->scbuf[c->cur ^ 1];
     p.prank = c->prank;
+    p.cell_order = c->cell_order;
     p.rcache = c->rcache;
     p.pop = c->pop;
     p.pcap = c->pcap;
@@ -239,7 +240,10 @@ void stage_bin(mpm_context* c) {
         mpm::launch_scene_ranges(p, c->stream);
         mpm::launch_block_order(p, c->stream);
     });
-    timed(c, KC_REORDER, [&] { mpm::launch_reorder<D>(p, c->stream, c->grid_particle); });
+    timed(c, KC_REORDER, [&] {
+        mpm::launch_cell_order(p, c->stream, c->grid_particle);
+        mpm::launch_reorder<D>(p, c->stream, c->grid_particle);
+    });
     c->cur ^= 1;
     c->binned = true;
     c->ever_binned = true;
@@ -567,6 +571,7 @@ void layout_buffers(mpm_context* c, Take&& take) {
         take(&c->scbuf[k], pcap);
     }
     take(&c->prank, pcap);
+    take(&c->cell_order, pcap);
     take(&c->rcache, (size_t)D * D * pcap);
     take(&c->pop, (size_t)mpm::op_record_floats(D) * pcap);
     take(&c->slot_count, nslots);
@@ -1355,8 +1360,8 @@ namespace mpm_host {
 UnitBytes unit_bytes(int D) {
     const int64_t nf = nfields(D), DD = (int64_t)D * D;
     UnitBytes u;
-    // two particle buffers (fields, id, key, scene) + rank, rotation cache, operator record
-    u.particle = 2 * (nf + 3) * 4 + 4 + DD * 4 + (int64_t)mpm::op_record_floats(D) * 4;
+    // two particle buffers (fields, id, key, scene) + rank, in-cell order, rotation cache, operator record
+    u.particle = 2 * (nf + 3) * 4 + 4 + 4 + DD * 4 + (int64_t)mpm::op_record_floats(D) * 4;
     // count, touched, active, owner + 64 cell counters
     u.slot = 4 * 4 + 64 * 4;
     // block tables, neighbours, cell ranges, tiles, 64 nodes of grid channels, partials

@@ -41,6 +41,7 @@ struct mpm_context {
     int cur = 0;
     int solo = -1;  // batch or arena substep with one active scene: it takes the single-scene CG
     int* prank = nullptr;
+    int* cell_order = nullptr;  // per particle: index scattered by binning rank (stable in-cell order)
     float* rcache = nullptr;
     float* pop = nullptr;
     int *slot_count = nullptr, *slot_touched = nullptr, *slot_active = nullptr;

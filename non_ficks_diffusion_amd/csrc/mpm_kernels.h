@@ -98,6 +98,7 @@ struct KP {
     int* scur;       // scene of each particle
     int* salt;
     int* prank;
+    int* cell_order;  // per particle slot of the binned layout: the index it is filled from
     float* rcache;   // rotation R per particle (sorted order), field-major
     float* pop;      // per particle (sorted order) op_record_floats(D) floats:
                      // X and the linearised CG force operator
@@ -186,6 +187,7 @@ template <int D> void launch_block_setup(const KP& p, hipStream_t s, int grid);
 void launch_scene_ranges(const KP& p, hipStream_t s);
 // block_order: every XCD range of the active blocks sorted by descending particle count
 void launch_block_order(const KP& p, hipStream_t s);
+void launch_cell_order(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_reorder(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_p2g(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_normalize(const KP& p, hipStream_t s, int grid);

@@ -608,6 +608,22 @@ __global__ __launch_bounds__(WG) void k_block_order(KP p) {
 }
 
 // scatter into (block, cell) order
+// The binning ranks come from atomics, so the order they give within a cell
+// depends on arrival.  Each particle writes its index into its cell's range
+// at that rank; k_reorder then places it by the number of indices in the
+// range below its own, which orders every cell by the particles' previous
+// positions (a stable sort by cell) and makes the layout, and every sum over
+// a cell's particles, the same from run to run.
+__global__ __launch_bounds__(WG) void k_cell_order(KP p) {
+    if (p.ctr->overflow != 0) return;
+    const int n = p.ctr->n_prev;
+    for (int i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) {
+        const int key = p.keycur[i];
+        if (key < 0) continue;
+        p.cell_order[p.cell_start[p.slot_active[key >> 6] * 65 + (key & 63)] + p.prank[i]] = i;
+    }
+}
+
 template <int D>
 __global__ __launch_bounds__(WG) void k_reorder(KP p) {
     const int n = p.ctr->n_prev;
@@ -615,9 +631,17 @@ __global__ __launch_bounds__(WG) void k_reorder(KP p) {
     for (int i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) {
         const int key = p.keycur[i];
         int dest;
-        if (overflow) dest = i;  // identity copy: the buffer swap leaves the state unchanged
-        else if (key < 0) continue;
-        else dest = p.cell_start[p.slot_active[key >> 6] * 65 + (key & 63)] + p.prank[i];
+        if (overflow) {
+            dest = i;  // identity copy: the buffer swap leaves the state unchanged
+        } else if (key < 0) {
+            continue;
+        } else {
+            const int cs = p.slot_active[key >> 6] * 65 + (key & 63);
+            const int start = p.cell_start[cs], end = p.cell_start[cs + 1];
+            int rank = 0;
+            for (int j = start; j < end; ++j) rank += p.cell_order[j] < i ? 1 : 0;
+            dest = start + rank;
+        }
 #pragma unroll
         for (int f = 0; f < PF<D>::N; ++f) p.palt[f * p.pcap + dest] = p.pcur[f * p.pcap + i];
         p.idalt[dest] = p.idcur[i];
@@ -2645,6 +2669,7 @@ void launch_scene_ranges(const KP& p, hipStream_t s) {
     hipLaunchKernelGGL(k_scene_ranges, dim3((p.nscenes + WG - 1) / WG), dim3(WG), 0, s, p);
 }
 void launch_block_order(const KP& p, hipStream_t s) { hipLaunchKernelGGL(k_block_order, dim3(16), dim3(WG), 0, s, p); }
+void launch_cell_order(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_cell_order, dim3(grid), dim3(WG), 0, s, p); }
 template <int D> void launch_reorder(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_reorder<D>, dim3(grid), dim3(WG), 0, s, p); }
 template <int D> void launch_p2g(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_p2g<D>, dim3(grid), dim3(BWG), 0, s, p); }
 template <int D> void launch_normalize(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_normalize<D>, dim3(grid), dim3(WG), 0, s, p); }

@@ -1,8 +1,8 @@
 """Batched scenes: every kernel of a substep covers all scenes of a batch
 (scene table, concatenated slots and particles, per-scene CG).  A scene must
-evolve in a batch exactly as it does alone (the FP32 bar of one substep
-chain, 1e-5 relative L2: only the order of particles inside a cell, which
-binning's atomics decide, differs between runs)."""
+evolve in a batch as it does alone (the FP32 bar of one substep chain, 1e-5
+relative L2: a batch reduces each scene's CG partials per block and then per
+scene, a scene alone per work-group, so the FP64 sums may round apart)."""
 import numpy as np
 import pytest
 

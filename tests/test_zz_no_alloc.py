@@ -4,9 +4,10 @@ One substep of a scene on particle arrays that the caller holds in device
 memory (here: buffers the test allocates through the HIP runtime the library
 links), with every working buffer in the library's static scratch, must give
 what mpm_create + mpm_upload_particles + mpm_initialize + mpm_substep give
-for the same particles (the FP32 bar of one substep, 1e-5 relative L2: only
-the order of particles inside a cell, which binning's atomics decide,
-differs; both CG solves converge).  Two calls in a row, the second with
+for the same particles (the FP32 bar of one substep, 1e-5 relative L2: the
+static scratch caps the block capacity at MPM_NOALLOC_MAX_BLOCKS, so the
+node kernels' launch grid, and with it the grouping of the CG's FP64
+partial sums, can differ from the handle's; both CG solves converge).  Two calls in a row, the second with
 fewer particles and a smaller grid, so the second runs on the scratch the
 first left behind; then the 2D path.  Sizes beyond the limits are refused
 before any GPU work.
