@@ -265,8 +265,11 @@ void stage_waiting(mpm_context* c) {
 // particles from which a batch substep with one active scene is launched
 // (single-scene CG kernels, polled) rather than replayed from the batch graph
 constexpr int64_t SOLO_LAUNCHED_MIN = 65536;
-// iterations per launch batch of the polled CG after the first
+// iterations per launch batch of the polled CG after the first; the batch
+// right after the first (the predicted count) covers the host's reading of
+// the first batch's poll and is usually all no-ops
 constexpr int CG_CHUNK = 4;
+constexpr int CG_COVER = 1;
 // A done poll copies Counters::overflow .. cg_iters (five ints) into a slot
 // of POLL_INTS pinned ints: [0] overflow, [1] finished solves, [4] the
 // largest iteration count among them.
@@ -327,12 +330,13 @@ int stage_cg(mpm_context* c, bool do_init, bool fixed = false) {
     bool pending[2] = {false, false};
     bool finished = false;
     while (launched < max_it) {
-        const int n = launched == 0 ? first : std::min(CG_CHUNK, max_it - launched);
+        const int n = launched == 0 ? first : std::min(launched == first ? CG_COVER : CG_CHUNK, max_it - launched);
         for (int j = 0; j < n; ++j) iteration(launched + j);
         launched += n;
-        // one copy reads the overflow flag, the done count and the iterations
-        HIPCHECK(hipMemcpyAsync(&c->h_done[POLL_INTS * slot], &c->ctr->overflow, 5 * sizeof(int), hipMemcpyDeviceToHost,
-                                c->stream));
+        // one wave stores the overflow flag, the done count and the
+        // iterations into the pinned slot (a kernel in the stream's order: no
+        // runtime copy)
+        mpm::launch_poll(p, c->stream, c->d_done + POLL_INTS * slot);
         HIPCHECK(hipEventRecord(c->done_ev[slot], c->stream));
         pending[slot] = true;
         boundary[slot] = launched;
@@ -620,9 +624,12 @@ int allocate_context(mpm_context* c) {
     layout_buffers(c, [&](auto** ptr, size_t count) { A(dalloc(c, ptr, count)); });
     if (rc == MPM_OK) {
         void* hp = nullptr;
-        if (hipHostMalloc(&hp, 2 * POLL_INTS * sizeof(int), hipHostMallocDefault) != hipSuccess)
+        // coherent: k_poll stores the counters into it from the device
+        if (hipHostMalloc(&hp, 2 * POLL_INTS * sizeof(int), hipHostMallocCoherent) != hipSuccess)
             rc = mpm_set_error(MPM_E_HIP, "hipHostMalloc failed");
         c->h_done = (int*)hp;
+        if (rc == MPM_OK && hipHostGetDevicePointer((void**)&c->d_done, hp, 0) != hipSuccess)
+            rc = mpm_set_error(MPM_E_HIP, "hipHostGetDevicePointer failed");
         for (int k = 0; k < 2 && rc == MPM_OK; ++k)
             if (hipEventCreateWithFlags(&c->done_ev[k], hipEventDisableTiming) != hipSuccess) rc = mpm_set_error(MPM_E_HIP, "hipEventCreate failed");
         hp = nullptr;

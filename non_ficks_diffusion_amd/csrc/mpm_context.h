@@ -64,6 +64,7 @@ struct mpm_context {
     unsigned* bu = nullptr;
     int cg_predict = 0;    // the next solve's first launch batch: the last solve's iterations plus one
     int* h_done = nullptr;  // pinned: per poll slot (8 ints), Counters::overflow .. cg_iters
+    int* d_done = nullptr;  // the device's address of h_done
     bool overflow_seen = false;  // the last substep's poll saw a block-capacity overflow
     hipEvent_t done_ev[2] = {nullptr, nullptr};
     // done polls that a solve running to its last chunk left unread; they are

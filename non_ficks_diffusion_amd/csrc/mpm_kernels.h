@@ -188,6 +188,7 @@ void launch_scene_ranges(const KP& p, hipStream_t s);
 // block_order: every XCD range of the active blocks sorted by descending particle count
 void launch_block_order(const KP& p, hipStream_t s);
 void launch_cell_order(const KP& p, hipStream_t s, int grid);
+void launch_poll(const KP& p, hipStream_t s, int* dst);
 template <int D> void launch_reorder(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_p2g(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_normalize(const KP& p, hipStream_t s, int grid);

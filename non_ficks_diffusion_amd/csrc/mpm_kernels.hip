@@ -2190,6 +2190,13 @@ __global__ __launch_bounds__(WG) void k_cg_node2(KP p, int k, int nparts) {
 
 // Batches: alpha = rho_k / s.q for every scene still iterating (one
 // work-group per scene over the scene's block partials)
+// The CG's done poll: Counters::overflow .. cg_iters (five ints) into the
+// host's pinned poll slot.
+__global__ __launch_bounds__(64) void k_poll(KP p, int* dst) {
+    const int t = (int)threadIdx.x;
+    if (t < 5) dst[t] = (&p.ctr->overflow)[t];
+}
+
 __global__ __launch_bounds__(WG) void k_cg_reduce_sq(KP p, int k) {
     const int si = blockIdx.x;
     CGState& cg = p.cg[si];
@@ -2669,6 +2676,7 @@ void launch_scene_ranges(const KP& p, hipStream_t s) {
     hipLaunchKernelGGL(k_scene_ranges, dim3((p.nscenes + WG - 1) / WG), dim3(WG), 0, s, p);
 }
 void launch_block_order(const KP& p, hipStream_t s) { hipLaunchKernelGGL(k_block_order, dim3(16), dim3(WG), 0, s, p); }
+void launch_poll(const KP& p, hipStream_t s, int* dst) { hipLaunchKernelGGL(k_poll, dim3(1), dim3(64), 0, s, p, dst); }
 void launch_cell_order(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_cell_order, dim3(grid), dim3(WG), 0, s, p); }
 template <int D> void launch_reorder(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_reorder<D>, dim3(grid), dim3(WG), 0, s, p); }
 template <int D> void launch_p2g(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_p2g<D>, dim3(grid), dim3(BWG), 0, s, p); }
