@@ -629,23 +629,42 @@ __global__ __launch_bounds__(WG) void k_reorder(KP p) {
     const int n = p.ctr->n_prev;
     const bool overflow = p.ctr->overflow != 0;
     for (int i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) {
+        // the particle's state loads with its key, before the destination is
+        // known (the buffers may alias as far as the compiler knows, so a load
+        // after a store would wait for it).  Only a placed particle (a live key
+        // and no overflow) reads the cell tables; a removed one (key < 0) or an
+        // identity copy reads cell 0's bounds (the table always has them: slot
+        // tables hold -1, and after an overflow indices past the capacity, so
+        // they are not read for these) and a removed one stores nothing.
         const int key = p.keycur[i];
-        int dest;
-        if (overflow) {
-            dest = i;  // identity copy: the buffer swap leaves the state unchanged
-        } else if (key < 0) {
-            continue;
-        } else {
-            const int cs = p.slot_active[key >> 6] * 65 + (key & 63);
-            const int start = p.cell_start[cs], end = p.cell_start[cs + 1];
-            int rank = 0;
-            for (int j = start; j < end; ++j) rank += p.cell_order[j] < i ? 1 : 0;
-            dest = start + rank;
-        }
+        float v[PF<D>::N];
 #pragma unroll
-        for (int f = 0; f < PF<D>::N; ++f) p.palt[f * p.pcap + dest] = p.pcur[f * p.pcap + i];
-        p.idalt[dest] = p.idcur[i];
-        p.salt[dest] = p.scur[i];
+        for (int f = 0; f < PF<D>::N; ++f) v[f] = p.pcur[f * p.pcap + i];
+        const int id = p.idcur[i], si = p.scur[i];
+        const bool live = overflow || key >= 0;
+        const bool placed = !overflow && key >= 0;
+        const int cs = placed ? p.slot_active[key >> 6] * 65 + (key & 63) : 0;
+        const int start = p.cell_start[cs], end = p.cell_start[cs + 1];
+        // rank: indices of the cell below i, eight loads in flight at a time
+        // (an index past the range reads its last entry and is not counted)
+        constexpr int U = 8;
+        int rank = 0;
+        if (placed) {
+            for (int j0 = start; j0 < end; j0 += U) {
+                int e[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) e[u] = p.cell_order[min(j0 + u, end - 1)];
+#pragma unroll
+                for (int u = 0; u < U; ++u) rank += (j0 + u < end && e[u] < i) ? 1 : 0;
+            }
+        }
+        // overflow: identity copy, the buffer swap leaves the state unchanged
+        const int dest = overflow ? i : start + rank;
+        if (!live) continue;
+#pragma unroll
+        for (int f = 0; f < PF<D>::N; ++f) p.palt[f * p.pcap + dest] = v[f];
+        p.idalt[dest] = id;
+        p.salt[dest] = si;
         p.keyalt[dest] = overflow ? key : 0;
     }
 }
