@@ -236,6 +236,11 @@ enum mpm_stage {
     MPM_STAGE_CG = 6,           /* RHS + Reset_Solver_Channels + CG solve (MPM_Example.cpp:700-711) */
     MPM_STAGE_G2P = 7           /* G2P, F and X update, removal (MPM_Example.cpp:631-660) */
 };
+/* Runs one stage.  Every stage but BIN and CONSTITUTIVE needs the particles
+ * as the last BIN placed them (MPM_E_NOT_READY after G2P), and CG also needs
+ * CONSTITUTIVE to have run after that BIN: the CG operator reads the
+ * per-particle operator records the constitutive update writes, in the
+ * order of the binning (MPM_E_NOT_READY otherwise). */
 int mpm_run_stage(mpm_handle h, int32_t stage, float dt);
 
 /* Grid channels readable after a stage (names follow MPM_Data.h:12-46). */
@@ -284,8 +289,10 @@ int32_t mpm_kernel_class_count(void);
  * operator application (MPM_CG_System::Force, MPM_CG_System.h:150-215) `reps`
  * times back to back on the current particle state and search direction,
  * bracketed by two HIP events on the handle's stream, and returns the
- * average milliseconds per launch.  Needs a completed substep; leaves the
- * state of that substep unchanged except the operator's tile buffer. */
+ * average milliseconds per launch.  Needs a completed substep whose operator
+ * records are current (not after mpm_time_stage of MPM_STAGE_CONSTITUTIVE,
+ * MPM_E_NOT_READY then); leaves the state of that substep unchanged except
+ * the operator's tile buffer. */
 int mpm_time_cg_operator(mpm_handle h, int32_t reps, float* avg_ms);
 
 /* Benchmark support (no counterpart in the reference): launches one

@@ -245,6 +245,7 @@ void stage_bin(mpm_context* c) {
         mpm::launch_reorder<D>(p, c->stream, c->grid_particle);
     });
     c->cur ^= 1;
+    c->records_fresh = false;
     c->binned = true;
     c->ever_binned = true;
 }
@@ -415,6 +416,7 @@ int run_stage(mpm_context* c, int stage, float dt) {
             break;
         case MPM_STAGE_CONSTITUTIVE:
             timed(c, KC_PFORCE, [&] { mpm::launch_particle_force<D>(p, c->stream, c->grid_block, true, false); });
+            c->records_fresh = c->binned;
             break;
         case MPM_STAGE_EXPLICIT_FORCE:
             timed(c, KC_PFORCE, [&] { mpm::launch_particle_force<D>(p, c->stream, c->grid_block, false, true); });
@@ -475,6 +477,7 @@ int substep(mpm_context* c, const float* dts, bool capture = false) {
         }
     }
     timed(c, KC_PFORCE, [&] { mpm::launch_particle_force<D>(p, c->stream, c->grid_block, true, true); });
+    c->records_fresh = true;
     timed(c, KC_GRID, [&] { mpm::launch_grid_update<D>(p, c->stream, c->grid_node, true, true, true, false); });
     int rc = stage_cg<D>(c, false, capture);
     if (rc != MPM_OK) return rc;
@@ -1752,6 +1755,10 @@ int mpm_run_stage(mpm_handle c, int32_t stage, float dt) {
         if (!s.uploaded) return mpm_set_error(MPM_E_NOT_READY, "mpm_run_stage: no particles uploaded");
     if (stage != MPM_STAGE_BIN && stage != MPM_STAGE_CONSTITUTIVE && !c->binned)
         return mpm_set_error(MPM_E_NOT_READY, "mpm_run_stage: particles moved since the last MPM_STAGE_BIN");
+    // the CG operator reads the records the constitutive update wrote, in the
+    // order of the binning it ran on
+    if (stage == MPM_STAGE_CG && !c->records_fresh)
+        return mpm_set_error(MPM_E_NOT_READY, "mpm_run_stage: MPM_STAGE_CG needs MPM_STAGE_CONSTITUTIVE after the last MPM_STAGE_BIN");
     int rc = c->D == 3 ? run_stage<3>(c, stage, dt) : run_stage<2>(c, stage, dt);
     if (rc) return rc;
     if (stage == MPM_STAGE_WAITING || stage == MPM_STAGE_P2G) c->initialized = true;
@@ -1924,6 +1931,8 @@ int mpm_kernel_times(mpm_handle c, int32_t nclasses, double* ms, int64_t* launch
 int mpm_time_cg_operator(mpm_handle c, int32_t reps, float* avg_ms) {
     if (!c || !avg_ms || reps <= 0) return mpm_set_error(MPM_E_INVALID, "mpm_time_cg_operator: bad argument");
     if (!c->stepped) return mpm_set_error(MPM_E_NOT_READY, "mpm_time_cg_operator: run a substep first");
+    if (!c->records_fresh)
+        return mpm_set_error(MPM_E_NOT_READY, "mpm_time_cg_operator: the operator records are stale (run a substep)");
     // the operator kernel skips finished solves: reopen them for the timed
     // launches (same dt, same search direction) and restore the state after
     const size_t n = (size_t)c->nscenes;
@@ -1996,6 +2005,8 @@ int mpm_time_stage(mpm_handle c, int32_t stage, int32_t reps, float* avg_ms) {
             if (c->D == 3) mpm::launch_p2g<3>(p, c->stream, c->grid_block);
             else mpm::launch_p2g<2>(p, c->stream, c->grid_block);
         } else if (stage == MPM_STAGE_CONSTITUTIVE) {  // the substep's fused constitutive update and force
+            // the records are not part of the restored state
+            c->records_fresh = false;
             if (c->D == 3) mpm::launch_particle_force<3>(p, c->stream, c->grid_block, true, true);
             else mpm::launch_particle_force<2>(p, c->stream, c->grid_block, true, true);
         } else {

@@ -98,6 +98,11 @@ struct mpm_context {
     bool binned = false;       // particles sit in the block ranges of the last binning
     bool ever_binned = false;  // particle ranges are the binned ones (not the upload ranges)
     bool stepped = false;      // a substep has run (operator timing needs its state)
+    // the operator records (pop) hold the constitutive update of the particles
+    // as the last binning placed them: set by the constitutive kernel on binned
+    // particles, cleared by binning (which reorders the particles but not the
+    // records) and by stage timing of the constitutive kernel
+    bool records_fresh = false;
     bool emitted = false;      // particles added since the last substep wait for their volumes
     std::vector<void*> allocations;
     // timing

@@ -50,11 +50,17 @@ constexpr int MAX_MATERIALS = 8;
 // contraction off, so keys, weights and the m > 0 test agree bit for bit
 // with a scalar CPU evaluation.
 // ---------------------------------------------------------------------------
-__device__ inline void bspline_weights(float X, float dx, float inv_dx, int& c, float w[3], float dw[3]) {
+// The cell c of coordinate X and X's offset from the stencil's first node in
+// cells (in [0.5, 1.5)).
+__device__ inline float stencil_offset(float X, float dx, float inv_dx, int& c) {
 #pragma clang fp contract(off)
     c = (int)floorf(X * inv_dx);
     const float center = ((float)c + 0.5f) * dx;
-    float x = (X - (center - dx)) * inv_dx;
+    return (X - (center - dx)) * inv_dx;
+}
+// The three weights and their derivatives at stencil offset x.
+__device__ inline void bspline_from_offset(float x, float inv_dx, float w[3], float dw[3]) {
+#pragma clang fp contract(off)
     w[0] = fabsf(0.5f * x * x - 1.5f * x + 1.125f);
     dw[0] = (x - 1.5f) * inv_dx;
     x -= 1.0f;
@@ -63,6 +69,9 @@ __device__ inline void bspline_weights(float X, float dx, float inv_dx, int& c, 
     x -= 1.0f;
     w[2] = fabsf(0.5f * x * x + 1.5f * x + 1.125f);
     dw[2] = (x + 1.5f) * inv_dx;
+}
+__device__ inline void bspline_weights(float X, float dx, float inv_dx, int& c, float w[3], float dw[3]) {
+    bspline_from_offset(stencil_offset(X, dx, inv_dx, c), inv_dx, w, dw);
 }
 
 __device__ inline int cell_of(float X, float inv_dx) {

@@ -774,7 +774,7 @@ __device__ inline bool cell_particle(const KP& p, const SceneParams& sc, int i, 
 // the record's layout: U = I, every entry of the diagonal block c, no pair
 // blocks (2D: c at (00,00), (00,11), (11,11)).
 template <int D>
-__device__ inline void store_operator_eos(const KP& p, int i, float c, const float* X);
+__device__ inline void store_operator_eos(const KP& p, int i, float c, const float* xo);
 
 // ============================================================================
 // Linearised force operator per particle.  MPM_CG_System::Force
@@ -789,12 +789,13 @@ __device__ inline void store_operator_eos(const KP& p, int i, float c, const flo
 // re-deriving the stress state, and reads one contiguous record.
 // ============================================================================
 template <int D> struct OpRec;
-// 3D: X[3], q[4] (unit quaternion of U), the singular values S[3] and the
+// 3D: x[3] (the stencil offsets of the particle's position in its binned
+// cell, in cells), q[4] (unit quaternion of U), the singular values S[3] and the
 // four scalars A = 2 mu V0, B = lambda V0 (J - 1), P = B J, Q = lambda V0 J^2
 // from which the 3x3 block on the diagonal entries and the three 2x2 pair
 // blocks follow, pad 2 (64 B: four float4).
 template <> struct OpRec<3> { static constexpr int W = 16; };  // 4 x float4
-// 2D: X[2], the full symmetric 4x4 operator (upper triangle, 10).
+// 2D: x[2] (stencil offsets), the full symmetric 4x4 operator (upper triangle, 10).
 template <> struct OpRec<2> { static constexpr int L = 10, W = 12; };  // 3 x float4
 static_assert(OpRec<3>::W == op_record_floats(3) && OpRec<2>::W == op_record_floats(2), "operator record size");
 
@@ -899,14 +900,14 @@ __device__ inline void rotate_out(const float* U, const float* Ah, float* out) {
 // record holds S and these four scalars (formed in FP64, stored in FP32);
 // apply_operator forms the block entries from them.
 template <int D>
-__device__ inline void store_operator(const KP& p, int i, const StressState<D>& st, float V0, const float* X,
+__device__ inline void store_operator(const KP& p, int i, const StressState<D>& st, float V0, const float* xo,
                                       const float* U, const double* S) {
     constexpr int DD = D * D;
     float rec[OpRec<D>::W];
 #pragma unroll
     for (int k = 0; k < OpRec<D>::W; ++k) rec[k] = 0.0f;
 #pragma unroll
-    for (int a = 0; a < D; ++a) rec[a] = X[a];
+    for (int a = 0; a < D; ++a) rec[a] = xo[a];
     if constexpr (D == 3) {
         quat_of(U, rec + 3);
         const double v0 = (double)V0, mu = (double)st.mu, lam = (double)st.lambda;
@@ -991,12 +992,12 @@ __device__ inline void apply_operator(const float* rec, const float* T, float* A
 }
 
 template <int D>
-__device__ inline void store_operator_eos(const KP& p, int i, float c, const float* X) {
+__device__ inline void store_operator_eos(const KP& p, int i, float c, const float* xo) {
     float rec[OpRec<D>::W];
 #pragma unroll
     for (int k = 0; k < OpRec<D>::W; ++k) rec[k] = 0.0f;
 #pragma unroll
-    for (int a = 0; a < D; ++a) rec[a] = X[a];
+    for (int a = 0; a < D; ++a) rec[a] = xo[a];
     if constexpr (D == 3) {
         // U = I, S = (1, 1, 1), A = B = P = 0, Q = c: every entry of the
         // diagonal block c, empty pair blocks
@@ -1031,9 +1032,15 @@ __device__ inline void grad_gather(const float (*w)[3], const float (*dw)[3], in
 #pragma unroll
             for (int j = 0; j < 3; ++j) {
                 const int t0 = base + i * LX + j * LY;
+                // the row's nine tile values load before its contractions
+                float u[3][3];
+#pragma unroll
+                for (int cc = 0; cc < 3; ++cc)
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) u[cc][k] = ut[cc][t0 + k];
 #pragma unroll
                 for (int cc = 0; cc < 3; ++cc) {
-                    const float u0 = ut[cc][t0], u1 = ut[cc][t0 + 1], u2 = ut[cc][t0 + 2];
+                    const float u0 = u[cc][0], u1 = u[cc][1], u2 = u[cc][2];
                     const float z0 = w[2][0] * u0 + w[2][1] * u1 + w[2][2] * u2;
                     const float z1 = dw[2][0] * u0 + dw[2][1] * u1 + dw[2][2] * u2;
                     y00[i][cc] += w[1][j] * z0;
@@ -1096,11 +1103,12 @@ __device__ inline void grad_scatter(const float (*w)[3], const float (*dw)[3], i
                 const int t0 = base + i * LX + j * LY;
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
+                    // a step's three reads go out before its writes
+                    float o[3];
 #pragma unroll
-                    for (int cc = 0; cc < 3; ++cc) {
-                        float* dst = q + cc * TN + t0 + k;
-                        *dst = fmaf(w[2][k], ya[cc], fmaf(dw[2][k], yb[cc], *dst));
-                    }
+                    for (int cc = 0; cc < 3; ++cc) o[cc] = q[cc * TN + t0 + k];
+#pragma unroll
+                    for (int cc = 0; cc < 3; ++cc) q[cc * TN + t0 + k] = fmaf(w[2][k], ya[cc], fmaf(dw[2][k], yb[cc], o[cc]));
                     MPM_STEP_FENCE();
                 }
             }
@@ -1333,8 +1341,21 @@ __device__ inline void scatter_force(const SceneParams& sc, const int* l, const 
 // MPM_Example.cpp:757-784); an EOS particle has no constitutive model and its
 // term is A = -(V0/rho) K rho^(gamma-1) I (MPM_Example.cpp:774-782).
 // ============================================================================
+// The operator record's position: particle i's stencil offsets, the same
+// arithmetic as the weights of its binned cell (binning and the record are
+// formed from the same position, and particles do not move between them).
+template <int D>
+__device__ inline void particle_offsets(const KP& p, const SceneParams& sc, int i, float* xo) {
+#pragma unroll
+    for (int a = 0; a < D; ++a) {
+        int c;
+        xo[a] = stencil_offset(p.pcur[(PF<D>::X + a) * p.pcap + i], sc.dx, sc.inv_dx, c);
+    }
+}
+
 template <int D, bool CONST, bool FORCE>
-__device__ __forceinline__ void particle_stress(const KP& p, const Material& mat, int i, float* A) {
+__device__ __forceinline__ void particle_stress(const KP& p, const SceneParams& sc, const Material& mat, int i,
+                                                float* A) {
     constexpr int DD = D * D;
     float Jp = p.pcur[PF<D>::J * p.pcap + i];
     const float V0 = p.pcur[PF<D>::VOL * p.pcap + i];
@@ -1342,10 +1363,9 @@ __device__ __forceinline__ void particle_stress(const KP& p, const Material& mat
         // EOS fluid (Jp slot = density): no constitutive model
         const float rg = powf(Jp, mat.gamma - 1.0f);
         if (CONST) {
-            float Xo[D];
-#pragma unroll
-            for (int a = 0; a < D; ++a) Xo[a] = p.pcur[(PF<D>::X + a) * p.pcap + i];
-            store_operator_eos<D>(p, i, V0 * mat.K * mat.gamma * rg, Xo);
+            float xo[D];
+            particle_offsets<D>(p, sc, i, xo);
+            store_operator_eos<D>(p, i, V0 * mat.K * mat.gamma * rg, xo);
             if (!FORCE) {
 #pragma unroll
                 for (int k = 0; k < DD; ++k) p.rcache[k * p.pcap + i] = (k % (D + 1) == 0) ? 1.0f : 0.0f;
@@ -1372,10 +1392,9 @@ __device__ __forceinline__ void particle_stress(const KP& p, const Material& mat
             p.pcur[PF<D>::J * p.pcap + i] = Jp;
             StressState<D> so;
             stress_state<D>(F, R, Jp, mat, so);
-            float Xo[D];
-#pragma unroll
-            for (int a = 0; a < D; ++a) Xo[a] = p.pcur[(PF<D>::X + a) * p.pcap + i];
-            store_operator<D>(p, i, so, V0, Xo, U, Sv);
+            float xo[D];
+            particle_offsets<D>(p, sc, i, xo);
+            store_operator<D>(p, i, so, V0, xo, U, Sv);
         } else {
 #pragma unroll
             for (int k = 0; k < DD; ++k) R[k] = p.rcache[k * p.pcap + i];
@@ -1439,7 +1458,7 @@ __global__ __launch_bounds__(BWG) void k_particle_force(KP p) {
                 for (int u = 0; u < nmat; ++u) {
                     if (i < p1 && (nmat == 1 || __float_as_int(p.pcur[PF<D>::MAT * p.pcap + i]) == u)) {
                         float A[DD];
-                        particle_stress<D, CONST, FORCE>(p, sc.mats[u], i, A);
+                        particle_stress<D, CONST, FORCE>(p, sc, sc.mats[u], i, A);
                         if (FORCE) {
 #pragma unroll
                             for (int k = 0; k < DD; ++k) stress[k][t] = A[k];
@@ -1619,8 +1638,6 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p, int k) {
         if (!sc.active || (!ONE && p.cg[si].done) || block_empty(p, b)) continue;
         const float beta = (RESTART || ONE) ? 0.0f : (float)p.cg[si].beta;
         if (threadIdx.x < NBR) nbr[threadIdx.x] = p.block_nbr[b * NBR + threadIdx.x];
-        int bb[D];
-        block_coords<D>(p, b, bb);
         static_assert((NW * D * LTN) % 4 == 0, "float4 zeroing");
         for (int t = threadIdx.x; t < NW * D * LTN / 4; t += BWG)
             reinterpret_cast<float4*>(qt)[t] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -1644,18 +1661,14 @@ __global__ __launch_bounds__(BWG) void k_cg_force(KP p, int k) {
         for (int i = i0 + wv; i < i1; i += NW) {
             const float* rec = p.pop + (int64_t)i * OpRec<D>::W;
             float w[D][3], dw[D][3];
-            bool same = true;
             {
+                // the record's stencil offsets, formed from the position that
+                // binned the particle into this cell
                 const float4 x4 = *reinterpret_cast<const float4*>(rec);
-                const float X[3] = {x4.x, x4.y, x4.z};
+                const float xo[3] = {x4.x, x4.y, x4.z};
 #pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    int cl;
-                    bspline_weights(X[a], sc.dx, sc.inv_dx, cl, w[a], dw[a]);
-                    same = same && cl + Dm::B - Dm::B * bb[a] == l[a];
-                }
+                for (int a = 0; a < D; ++a) bspline_from_offset(xo[a], sc.inv_dx, w[a], dw[a]);
             }
-            if (!same) continue;
             float T[DD];
             grad_gather<D>(w, dw, base, ut, T);
             float A[DD];  // V0 F dP(T^T F)^T
@@ -1781,8 +1794,6 @@ __global__ __launch_bounds__(WG) void k_cg_force_cell(KP p, int k) {
         if (!sc.active || (!ONE && p.cg[si].done) || block_empty(p, b)) continue;
         const float beta = (RESTART || ONE) ? 0.0f : (float)p.cg[si].beta;
         if (c < NBR) nbr[c] = p.block_nbr[b * NBR + c];
-        int bb[D];
-        block_coords<D>(p, b, bb);
         for (int t = c; t < D * LTN; t += 64) qt[t] = 0.0f;
         MPM_STEP_FENCE();
         for (int t = c; t < TN; t += 64) {
@@ -1810,18 +1821,14 @@ __global__ __launch_bounds__(WG) void k_cg_force_cell(KP p, int k) {
         for (int i = i0; i < i1; ++i) {
             const float* rec = p.pop + (int64_t)i * OpRec<D>::W;
             float w[D][3], dw[D][3];
-            bool same = true;
             {
+                // the record's stencil offsets, formed from the position that
+                // binned the particle into this cell
                 const float4 x4 = *reinterpret_cast<const float4*>(rec);
-                const float X[3] = {x4.x, x4.y, x4.z};
+                const float xo[3] = {x4.x, x4.y, x4.z};
 #pragma unroll
-                for (int a = 0; a < D; ++a) {
-                    int cl;
-                    bspline_weights(X[a], sc.dx, sc.inv_dx, cl, w[a], dw[a]);
-                    same = same && cl + Dm::B - Dm::B * bb[a] == l[a];
-                }
+                for (int a = 0; a < D; ++a) bspline_from_offset(xo[a], sc.inv_dx, w[a], dw[a]);
             }
-            if (!same) continue;
             float T[DD];
             grad_gather<D>(w, dw, base, utile, T);
             float A[DD];

@@ -306,6 +306,22 @@ def test_call_order_and_argument_errors():
     assert code(s.download, n - 1) == E_CAPACITY  # cap below the live count
     got = s.download()
     assert len(got["ids"]) == n
+    # the CG operator reads the records of the constitutive update on the
+    # current binning: a staged CG needs CONSTITUTIVE after the last BIN, and
+    # operator timing needs records that stage timing has not rewritten
+    s.run_stage(M.STAGE_BIN)
+    s.run_stage(M.STAGE_P2G)
+    assert code(s.run_stage, M.STAGE_CG, 1e-4) == E_NOT_READY
+    s.run_stage(M.STAGE_CONSTITUTIVE)
+    s.run_stage(M.STAGE_EXPLICIT_FORCE, 1e-4)
+    s.run_stage(M.STAGE_COLLISION, 1e-4)
+    s.run_stage(M.STAGE_CG, 1e-4)
+    s.run_stage(M.STAGE_G2P, 1e-4)
+    assert s.time_cg_operator(2) > 0.0
+    assert s.time_stage(M.STAGE_CONSTITUTIVE, 2) > 0.0
+    assert code(s.time_cg_operator, 2) == E_NOT_READY
+    s.substep(1e-4)
+    assert s.time_cg_operator(2) > 0.0
     s.close()
 
     b = M.Batch([d, d], [n, n])
