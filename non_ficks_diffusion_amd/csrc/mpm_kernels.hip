@@ -285,35 +285,43 @@ __global__ __launch_bounds__(WG) void k_bin(KP p) {
     // whole waves iterate together so the ballots below are convergent
     for (int base = blockIdx.x * WG + (threadIdx.x & ~63); base < n; base += gridDim.x * WG) {
         const int i = base + lane;
-        bool valid = i < n && p.keycur[i] >= 0;
-        int c[D];
-        int si = 0;
-        if (valid) {
-            si = p.scur[i];
-            const SceneParams& sc = p.scn[si];
-            bool finite = true;
+        // the key, scene and position load together (a lane past the end
+        // reads the last particle and is not valid), then the scene's
+        // parameters (scene 0's for a lane that is not live); no branch
+        // separates them, so no load waits on another
+        const int ic = min(i, n - 1);
+        const int key0 = p.keycur[ic], si0 = p.scur[ic];
+        float X[D];
 #pragma unroll
-            for (int a = 0; a < D; ++a) {
-                const float X = p.pcur[(PF<D>::X + a) * p.pcap + i];
-                finite = finite && isfinite(X);
-                c[a] = finite ? cell_of(X, sc.inv_dx) : -1;
-                valid = valid && finite && c[a] >= 0 && c[a] <= sc.counts - 1;
-            }
-            if (!valid) {
-                p.keycur[i] = -1;
-                if (!finite) atomicOr(&p.sst[si].nonfinite, 1);
-            }
+        for (int a = 0; a < D; ++a) X[a] = p.pcur[(PF<D>::X + a) * p.pcap + ic];
+        const bool live = i < n && key0 >= 0;
+        const int si = live ? si0 : 0;
+        const SceneParams& sc = p.scn[si];
+        const float inv_dx = sc.inv_dx;
+        const int counts = sc.counts, nb = sc.nb, slot_base = sc.slot_base;
+        bool finite = true;
+#pragma unroll
+        for (int a = 0; a < D; ++a) finite = finite & (bool)isfinite(X[a]);
+        int c[D];
+        bool valid = live && finite;
+#pragma unroll
+        for (int a = 0; a < D; ++a) {
+            c[a] = finite ? cell_of(X[a], inv_dx) : -1;
+            valid = valid & (c[a] >= 0) & (c[a] <= counts - 1);
+        }
+        if (live && !valid) {
+            p.keycur[i] = -1;
+            if (!finite) atomicOr(&p.sst[si].nonfinite, 1);
         }
         int slot = -1, key = -1;
         if (valid) {
-            const SceneParams& sc = p.scn[si];
             int local = 0, cell = 0;
 #pragma unroll
             for (int a = 0; a < D; ++a) {
-                local = local * sc.nb + (c[a] + B) / B;
+                local = local * nb + (c[a] + B) / B;
                 cell = cell * B + (c[a] + B) % B;
             }
-            slot = sc.slot_base + local;
+            slot = slot_base + local;
             key = slot * 64 + cell;
         }
         // wave-aggregated atomics: one per distinct (block, cell) for the rank in
@@ -345,13 +353,12 @@ __global__ __launch_bounds__(WG) void k_bin(KP p) {
             todo &= ~grp;
         }
         if (valid) {
-            const SceneParams& sc = p.scn[si];
             p.prank[i] = rank;
             p.keycur[i] = key;
             int lo[3] = {0, 0, 0}, hi[3] = {0, 0, 0};
 #pragma unroll
             for (int a = 0; a < D; ++a) { lo[a] = (c[a] + B - 1) / B; hi[a] = (c[a] + B + 1) / B; }
-            const int nb = sc.nb, sb = sc.slot_base;
+            const int sb = slot_base;
             if constexpr (D == 3) {
                 for (int x = lo[0]; x <= hi[0]; ++x)
                     for (int y = lo[1]; y <= hi[1]; ++y)
@@ -2425,14 +2432,21 @@ __global__ __launch_bounds__(WG) void k_g2p_reduce(KP p) {
 #pragma unroll
     for (int u = 0; u < U; ++u) { ba[u] = 0u; ra[u] = 0u; }
     for (int base = b0 + (int)threadIdx.x; base < b1; base += U * WG) {
+        // the chunk's loads all go out before the first use (a block past the
+        // range reads the range's last one and is not counted)
+        unsigned bv[U], rv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
-            const int b = base + u * WG;
-            if (b < b1) {
-                ba[u] = max(ba[u], p.bu[b]);
-                ra[u] += p.bu[p.bcap + b];
-            }
+            const int b = min(base + u * WG, b1 - 1);
+            bv[u] = p.bu[b];
+            rv[u] = p.bu[p.bcap + b];
         }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (base + u * WG < b1) {
+                ba[u] = max(ba[u], bv[u]);
+                ra[u] += rv[u];
+            }
     }
     unsigned best = 0u, removed = 0u;
 #pragma unroll
