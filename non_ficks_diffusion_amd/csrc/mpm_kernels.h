@@ -15,7 +15,7 @@ namespace mpm {
 constexpr int WG = 256;          // work-group size of every kernel
 // floats per particle of the CG force operator record (see OpRec)
 constexpr int op_record_floats(int D) { return D == 3 ? 16 : 12; }
-constexpr int SCAN_PER_THREAD = 16;
+constexpr int SCAN_PER_THREAD = 4;  // one int4 per array and thread
 constexpr int SCAN_TILE = WG * SCAN_PER_THREAD;
 constexpr int MAX_PLANES = 8;
 constexpr unsigned MPM_FLAG_INTERIOR_DEV = 1u;  // Cell_Type_Interior (Flag_Setup_Helper.h:32)

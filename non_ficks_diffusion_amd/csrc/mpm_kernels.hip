@@ -397,6 +397,26 @@ __device__ inline int wg_exclusive_scan(int v, int& total) {
 // Slots scanned: [0, slot_hw) -- an arena keeps its scenes' ranges low.
 __device__ inline int scanned_slots(const KP& p) { return min(p.nslots, p.ctr->slot_hw); }
 
+// A thread's SCAN_PER_THREAD consecutive slots from `base`: marks and particle
+// counts, one int4 each where all of them are below ns (base is a multiple
+// of 4 and the arrays are 256-byte aligned), zeros past ns.
+__device__ inline void load_slots(const KP& p, int base, int ns, int* act, int* cnt) {
+    static_assert(SCAN_PER_THREAD == 4, "one int4 per array");
+    if (base + SCAN_PER_THREAD <= ns) {
+        const int4 a = *reinterpret_cast<const int4*>(p.slot_touched + base);
+        const int4 c = *reinterpret_cast<const int4*>(p.slot_count + base);
+        act[0] = a.x; act[1] = a.y; act[2] = a.z; act[3] = a.w;
+        cnt[0] = c.x; cnt[1] = c.y; cnt[2] = c.z; cnt[3] = c.w;
+    } else {
+#pragma unroll
+        for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+            const int s = base + k;
+            act[k] = s < ns ? p.slot_touched[s] : 0;
+            cnt[k] = s < ns ? p.slot_count[s] : 0;
+        }
+    }
+}
+
 __global__ __launch_bounds__(WG) void k_scan_reduce(KP p) {
     const int ns = scanned_slots(p);
     if ((int64_t)blockIdx.x * SCAN_TILE >= ns) {  // beyond the scanned slots: empty part
@@ -407,14 +427,13 @@ __global__ __launch_bounds__(WG) void k_scan_reduce(KP p) {
         return;
     }
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
+    int at[SCAN_PER_THREAD], ct[SCAN_PER_THREAD];
+    load_slots(p, base, ns, at, ct);
     int act = 0, cnt = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_PER_THREAD; ++k) {
-        const int s = base + k;
-        if (s < ns) {
-            act += p.slot_touched[s] != 0;
-            cnt += p.slot_count[s];
-        }
+        act += at[k] != 0;
+        cnt += ct[k];
     }
     int ta, tc;
     wg_exclusive_scan(act, ta);
@@ -468,12 +487,11 @@ __global__ __launch_bounds__(WG) void k_scan_apply(KP p) {
     if ((int64_t)blockIdx.x * SCAN_TILE >= ns) return;
     const int base = blockIdx.x * SCAN_TILE + threadIdx.x * SCAN_PER_THREAD;
     int act[SCAN_PER_THREAD], cnt[SCAN_PER_THREAD];
+    load_slots(p, base, ns, act, cnt);
     int sa = 0, sc = 0;
 #pragma unroll
     for (int k = 0; k < SCAN_PER_THREAD; ++k) {
-        const int s = base + k;
-        act[k] = (s < ns) ? (p.slot_touched[s] != 0) : 0;
-        cnt[k] = (s < ns) ? p.slot_count[s] : 0;
+        act[k] = act[k] != 0;
         sa += act[k];
         sc += cnt[k];
     }
@@ -483,25 +501,37 @@ __global__ __launch_bounds__(WG) void k_scan_apply(KP p) {
     // after an overflow no block is set up, so the binning's cell counters of
     // the touched slots are cleared here instead of by the block set-up
     const bool overflow = p.ctr->overflow != 0;
+    int sact[SCAN_PER_THREAD];
 #pragma unroll
     for (int k = 0; k < SCAN_PER_THREAD; ++k) {
         const int s = base + k;
-        if (s >= ns) break;
-        if (act[k]) {
-            p.slot_active[s] = ra;
+        sact[k] = act[k] ? ra : -1;
+        if (act[k] && s < ns) {
             if (ra < p.bcap) {
                 p.block_slot[ra] = s;
                 p.block_pstart[ra] = rc;
             }
             if (overflow)
                 for (int cell = 0; cell < 64; ++cell) p.cell_count[s * 64 + cell] = 0;
-        } else {
-            p.slot_active[s] = -1;
         }
-        p.slot_touched[s] = 0;
-        p.slot_count[s] = 0;
         ra += act[k];
         rc += cnt[k];
+    }
+    // the thread's slots: active index, and the binning's marks cleared
+    if (base + SCAN_PER_THREAD <= ns) {
+        *reinterpret_cast<int4*>(p.slot_active + base) = make_int4(sact[0], sact[1], sact[2], sact[3]);
+        *reinterpret_cast<int4*>(p.slot_touched + base) = make_int4(0, 0, 0, 0);
+        *reinterpret_cast<int4*>(p.slot_count + base) = make_int4(0, 0, 0, 0);
+    } else {
+#pragma unroll
+        for (int k = 0; k < SCAN_PER_THREAD; ++k) {
+            const int s = base + k;
+            if (s < ns) {
+                p.slot_active[s] = sact[k];
+                p.slot_touched[s] = 0;
+                p.slot_count[s] = 0;
+            }
+        }
     }
 }
 
