@@ -237,13 +237,9 @@ void stage_bin(mpm_context* c) {
     timed(c, KC_SCAN, [&] { mpm::launch_scan(p, c->stream); });
     timed(c, KC_SETUP, [&] {
         mpm::launch_block_setup<D>(p, c->stream, c->grid_block);
-        mpm::launch_scene_ranges(p, c->stream);
-        mpm::launch_block_order(p, c->stream);
+        mpm::launch_setup_order(p, c->stream, c->grid_particle);
     });
-    timed(c, KC_REORDER, [&] {
-        mpm::launch_cell_order(p, c->stream, c->grid_particle);
-        mpm::launch_reorder<D>(p, c->stream, c->grid_particle);
-    });
+    timed(c, KC_REORDER, [&] { mpm::launch_reorder<D>(p, c->stream, c->grid_particle); });
     c->cur ^= 1;
     c->records_fresh = false;
     c->binned = true;

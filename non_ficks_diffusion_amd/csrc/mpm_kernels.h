@@ -184,10 +184,9 @@ template <int D> void launch_bin(const KP& p, hipStream_t s, int grid);
 void launch_scan(const KP& p, hipStream_t s);
 void launch_copy(const float4* src, float4* dst, int64_t n4, hipStream_t s);
 template <int D> void launch_block_setup(const KP& p, hipStream_t s, int grid);
-void launch_scene_ranges(const KP& p, hipStream_t s);
 // block_order: every XCD range of the active blocks sorted by descending particle count
-void launch_block_order(const KP& p, hipStream_t s);
-void launch_cell_order(const KP& p, hipStream_t s, int grid);
+// block order, scene ranges and the in-cell order scatter (cell_groups work-groups)
+void launch_setup_order(const KP& p, hipStream_t s, int cell_groups);
 void launch_poll(const KP& p, hipStream_t s, int* dst);
 template <int D> void launch_reorder(const KP& p, hipStream_t s, int grid);
 template <int D> void launch_p2g(const KP& p, hipStream_t s, int grid);

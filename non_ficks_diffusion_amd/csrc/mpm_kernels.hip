@@ -589,8 +589,8 @@ __global__ __launch_bounds__(WG) void k_block_setup(KP p) {
 
 // Per scene: its active block range and particle range (active blocks are
 // in slot order, so each scene's blocks are contiguous).
-__global__ __launch_bounds__(WG) void k_scene_ranges(KP p) {
-    const int si = blockIdx.x * WG + threadIdx.x;
+__device__ inline void scene_ranges(const KP& p, int bid) {
+    const int si = bid * WG + (int)threadIdx.x;
     if (si >= p.nscenes) return;
     const int nb = p.ctr->nblocks;
     auto lower = [&](int slot) {
@@ -622,10 +622,10 @@ __global__ __launch_bounds__(WG) void k_scene_ranges(KP p) {
 // dispatch order then start the long blocks first and leave short ones for
 // the tail.  Only which work-group computes a block changes, never a
 // block's result.
-__global__ __launch_bounds__(WG) void k_block_order(KP p) {
+__device__ inline void block_order(const KP& p, int bid) {
     constexpr int NBK = WG;  // buckets, one per thread for the scan
     __shared__ int hist[NBK];
-    const int n = p.ctr->nblocks, x = (int)(blockIdx.x & 7), by_cell = (int)(blockIdx.x >> 3);
+    const int n = p.ctr->nblocks, x = bid & 7, by_cell = bid >> 3;
     const int lo = (int)(((int64_t)n * x) >> 3), hi = (int)(((int64_t)n * (x + 1)) >> 3);
     int* order = p.block_order + (by_cell ? p.bcap : 0);
     auto key = [&](int b) {  // the longest first
@@ -651,14 +651,28 @@ __global__ __launch_bounds__(WG) void k_block_order(KP p) {
 // range below its own, which orders every cell by the particles' previous
 // positions (a stable sort by cell) and makes the layout, and every sum over
 // a cell's particles, the same from run to run.
-__global__ __launch_bounds__(WG) void k_cell_order(KP p) {
+__device__ inline void cell_order(const KP& p, int bid, int nbid) {
     if (p.ctr->overflow != 0) return;
     const int n = p.ctr->n_prev;
-    for (int i = blockIdx.x * WG + threadIdx.x; i < n; i += gridDim.x * WG) {
+    for (int i = bid * WG + (int)threadIdx.x; i < n; i += nbid * WG) {
         const int key = p.keycur[i];
         if (key < 0) continue;
         p.cell_order[p.cell_start[p.slot_active[key >> 6] * 65 + (key & 63)] + p.prank[i]] = i;
     }
+}
+
+// The three set-up steps that follow the block set-up and do not depend on
+// one another, in one launch: work-groups 0-15 sort the blocks (the XCD of
+// work-group w is w mod 8, as for the block kernels), the next ceil(scenes /
+// WG) find the scenes' ranges, the rest scatter the particles' indices into
+// their cells.
+constexpr int ORDER_GROUPS = 16;
+__global__ __launch_bounds__(WG) void k_setup_order(KP p) {
+    const int sr = (p.nscenes + WG - 1) / WG;
+    const int bid = (int)blockIdx.x;
+    if (bid < ORDER_GROUPS) block_order(p, bid);
+    else if (bid < ORDER_GROUPS + sr) scene_ranges(p, bid - ORDER_GROUPS);
+    else cell_order(p, bid - ORDER_GROUPS - sr, (int)gridDim.x - ORDER_GROUPS - sr);
 }
 
 template <int D>
@@ -2742,12 +2756,11 @@ void launch_scan(const KP& p, hipStream_t s) {
     hipLaunchKernelGGL(k_scan_apply, dim3(p.nparts), dim3(WG), 0, s, p);
 }
 template <int D> void launch_block_setup(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_block_setup<D>, dim3(grid), dim3(WG), 0, s, p); }
-void launch_scene_ranges(const KP& p, hipStream_t s) {
-    hipLaunchKernelGGL(k_scene_ranges, dim3((p.nscenes + WG - 1) / WG), dim3(WG), 0, s, p);
+void launch_setup_order(const KP& p, hipStream_t s, int cell_groups) {
+    const int grid = ORDER_GROUPS + (p.nscenes + WG - 1) / WG + cell_groups;
+    hipLaunchKernelGGL(k_setup_order, dim3(grid), dim3(WG), 0, s, p);
 }
-void launch_block_order(const KP& p, hipStream_t s) { hipLaunchKernelGGL(k_block_order, dim3(16), dim3(WG), 0, s, p); }
 void launch_poll(const KP& p, hipStream_t s, int* dst) { hipLaunchKernelGGL(k_poll, dim3(1), dim3(64), 0, s, p, dst); }
-void launch_cell_order(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_cell_order, dim3(grid), dim3(WG), 0, s, p); }
 template <int D> void launch_reorder(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_reorder<D>, dim3(grid), dim3(WG), 0, s, p); }
 template <int D> void launch_p2g(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_p2g<D>, dim3(grid), dim3(BWG), 0, s, p); }
 template <int D> void launch_normalize(const KP& p, hipStream_t s, int grid) { hipLaunchKernelGGL(k_normalize<D>, dim3(grid), dim3(WG), 0, s, p); }
