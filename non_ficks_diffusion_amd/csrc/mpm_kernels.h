@@ -101,7 +101,7 @@ struct KP {
     int* cell_order;  // per particle slot of the binned layout: the index it is filled from
     float* rcache;   // rotation R per particle (sorted order), field-major
     float* pop;      // per particle (sorted order) op_record_floats(D) floats:
-                     // X and the linearised CG force operator
+                     // the stencil offsets and the linearised CG force operator
     int64_t pcap;
     // slots and blocks
     int nslots;        // all scenes

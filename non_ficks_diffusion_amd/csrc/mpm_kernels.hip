@@ -835,7 +835,7 @@ __device__ inline void store_operator_eos(const KP& p, int i, float c, const flo
 // T -> A is linear and fixed for the whole solve, so it is evaluated once
 // per substep as a D^2 x D^2 matrix (columns = A of the unit T's), which is
 // symmetric (dP is the Hessian of the energy: V0 dP(dF):dF' = V0 dP(dF'):dF)
-// and is stored after the particle position as its upper triangle.
+// and is stored after the particle's stencil offsets as its upper triangle.
 // One CG application then costs D^4 multiply-adds per particle instead of
 // re-deriving the stress state, and reads one contiguous record.
 // ============================================================================
